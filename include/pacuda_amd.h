@@ -1,0 +1,119 @@
+/* SPDX-License-Identifier: 0BSD */
+/*
+ * librecurrence.so for MI355X: entry points beyond runner/pacman_recurrence.h.
+ *
+ *  - pacman_recurrence_v1_test_sources: the test-only source entry the reference defines outside
+ *    its header (runner/recurrence_runner.cu:1811-1834), used by tests/check_sources.py:95.
+ *  - the fused evaluation (capability bit 3, PACUDA_CAP_FUSED): whole plan runs, from device-
+ *    resident external momenta to device-resident |M|^2 outputs, of one or many plans in one
+ *    persistent launch. No reference counterpart; the reference's nearest path is a
+ *    pacman_recurrence_v1_run(KEEP | FORMS_ON_DEVICE) followed by
+ *    pacman_recurrence_v1_reduce_resolved(OUTPUT_ZERO) and the caller's helicity sum
+ *    (umami/umami_pacuda.c:644-747, 853-892), whose results it reproduces bit for bit.
+ *  - device memory helpers, so a caller without a GPU framework can hold inputs in HBM.
+ */
+#ifndef PACUDA_AMD_H
+#define PACUDA_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pacman_recurrence.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* runner/recurrence_runner.cu:1811: the source wavefunctions of `count` (template code, mass,
+ * momentum) items on the device; 4 components per item in out_re/out_im (unused ones 0) */
+int pacman_recurrence_v1_test_sources(const uint32_t *codes, const double *masses, const double *momenta,
+                                      uint32_t count, double *out_re, double *out_im);
+
+/* bit 3 of pacman_recurrence_v1_capabilities(): pacuda_plan_create / pacuda_evaluate */
+#define PACUDA_CAP_FUSED 8u
+
+typedef struct pacuda_plan pacuda_plan;
+
+/*
+ * A recorded plan and its reduction, as one colour of a .pacuda file holds them
+ * (convert/src/format.rs:15-24): the words with their widths, the table sizes, the scales and
+ * factors, the momentum forms over the external legs, the reduction of amplitude planes into
+ * helicity outputs, and the process group factor. Every plane the program reads must be written
+ * by it first (sources, clears, initialising rows); the amplitudes start at zero.
+ */
+typedef struct {
+    uint32_t abi_version;  /* PACMAN_RECURRENCE_ABI */
+    uint32_t struct_size;  /* sizeof(pacuda_plan_desc) */
+    const uint64_t *words;
+    uint32_t word_count;
+    uint8_t width[8];
+    uint32_t current_planes, amplitude_planes, form_count;
+    const double *scale_re, *scale_im;
+    uint32_t scale_count;
+    const double *factor_re, *factor_im;
+    uint32_t factor_count;
+    uint32_t external_count;
+    uint32_t term_count;
+    const uint32_t *form_terms;      /* (first term, term count) per form */
+    const uint32_t *term_slots;
+    const double *term_coefficients;
+    uint32_t outputs;                /* helicity outputs */
+    const uint32_t *offsets;         /* outputs + 1 */
+    const uint32_t *planes;
+    const double *weights;
+    double numerator, denominator;   /* group factor: v * num / den when they differ */
+    uint32_t no_sort;                /* 1: the contribution rows in plan order */
+} pacuda_plan_desc;
+
+/* 0 OK, 2 invalid description, 3 device error */
+int pacuda_plan_create(const pacuda_plan_desc *desc, pacuda_plan **plan);
+void pacuda_plan_destroy(pacuda_plan *plan);
+/* the plan's instruction count after compilation and the scratch doubles one wavefront needs */
+int pacuda_plan_info(const pacuda_plan *plan, uint32_t *instructions, uint64_t *slot_doubles);
+
+/*
+ * One batch of points of one plan. external: device, p(leg, k) of point i at
+ * external[(leg * 4 + k) * ext_stride + i] (the momenta the forms read, crossing applied);
+ * resolved: device [point][outputs] (each output from zero, before the group factor) or NULL;
+ * total: device, per point, 0 + v[0] + v[1] + ... with v = (r * num) / den when num != den,
+ * else r, or NULL.
+ */
+typedef struct {
+    const pacuda_plan *plan;
+    const double *external;
+    uint32_t points, ext_stride;
+    double *resolved;
+    double *total;
+} pacuda_item;
+
+typedef struct {
+    uint32_t abi_version;   /* PACMAN_RECURRENCE_ABI */
+    uint32_t struct_size;   /* sizeof(pacuda_launch) */
+    uint32_t kernel;        /* PACMAN_RECURRENCE_KERNEL_* */
+    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 4 */
+    uint32_t block;         /* threads per workgroup, a multiple of 64 up to 1024, 0: 256 */
+    uint32_t reserved;
+    void *stream;           /* hipStream_t to queue on, or NULL: the library's stream */
+    float *kernel_ms;       /* not NULL: synchronise and store the fused kernel's time (HIP events) */
+} pacuda_launch;
+
+/*
+ * Evaluates every item in one persistent launch: each wavefront takes 64-point units of the
+ * items off a device work queue and runs forms, sources, program and reduction for them with its
+ * arena in an on-device scratch slot. Results equal the v1 path bit for bit for the same kernel.
+ * Asynchronous on the stream unless kernel_ms is set. 0 OK, 2 invalid, 3 device error.
+ */
+int pacuda_evaluate(const pacuda_item *items, uint32_t count, const pacuda_launch *launch);
+
+/* device memory for callers without a GPU framework (NULL / nonzero on failure) */
+void *pacuda_device_alloc(size_t bytes);
+void pacuda_device_free(void *p);
+int pacuda_copy_to_device(void *dst, const void *src, size_t bytes);
+int pacuda_copy_to_host(void *dst, const void *src, size_t bytes);
+int pacuda_synchronize(void);
+/* compute units of device 0 (the persistent grid is sized from it) */
+int pacuda_device_compute_units(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

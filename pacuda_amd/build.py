@@ -1,0 +1,62 @@
+# SPDX-License-Identifier: 0BSD
+"""Builds librecurrence.so for gfx950 in-tree (pacuda_amd/lib/) with hipcc, and the CPU oracle
+(oracle/, with make) that the tests and the benchmark's CPU baseline use as the checker.
+
+Flags: -ffp-contract=off (no fused multiply-add unless written: the f64 kernel is bit-exact to
+the CPU), denormals kept, packed FP32 instructions off."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_DIR = os.path.join(HERE, "lib")
+LIBRARY = os.path.join(LIB_DIR, "librecurrence.so")
+SOURCES = ["runner.hip", "fused.hip"]
+ARCH = os.environ.get("PACUDA_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}",
+    "-ffp-contract=off", "-fno-fast-math", "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if c and (os.path.isabs(c) and os.path.exists(c) or not os.path.isabs(c)):
+            return c
+    return "hipcc"
+
+
+def build_library(verbose=False):
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = LIBRARY + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
+           *[os.path.join(HERE, "csrc", s) for s in SOURCES], "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIBRARY)
+    return LIBRARY
+
+
+def build_oracle(verbose=False):
+    targets = ["all"]
+    if os.path.exists("/root/reference/runner/rectest.c"):
+        targets.append("ref")
+    cmd = ["make", "-s", "-C", os.path.join(ROOT, "oracle"), *targets]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=False):
+    build_oracle(verbose)
+    return build_library(verbose)
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

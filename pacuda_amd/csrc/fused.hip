@@ -1,0 +1,446 @@
+// SPDX-License-Identifier: 0BSD
+//
+// The fused evaluation (include/pacuda_amd.h): whole plan runs of one or many plans in one
+// persistent launch.
+//
+// Layout in HBM: the external momenta stay where the caller put them (SoA planes of points);
+// each resident wavefront owns a scratch slot holding its arena for 64 points, plane-major with
+// 64 doubles (512 bytes) per plane: currents re/im, amplitudes re/im, momentum forms. A slot is
+// reused for every 64-point unit the wavefront takes, so the arena's footprint is
+// (resident wavefronts x slot size), not (points x plan size): it stays in the L2 of the XCD
+// the wavefront runs on and in the MALL, and HBM carries the momenta in and |M|^2 out.
+//
+// Work queue: the items' 64-point units are numbered item by item; a wavefront takes the next
+// unit with one atomic add of lane 0, so plans of uneven length and items of uneven size
+// balance across the compute units. The grid is sized to the resident wavefronts only; every
+// wavefront leaves when the counter passes the last unit.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "pacman_recurrence.h"
+#include "pacuda_amd.h"
+#include "program.hpp"
+#include "runtime.hpp"
+
+using namespace pacuda;
+
+namespace {
+
+struct PlanDev {
+    const Instr *prog;
+    uint32_t count, sources;
+    const double *c64;  // scales re, im, factors re, im (f64)
+    const double *cdf;  // the same in the df64 layout
+    uint32_t scale_count, factor_count;
+    uint32_t current_planes, amplitude_planes, form_count, outputs;
+    const uint32_t *form_terms, *term_slots;
+    const double *coefficients;
+    const uint32_t *offsets, *planes;
+    const double *weights;
+    double numerator, denominator;
+    uint32_t factor, pad;
+};
+
+struct ItemDev {
+    const PlanDev *plan;
+    const double *external;
+    double *resolved, *total;
+    uint32_t points, ext_stride;
+};
+
+constexpr int LANES = 64;
+
+template <bool DF> PCD double amplitude_value(const double *p, size_t at) {
+    if constexpr (DF) {
+        float2 v = reinterpret_cast<const float2 *>(p)[at];
+        return (double)v.x + (double)v.y;
+    } else {
+        return p[at];
+    }
+}
+
+__global__ void reset_counter(uint32_t *counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
+}
+
+template <class K>
+__global__ void __launch_bounds__(256) fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start,
+                                                    uint32_t item_count, uint32_t *counter, double *scratch,
+                                                    size_t slot_doubles) {
+    typedef typename K::real R;
+    constexpr bool DF = !__is_same(R, double) || !__is_same(typename K::den, double);
+    constexpr bool DF_FORMS = __is_same(typename K::den, D);
+    const uint32_t lane = threadIdx.x & (LANES - 1);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / LANES) + threadIdx.x / LANES);
+    double *slot = scratch + (size_t)wave * slot_doubles;
+    const uint32_t units = cnst(unit_start)[item_count];
+    for (;;) {
+        uint32_t taken = 0;
+        if (lane == 0) taken = atomicAdd(counter, 1u);
+        const uint32_t unit = __builtin_amdgcn_readfirstlane(taken);
+        if (unit >= units) break;
+        // the item of this unit (uniform binary search)
+        uint32_t lo = 0, hi = item_count;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) / 2;
+            if (cnst(unit_start)[mid] <= unit) lo = mid;
+            else hi = mid;
+        }
+        const ItemDev it = cnst(items)[lo];
+        const PlanDev pl = *cnst(it.plan);
+        const uint32_t i = (unit - cnst(unit_start)[lo]) * LANES + lane;
+        const bool active = i < it.points;
+        const size_t ie = active ? i : it.points - 1;  // inactive lanes repeat the last point
+        const uint32_t C = pl.current_planes, A = pl.amplitude_planes, F = pl.form_count;
+        Arena a;
+        a.cre = slot;
+        a.cim = slot + (size_t)C * LANES;
+        a.fcre = reinterpret_cast<float *>(a.cre);
+        a.fcim = reinterpret_cast<float *>(a.cim);
+        a.are = a.cim + (size_t)C * LANES;
+        a.aim = a.are + (size_t)A * LANES;
+        double *mom = a.aim + (size_t)A * LANES;
+        a.mom = mom;
+        a.ld = LANES;
+        // the momentum forms in f64 (runner/recurrence_runner.cu:745-764)
+        for (uint32_t f = 0; f < F; f++) {
+            const uint32_t first = cnst(pl.form_terms)[2 * f], n = cnst(pl.form_terms)[2 * f + 1];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                double v = 0.0;
+                for (uint32_t t = 0; t < n; t++) {
+                    double term = cnst(pl.coefficients)[first + t] *
+                                  it.external[((size_t)cnst(pl.term_slots)[first + t] * 4 + k) * it.ext_stride + ie];
+                    v = t == 0 ? 0.0 + term : v + term;
+                }
+                mom[(f * 4 + k) * LANES + lane] = v;
+            }
+        }
+        for (uint32_t q = 0; q < A; q++) {
+            a.are[q * LANES + lane] = 0.0;
+            a.aim[q * LANES + lane] = 0.0;
+        }
+        // sources in f64 from the f64 constants, then the program in the kernel's layout
+        const uint32_t S = pl.scale_count, X = pl.factor_count;
+        a.sre = pl.c64; a.sim = pl.c64 + S; a.fre = pl.c64 + 2 * S; a.fim = pl.c64 + 2 * S + X;
+        run_sources_point<R>(pl.prog, pl.sources, a, lane);
+        if constexpr (DF_FORMS) {
+            for (uint32_t q = 0; q < 4 * F; q++) {
+                size_t at = q * LANES + lane;
+                str(mom, at, narrow<D>(mom[at]));
+            }
+        }
+        a.sre = pl.cdf; a.sim = pl.cdf + S; a.fre = pl.cdf + 2 * S; a.fim = pl.cdf + 2 * S + X;
+        interpret<K>(pl.prog + pl.sources, pl.count - pl.sources, a, lane);
+        // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
+        double t = 0.0;
+        for (uint32_t o = 0; o < pl.outputs; o++) {
+            double acc = 0.0;
+            for (uint32_t k = cnst(pl.offsets)[o]; k < cnst(pl.offsets)[o + 1]; k++) {
+                size_t at = (size_t)cnst(pl.planes)[k] * LANES + lane;
+                double re = amplitude_value<DF>(a.are, at), im = amplitude_value<DF>(a.aim, at);
+                acc = acc + cnst(pl.weights)[k] * __builtin_fma(re, re, im * im);
+            }
+            if (it.resolved && active) it.resolved[(size_t)i * pl.outputs + o] = acc;
+            double v = pl.factor ? (acc * pl.numerator) / pl.denominator : acc;
+            t = t + v;
+        }
+        if (it.total && active) it.total[i] = t;
+    }
+}
+
+}  // namespace
+
+struct pacuda_plan {
+    PlanDev host;         // with device pointers
+    PlanDev *device = nullptr;
+    std::vector<void *> allocations;
+    uint64_t slot_doubles = 0;
+};
+
+namespace {
+
+template <class T> T *upload(pacuda_plan *p, const T *src, size_t n, bool *ok) {
+    if (!*ok) return nullptr;
+    T *d = nullptr;
+    if (hip_failed(hipMalloc((void **)&d, std::max<size_t>(n, 1) * sizeof(T)), "plan") ||
+        (n && hip_failed(hipMemcpy(d, src, n * sizeof(T), hipMemcpyHostToDevice), "plan"))) {
+        *ok = false;
+        return nullptr;
+    }
+    p->allocations.push_back(d);
+    return d;
+}
+
+// every current plane the program reads has been written by it before
+bool reads_after_writes(const std::vector<Instr> &ops, uint32_t planes) {
+    std::vector<char> w(planes, 0);
+    auto written = [&](uint32_t base, uint32_t n) {
+        for (uint32_t k = 0; k < n; k++)
+            if (!w[base + k]) return false;
+        return true;
+    };
+    auto mark = [&](uint32_t base, uint32_t n) { for (uint32_t k = 0; k < n; k++) w[base + k] = 1; };
+    for (const Instr &o : ops) {
+        const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff, flags = o.w[0] >> 16;
+        if (kind == K_SOURCE) {
+            uint32_t family = o.w[5] & 7;
+            mark(o.w[1], family == 0 ? 1 : family == 1 ? 2 : 4);
+        } else if (kind == K_CLEAR) {
+            mark(o.w[1], o.w[2]);
+        } else if (kind == K_CONTRIB) {
+            int c0, c1, cd;
+            contribution_shape(op, &c0, &c1, &cd);
+            if (!written(o.w[2], c0) || !written(o.w[3], c1)) return false;
+            if ((flags & F_FIRST) && !(flags & F_INIT) && !written(o.w[4], cd)) return false;
+            if (flags & F_LAST) mark(o.w[4], cd);
+        } else if (kind == K_PROP) {
+            if (!written(o.w[1], op == 9 ? 4 : 2)) return false;
+        } else if (kind == K_IDENTITY) {
+            if (!written(o.w[1], o.w[2])) return false;
+        } else if (kind == K_CLOSURE) {
+            if (!written(o.w[1], o.w[6]) || !written(o.w[2], o.w[6])) return false;
+        }
+    }
+    return true;
+}
+
+struct FusedState {
+    std::mutex lock;
+    double *scratch = nullptr;
+    size_t scratch_doubles = 0;
+    uint32_t *counter = nullptr;
+    ItemDev *items = nullptr;
+    uint32_t *unit_start = nullptr;
+    size_t item_capacity = 0;
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+FusedState fs;
+
+int compute_units() {
+    static int cus = 0;
+    if (!cus) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || v <= 0) v = 256;
+        cus = v;
+    }
+    return cus;
+}
+
+}  // namespace
+
+extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) {
+    if (!out) return PACMAN_RECURRENCE_INVALID;
+    *out = nullptr;
+    if (!d || d->abi_version != PACMAN_RECURRENCE_ABI || d->struct_size != sizeof(pacuda_plan_desc) || !d->words ||
+        !d->offsets || (d->form_count && !d->form_terms) || (d->term_count && (!d->term_slots || !d->term_coefficients)) ||
+        (d->scale_count && (!d->scale_re || !d->scale_im)) || (d->factor_count && (!d->factor_re || !d->factor_im)))
+        return PACMAN_RECURRENCE_INVALID;
+    const uint32_t count = d->offsets[d->outputs];
+    if (d->offsets[0] != 0 || (count && (!d->planes || !d->weights))) return PACMAN_RECURRENCE_INVALID;
+    for (uint32_t o = 0; o < d->outputs; o++)
+        if (d->offsets[o] > d->offsets[o + 1]) return PACMAN_RECURRENCE_INVALID;
+    for (uint32_t k = 0; k < count; k++)
+        if (d->planes[k] >= d->amplitude_planes) return PACMAN_RECURRENCE_INVALID;
+    for (uint32_t f = 0; f < d->form_count; f++)
+        if (d->form_terms[2 * f] > d->term_count || d->form_terms[2 * f + 1] > d->term_count - d->form_terms[2 * f])
+            return PACMAN_RECURRENCE_INVALID;
+    for (uint32_t t = 0; t < d->term_count; t++)
+        if (d->term_slots[t] >= d->external_count) return PACMAN_RECURRENCE_INVALID;
+    HostProgram hp = build_program(d->words, d->word_count, d->width, !d->no_sort);
+    if (hp.error) {
+        fprintf(stderr, "pacman recurrence runner: %s\n", hp.error);
+        return PACMAN_RECURRENCE_INVALID;
+    }
+    if (!program_fits(hp.need, d->current_planes, d->amplitude_planes, d->form_count, d->scale_count, d->factor_count)) {
+        fprintf(stderr, "pacman recurrence runner: the program indexes past the plan's planes or tables\n");
+        return PACMAN_RECURRENCE_INVALID;
+    }
+    if (!reads_after_writes(hp.ops, d->current_planes)) {
+        fprintf(stderr, "pacman recurrence runner: the program reads a current plane before writing it\n");
+        return PACMAN_RECURRENCE_INVALID;
+    }
+    pacuda_plan *p = new pacuda_plan();
+    const uint32_t S = d->scale_count, X = d->factor_count;
+    std::vector<double> c64(2 * (size_t)(S + X)), cdf;
+    for (uint32_t k = 0; k < S; k++) { c64[k] = d->scale_re[k]; c64[S + k] = d->scale_im[k]; }
+    for (uint32_t k = 0; k < X; k++) { c64[2 * S + k] = d->factor_re[k]; c64[2 * S + X + k] = d->factor_im[k]; }
+    cdf = c64;
+    for (double &v : cdf) {  // df64 (hi, lo) = ((float)x, (float)(x - hi)) in the double's 8 bytes
+        float hl[2];
+        hl[0] = (float)v;
+        hl[1] = (float)(v - (double)hl[0]);
+        memcpy(&v, hl, 8);
+    }
+    bool ok = true;
+    PlanDev &h = p->host;
+    memset(&h, 0, sizeof h);
+    h.prog = upload(p, hp.ops.data(), hp.ops.size(), &ok);
+    h.count = (uint32_t)hp.ops.size();
+    h.sources = hp.sources;
+    h.c64 = upload(p, c64.data(), c64.size(), &ok);
+    h.cdf = upload(p, cdf.data(), cdf.size(), &ok);
+    h.scale_count = S;
+    h.factor_count = X;
+    h.current_planes = d->current_planes;
+    h.amplitude_planes = d->amplitude_planes;
+    h.form_count = d->form_count;
+    h.outputs = d->outputs;
+    h.form_terms = upload(p, d->form_terms, 2 * (size_t)d->form_count, &ok);
+    h.term_slots = upload(p, d->term_slots, d->term_count, &ok);
+    h.coefficients = upload(p, d->term_coefficients, d->term_count, &ok);
+    h.offsets = upload(p, d->offsets, (size_t)d->outputs + 1, &ok);
+    h.planes = upload(p, d->planes, count, &ok);
+    h.weights = upload(p, d->weights, count, &ok);
+    h.numerator = d->numerator;
+    h.denominator = d->denominator;
+    h.factor = d->numerator != d->denominator;
+    p->device = upload(p, &h, 1, &ok);
+    p->slot_doubles = (uint64_t)LANES * (2ull * d->current_planes + 2ull * d->amplitude_planes + 4ull * d->form_count);
+    if (!ok) {
+        pacuda_plan_destroy(p);
+        return PACMAN_RECURRENCE_DEVICE_ERROR;
+    }
+    *out = p;
+    return PACMAN_RECURRENCE_OK;
+}
+
+extern "C" void pacuda_plan_destroy(pacuda_plan *p) {
+    if (!p) return;
+    (void)hipDeviceSynchronize();
+    for (void *a : p->allocations) (void)hipFree(a);
+    delete p;
+}
+
+extern "C" int pacuda_plan_info(const pacuda_plan *p, uint32_t *instructions, uint64_t *slot_doubles) {
+    if (!p) return PACMAN_RECURRENCE_INVALID;
+    if (instructions) *instructions = p->host.count;
+    if (slot_doubles) *slot_doubles = p->slot_doubles;
+    return PACMAN_RECURRENCE_OK;
+}
+
+extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const pacuda_launch *l) {
+    if (!l || l->abi_version != PACMAN_RECURRENCE_ABI || l->struct_size != sizeof(pacuda_launch) ||
+        l->kernel > PACMAN_RECURRENCE_KERNEL_MIXED64 || (l->block % LANES) != 0 || l->block > 1024 ||
+        l->waves_per_cu > 64 || (count && !items))
+        return PACMAN_RECURRENCE_INVALID;
+    if (count == 0) return PACMAN_RECURRENCE_OK;
+    std::lock_guard<std::mutex> guard(fs.lock);
+    std::vector<ItemDev> host(count);
+    std::vector<uint32_t> starts(count + 1, 0);
+    uint64_t slot = 0, units = 0;
+    for (uint32_t k = 0; k < count; k++) {
+        const pacuda_item &it = items[k];
+        if (!it.plan || (it.points && (!it.external || it.ext_stride < it.points))) return PACMAN_RECURRENCE_INVALID;
+        host[k].plan = it.plan->device;
+        host[k].external = it.external;
+        host[k].resolved = it.resolved;
+        host[k].total = it.total;
+        host[k].points = it.points;
+        host[k].ext_stride = it.ext_stride;
+        starts[k] = (uint32_t)units;
+        units += (it.points + LANES - 1) / LANES;
+        slot = std::max<uint64_t>(slot, it.plan->slot_doubles);
+    }
+    if (units >= 0xffffffffull) return PACMAN_RECURRENCE_INVALID;
+    starts[count] = (uint32_t)units;
+    if (units == 0) return PACMAN_RECURRENCE_OK;
+    const uint32_t block = l->block ? l->block : 256;
+    const uint32_t per_cu = l->waves_per_cu ? l->waves_per_cu : 4;
+    const uint32_t waves_per_block = block / LANES;
+    uint64_t waves = (uint64_t)compute_units() * per_cu;
+    waves = std::min<uint64_t>(waves, units);  // no more wavefronts than units
+    const uint32_t grid = (uint32_t)((waves + waves_per_block - 1) / waves_per_block);
+    waves = (uint64_t)grid * waves_per_block;
+    const size_t need = (size_t)waves * slot;
+    hipStream_t s = l->stream ? (hipStream_t)l->stream : runner_stream();
+    if (!s) return PACMAN_RECURRENCE_DEVICE_ERROR;
+    if (need > fs.scratch_doubles) {
+        if (fs.scratch) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(fs.scratch);
+        }
+        fs.scratch = nullptr;
+        fs.scratch_doubles = 0;
+        if (hip_failed(hipMalloc((void **)&fs.scratch, need * sizeof(double)), "scratch")) return PACMAN_RECURRENCE_DEVICE_ERROR;
+        fs.scratch_doubles = need;
+    }
+    if (count > fs.item_capacity) {
+        (void)hipDeviceSynchronize();
+        if (fs.items) (void)hipFree(fs.items);
+        if (fs.unit_start) (void)hipFree(fs.unit_start);
+        fs.items = nullptr;
+        fs.unit_start = nullptr;
+        fs.item_capacity = 0;
+        if (hip_failed(hipMalloc((void **)&fs.items, count * sizeof(ItemDev)), "items") ||
+            hip_failed(hipMalloc((void **)&fs.unit_start, (count + 1) * sizeof(uint32_t)), "items"))
+            return PACMAN_RECURRENCE_DEVICE_ERROR;
+        fs.item_capacity = count;
+    }
+    if (!fs.counter && hip_failed(hipMalloc((void **)&fs.counter, 64), "counter")) return PACMAN_RECURRENCE_DEVICE_ERROR;
+    if (!fs.start && (hip_failed(hipEventCreate(&fs.start), "event") || hip_failed(hipEventCreate(&fs.stop), "event")))
+        return PACMAN_RECURRENCE_DEVICE_ERROR;
+    // the item table: no earlier launch on the stream may still read it, and the host copies go
+    // out of scope on return, so wait for the stream and copy synchronously
+    if (hip_failed(hipStreamSynchronize(s), "items") ||
+        hip_failed(hipMemcpy(fs.items, host.data(), count * sizeof(ItemDev), hipMemcpyHostToDevice), "items") ||
+        hip_failed(hipMemcpy(fs.unit_start, starts.data(), (count + 1) * sizeof(uint32_t), hipMemcpyHostToDevice), "items"))
+        return PACMAN_RECURRENCE_DEVICE_ERROR;
+    reset_counter<<<1, 64, 0, s>>>(fs.counter);
+    if (l->kernel_ms) (void)hipEventRecord(fs.start, s);
+    switch (l->kernel) {
+    case PACMAN_RECURRENCE_KERNEL_F64:
+        fused_kernel<KF64><<<grid, block, 0, s>>>(fs.items, fs.unit_start, count, fs.counter, fs.scratch, slot);
+        break;
+    case PACMAN_RECURRENCE_KERNEL_MIXED:
+        fused_kernel<KMixed><<<grid, block, 0, s>>>(fs.items, fs.unit_start, count, fs.counter, fs.scratch, slot);
+        break;
+    case PACMAN_RECURRENCE_KERNEL_MIXED64:
+        fused_kernel<KMixed64><<<grid, block, 0, s>>>(fs.items, fs.unit_start, count, fs.counter, fs.scratch, slot);
+        break;
+    default:
+        fused_kernel<KDF64><<<grid, block, 0, s>>>(fs.items, fs.unit_start, count, fs.counter, fs.scratch, slot);
+        break;
+    }
+    if (hip_failed(hipGetLastError(), "fused launch")) return PACMAN_RECURRENCE_DEVICE_ERROR;
+    if (l->kernel_ms) {
+        if (hip_failed(hipEventRecord(fs.stop, s), "event") || hip_failed(hipEventSynchronize(fs.stop), "kernel"))
+            return PACMAN_RECURRENCE_DEVICE_ERROR;
+        float ms = 0;
+        if (hip_failed(hipEventElapsedTime(&ms, fs.start, fs.stop), "event")) return PACMAN_RECURRENCE_DEVICE_ERROR;
+        *l->kernel_ms = ms;
+    }
+    return PACMAN_RECURRENCE_OK;
+}
+
+extern "C" void *pacuda_device_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hip_failed(hipMalloc(&p, bytes ? bytes : 1), "alloc")) return nullptr;
+    return p;
+}
+extern "C" void pacuda_device_free(void *p) {
+    if (p) (void)hipFree(p);
+}
+extern "C" int pacuda_copy_to_device(void *dst, const void *src, size_t bytes) {
+    hipStream_t s = runner_stream();
+    if (!s || hip_failed(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "copy") ||
+        hip_failed(hipStreamSynchronize(s), "copy"))
+        return PACMAN_RECURRENCE_DEVICE_ERROR;
+    return PACMAN_RECURRENCE_OK;
+}
+extern "C" int pacuda_copy_to_host(void *dst, const void *src, size_t bytes) {
+    hipStream_t s = runner_stream();
+    if (!s || hip_failed(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s), "copy") ||
+        hip_failed(hipStreamSynchronize(s), "copy"))
+        return PACMAN_RECURRENCE_DEVICE_ERROR;
+    return PACMAN_RECURRENCE_OK;
+}
+extern "C" int pacuda_synchronize(void) {
+    return hip_failed(hipDeviceSynchronize(), "synchronize") ? PACMAN_RECURRENCE_DEVICE_ERROR : PACMAN_RECURRENCE_OK;
+}
+extern "C" int pacuda_device_compute_units(void) { return compute_units(); }

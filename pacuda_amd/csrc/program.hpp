@@ -1,0 +1,194 @@
+// SPDX-License-Identifier: 0BSD
+//
+// Plan words -> device instructions (host side), and the interpreter loop (device side) that
+// one lane runs for its point.
+//
+// Host: decode the 64-bit words with the plan's field widths (runner/recurrence_runner.cu:
+// 914-975), stably sort each run of contribution rows by destination when that cannot change a
+// value (:977-1007), then emit one 32-byte instruction per row with F_FIRST/F_LAST marking the
+// rows of one destination that a lane accumulates in registers (:1009-1054).
+//
+// Device: the instruction stream is the same for every lane of a wavefront, so each
+// instruction is fetched with scalar loads into SGPRs and dispatched by a uniform branch; the
+// arena values are vector loads of one plane for 64 consecutive points.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "device.hpp"
+
+namespace pacuda {
+
+enum { K_CLEAR = 1, K_CONTRIB, K_PROP, K_IDENTITY, K_CLOSURE, K_SOURCE, K_SYMJIT };
+enum { F_INIT = 1, F_FIRST = 2, F_LAST = 4 };
+
+// w[0] = kind | op << 8 | flags << 16; contributions: w[1] scale slot, w[2] parent 0, w[3]
+// parent 1, w[4] destination, w[5] form 0, w[6] form 1, w[7] factor; clear: w[1] base, w[2]
+// count; propagator: w[1] base, w[2] form, w[3] factor; identity: w[1] base, w[2] count, w[3]
+// factor; closure: w[1] parent 0, w[2] parent 1, w[3] amplitude, w[4] factor, w[5] first
+// component factor, w[6] count; source: w[1] destination, w[2] form, w[3] factor, w[4] mass
+// slot, w[5] template.
+struct alignas(32) Instr {
+    uint32_t w[8];
+};
+
+struct HostRow {
+    int op;
+    uint32_t f[8];
+    std::vector<uint32_t> data;
+};
+
+// a program ready for upload, or an error message; `need` holds one more than the largest index
+// the program touches per table (the call's tables must be at least that long)
+enum { N_CURRENT, N_AMPLITUDE, N_FORM, N_SCALE, N_FACTOR, N_COUNT };
+struct HostProgram {
+    std::vector<Instr> ops;
+    uint32_t sources = 0;
+    uint64_t need[N_COUNT] = {0, 0, 0, 0, 0};
+    const char *error = nullptr;
+};
+// true if a call with these table sizes keeps every index of the program in bounds
+inline bool program_fits(const uint64_t *need, uint64_t currents, uint64_t amplitudes, uint64_t forms, uint64_t scales,
+                         uint64_t factors) {
+    return need[N_CURRENT] <= currents && need[N_AMPLITUDE] <= amplitudes && need[N_FORM] <= forms &&
+           need[N_SCALE] <= scales && need[N_FACTOR] <= factors;
+}
+
+HostProgram build_program(const uint64_t *words, uint32_t count, const uint8_t *width, bool sort);
+void contribution_shape(int op, int *c0, int *c1, int *cd);
+
+// ---- the interpreter (device) ----------------------------------------------------------------
+
+template <class K, int OP>
+PCD void contribution(const Arena &a, const Instr &o, int flags, Cx<typename K::real> *acc, size_t i) {
+    typedef typename K::real R;
+    typedef typename K::den Den;
+    typedef Cx<R> C;
+    constexpr int c0 = Shape<OP>::c0, c1 = Shape<OP>::c1, cd = Shape<OP>::cd;
+    const size_t ld = a.ld;
+    const uint32_t p0 = o.w[2], p1 = o.w[3], dst = o.w[4];
+    C scale = mul(cx(ldc<R>(a.sre, o.w[1]), ldc<R>(a.sim, o.w[1])), cx(ldc<R>(a.fre, o.w[7]), ldc<R>(a.fim, o.w[7])));
+    C l[c0], r[c1], v[cd];
+    R q0[4], q1[4];
+#pragma unroll
+    for (int k = 0; k < c0; k++) l[k] = Cur<R>::ld(a, (p0 + k) * ld + i);
+#pragma unroll
+    for (int k = 0; k < c1; k++) r[k] = Cur<R>::ld(a, (p1 + k) * ld + i);
+    if constexpr (OP == 6) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            q0[k] = Form<R, Den>::ld(a.mom, (o.w[5] * 4 + k) * ld + i);
+            q1[k] = Form<R, Den>::ld(a.mom, (o.w[6] * 4 + k) * ld + i);
+        }
+    }
+    contribution_values<OP>(l, r, q0, q1, v);
+    if (!(flags & F_FIRST)) {
+#pragma unroll
+        for (int k = 0; k < cd; k++) acc[k] = add(acc[k], mul(v[k], scale));
+    } else if (flags & F_INIT) {
+#pragma unroll
+        for (int k = 0; k < cd; k++) acc[k] = mul(v[k], scale);
+    } else {
+#pragma unroll
+        for (int k = 0; k < cd; k++) acc[k] = add(Cur<R>::ld(a, (dst + k) * ld + i), mul(v[k], scale));
+    }
+    if (flags & F_LAST) {
+#pragma unroll
+        for (int k = 0; k < cd; k++) Cur<R>::st(a, (dst + k) * ld + i, acc[k]);
+    }
+}
+
+template <class K, int OP>
+PCD void propagator(const Arena &a, const Instr &o, size_t i) {
+    typedef typename K::real R;
+    typedef typename K::den Den;
+    typedef Cx<R> C;
+    constexpr int m = OP == 9 ? 4 : 2;
+    const size_t ld = a.ld;
+    const uint32_t base = o.w[1];
+    R p[4];
+    Den q[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        size_t at = (o.w[2] * 4 + k) * ld + i;
+        q[k] = ldr<Den>(a.mom, at);
+        p[k] = Form<R, Den>::ld(a.mom, at);
+    }
+    R reciprocal = ToReal<R, Den>::of(reciprocal_minkowski_norm(q[0], q[1], q[2], q[3]));
+    C c[m], v[m];
+#pragma unroll
+    for (int k = 0; k < m; k++) c[k] = Cur<R>::ld(a, (base + k) * ld + i);
+    propagator_values<OP>(c, p, reciprocal, cx(ldc<R>(a.fre, o.w[3]), ldc<R>(a.fim, o.w[3])), v);
+#pragma unroll
+    for (int k = 0; k < m; k++) Cur<R>::st(a, (base + k) * ld + i, v[k]);
+}
+
+// One lane runs `count` instructions for its point i (runner/recurrence_runner.cu:787-838).
+template <class K>
+__device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count, const Arena &a, size_t i) {
+    typedef typename K::real R;
+    typedef typename K::amp Amp;
+    typedef Cx<R> C;
+    const size_t ld = a.ld;
+    C acc[6];
+    for (uint32_t n = 0; n < count; n++) {
+        const Instr o = cnst(prog)[n];
+        const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff, flags = o.w[0] >> 16;
+        if (kind == K_CONTRIB) {
+            switch (op) {
+            case 6: contribution<K, 6>(a, o, flags, acc, i); break;
+            case 4: contribution<K, 4>(a, o, flags, acc, i); break;
+            case 5: contribution<K, 5>(a, o, flags, acc, i); break;
+            case 2: contribution<K, 2>(a, o, flags, acc, i); break;
+            default: contribution<K, 3>(a, o, flags, acc, i); break;
+            }
+        } else if (kind == K_PROP) {
+            if (op == 9) propagator<K, 9>(a, o, i);
+            else if (op == 7) propagator<K, 7>(a, o, i);
+            else propagator<K, 8>(a, o, i);
+        } else if (kind == K_CLOSURE) {
+            R rfr = ldc<R>(a.fre, o.w[4]), rfi = ldc<R>(a.fim, o.w[4]);
+            size_t at = o.w[3] * ld + i;
+            Amp dr = ldr<Amp>(a.are, at), di = ldr<Amp>(a.aim, at);
+            for (uint32_t k = 0; k < o.w[6]; k++) {
+                C v = closure_increment(Cur<R>::ld(a, (o.w[1] + k) * ld + i), Cur<R>::ld(a, (o.w[2] + k) * ld + i),
+                                        ldc<R>(a.fre, o.w[5] + k), ldc<R>(a.fim, o.w[5] + k), rfr, rfi);
+                dr = dr + widen(v.re);
+                di = di + widen(v.im);
+            }
+            str(a.are, at, dr);
+            str(a.aim, at, di);
+        } else if (kind == K_IDENTITY) {
+            R fr = ldc<R>(a.fre, o.w[3]), fi = ldc<R>(a.fim, o.w[3]);
+            for (uint32_t k = 0; k < o.w[2]; k++) {
+                size_t at = (o.w[1] + k) * ld + i;
+                Cur<R>::st(a, at, identity_value(fr, fi, Cur<R>::ld(a, at)));
+            }
+        } else if (kind == K_CLEAR) {
+            for (uint32_t k = 0; k < o.w[2]; k++) Cur<R>::st(a, (o.w[1] + k) * ld + i, cx(R(0.0), R(0.0)));
+        }
+    }
+}
+
+// The leading source instructions for point i, in f64 from f64 forms, scales and factors, then
+// narrowed to the kernel's current layout (runner/recurrence_runner.cu:766-785).
+template <class R>
+__device__ inline void run_sources_point(const Instr *__restrict__ prog, uint32_t count, const Arena &a, size_t i) {
+    const size_t ld = a.ld;
+    for (uint32_t n = 0; n < count; n++) {
+        const Instr o = cnst(prog)[n];
+        double p[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) p[k] = a.mom[(o.w[2] * 4 + k) * ld + i];
+        double fr = ldc<double>(a.fre, o.w[3]), fi = ldc<double>(a.fim, o.w[3]), mass = ldc<double>(a.sre, o.w[4]);
+        W v[4];
+        int c = source_values(o.w[5], mass, p, v);
+        for (int k = 0; k < c; k++) {
+            W s = cx(fr * v[k].re - fi * v[k].im, fr * v[k].im + fi * v[k].re);
+            Cur<R>::st(a, (o.w[1] + k) * ld + i, cx(narrow<R>(s.re), narrow<R>(s.im)));
+        }
+    }
+}
+
+}  // namespace pacuda

@@ -1,0 +1,320 @@
+# SPDX-License-Identifier: 0BSD
+"""Recurrence plans in the runner's op vocabulary, and their 64-bit words.
+
+The reference records its plans from pyAmpliCol/RustiCol LC artifacts (convert/src/record.rs),
+neither of which exists here; this module generates the same kind of plan itself: a
+colour-ordered Berends-Giele recursion over all helicity configurations of one colour ordering,
+written with the rows of runner/recurrence_runner.cu:15-36 (legend convert/CONVENTIONS.md):
+
+  op 12 source        the external wavefunction of a leg and helicity (HELAS conventions)
+  op 6  three-vector  the colour-ordered three-gluon vertex of two sub-currents
+  op 5  wedge         J1 ^ J2, the auxiliary antisymmetric-tensor current of the four-gluon vertex
+  op 10 identity      the tensor's constant propagator (i/2)
+  op 4  tensor-vector T . J, the tensor's coupling back to a vector current
+  op 9  propagator    -i / P^2 (Feynman gauge)
+  op 11 closure       A = eps(leg 0) . K(1..n-1), K the amputated current of the other legs
+
+The four-gluon vertex V4(J1, J2, J3) = (i/2)[2 (J1.J3) J2 - (J1.J2) J3 - (J2.J3) J1] equals
+(i/2)[op4(J1^J2, J3) - op4(J2^J3, J1)] (op4(A^B, C) = B (A.C) - A (B.C)), so the sum over the
+three-way splits of a range becomes tensor currents of the sub-ranges. The three-gluon vertex in
+Dixon's colour-ordered normalisation is (i/sqrt 2) times minus op 6 for conserved currents.
+With these constants |A|^2 of every MHV configuration equals the Parke-Taylor value
+s_ij^4 / |s_12 s_23 ... s_n1| (g = 1); tests/test_physics.py checks it.
+
+Word layout and widths follow runner/recurrence_runner.cu:10-36 and convert/src/program.rs:117-162
+(widths as RustiCol sets them: rusticol/patches/0004-...patch:498-504)."""
+import itertools
+import math
+
+import numpy as np
+
+SQRT_HALF = math.sqrt(0.5)
+
+# the 10-bit source template (runner/recurrence_runner.cu:25-28)
+FAMILY_SCALAR, FAMILY_WEYL, FAMILY_DIRAC, FAMILY_VECTOR = 0, 1, 2, 3
+
+
+def source_code(family, orientation=0, helicity=0, chirality=0):
+    return family | orientation << 3 | (helicity + 2) << 5 | (chirality + 1) << 8
+
+
+def bits(v):
+    """64 - clz(max(v, 1)): the width of a field holding values below v (or v itself)."""
+    return max(int(v), 1).bit_length()
+
+
+class Plan:
+    """A plan run template: rows, table sizes, constants, forms, reduction, helicity labels."""
+
+    def __init__(self, legs):
+        self.legs = legs
+        self.rows = []
+        self.current_planes = 0
+        self.amplitude_planes = 0
+        self.forms = []          # per form: [(leg, coefficient)]
+        self._form_index = {}
+        self.scales = []         # complex
+        self.factors = []        # complex
+        self._factor_index = {}
+        self.outputs = []        # per output: [(amplitude plane, weight)]
+        self.helicities = []     # per output: the helicity of each leg (source-code helicities)
+        self.numerator = 1
+        self.denominator = 1
+        self.crossing = None     # per leg: (input leg, sign)
+        self.name = ""
+
+    # ---- tables ----
+    def current(self, components):
+        base = self.current_planes
+        self.current_planes += components
+        return base
+
+    def amplitude(self):
+        a = self.amplitude_planes
+        self.amplitude_planes += 1
+        return a
+
+    def form(self, legs):
+        key = tuple(legs)
+        if key not in self._form_index:
+            self._form_index[key] = len(self.forms)
+            self.forms.append([(leg, 1.0) for leg in key])
+        return self._form_index[key]
+
+    def scale(self, value):
+        self.scales.append(complex(value))
+        return len(self.scales) - 1
+
+    def factor(self, value):
+        value = complex(value)
+        key = (value.real, value.imag)
+        if key not in self._factor_index:
+            self._factor_index[key] = len(self.factors)
+            self.factors.append(value)
+        return self._factor_index[key]
+
+    def factor_run(self, values):
+        """consecutive factors (a closure's component factors), not shared"""
+        first = len(self.factors)
+        for v in values:
+            self.factors.append(complex(v))
+        return first
+
+    # ---- words ----
+    def widths(self):
+        w = [0] * 8
+        w[0] = bits(self.current_planes)
+        w[1] = bits(max(self.amplitude_planes - 1, 0))
+        w[2] = bits(max(len(self.forms) - 1, 0))
+        w[3] = bits(max(len(self.factors) - 1, 0))
+        w[4] = bits(max(len(self.scales) - 1, 0))
+        w[5] = 4
+        return w
+
+    def words(self):
+        return encode(self.rows, self.widths())
+
+    # ---- the arrays the runner takes ----
+    def arrays(self):
+        """numpy arrays of the tables (scales, factors, forms, reduction)."""
+        form_terms, slots, coeffs = [], [], []
+        for f in self.forms:
+            form_terms += [len(slots), len(f)]
+            for leg, c in f:
+                slots.append(leg)
+                coeffs.append(c)
+        offsets, planes, weights = [0], [], []
+        for out in self.outputs:
+            for p, w in out:
+                planes.append(p)
+                weights.append(w)
+            offsets.append(len(planes))
+        return dict(
+            words=np.array(self.words(), dtype=np.uint64),
+            width=np.array(self.widths(), dtype=np.uint8),
+            scale_re=np.array([s.real for s in self.scales]), scale_im=np.array([s.imag for s in self.scales]),
+            factor_re=np.array([f.real for f in self.factors]), factor_im=np.array([f.imag for f in self.factors]),
+            form_terms=np.array(form_terms, dtype=np.uint32), term_slots=np.array(slots, dtype=np.uint32),
+            term_coefficients=np.array(coeffs, dtype=np.float64),
+            offsets=np.array(offsets, dtype=np.uint32), planes=np.array(planes, dtype=np.uint32),
+            weights=np.array(weights, dtype=np.float64),
+        )
+
+    def counts(self):
+        ops = {}
+        for r in self.rows:
+            key = r[1] if r[0] == "contrib" else r[0]
+            ops[key] = ops.get(key, 0) + 1
+        return ops
+
+
+def encode(rows, width):
+    """rows -> 64-bit words, fields LSB first after the 4-bit op (convert/src/program.rs:117-162)."""
+    P, A, F, X, S, K = width[:6]
+    words = []
+    for n, r in enumerate(rows):
+        kind = r[0]
+        if kind == "clear":
+            op, fields = 1, [(r[1], P), (r[2], P)]
+        elif kind == "contrib":
+            _, op, init, slot, p0, p1, dst, f0, f1, x = r
+            fields = [(init, 1), (slot, S), (p0, P), (p1, P), (dst, P), (f0, F), (f1, F), (x, X)]
+        elif kind == "prop":
+            _, op, base, form, x = r
+            fields = [(base, P), (form, F), (x, X)]
+        elif kind == "identity":
+            op, fields = 10, [(r[1], P), (r[2], K), (r[3], X)]
+        elif kind == "closure":
+            _, p0, p1, amp, x, first, count = r
+            op, fields = 11, [(p0, P), (p1, P), (amp, A), (x, X), (first, X), (count, K)]
+        elif kind == "source":
+            _, dst, form, x, mass, code = r
+            op, fields = 12, [(dst, P), (form, F), (x, X), (mass, S), (code, 10)]
+        else:
+            raise ValueError(f"row {n}: {kind}")
+        word, at = op, 4
+        for value, b in fields:
+            if value >> b:
+                raise ValueError(f"row {n}: {value} does not fit {b} bits")
+            word |= int(value) << at
+            at += b
+        if at > 64:
+            raise ValueError(f"row {n}: {at} bits")
+        words.append(word)
+    return words
+
+
+def decode_fields(word, width):
+    """the op and fields of one word (for tests); inverse of encode for ops 1-12"""
+    P, A, F, X, S, K = width[:6]
+    op = word & 15
+    w = word >> 4
+    layout = {1: [P, P], 10: [P, K, X], 11: [P, P, A, X, X, K], 12: [P, F, X, S, 10]}
+    for o in range(2, 7):
+        layout[o] = [1, S, P, P, P, F, F, X]
+    for o in range(7, 10):
+        layout[o] = [P, F, X]
+    out = []
+    for b in layout[op]:
+        out.append(w & ((1 << b) - 1))
+        w >>= b
+    return op, out
+
+
+# ---- n-gluon colour-ordered plan ------------------------------------------------------------
+
+def gluon_plan(n, incoming=(0, 1), weight=1.0):
+    """The leading-colour plan of the colour ordering (0, 1, ..., n-1) of n gluons, every helicity.
+
+    Leg 0 closes the recursion. Output o has helicities h[leg] = +1 if bit (n-1-leg) of o is set,
+    else -1 (source-code helicities: an incoming leg's physical helicity; the source flips it for
+    the negative energy of the crossed momentum). Output weights are `weight`."""
+    if n < 3:
+        raise ValueError("n >= 3")
+    p = Plan(n)
+    p.name = f"{n}g"
+    p.crossing = [(leg, -1.0 if leg in incoming else 1.0) for leg in range(n)]
+    mass0 = p.scale(0.0)
+    g3 = p.scale(complex(0.0, -SQRT_HALF))   # (i/sqrt 2) x (-1): op 6 is minus the vertex bracket
+    unit = p.scale(1.0)
+    one = p.factor(1.0)
+    minus_one = p.factor(-1.0)
+    tensor_prop = p.factor(complex(0.0, 0.5))  # the four-gluon vertex's i/2
+    hels = (-1, 1)
+    rows = p.rows
+    # sources: every leg, both helicities
+    J = {}
+    for leg in range(n):
+        f = p.form([leg])
+        for h in hels:
+            base = p.current(4)
+            rows.append(("source", base, f, one, mass0, source_code(FAMILY_VECTOR, 0, h, 0)))
+            J[(leg, leg, (h,))] = base
+    first, last = 1, n - 1
+    T = {}
+    for L in range(2, last - first + 2):
+        contrib, identities, props = [], [], []
+        ranges = [(a, a + L - 1) for a in range(first, last - L + 2)]
+        full = L == last - first + 1
+        for a, b in ranges:
+            for H in itertools.product(hels, repeat=L):
+                # the tensor current of this range (needed for ranges shorter than the full one)
+                if not full:
+                    t = p.current(6)
+                    init = 1
+                    for m in range(a, b):
+                        hl, hr = H[: m - a + 1], H[m - a + 1:]
+                        contrib.append(("contrib", 5, init, unit, J[(a, m, hl)], J[(m + 1, b, hr)], t, 0, 0, one))
+                        init = 0
+                    T[(a, b, H)] = t
+                    identities.append(("identity", t, 6, tensor_prop))
+                # the vector current: three-vertex splits, then the tensor-vector terms
+                j = p.current(4)
+                init = 1
+                for m in range(a, b):
+                    hl, hr = H[: m - a + 1], H[m - a + 1:]
+                    contrib.append(("contrib", 6, init, g3, J[(a, m, hl)], J[(m + 1, b, hr)], j,
+                                    p.form(range(a, m + 1)), p.form(range(m + 1, b + 1)), one))
+                    init = 0
+                for m in range(a, b):
+                    hl, hr = H[: m - a + 1], H[m - a + 1:]
+                    if m - a + 1 >= 2:
+                        contrib.append(("contrib", 4, 0, unit, T[(a, m, hl)], J[(m + 1, b, hr)], j, 0, 0, one))
+                    if b - m >= 2:
+                        contrib.append(("contrib", 4, 0, unit, T[(m + 1, b, hr)], J[(a, m, hl)], j, 0, 0, minus_one))
+                J[(a, b, H)] = j
+                if not full:
+                    props.append(("prop", 9, j, p.form(range(a, b + 1)), one))
+        rows += contrib + identities + props
+    # closures: A(h0, H) = eps0(h0) . K(1..n-1, H), metric (+, -, -, -) as component factors
+    metric = p.factor_run([1.0, -1.0, -1.0, -1.0])
+    amp_of = {}
+    for h0 in hels:
+        for H in itertools.product(hels, repeat=n - 1):
+            a = p.amplitude()
+            amp_of[(h0,) + H] = a
+            rows.append(("closure", J[(0, 0, (h0,))], J[(first, last, H)], a, one, metric, 4))
+    for o in range(2 ** n):
+        hel = tuple(1 if (o >> (n - 1 - leg)) & 1 else -1 for leg in range(n))
+        p.outputs.append([(amp_of[hel], weight)])
+        p.helicities.append(hel)
+    return p
+
+
+# ---- flop count (per point) ----------------------------------------------------------------
+# A real add, subtract or multiply counts 1, an fma 2, a divide or square root 1 (also counted
+# apart); negation and multiplication by i count 0 (the operations of runner/recurrence_runner.cu
+# :229-330 as written).
+CONTRIB_VALUES = {2: 60, 3: 60, 4: 88, 5: 84, 6: 142}
+CONTRIB_OUT = {2: 2, 3: 2, 4: 4, 5: 6, 6: 4}
+SOURCE_FLOPS = {FAMILY_SCALAR: 6, FAMILY_WEYL: 20, FAMILY_DIRAC: 30, FAMILY_VECTOR: 22}
+
+
+def plan_flops(plan):
+    """(flops, divides) per point of one plan run: forms, sources, program and reduction."""
+    flops, divs = 0, 0
+    for f in plan.forms:
+        flops += 4 * (2 * len(f) - 1)
+    for r in plan.rows:
+        k = r[0]
+        if k == "contrib":
+            op, init = r[1], r[2]
+            cd = CONTRIB_OUT[op]
+            flops += 6 + CONTRIB_VALUES[op] + 6 * cd + (0 if init else 2 * cd)
+        elif k == "prop":
+            op = r[1]
+            flops += 7 + (8 + 24 if op == 9 else 2 + 28 + 12)
+            divs += 1
+        elif k == "identity":
+            flops += 6 * r[2]
+        elif k == "closure":
+            flops += 20 * r[6]
+        elif k == "source":
+            code = r[5]
+            fam = code & 7
+            comps = {0: 1, 1: 2, 2: 4, 3: 4}[fam]
+            flops += SOURCE_FLOPS[fam] + 6 * comps
+            divs += 3
+    for out in plan.outputs:
+        flops += 5 * len(out) + 1
+    return flops, divs

@@ -1,0 +1,160 @@
+# SPDX-License-Identifier: 0BSD
+"""GPU parity: librecurrence.so on gfx950 against the oracle, bit for bit.
+
+- the reference's own synthetic-plan client runner/rectest.c (built into oracle/_ref) passes its
+  internal host-vs-device KATs against the HIP library for every kernel, and its printed
+  amplitudes equal the oracle's emulation of the same v1 calls byte for byte;
+- the source wavefunctions (pacman_recurrence_v1_test_sources) equal the oracle's bit for bit on
+  the case generators of tests/check_sources.py and rusticol/patches/0009 (restated here);
+- generated n-gluon plans through the v1 path and the fused path equal the oracle's whole plan
+  run bit for bit in every kernel (f64, df64, mixed, mixed64), on ragged point counts."""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from pacuda_amd import abi, plans, phase_space as ps
+import harness
+from harness import V1, FusedPlan, DeviceArray, fused_evaluate, bits_equal
+import refcpu
+import source_cases
+
+pytestmark = pytest.mark.gpu
+
+KERNEL_ARGS = [("f64",), ("f64", "nosort"), ("df64",), (), ("mixed",), ("mixed64",)]
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return abi.load()
+
+
+@pytest.mark.parametrize("args", KERNEL_ARGS, ids=lambda a: "-".join(a) or "default")
+def test_rectest_against_hip_library(args):
+    print("rectest", args, flush=True)
+    code, out, err = harness.rectest(abi.LIBRARY, *args)
+    assert code == 0, f"rectest {args}: exit {code}: {err}"
+    code_o, out_o, err_o = harness.rectest(refcpu.ABI_LIBRARY, *args)
+    assert code_o == 0, err_o
+    assert len(out.split()) == 3 * 300 * 2
+    assert out == out_o, "HIP amplitudes differ from the oracle's"
+
+
+def test_capabilities(lib):
+    caps = lib.pacman_recurrence_v1_capabilities()
+    assert caps & abi.CAP_SOURCES and caps & abi.CAP_FORMS and caps & abi.CAP_FUSED
+    assert not caps & abi.CAP_SYMJIT
+    assert lib.pacman_recurrence_v1_symjit_kernel(b"0" * 64) == -1
+
+
+def test_sources_bitwise(lib):
+    cases = source_cases.legend_cases(seed=3, per_family=300) + source_cases.patch_cases()
+    codes = np.array([c[0] for c in cases], dtype=np.uint32)
+    masses = np.array([c[1] for c in cases])
+    mom = np.array([c[2] for c in cases]).reshape(-1)
+    n = len(cases)
+    print("sources", n, flush=True)
+    re, im = np.zeros(4 * n), np.zeros(4 * n)
+    st = lib.pacman_recurrence_v1_test_sources(abi.ptr(codes, abi.u32p), abi.ptr(masses), abi.ptr(mom), n,
+                                               abi.ptr(re), abi.ptr(im))
+    assert st == 0
+    _, ore, oim = refcpu.test_sources(codes, masses, mom)
+    bad = np.nonzero((re.view(np.uint64) != ore.reshape(-1).view(np.uint64)) |
+                     (im.view(np.uint64) != oim.reshape(-1).view(np.uint64)))[0]
+    assert len(bad) == 0, f"{len(bad)} components differ, first case {bad[0] // 4}: {cases[bad[0] // 4]}"
+
+
+def _points(n, count, seed):
+    mom, _ = ps.generate(n, count, seed)
+    return mom
+
+
+@pytest.fixture(scope="module")
+def gg_ggg():
+    plan = plans.gluon_plan(5)
+    mom = _points(5, 16384, 1)
+    return plan, ps.external_planes(mom, plan.crossing)
+
+
+def test_v1_f64_gluon5_matches_oracle(lib, gg_ggg):
+    plan, ext = gg_ggg
+    v1 = V1(lib)
+    st = v1.configure_kernel(abi.KERNEL_F64)
+    assert st == 0
+    for points, stride in [(16384, 16384), (1000, 1024), (1, 64), (333, 333)]:
+        print("v1 f64 points", points, "stride", stride, flush=True)
+        got = v1.run_plan(plan, ext, points, stride)
+        want, _ = refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), ext[:, :points], points, kernel=1)
+        assert bits_equal(got, want), f"points {points}: max |diff| {np.max(np.abs(got - want))}"
+
+
+@pytest.mark.parametrize("kernel", ["f64", "df64", "mixed", "mixed64"])
+def test_fused_gluon5_matches_oracle(lib, gg_ggg, kernel):
+    plan, ext = gg_ggg
+    k = abi.KERNELS[kernel]
+    fp = FusedPlan(lib, plan)
+    points = 16384 - 37  # a ragged last unit
+    d_ext = DeviceArray(lib, ext.nbytes)
+    d_ext.upload(ext)
+    outputs = plan.amplitude_planes
+    d_res = DeviceArray(lib, points * outputs * 8)
+    d_tot = DeviceArray(lib, points * 8)
+    print("fused", kernel, points, flush=True)
+    fused_evaluate(lib, [(fp, d_ext, points, ext.shape[1], d_res, d_tot)], k)
+    res = d_res.download((points, outputs))
+    tot = d_tot.download(points)
+    want_res, want_tot = refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), ext[:, :points], points, kernel=k)
+    assert bits_equal(res, want_res), f"{kernel}: resolved differs, max {np.max(np.abs(res - want_res))}"
+    assert bits_equal(tot, want_tot), f"{kernel}: totals differ"
+    for d in (d_ext, d_res, d_tot):
+        d.free()
+    fp.destroy()
+
+
+def test_fused_multi_plan_queue(lib):
+    """several plans and uneven point counts in one persistent launch == the oracle per item"""
+    jobs, wants, devs, fps = [], [], [], []
+    for n, points, seed in [(4, 70, 11), (5, 1000, 12), (4, 1, 13), (5, 129, 14), (6, 200, 15), (5, 0, 16)]:
+        plan = plans.gluon_plan(n)
+        ext = ps.external_planes(_points(n, max(points, 1), seed), plan.crossing)
+        fp = FusedPlan(lib, plan)
+        d_ext = DeviceArray(lib, ext.nbytes)
+        d_ext.upload(ext)
+        d_tot = DeviceArray(lib, max(points, 1) * 8)
+        jobs.append((fp, d_ext, points, ext.shape[1], None, d_tot))
+        devs += [d_ext, d_tot]
+        fps.append(fp)
+        if points:
+            _, want = refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), ext[:, :points], points, kernel=1)
+        else:
+            want = np.zeros(0)
+        wants.append(want)
+    print("multi-plan launch", flush=True)
+    fused_evaluate(lib, jobs, abi.KERNEL_F64, waves_per_cu=1)
+    for (fp, _, points, _, _, d_tot), want in zip(jobs, wants):
+        got = d_tot.download(points)
+        assert bits_equal(got, want), f"{fp.plan.name} {points} points"
+    for d in devs:
+        d.free()
+    for fp in fps:
+        fp.destroy()
+
+
+def test_v1_other_kernels_in_fresh_processes(gg_ggg, tmp_path):
+    """the v1 path of df64, mixed and mixed64 (one configuration per process) == oracle"""
+    plan, ext = gg_ggg
+    points = 4096 + 5
+    np.save(tmp_path / "ext.npy", ext[:, :points])
+    for kernel in ["df64", "mixed", "mixed64"]:
+        print("v1 driver", kernel, flush=True)
+        out = tmp_path / f"{kernel}.npy"
+        p = subprocess.run([sys.executable, os.path.join(os.path.dirname(__file__), "v1_driver.py"), kernel,
+                            str(tmp_path / "ext.npy"), str(out), "5"], capture_output=True, text=True, timeout=300)
+        assert p.returncode == 0, p.stderr
+        got = np.load(out)
+        want, _ = refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), ext[:, :points], points,
+                                  kernel=abi.KERNELS[kernel])
+        assert bits_equal(got, want), f"{kernel}: max |diff| {np.max(np.abs(got - want))}"
