@@ -1,0 +1,127 @@
+# SPDX-License-Identifier: 0BSD
+"""Host-side helpers over the fused entry points of librecurrence.so (include/pacuda_amd.h):
+device buffers, compiled plans, and one-launch evaluation of many (plan, points) items."""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import ptr, u32p, u64p
+
+
+class DeviceArray:
+    """A device buffer owned by the HIP library (pacuda_device_alloc)."""
+
+    def __init__(self, lib, nbytes):
+        self.lib = lib
+        self.nbytes = int(nbytes)
+        self.ptr = lib.pacuda_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise RuntimeError(f"device allocation of {nbytes} bytes failed")
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        if self.lib.pacuda_copy_to_device(self.ptr, a.ctypes.data, a.nbytes) != 0:
+            raise RuntimeError("copy to device failed")
+
+    def download(self, shape, dtype=np.float64):
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("shape larger than the buffer")
+        if self.lib.pacuda_copy_to_host(out.ctypes.data, self.ptr, out.nbytes) != 0:
+            raise RuntimeError("copy to host failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.lib.pacuda_device_free(self.ptr)
+            self.ptr = None
+
+
+class Device:
+    def __init__(self, lib=None):
+        self.lib = lib or abi.load()
+
+    def array(self, a):
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self.lib, a.nbytes)
+        d.upload(a)
+        return d
+
+    def empty(self, count, dtype=np.float64):
+        return DeviceArray(self.lib, int(count) * np.dtype(dtype).itemsize)
+
+    def synchronize(self):
+        if self.lib.pacuda_synchronize() != 0:
+            raise RuntimeError("device synchronisation failed")
+
+
+class FusedPlan:
+    """A plan (pacuda_amd.plans.Plan) compiled into the library (pacuda_plan_create)."""
+
+    def __init__(self, lib, plan, no_sort=0):
+        self.lib = lib
+        self.plan = plan
+        a = plan.arrays()
+        self.arrays = a
+        d = abi.PlanDesc()
+        d.abi_version = abi.ABI
+        d.struct_size = C.sizeof(abi.PlanDesc)
+        d.words = ptr(a["words"], u64p)
+        d.word_count = len(a["words"])
+        for k in range(8):
+            d.width[k] = int(a["width"][k])
+        d.current_planes = plan.current_planes
+        d.amplitude_planes = plan.amplitude_planes
+        d.form_count = len(plan.forms)
+        d.scale_re, d.scale_im, d.scale_count = ptr(a["scale_re"]), ptr(a["scale_im"]), len(a["scale_re"])
+        d.factor_re, d.factor_im, d.factor_count = ptr(a["factor_re"]), ptr(a["factor_im"]), len(a["factor_re"])
+        d.external_count = plan.legs
+        d.term_count = len(a["term_slots"])
+        d.form_terms, d.term_slots = ptr(a["form_terms"], u32p), ptr(a["term_slots"], u32p)
+        d.term_coefficients = ptr(a["term_coefficients"])
+        d.outputs = len(a["offsets"]) - 1
+        d.offsets, d.planes, d.weights = ptr(a["offsets"], u32p), ptr(a["planes"], u32p), ptr(a["weights"])
+        d.numerator, d.denominator = float(plan.numerator), float(plan.denominator)
+        d.no_sort = no_sort
+        h = C.c_void_p()
+        st = lib.pacuda_plan_create(C.byref(d), C.byref(h))
+        if st != 0:
+            raise RuntimeError(f"pacuda_plan_create: status {st}")
+        self.handle = h
+
+    def info(self):
+        n, slot = C.c_uint32(), C.c_uint64()
+        self.lib.pacuda_plan_info(self.handle, C.byref(n), C.byref(slot))
+        return n.value, slot.value
+
+    def destroy(self):
+        if self.handle:
+            self.lib.pacuda_plan_destroy(self.handle)
+            self.handle = None
+
+
+def evaluate(lib, jobs, kernel, waves_per_cu=0, block=0, timed=False):
+    """jobs: [(FusedPlan, DeviceArray external, points, ext_stride, DeviceArray|None resolved,
+    DeviceArray|None total)]; one persistent launch. Returns the kernel's time in ms (HIP events)
+    when timed, else waits for the device and returns 0."""
+    items = (abi.Item * max(1, len(jobs)))()
+    for k, (fp, ext, points, stride, res, tot) in enumerate(jobs):
+        items[k].plan = fp.handle
+        items[k].external = ext.ptr
+        items[k].points = points
+        items[k].ext_stride = stride
+        items[k].resolved = res.ptr if res is not None else None
+        items[k].total = tot.ptr if tot is not None else None
+    ms = C.c_float(0)
+    l = abi.Launch(abi.ABI, C.sizeof(abi.Launch), kernel, waves_per_cu, block, 0, None,
+                   C.pointer(ms) if timed else None)
+    st = lib.pacuda_evaluate(items, len(jobs), C.byref(l))
+    if st != 0:
+        raise RuntimeError(f"pacuda_evaluate: status {st}")
+    if not timed:
+        if lib.pacuda_synchronize() != 0:
+            raise RuntimeError("device synchronisation failed")
+    return ms.value

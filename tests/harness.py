@@ -103,91 +103,8 @@ def helicity_sum(resolved, numerator=1.0, denominator=1.0):
     return t
 
 
-class DeviceArray:
-    """A device buffer owned by the HIP library (pacuda_device_alloc)."""
-
-    def __init__(self, lib, nbytes):
-        self.lib = lib
-        self.nbytes = nbytes
-        self.ptr = lib.pacuda_device_alloc(nbytes)
-        if not self.ptr:
-            raise RuntimeError("device allocation failed")
-
-    def upload(self, a):
-        a = np.ascontiguousarray(a)
-        assert a.nbytes <= self.nbytes
-        assert self.lib.pacuda_copy_to_device(self.ptr, a.ctypes.data, a.nbytes) == 0
-
-    def download(self, shape, dtype=np.float64):
-        out = np.empty(shape, dtype=dtype)
-        assert out.nbytes <= self.nbytes
-        assert self.lib.pacuda_copy_to_host(out.ctypes.data, self.ptr, out.nbytes) == 0
-        return out
-
-    def free(self):
-        if self.ptr:
-            self.lib.pacuda_device_free(self.ptr)
-            self.ptr = None
-
-
-class FusedPlan:
-    def __init__(self, lib, plan, no_sort=0):
-        self.lib = lib
-        self.plan = plan
-        a = plan.arrays()
-        self.arrays = a
-        d = abi.PlanDesc()
-        d.abi_version = abi.ABI
-        d.struct_size = C.sizeof(abi.PlanDesc)
-        d.words = ptr(a["words"], u64p)
-        d.word_count = len(a["words"])
-        for k in range(8):
-            d.width[k] = int(a["width"][k])
-        d.current_planes = plan.current_planes
-        d.amplitude_planes = plan.amplitude_planes
-        d.form_count = len(plan.forms)
-        d.scale_re, d.scale_im, d.scale_count = ptr(a["scale_re"]), ptr(a["scale_im"]), len(a["scale_re"])
-        d.factor_re, d.factor_im, d.factor_count = ptr(a["factor_re"]), ptr(a["factor_im"]), len(a["factor_re"])
-        d.external_count = plan.legs
-        d.term_count = len(a["term_slots"])
-        d.form_terms, d.term_slots = ptr(a["form_terms"], u32p), ptr(a["term_slots"], u32p)
-        d.term_coefficients = ptr(a["term_coefficients"])
-        d.outputs = len(a["offsets"]) - 1
-        d.offsets, d.planes, d.weights = ptr(a["offsets"], u32p), ptr(a["planes"], u32p), ptr(a["weights"])
-        d.numerator, d.denominator = float(plan.numerator), float(plan.denominator)
-        d.no_sort = no_sort
-        h = C.c_void_p()
-        st = lib.pacuda_plan_create(C.byref(d), C.byref(h))
-        if st != 0:
-            raise RuntimeError(f"pacuda_plan_create: status {st}")
-        self.handle = h
-
-    def destroy(self):
-        if self.handle:
-            self.lib.pacuda_plan_destroy(self.handle)
-            self.handle = None
-
-
-def fused_evaluate(lib, jobs, kernel, waves_per_cu=0, block=0, timed=False):
-    """jobs: [(FusedPlan, DeviceArray external, points, ext_stride, DeviceArray|None resolved,
-    DeviceArray|None total)]; returns the kernel time in ms when timed"""
-    items = (abi.Item * len(jobs))()
-    for k, (fp, ext, points, stride, res, tot) in enumerate(jobs):
-        items[k].plan = fp.handle
-        items[k].external = ext.ptr
-        items[k].points = points
-        items[k].ext_stride = stride
-        items[k].resolved = res.ptr if res is not None else None
-        items[k].total = tot.ptr if tot is not None else None
-    ms = C.c_float(0)
-    l = abi.Launch(abi.ABI, C.sizeof(abi.Launch), kernel, waves_per_cu, block, 0, None,
-                   C.pointer(ms) if timed else None)
-    st = lib.pacuda_evaluate(items, len(jobs), C.byref(l))
-    if st != 0:
-        raise RuntimeError(f"pacuda_evaluate: status {st}")
-    if not timed:
-        assert lib.pacuda_synchronize() == 0
-    return ms.value
+from pacuda_amd.runtime import DeviceArray, FusedPlan  # noqa: E402,F401
+from pacuda_amd.runtime import evaluate as fused_evaluate  # noqa: E402,F401
 
 
 def rectest(runner_library, *args, timeout=120):

@@ -1,0 +1,135 @@
+# SPDX-License-Identifier: 0BSD
+"""The generated plans (pacuda_amd/plans.py) run on the CPU oracle against closed forms.
+
+- every MHV and anti-MHV helicity configuration of 4, 5 and 6 gluons equals Parke-Taylor,
+  |A|^2 = s_ij^4 / |s_12 s_23 ... s_n1|, to 1e-9 relative (g = 1);
+- all-plus and one-minus configurations vanish (to 1e-20 of the point's total);
+- the colour-ordered amplitudes obey reflection, |A(1..n)|^2 = |A(n..1)|^2, and the U(1)
+  decoupling identity sum over cyclic insertions of leg 1 of A = 0, both on NMHV configurations
+  of six gluons, which Parke-Taylor does not cover;
+- the plan's words decode back to its rows (convert/src/program.rs:60-162).
+A wrong sign or constant in the three- or four-gluon vertex, a swapped operand in an op, or a
+dropped propagator fails the Parke-Taylor check."""
+import numpy as np
+import pytest
+
+from pacuda_amd import plans, phase_space as ps
+import harness
+import refcpu
+
+
+def _mdot(a, b):
+    return a[..., 0] * b[..., 0] - a[..., 1] * b[..., 1] - a[..., 2] * b[..., 2] - a[..., 3] * b[..., 3]
+
+
+def _run(plan, mom, kernel=1):
+    ext = ps.external_planes(mom, plan.crossing)
+    res, tot = refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), ext, mom.shape[0], kernel=kernel)
+    return ext, res, tot
+
+
+@pytest.mark.parametrize("n", [4, 5, 6])
+def test_parke_taylor(n):
+    plan = plans.gluon_plan(n)
+    mom, _ = ps.generate(n, 48, seed=100 + n)
+    ext, res, tot = _run(plan, mom)
+    P = mom.shape[0]
+    q = ext.reshape(n, 4, P).transpose(2, 0, 1)
+    s = np.empty((P, n, n))
+    for i in range(n):
+        for j in range(n):
+            s[:, i, j] = 2 * _mdot(q[:, i], q[:, j])
+    den = np.abs(np.prod([s[:, k, (k + 1) % n] for k in range(n)], axis=0))
+    checked = 0
+    for o, hel in enumerate(plan.helicities):
+        out = [(-h if plan.crossing[l][1] < 0 else h) for l, h in enumerate(hel)]  # outgoing helicities
+        neg = [l for l in range(n) if out[l] < 0]
+        pos = [l for l in range(n) if out[l] > 0]
+        if len(neg) == 2 or len(pos) == 2:
+            i, j = neg if len(neg) == 2 else pos
+            want = s[:, i, j] ** 4 / den
+            assert np.allclose(res[:, o], want, rtol=1e-9, atol=0), (hel, np.max(np.abs(res[:, o] / want - 1)))
+            checked += 1
+        elif len(neg) <= 1 or len(pos) <= 1:
+            assert np.all(np.abs(res[:, o]) <= 1e-20 * np.abs(tot)), hel
+    assert checked == (2 * n * (n - 1) // 2 if n > 4 else 6)
+
+
+def _amplitudes(plan, mom):
+    """complex amplitudes of every helicity configuration (v1 emulation, kept amplitudes)"""
+    ext = ps.external_planes(mom, plan.crossing)
+    lib = refcpu.load()
+    lib.refcpu_reset()
+    v1 = harness.V1(lib, prefix="refcpu_v1_")
+    assert v1.configure_kernel(1) == 0
+    re, im = v1.run_plan(plan, ext, mom.shape[0], keep=False)
+    lib.refcpu_reset()
+    return re + 1j * im  # [amplitude plane][point]
+
+
+def test_reflection_and_u1_decoupling_six_gluons():
+    n = 6
+    plan = plans.gluon_plan(n)
+    mom, _ = ps.generate(n, 16, seed=7)
+    # the amplitude of ordering sigma: permute the momenta so that plan leg k reads leg sigma[k]
+    def amps(order):
+        return _amplitudes(plan, mom[:, list(order), :])
+    # helicity configurations are the source-code helicities of the plan's legs; permuting
+    # momenta permutes which physical leg each plan leg carries (crossing signs follow legs 0, 1)
+    base = amps(range(n))
+    nmhv = [o for o, h in enumerate(plan.helicities) if sum(1 for x in h if x < 0) == 3]
+    assert nmhv
+    # reflection: A(0, 5, 4, 3, 2, 1) has the same modulus as A(0..5) for the reflected labels
+    # (crossing is applied by plan leg, so reflect only among the outgoing legs 2..5 and keep
+    # the incoming 0, 1: A(0,1,2,3,4,5) and A(1,0,5,4,3,2) are reflections of each other)
+    order = [1, 0, 5, 4, 3, 2]
+    plan_r = plans.gluon_plan(n, incoming=(0, 1))
+    refl = _amplitudes(plan_r, mom[:, order, :])
+    # the momenta at plan leg k are physical leg order[k]; their crossing: legs 0, 1 incoming
+    floor = 1e-10 * np.max(np.abs(base))  # vanishing configurations are roundoff
+    for o, hel in enumerate(plan.helicities):
+        hel_r = tuple(hel[order[k]] for k in range(n))
+        o_r = plan.helicities.index(hel_r)
+        assert np.allclose(np.abs(base[o]), np.abs(refl[o_r]), rtol=1e-9, atol=floor), hel
+    # U(1) decoupling: sum over the insertions of physical leg 1 into (0, 2, 3, 4, 5) cyclically
+    others = [0, 2, 3, 4, 5]
+    total, largest = None, None
+    for pos in range(1, n):
+        order = others[:pos] + [1] + others[pos:]
+        pl = plans.gluon_plan(n, incoming=tuple(k for k in range(n) if order[k] in (0, 1)))
+        a = _amplitudes(pl, mom[:, order, :])
+        idx = [pl.helicities.index(tuple(hel[order[k]] for k in range(n))) for hel in plan.helicities]
+        a = a[idx]
+        total = a if total is None else total + a
+        largest = np.abs(a) if largest is None else np.maximum(largest, np.abs(a))
+    for o in nmhv:
+        # relative to the largest term of the sum, point by point
+        assert np.all(np.abs(total[o]) <= 1e-11 * largest[o] + floor), plan.helicities[o]
+
+
+@pytest.mark.parametrize("n", [4, 5, 6, 7])
+def test_words_round_trip(n):
+    plan = plans.gluon_plan(n)
+    width = plan.widths()
+    words = plan.words()
+    assert len(words) == len(plan.rows)
+    for w, r in zip(words, plan.rows):
+        assert w < (1 << 64)
+        op, fields = plans.decode_fields(w, width)
+        if r[0] == "contrib":
+            assert op == r[1] and fields == list(r[2:])
+        elif r[0] == "prop":
+            assert op == r[1] and fields == list(r[2:])
+        elif r[0] == "source":
+            assert op == 12 and fields == list(r[1:])
+        elif r[0] == "closure":
+            assert op == 11 and fields == list(r[1:])
+        elif r[0] == "identity":
+            assert op == 10 and fields == list(r[1:])
+
+
+def test_plan_sizes_five_gluons():
+    plan = plans.gluon_plan(5)
+    c = plan.counts()
+    assert (c[6], c[5], c[4], c["identity"], c["prop"], c["closure"], c["source"]) == (92, 44, 96, 28, 28, 32, 10)
+    assert plan.current_planes == 384 and plan.amplitude_planes == 32 and len(plan.forms) == 10
