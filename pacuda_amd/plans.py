@@ -231,14 +231,35 @@ def gluon_plan(n, incoming=(0, 1), weight=1.0):
             rows.append(("source", base, f, one, mass0, source_code(FAMILY_VECTOR, 0, h, 0)))
             J[(leg, leg, (h,))] = base
     first, last = 1, n - 1
+    _gluon_currents(p, rows, J, first, last, True, unit, g3, one, minus_one, tensor_prop)
+    # closures: A(h0, H) = eps0(h0) . K(1..n-1, H), metric (+, -, -, -) as component factors
+    metric = p.factor_run([1.0, -1.0, -1.0, -1.0])
+    amp_of = {}
+    for h0 in hels:
+        for H in itertools.product(hels, repeat=n - 1):
+            a = p.amplitude()
+            amp_of[(h0,) + H] = a
+            rows.append(("closure", J[(0, 0, (h0,))], J[(first, last, H)], a, one, metric, 4))
+    for o in range(2 ** n):
+        hel = tuple(1 if (o >> (n - 1 - leg)) & 1 else -1 for leg in range(n))
+        p.outputs.append([(amp_of[hel], weight)])
+        p.helicities.append(hel)
+    return p
+
+
+def _gluon_currents(p, rows_out, J, first, last, full_is_amputated, unit, g3, one, minus_one, tensor_prop):
+    """Berends-Giele gluon currents J[(a, b, H)] of every range inside legs first..last (sources
+    already in J), level by level: tensor currents of sub-ranges (op 5 + op 10), vector currents
+    from three-vertex splits (op 6) and tensor-vector terms (op 4), propagators (op 9). The full
+    range is propagated unless `full_is_amputated`."""
+    hels = (-1, 1)
     T = {}
     for L in range(2, last - first + 2):
         contrib, identities, props = [], [], []
-        ranges = [(a, a + L - 1) for a in range(first, last - L + 2)]
         full = L == last - first + 1
-        for a, b in ranges:
+        for a in range(first, last - L + 2):
+            b = a + L - 1
             for H in itertools.product(hels, repeat=L):
-                # the tensor current of this range (needed for ranges shorter than the full one)
                 if not full:
                     t = p.current(6)
                     init = 1
@@ -248,7 +269,6 @@ def gluon_plan(n, incoming=(0, 1), weight=1.0):
                         init = 0
                     T[(a, b, H)] = t
                     identities.append(("identity", t, 6, tensor_prop))
-                # the vector current: three-vertex splits, then the tensor-vector terms
                 j = p.current(4)
                 init = 1
                 for m in range(a, b):
@@ -263,21 +283,85 @@ def gluon_plan(n, incoming=(0, 1), weight=1.0):
                     if b - m >= 2:
                         contrib.append(("contrib", 4, 0, unit, T[(m + 1, b, hr)], J[(a, m, hl)], j, 0, 0, minus_one))
                 J[(a, b, H)] = j
-                if not full:
+                if not (full and full_is_amputated):
                     props.append(("prop", 9, j, p.form(range(a, b + 1)), one))
-        rows += contrib + identities + props
-    # closures: A(h0, H) = eps0(h0) . K(1..n-1, H), metric (+, -, -, -) as component factors
-    metric = p.factor_run([1.0, -1.0, -1.0, -1.0])
-    amp_of = {}
-    for h0 in hels:
-        for H in itertools.product(hels, repeat=n - 1):
-            a = p.amplitude()
-            amp_of[(h0,) + H] = a
-            rows.append(("closure", J[(0, 0, (h0,))], J[(first, last, H)], a, one, metric, 4))
-    for o in range(2 ** n):
-        hel = tuple(1 if (o >> (n - 1 - leg)) & 1 else -1 for leg in range(n))
-        p.outputs.append([(amp_of[hel], weight)])
-        p.helicities.append(hel)
+        rows_out += contrib + identities + props
+    return T
+
+
+QUARK_GLUON = SQRT_HALF  # the colour-ordered quark-gluon coupling (Dixon's normalisation, g = 1)
+
+
+def quark_line_plan(n, incoming=(0, 1), weight=1.0):
+    """The colour ordering (qbar, g_1, ..., g_{n-2}, q) of one massless quark line and n-2 gluons,
+    every helicity configuration, as Weyl currents (ops 2/3, 7/8; convert/CONVENTIONS.md:80-90).
+
+    Legs in the all-outgoing convention: 0 the antiquark (closing the recursion), 1..n-2 the
+    gluons, n-1 the quark; `incoming` legs get crossed (negative-energy) momenta. The fermion
+    current is a row spinor grown from the quark: a line of chirality +1 alternates vertex op 2
+    (sigma) and propagator op 8 (sigma-bar); chirality -1 alternates op 3 and op 7; the antiquark
+    closes with the opposite chirality (op 11 over 2 components, factors 1). With the coupling
+    1/sqrt 2 every non-vanishing amplitude has |A|^2 = |s_{f- j}|^3 |s_{f+ j}| / |s_01 s_12 ...
+    s_(n-1)0| (MHV, j the odd gluon; tests/test_physics.py). Output o: chirality -1 for o < 2^(n-2),
+    then +1; within, gluon helicities as bits (leg 1 most significant)."""
+    if n < 3:
+        raise ValueError("n >= 3")
+    p = Plan(n)
+    p.name = f"qqbar+{n - 2}g"
+    p.crossing = [(leg, -1.0 if leg in incoming else 1.0) for leg in range(n)]
+    mass0 = p.scale(0.0)
+    g3 = p.scale(complex(0.0, -SQRT_HALF))
+    unit = p.scale(1.0)
+    gq = p.scale(QUARK_GLUON)
+    one = p.factor(1.0)
+    minus_one = p.factor(-1.0)
+    tensor_prop = p.factor(complex(0.0, 0.5))
+    rows = p.rows
+    J = {}
+    for leg in range(1, n - 1):
+        f = p.form([leg])
+        for h in (-1, 1):
+            base = p.current(4)
+            rows.append(("source", base, f, one, mass0, source_code(FAMILY_VECTOR, 0, h, 0)))
+            J[(leg, leg, (h,))] = base
+    quark, antiquark, code_hel = {}, {}, {}
+    for ch in (-1, 1):
+        hq = -ch if (n - 1) in incoming else ch
+        base = p.current(2)
+        rows.append(("source", base, p.form([n - 1]), one, mass0, source_code(FAMILY_WEYL, 0, hq, ch)))
+        quark[ch] = base
+        cb = -ch
+        hb = -cb if 0 in incoming else cb
+        base = p.current(2)
+        rows.append(("source", base, p.form([0]), one, mass0, source_code(FAMILY_WEYL, 1, hb, cb)))
+        antiquark[ch] = base
+        code_hel[ch] = (hb, hq)
+    if n > 3:
+        _gluon_currents(p, rows, J, 1, n - 2, False, unit, g3, one, minus_one, tensor_prop)
+    psi = {(n - 1, ch, ()): quark[ch] for ch in (-1, 1)}
+    for a in range(n - 2, 0, -1):
+        contrib, props = [], []
+        for ch in (-1, 1):
+            vertex, propagator = (2, 8) if ch == 1 else (3, 7)
+            for H in itertools.product((-1, 1), repeat=n - 1 - a):
+                d = p.current(2)
+                init = 1
+                for m in range(a, n - 1):
+                    hl, hr = H[: m - a + 1], H[m - a + 1:]
+                    contrib.append(("contrib", vertex, init, gq, psi[(m + 1, ch, hr)], J[(a, m, hl)], d, 0, 0, one))
+                    init = 0
+                psi[(a, ch, H)] = d
+                if a > 1:
+                    props.append(("prop", propagator, d, p.form(range(a, n)), one))
+        rows += contrib + props
+    components = p.factor_run([1.0, 1.0])
+    for ch in (-1, 1):
+        for H in itertools.product((-1, 1), repeat=n - 2):
+            amp = p.amplitude()
+            rows.append(("closure", antiquark[ch], psi[(1, ch, H)], amp, one, components, 2))
+            p.outputs.append([(amp, weight)])
+            hb, hq = code_hel[ch]
+            p.helicities.append((hb,) + H + (hq,))
     return p
 
 

@@ -133,3 +133,84 @@ def test_plan_sizes_five_gluons():
     c = plan.counts()
     assert (c[6], c[5], c[4], c["identity"], c["prop"], c["closure"], c["source"]) == (92, 44, 96, 28, 28, 32, 10)
     assert plan.current_planes == 384 and plan.amplitude_planes == 32 and len(plan.forms) == 10
+
+
+@pytest.mark.parametrize("n", [4, 5, 6])
+def test_quark_line_mhv(n):
+    """q qbar + (n-2) gluons: every non-vanishing helicity amplitude equals the MHV closed form
+    |s_{f- j}|^3 |s_{f+ j}| / |s_01 s_12 ... s_(n-1)0| (j the gluon of odd helicity, f-/f+ the
+    fermions of negative/positive helicity; Dixon, hep-ph/9601359), and the configurations with
+    all gluons of one helicity vanish; both beams crossed as for q qbar -> gluons."""
+    plan = plans.quark_line_plan(n, incoming=(0, n - 1))
+    mom, _ = ps.generate(n, 32, seed=200 + n)
+    # plan legs: 0 antiquark slot <- beam 0, n-1 quark slot <- beam 1, gluons <- the outgoing partons
+    order = [0] + list(range(2, n)) + [1]
+    mom = mom[:, order, :]
+    ext, res, tot = _run(plan, mom)
+    P = mom.shape[0]
+    q = ext.reshape(n, 4, P).transpose(2, 0, 1)
+    s = np.array([[2 * _mdot(q[:, i], q[:, j]) for j in range(n)] for i in range(n)])
+    den = np.abs(np.prod([s[k, (k + 1) % n] for k in range(n)], axis=0))
+    checked = 0
+    for o, hel in enumerate(plan.helicities):
+        out = [(-h if plan.crossing[l][1] < 0 else h) for l, h in enumerate(hel)]
+        fneg = 0 if out[0] < 0 else n - 1
+        fpos = n - 1 - fneg
+        assert out[0] == -out[n - 1]
+        gl = out[1:n - 1]
+        negg = [l + 1 for l, h in enumerate(gl) if h < 0]
+        posg = [l + 1 for l, h in enumerate(gl) if h > 0]
+        if len(negg) == 1:
+            j = negg[0]
+            want = np.abs(s[fneg, j]) ** 3 * np.abs(s[fpos, j]) / den
+        elif len(posg) == 1:
+            j = posg[0]
+            want = np.abs(s[fpos, j]) ** 3 * np.abs(s[fneg, j]) / den
+        elif not negg or not posg:
+            assert np.all(np.abs(res[:, o]) <= 1e-20 * np.abs(tot)), hel
+            continue
+        else:
+            continue
+        assert np.allclose(res[:, o], want, rtol=1e-9, atol=0), (hel, np.max(np.abs(res[:, o] / want - 1)))
+        checked += 1
+    assert checked >= 4
+
+
+def test_pp_jjj_subprocesses_closed_forms():
+    """every subprocess of pacuda_amd/processes.py, with its crossing: each non-vanishing
+    helicity amplitude equals Parke-Taylor (gluons) or the quark-line MHV form"""
+    from pacuda_amd import processes
+    subs = processes.subprocesses()
+    assert len(subs) == 29
+    mom, _ = ps.generate(5, 8, seed=300)
+    n = 5
+    for sp in subs:
+        plan = sp.plan
+        ext, res, tot = _run(plan, mom)
+        assert np.all(tot > 0), sp.name
+        q = ext.reshape(n, 4, 8).transpose(2, 0, 1)
+        s = np.array([[2 * _mdot(q[:, i], q[:, j]) for j in range(n)] for i in range(n)])
+        den = np.abs(np.prod([s[k, (k + 1) % n] for k in range(n)], axis=0))
+        quark = plan.name.startswith("qqbar")
+        for o, hel in enumerate(plan.helicities):
+            out = [(-h if plan.crossing[l][1] < 0 else h) for l, h in enumerate(hel)]
+            if quark:
+                fneg = 0 if out[0] < 0 else 4
+                fpos = 4 - fneg
+                negg = [l + 1 for l, h in enumerate(out[1:4]) if h < 0]
+                posg = [l + 1 for l, h in enumerate(out[1:4]) if h > 0]
+                if len(negg) == 1:
+                    want = np.abs(s[fneg, negg[0]]) ** 3 * np.abs(s[fpos, negg[0]]) / den
+                elif len(posg) == 1:
+                    want = np.abs(s[fpos, posg[0]]) ** 3 * np.abs(s[fneg, posg[0]]) / den
+                else:
+                    want = np.zeros(8)
+            else:
+                neg = [l for l in range(n) if out[l] < 0]
+                pos = [l for l in range(n) if out[l] > 0]
+                if len(neg) == 2 or len(pos) == 2:
+                    i, j = neg if len(neg) == 2 else pos
+                    want = s[i, j] ** 4 / den
+                else:
+                    want = np.zeros(8)
+            assert np.allclose(res[:, o], want, rtol=1e-9, atol=1e-18 * np.max(tot)), (sp.name, hel)
