@@ -113,6 +113,30 @@ int pacuda_synchronize(void);
 /* compute units of device 0 (the persistent grid is sized from it) */
 int pacuda_device_compute_units(void);
 
+/*
+ * The surrogate of log|M|^2: a multilayer perceptron `inputs -> hidden (x layers, SiLU) -> 1`
+ * with BF16 matrix-core products and FP32 accumulation, FP32 master weights and Adam, on the
+ * GPU. No reference counterpart. Parameters (fp32) per hidden layer l: W_l [hidden][fan_in]
+ * (fan_in = inputs for l = 0, else hidden), b_l [hidden]; then w_out [hidden], b_out.
+ * hidden: 64, 128 or 256; batches are multiples of 64; x, y and target are device pointers
+ * (x [batch][inputs] fp32, y and target [batch] fp32).
+ */
+typedef struct pacuda_mlp pacuda_mlp;
+int pacuda_mlp_create(uint32_t inputs, uint32_t hidden, uint32_t layers, uint32_t max_batch, pacuda_mlp **out);
+void pacuda_mlp_destroy(pacuda_mlp *mlp);
+uint64_t pacuda_mlp_param_count(const pacuda_mlp *mlp);
+int pacuda_mlp_set_params(pacuda_mlp *mlp, const float *host, uint64_t count);
+int pacuda_mlp_get_params(const pacuda_mlp *mlp, float *host, uint64_t count);
+int pacuda_mlp_get_grads(const pacuda_mlp *mlp, float *host, uint64_t count);
+/* y = network(x); with `train` the activations are kept for pacuda_mlp_backward (batch up to
+ * max_batch). Asynchronous on the library's stream. */
+int pacuda_mlp_forward(pacuda_mlp *mlp, const float *x, uint32_t batch, float *y, uint32_t train);
+/* gradients of mean((y - target)^2) after a training forward of the same batch; loss (host,
+ * may be NULL) receives the loss */
+int pacuda_mlp_backward(pacuda_mlp *mlp, const float *y, const float *target, uint32_t batch, float *loss);
+/* one Adam step (step counts from 1) on the master weights, then the bf16 copies */
+int pacuda_mlp_adam(pacuda_mlp *mlp, float lr, float beta1, float beta2, float eps, uint32_t step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,6 +116,15 @@ EXTENSION_SYMBOLS = [
     ("pacuda_copy_to_host", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("pacuda_synchronize", C.c_int, []),
     ("pacuda_device_compute_units", C.c_int, []),
+    ("pacuda_mlp_create", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("pacuda_mlp_destroy", None, [C.c_void_p]),
+    ("pacuda_mlp_param_count", C.c_uint64, [C.c_void_p]),
+    ("pacuda_mlp_set_params", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint64]),
+    ("pacuda_mlp_get_params", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint64]),
+    ("pacuda_mlp_get_grads", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint64]),
+    ("pacuda_mlp_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
+    ("pacuda_mlp_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
+    ("pacuda_mlp_adam", C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32]),
 ]
 
 

@@ -35,9 +35,23 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.owner:
             self.lib.pacuda_device_free(self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    owner = True
+
+    @classmethod
+    def view(cls, base, offset_bytes, nbytes=None):
+        """a non-owning window into `base` starting `offset_bytes` in"""
+        v = cls.__new__(cls)
+        v.lib = base.lib
+        v.nbytes = base.nbytes - offset_bytes if nbytes is None else nbytes
+        if offset_bytes < 0 or offset_bytes + v.nbytes > base.nbytes:
+            raise ValueError("view outside the buffer")
+        v.ptr = base.ptr + offset_bytes
+        v.owner = False
+        return v
 
 
 class Device:

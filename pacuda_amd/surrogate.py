@@ -1,0 +1,64 @@
+# SPDX-License-Identifier: 0BSD
+"""The surrogate network on the GPU (include/pacuda_amd.h, pacuda_mlp_*): a Python handle over the
+library's BF16-MFMA forward, backward and Adam kernels, and the phase-space features it reads."""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import abi
+from .runtime import DeviceArray
+
+F32P = C.POINTER(C.c_float)
+
+
+class MLP:
+    def __init__(self, inputs, hidden=256, layers=4, max_batch=4096, lib=None):
+        self.lib = lib or abi.load()
+        self.inputs, self.hidden, self.layers, self.max_batch = inputs, hidden, layers, max_batch
+        h = C.c_void_p()
+        st = self.lib.pacuda_mlp_create(inputs, hidden, layers, max_batch, C.byref(h))
+        if st != 0:
+            raise RuntimeError(f"pacuda_mlp_create: status {st}")
+        self.handle = h
+        self.count = self.lib.pacuda_mlp_param_count(h)
+        self.step = 0
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.count
+        if self.lib.pacuda_mlp_set_params(self.handle, p.ctypes.data_as(F32P), self.count) != 0:
+            raise RuntimeError("pacuda_mlp_set_params failed")
+        self.step = 0
+
+    def params(self):
+        out = np.empty(self.count, dtype=np.float32)
+        if self.lib.pacuda_mlp_get_params(self.handle, out.ctypes.data_as(F32P), self.count) != 0:
+            raise RuntimeError("pacuda_mlp_get_params failed")
+        return out
+
+    def grads(self):
+        out = np.empty(self.count, dtype=np.float32)
+        if self.lib.pacuda_mlp_get_grads(self.handle, out.ctypes.data_as(F32P), self.count) != 0:
+            raise RuntimeError("pacuda_mlp_get_grads failed")
+        return out
+
+    def forward(self, d_x, batch, d_y, train=False):
+        if self.lib.pacuda_mlp_forward(self.handle, d_x.ptr, batch, d_y.ptr, 1 if train else 0) != 0:
+            raise RuntimeError("pacuda_mlp_forward failed")
+
+    def backward(self, d_y, d_t, batch):
+        loss = C.c_float()
+        if self.lib.pacuda_mlp_backward(self.handle, d_y.ptr, d_t.ptr, batch, C.byref(loss)) != 0:
+            raise RuntimeError("pacuda_mlp_backward failed")
+        return loss.value
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.step += 1
+        if self.lib.pacuda_mlp_adam(self.handle, lr, beta1, beta2, eps, self.step) != 0:
+            raise RuntimeError("pacuda_mlp_adam failed")
+
+    def destroy(self):
+        if self.handle:
+            self.lib.pacuda_mlp_destroy(self.handle)
+            self.handle = None
