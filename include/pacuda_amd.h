@@ -137,6 +137,22 @@ int pacuda_mlp_backward(pacuda_mlp *mlp, const float *y, const float *target, ui
 /* one Adam step (step counts from 1) on the master weights, then the bf16 copies */
 int pacuda_mlp_adam(pacuda_mlp *mlp, float lr, float beta1, float beta2, float eps, uint32_t step);
 
+/*
+ * RAMBO phase space of 2 -> legs-2 massless partons in f64 on the device (convert/src/record.rs:
+ * 17-52 on a counter-based stream: point `first + i` of stream `seed`), written as the runner's
+ * external momenta external[(l * 4 + k) * stride + i] = crossing_sign[l] * p[crossing_src[l]][k];
+ * pass[i] = 1 when the outgoing partons pass pT > pt_min, |eta| < eta_max, Delta R > dr_min;
+ * features (or NULL): [points][3 (legs - 2) + 2] floats in [0, 1], the surrogate's inputs.
+ * All pointers but the crossing tables are device pointers.
+ */
+int pacuda_rambo(uint64_t seed, uint64_t first, uint32_t points, uint32_t legs, double ecm,
+                 const uint32_t *crossing_src, const double *crossing_sign, double *external, uint32_t stride,
+                 float *features, uint8_t *pass, double pt_min, double eta_max, double dr_min);
+/* out[r * count + j] = external[r * stride + index[j]] for rows r < rows; features likewise
+ * ([j][width] <- [index[j]][width]) when features is not NULL (device pointers) */
+int pacuda_gather(const double *external, uint32_t stride, uint32_t rows, const uint32_t *index, uint32_t count,
+                  double *out, const float *features, uint32_t width, float *features_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,19 @@ from .runtime import DeviceArray
 F32P = C.POINTER(C.c_float)
 
 
+def he_init(inputs, hidden, layers, seed=0):
+    """He-normal weights, zero biases, in the library's parameter layout (float32)"""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for l in range(layers):
+        fan = inputs if l == 0 else hidden
+        parts.append(rng.normal(0, np.sqrt(2.0 / fan), hidden * fan))
+        parts.append(np.zeros(hidden))
+    parts.append(rng.normal(0, np.sqrt(1.0 / hidden), hidden))
+    parts.append(np.zeros(1))
+    return np.concatenate(parts).astype(np.float32)
+
+
 class MLP:
     def __init__(self, inputs, hidden=256, layers=4, max_batch=4096, lib=None):
         self.lib = lib or abi.load()
