@@ -1,16 +1,18 @@
 # SPDX-License-Identifier: 0BSD
 """Benchmark of the recurrence runner on one MI355X.
 
-Workload (the headline configuration): g g > g g g, leading colour, one colour ordering with all
-32 helicity configurations, f64 (bit-exact to the CPU), 2^22 RAMBO points (seed 2) at
+Headline workload (`value`): g g > g g g, leading colour, one colour ordering with all 32
+helicity configurations, f64 (bit-exact to the CPU oracle), 2^22 RAMBO points (seed 2) at
 sqrt(s) = 1 TeV with pT > 20 GeV, |eta| < 2.5, dR > 0.4. A step is one pass of the hot path over
 the batch: the fused evaluation (forms, sources, the plan's program, the helicity reduction and
 sum) of every point, momenta already in HBM, |M|^2 per point back into HBM.
 
-Prints one JSON line: value = points/s; roofline of the fused f64 kernel (its time from HIP events
-recorded by the library on the kernel's stream); the CPU oracle (the same plan interpreter,
-OpenMP) timed on a bounded sample of the same points on this host; the other precision modes'
-throughput and error against f64."""
+The JSON line also carries: the roofline of the fused f64 kernel (its time from HIP events the
+library records on the kernel's stream); the CPU oracle (the same plan interpreter, OpenMP)
+timed on a bounded sample of the same points; df64/mixed/mixed64 throughput and error against
+f64; g g > g g g g (2^21 points, f64 and mixed, the persistent grid tuned by a sweep); the
+p p > j j j batch (every subprocess with at most one quark line in one persistent launch, Zipf
+point counts summing to 2^22); and surrogate vs plain unweighting of g g > g g g g."""
 import argparse
 import json
 import os
@@ -22,35 +24,68 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from pacuda_amd import abi, plans, phase_space as ps, runtime  # noqa: E402
+from pacuda_amd import abi, plans, processes, phase_space as ps  # noqa: E402
+from pacuda_amd.runtime import Device, FusedPlan, evaluate  # noqa: E402
+from pacuda_amd.unweight import Generator, Unweighting  # noqa: E402
 
 METRIC = "|M|^2 points/s (f64 bit-exact) + unweighted events/s, surrogate vs plain"
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
-FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed)
 
 
-def cpu_baseline(plan, ext, gpu_total, sample):
-    """the oracle over the first `sample` points, all host threads and one thread"""
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def points_on_device(dev, plan, n, seed):
+    """n cut-passing RAMBO points of stream `seed`, generated and compacted on the GPU"""
+    gen = Generator(dev, plan)
+    draw = int(n * 1.2) + 4096
+    ext, feat, ok = gen.draw(seed, 0, draw)
+    idx = np.nonzero(ok)[0]
+    if len(idx) < n:
+        raise RuntimeError("too few points passed the cuts")
+    out, _ = gen.gather(ext, None, draw, idx[:n])
+    ext.free()
+    feat.free()
+    return out, int(idx[n - 1]) + 1
+
+
+def timed_steps(lib, jobs, kernel, steps, waves_per_cu=0, block=0):
+    """wall seconds of `steps` launches (synchronised) and the mean kernel ms (HIP events)"""
+    ms = []
+    t = time.perf_counter()
+    for _ in range(steps):
+        ms.append(evaluate(lib, jobs, kernel, waves_per_cu=waves_per_cu, block=block, timed=True))
+    lib.pacuda_synchronize()
+    return time.perf_counter() - t, float(np.mean(ms))
+
+
+def cpu_baseline(plan, ext_host, gpu_total, sample):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refcpu
     counts = dict(current_planes=plan.current_planes, amplitude_planes=plan.amplitude_planes,
                   form_count=len(plan.forms), external_count=plan.legs)
     arrays = plan.arrays()
     threads = max(1, min(16, os.cpu_count() or 1))
-    sub = np.ascontiguousarray(ext[:, :sample])
-    refcpu.evaluate(arrays, counts, sub[:, :4096], 4096, kernel=1, threads=threads, resolved=False)  # warm-up
+    refcpu.evaluate(arrays, counts, ext_host[:, :4096].copy(), 4096, kernel=1, threads=threads, resolved=False)
     t = time.perf_counter()
-    _, tot = refcpu.evaluate(arrays, counts, sub, sample, kernel=1, threads=threads, resolved=False)
+    _, tot = refcpu.evaluate(arrays, counts, ext_host, sample, kernel=1, threads=threads, resolved=False)
     dt_all = time.perf_counter() - t
     one = min(sample, 1 << 15)
     t = time.perf_counter()
-    refcpu.evaluate(arrays, counts, np.ascontiguousarray(sub[:, :one]), one, kernel=1, threads=1, resolved=False)
+    refcpu.evaluate(arrays, counts, np.ascontiguousarray(ext_host[:, :one]), one, kernel=1, threads=1, resolved=False)
     dt_one = time.perf_counter() - t
     exact = bool(np.array_equal(tot.view(np.uint64), gpu_total[:sample].view(np.uint64)))
     return {"value": sample / dt_all, "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample} of the same points, f64, oracle/refcpu.cpp (OpenMP, {threads} threads); "
-                      f"single thread {one / dt_one:.4g} points/s",
+            "sample": f"the first {sample} of the benchmark's points, f64, oracle/refcpu.cpp plan interpreter "
+                      f"(OpenMP, {threads} threads, {dt_all:.1f} s); single thread {one / dt_one:.4g} points/s",
             "single_thread_points_per_s": one / dt_one, "f64_bitexact_on_sample": exact}
+
+
+def mode_errors(got, ref):
+    rel = np.abs(got - ref) / np.abs(ref)
+    return {"max_rel_err_vs_f64": float(np.max(rel)), "median_rel_err_vs_f64": float(np.median(rel)),
+            "max_err_rel_to_max_f64": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))}
 
 
 def main():
@@ -59,75 +94,140 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--points", type=int, default=1 << 22)
-    ap.add_argument("--legs", type=int, default=5)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-modes", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    ap.add_argument("--candidates", type=int, default=1 << 24)
+    ap.add_argument("--train-points", type=int, default=1 << 20)
+    ap.add_argument("--epochs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
-              "points with no exchange during evaluation, see DESIGN.md)", file=sys.stderr)
+              "points with no exchange during evaluation)", file=sys.stderr)
         sys.exit(2)
 
-    plan = plans.gluon_plan(args.legs)
+    dev = Device()
+    lib = dev.lib
+    plan = plans.gluon_plan(5)
     n = args.points
     t0 = time.perf_counter()
-    mom, drawn = ps.generate(args.legs, n, seed=2)
-    ext = ps.external_planes(mom, plan.crossing)
-    gen_s = time.perf_counter() - t0
-    print(f"bench: {n} points generated in {gen_s:.1f} s ({drawn} drawn)", file=sys.stderr, flush=True)
-
-    lib = abi.load()
-    dev = runtime.Device(lib)
-    fp = runtime.FusedPlan(lib, plan)
-    d_ext = dev.array(ext)
+    d_ext, drawn = points_on_device(dev, plan, n, seed=2)
+    log(f"{n} points generated on the GPU in {time.perf_counter() - t0:.2f} s ({drawn} drawn)")
+    fp = FusedPlan(lib, plan)
     d_tot = dev.empty(n)
     flops, divs = plans.plan_flops(plan)
-
-    def step(kernel, timed=True):
-        return runtime.evaluate(lib, [(fp, d_ext, n, n, None, d_tot)], kernel, waves_per_cu=args.waves_per_cu,
-                                timed=timed)
-
+    jobs = [(fp, d_ext, n, n, None, d_tot)]
     k64 = abi.KERNEL_F64
     for _ in range(args.warmup):
-        step(k64)
-    dev.synchronize()
-    ms = []
-    t = time.perf_counter()
-    for _ in range(args.steps):
-        ms.append(step(k64))
-    dev.synchronize()
-    wall = time.perf_counter() - t
+        evaluate(lib, jobs, k64, waves_per_cu=args.waves_per_cu, timed=True)
+    wall, kernel_ms = timed_steps(lib, jobs, k64, args.steps, args.waves_per_cu)
     total64 = d_tot.download(n)
     value = n * args.steps / wall
-    kernel_ms = float(np.mean(ms))
     achieved = flops * n / (kernel_ms * 1e-3) / 1e12
-    print(f"bench: f64 {value:.4g} points/s, kernel {kernel_ms:.3f} ms", file=sys.stderr, flush=True)
+    log(f"f64 {value:.4g} points/s, kernel {kernel_ms:.3f} ms, {achieved:.2f} TFLOP/s")
 
     modes = {}
     if not args.no_modes:
-        ref = total64
         for name in ("df64", "mixed", "mixed64"):
             k = abi.KERNELS[name]
-            step(k)
-            times = [step(k) for _ in range(3)]
-            got = d_tot.download(n)
-            rel = np.abs(got - ref) / np.abs(ref)
-            modes[name] = {"points_per_s": n / (np.mean(times) * 1e-3), "kernel_ms": float(np.mean(times)),
-                           "max_rel_err_vs_f64": float(np.max(rel)), "median_rel_err_vs_f64": float(np.median(rel)),
-                           "max_err_rel_to_max_f64": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))}
-            print(f"bench: {name} {modes[name]['points_per_s']:.4g} points/s", file=sys.stderr, flush=True)
+            evaluate(lib, jobs, k, timed=True)
+            _, ms = timed_steps(lib, jobs, k, 3)
+            modes[name] = {"points_per_s": n / (ms * 1e-3), "kernel_ms": ms, **mode_errors(d_tot.download(n), total64)}
+            log(f"{name} {modes[name]['points_per_s']:.4g} points/s")
+
+    extra = {}
+    if not args.no_extra:
+        # g g > g g g g: 2^21 points, the persistent grid swept (waves per CU, block)
+        p6 = plans.gluon_plan(6)
+        n6 = 1 << 21
+        e6, _ = points_on_device(dev, p6, n6, seed=5)
+        f6 = FusedPlan(lib, p6)
+        t6 = dev.empty(n6)
+        j6 = [(f6, e6, n6, n6, None, t6)]
+        sweep = {}
+        for wpc in (1, 2, 4, 8):
+            for block in (64, 256):
+                evaluate(lib, j6, k64, waves_per_cu=wpc, block=block, timed=True)
+                _, ms = timed_steps(lib, j6, k64, 2, wpc, block)
+                sweep[f"{wpc}x{block}"] = ms
+        best = min(sweep, key=sweep.get)
+        wpc, block = (int(v) for v in best.split("x"))
+        _, ms64 = timed_steps(lib, j6, k64, 3, wpc, block)
+        ref6 = t6.download(n6)
+        f6flops = plans.plan_flops(p6)[0]
+        res6 = {"points": n6, "plan_rows": len(p6.rows), "flops_per_point": f6flops, "tuned": best, "sweep_ms": sweep,
+                "f64_points_per_s": n6 / (ms64 * 1e-3),
+                "f64_tflops": f6flops * n6 / (ms64 * 1e-3) / 1e12}
+        for name in ("mixed", "mixed64", "df64"):
+            k = abi.KERNELS[name]
+            evaluate(lib, j6, k, waves_per_cu=wpc, block=block, timed=True)
+            _, ms = timed_steps(lib, j6, k, 3, wpc, block)
+            res6[name] = {"points_per_s": n6 / (ms * 1e-3), **mode_errors(t6.download(n6), ref6)}
+        extra["gg_gggg"] = res6
+        log(f"gg>gggg f64 {res6['f64_points_per_s']:.4g} points/s tuned {best}")
+        for d in (e6, t6):
+            d.free()
+        f6.destroy()
+
+        # p p > j j j: every subprocess, Zipf(1.2) counts summing to 2^22, one persistent launch
+        subs = processes.subprocesses()
+        counts = processes.zipf_counts(len(subs), 1 << 22, s=1.2, seed=1)
+        pjobs, holds = [], []
+        for k, (sp, c) in enumerate(zip(subs, counts)):
+            e, _ = points_on_device(dev, sp.plan, int(c), seed=100 + k)
+            f = FusedPlan(lib, sp.plan)
+            t = dev.empty(int(c))
+            pjobs.append((f, e, int(c), int(c), None, t))
+            holds.append((f, e, t))
+        evaluate(lib, pjobs, k64, timed=True)
+        _, ms_p = timed_steps(lib, pjobs, k64, 3)
+        refs = [j[5].download(j[2]) for j in pjobs]
+        pp = {"subprocesses": len(subs), "points": int(counts.sum()), "f64_points_per_s": int(counts.sum()) / (ms_p * 1e-3),
+              "counts_min_max": [int(counts.min()), int(counts.max())]}
+        for name in ("mixed", "mixed64"):
+            k = abi.KERNELS[name]
+            evaluate(lib, pjobs, k, timed=True)
+            _, ms = timed_steps(lib, pjobs, k, 3)
+            errs = {sp.name: mode_errors(j[5].download(j[2]), r)["max_err_rel_to_max_f64"]
+                    for sp, j, r in zip(subs, pjobs, refs)}
+            pp[name] = {"points_per_s": int(counts.sum()) / (ms * 1e-3), "worst_subprocess_err": max(errs.values()),
+                        "err_rel_to_max_f64_per_subprocess": errs}
+        extra["pp_jjj"] = pp
+        log(f"pp>jjj f64 {pp['f64_points_per_s']:.4g} points/s over {len(subs)} subprocesses")
+        for f, e, t in holds:
+            e.free()
+            t.free()
+            f.destroy()
+
+        # surrogate vs plain unweighting of g g > g g g g
+        t = time.perf_counter()
+        uw = Unweighting(legs=6, hidden=256, layers=4, batch=4096, lib=lib)
+        x, logm2 = uw.training_set(args.train_points, seed=3)
+        hist = uw.train(x, logm2, epochs=args.epochs, lr=1e-3)
+        train_s = time.perf_counter() - t
+        log(f"surrogate trained in {train_s:.1f} s, loss {hist[0]:.3f} -> {hist[-1]:.3f}")
+        res = uw.run(args.candidates, seed=4)
+        res["training"] = {"points": args.train_points, "epochs": args.epochs, "seconds": train_s,
+                           "loss_first_last": [hist[0], hist[-1]],
+                           "rms_log_error": float(np.sqrt(hist[-1]) * uw.std)}
+        extra["unweighting"] = res
+        log(f"unweighting: plain {res['plain']['events_per_s']:.4g} ev/s, two-stage "
+            f"{res['two_stage']['events_per_s']:.4g} ev/s, gain {res['gain']:.2f}, pull {res['sigma_pull']:.2f}")
+        uw.destroy()
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(plan, ext, total64, min(args.cpu_sample, n))
-        print(f"bench: cpu {cpu['value']:.4g} points/s", file=sys.stderr, flush=True)
+        sample = min(args.cpu_sample, n)
+        ext_host = d_ext.download((plan.legs * 4, n))[:, :sample].copy()
+        cpu = cpu_baseline(plan, ext_host, total64, sample)
+        log(f"cpu {cpu['value']:.4g} points/s")
 
     line = {
         "metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (RAMBO, seed 2)",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (RAMBO on the GPU, seed 2)",
         "config": {"workload": f"g g > g g g LC plan run (one colour ordering, {len(plan.outputs)} helicities summed), "
                                f"f64, {n} RAMBO points, sqrt(s)=1 TeV, pT>20 GeV, |eta|<2.5, dR>0.4",
                    "points": n, "plan_rows": len(plan.rows), "current_planes": plan.current_planes,
@@ -139,9 +239,14 @@ def main():
                      "algorithmic_bytes_per_point": 20 * 8 + 8},
         "cpu_baseline": cpu,
         "modes": modes,
-        "unweighting": None,
+        "unweighted_events_per_s": ({"plain": extra["unweighting"]["plain"]["events_per_s"],
+                                     "surrogate": extra["unweighting"]["two_stage"]["events_per_s"],
+                                     "gain": extra["unweighting"]["gain"]} if "unweighting" in extra else None),
+        "extra": extra,
     }
     print(json.dumps(line), flush=True)
+    d_ext.free()
+    d_tot.free()
     fp.destroy()
 
 

@@ -43,6 +43,20 @@ def build_library(verbose=False):
     return LIBRARY
 
 
+CHECK_TOOL = os.path.join(LIB_DIR, "pacuda-check")
+
+
+def build_check_tool(verbose=False):
+    """pacuda-check: the replay of .pacuda recordings through a runner library (host C++)"""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+           os.path.join(HERE, "csrc", "pacuda_check.cpp"), "-o", CHECK_TOOL + ".tmp", "-ldl"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(CHECK_TOOL + ".tmp", CHECK_TOOL)
+
+
 def build_oracle(verbose=False):
     targets = ["all"]
     if os.path.exists("/root/reference/runner/rectest.c"):
@@ -55,6 +69,7 @@ def build_oracle(verbose=False):
 
 def build(verbose=False):
     build_oracle(verbose)
+    build_check_tool(verbose)
     return build_library(verbose)
 
 

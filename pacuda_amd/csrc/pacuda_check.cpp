@@ -1,0 +1,375 @@
+// SPDX-License-Identifier: 0BSD
+//
+// pacuda-check FILE.pacuda --runner LIBRARY [--kernel f64|df64|mixed|mixed64] [--tile N]
+//              [--host-reduce] [--dump]
+//
+// Replays the reference points of every (process, colour) of a .pacuda v2 recording through a
+// runner library's C ABI and compares them with the recorded CPU values, as
+// `pacuda-convert check` does (convert/src/main.rs:100-126, convert/src/replay.rs:156-298):
+// the momenta crossed into source-major tiles, a FORMS_ON_DEVICE run (KEEP unless
+// --host-reduce), the resolved reduction into [point][helicity] (or the host's), the group
+// factor; then the count of values differing in bits, the worst error relative to the point's
+// total and the worst relative error of a point's total. With the f64 kernel any differing
+// value is a failure (exit 1). The file grammar is convert/src/format.rs:6-29 (little endian,
+// no padding, counted arrays, u32-length strings); the reader rejects trailing bytes and the
+// inconsistencies umami/umami_pacuda.c:253-257 rejects.
+#include <dlfcn.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pacman_recurrence.h"
+
+namespace {
+
+struct Reader {
+    const std::vector<uint8_t> &b;
+    size_t at = 0;
+    explicit Reader(const std::vector<uint8_t> &bytes) : b(bytes) {}
+    const uint8_t *take(size_t n) {
+        if (n > b.size() - at) throw std::runtime_error("truncated file");
+        const uint8_t *p = b.data() + at;
+        at += n;
+        return p;
+    }
+    template <class T> T get() {
+        T v;
+        memcpy(&v, take(sizeof v), sizeof v);
+        return v;
+    }
+    uint32_t u32() { return get<uint32_t>(); }
+    std::string str() {
+        uint32_t n = u32();
+        const uint8_t *p = take(n);
+        return std::string((const char *)p, n);
+    }
+    template <class T> std::vector<T> vec(size_t n) {
+        if (n > (b.size() - at) / sizeof(T)) throw std::runtime_error("truncated file");
+        std::vector<T> v(n);
+        if (n) memcpy(v.data(), take(n * sizeof(T)), n * sizeof(T));
+        return v;
+    }
+    std::vector<uint32_t> runs() { return vec<uint32_t>(2 * (size_t)u32()); }
+};
+
+struct Colour {
+    std::string id;
+    uint32_t tile;
+    std::vector<uint64_t> words;
+    uint8_t width[8];
+    uint32_t current_planes, amplitude_planes, momentum_planes;
+    std::vector<double> scale_re, scale_im, factor_re, factor_im;
+    std::vector<uint32_t> zero_runs, download_runs;
+    uint32_t external_count;
+    std::vector<uint32_t> form_terms, term_slots;
+    std::vector<double> term_coefficients;
+    std::vector<uint32_t> offsets, planes;
+    std::vector<double> weights;
+    uint32_t points;
+    std::vector<double> momenta, values;
+};
+
+struct Process {
+    std::string key;
+    uint32_t legs, helicities;
+    std::string alpha_s_name;
+    double alpha_s;
+    int32_t qcd_power;
+    int64_t num, den;
+    std::vector<uint32_t> cross_leg;
+    std::vector<double> cross_sign;
+    std::vector<Colour> colours;
+};
+
+struct File {
+    uint32_t kernel, batch;
+    std::vector<std::string> sha256;
+    std::vector<Process> processes;
+};
+
+File read_file(const std::vector<uint8_t> &bytes) {
+    Reader r(bytes);
+    static const uint8_t magic[8] = {'P', 'A', 'C', 'U', 'D', 'A', 0, 1};
+    if (memcmp(r.take(8), magic, 8)) throw std::runtime_error("not a .pacuda file");
+    if (r.u32() != 2) throw std::runtime_error("not a version 2 .pacuda file");
+    File f;
+    f.kernel = r.u32();
+    f.batch = r.u32();
+    for (uint32_t n = r.u32(), k = 0; k < n; k++) f.sha256.emplace_back((const char *)r.take(64), 64);
+    for (uint32_t np = r.u32(), k = 0; k < np; k++) {
+        Process p;
+        p.key = r.str();
+        p.legs = r.u32();
+        p.helicities = r.u32();
+        p.alpha_s_name = r.str();
+        p.alpha_s = r.get<double>();
+        p.qcd_power = r.get<int32_t>();
+        for (uint32_t n = r.u32(), j = 0; j < n; j++) {
+            r.str();
+            r.get<double>();
+            r.get<double>();
+        }
+        p.num = r.get<int64_t>();
+        p.den = r.get<int64_t>();
+        uint32_t crossing = r.u32();
+        if (crossing && crossing != p.legs) throw std::runtime_error("process " + p.key + ": a crossing of the wrong length");
+        for (uint32_t j = 0; j < crossing; j++) {
+            p.cross_leg.push_back(r.u32());
+            p.cross_sign.push_back(r.get<double>());
+            if (p.cross_leg.back() >= p.legs) throw std::runtime_error("process " + p.key + ": a crossing leg out of range");
+        }
+        for (uint32_t nc = r.u32(), j = 0; j < nc; j++) {
+            Colour c;
+            c.id = r.str();
+            c.tile = r.u32();
+            c.words = r.vec<uint64_t>(r.u32());
+            memcpy(c.width, r.take(8), 8);
+            c.current_planes = r.u32();
+            c.amplitude_planes = r.u32();
+            c.momentum_planes = r.u32();
+            uint32_t scales = r.u32();
+            c.scale_re = r.vec<double>(scales);
+            c.scale_im = r.vec<double>(scales);
+            uint32_t factors = r.u32();
+            c.factor_re = r.vec<double>(factors);
+            c.factor_im = r.vec<double>(factors);
+            c.zero_runs = r.runs();
+            c.download_runs = r.runs();
+            c.external_count = r.u32();
+            c.form_terms = r.runs();
+            uint32_t terms = r.u32();
+            c.term_slots = r.vec<uint32_t>(terms);
+            c.term_coefficients = r.vec<double>(terms);
+            uint32_t outputs = r.u32();
+            c.offsets = r.vec<uint32_t>((size_t)outputs + 1);
+            c.planes = r.vec<uint32_t>(c.offsets.back());
+            c.weights = r.vec<double>(c.offsets.back());
+            c.points = r.u32();
+            c.momenta = r.vec<double>((size_t)c.points * p.legs * 4);
+            c.values = r.vec<double>((size_t)c.points * p.helicities);
+            if (outputs != p.helicities || c.external_count != p.legs || c.tile == 0 || c.offsets[0] != 0)
+                throw std::runtime_error("process " + p.key + " colour " + c.id + ": inconsistent template");
+            for (uint32_t o = 0; o < outputs; o++)
+                if (c.offsets[o] > c.offsets[o + 1]) throw std::runtime_error("colour " + c.id + ": offsets not monotone");
+            if (c.form_terms.size() * 2 != (size_t)c.momentum_planes) throw std::runtime_error("colour " + c.id + ": forms");
+            p.colours.push_back(std::move(c));
+        }
+        f.processes.push_back(std::move(p));
+    }
+    if (r.at != bytes.size()) throw std::runtime_error("trailing bytes");
+    return f;
+}
+
+struct Runner {
+    int (*run)(const pacman_recurrence_call *);
+    int (*reduce_resolved)(const pacman_recurrence_reduce_resolved *);
+    int (*configure)(const pacman_recurrence_options *);
+    int (*symjit_kernel)(const char *);
+};
+
+// op 14 kernel indices rewritten from the file's table to the runner's (convert/src/replay.rs:133-153)
+std::vector<uint64_t> remap_kernels(const std::vector<uint64_t> &words, const std::vector<uint32_t> &map) {
+    std::vector<uint64_t> out = words;
+    for (size_t n = 0; n < out.size();) {
+        uint64_t w = out[n];
+        if ((w & 15) == 14) {
+            uint64_t kernel = (w >> 4) & 0xffff, codes = ((w >> 20) & 0xff) + ((w >> 28) & 0xff);
+            if (kernel >= map.size()) throw std::runtime_error("op 14 names a kernel outside the file's table");
+            out[n] = (w & ~(0xffffull << 4)) | (uint64_t)map[kernel] << 4;
+            n += 1 + (codes + 1) / 2;
+        } else {
+            n++;
+        }
+    }
+    return out;
+}
+
+std::vector<double> evaluate(const Runner &rn, const std::vector<uint32_t> &map, const Process &p, const Colour &c,
+                             size_t tile, bool host_reduce) {
+    const size_t n = p.legs, h = p.helicities, points = c.points;
+    std::vector<uint64_t> words = remap_kernels(c.words, map);
+    std::vector<double> values(points * h, 0.0);
+    const size_t stride = tile;
+    std::vector<double> cre(c.current_planes * stride), cim(cre.size()), are(c.amplitude_planes * stride), aim(are.size()),
+        mom(std::max<size_t>(1, c.momentum_planes * stride)), ext(n * 4 * tile);
+    for (size_t start = 0; start < points; start += tile) {
+        const size_t count = std::min(tile, points - start);
+        for (size_t q = 0; q < count; q++)
+            for (size_t leg = 0; leg < n; leg++) {
+                size_t src = p.cross_leg.empty() ? leg : p.cross_leg[leg];
+                double sign = p.cross_sign.empty() ? 1.0 : p.cross_sign[leg];
+                for (size_t k = 0; k < 4; k++) ext[(leg * 4 + k) * count + q] = sign * c.momenta[((start + q) * n + src) * 4 + k];
+            }
+        pacman_recurrence_call call;
+        memset(&call, 0, sizeof call);
+        call.abi_version = PACMAN_RECURRENCE_ABI;
+        call.struct_size = sizeof call;
+        call.words = words.data();
+        call.word_count = (uint32_t)words.size();
+        memcpy(call.width, c.width, 8);
+        call.current_re = cre.data();
+        call.current_im = cim.data();
+        call.amplitude_re = are.data();
+        call.amplitude_im = aim.data();
+        call.momenta = mom.data();
+        call.scale_re = c.scale_re.data();
+        call.scale_im = c.scale_im.data();
+        call.factor_re = c.factor_re.data();
+        call.factor_im = c.factor_im.data();
+        call.download_amplitude = c.download_runs.data();
+        call.current_planes = c.current_planes;
+        call.amplitude_planes = c.amplitude_planes;
+        call.momentum_planes = c.momentum_planes;
+        call.scale_count = (uint32_t)c.scale_re.size();
+        call.factor_count = (uint32_t)c.factor_re.size();
+        call.download_amplitude_runs = (uint32_t)(c.download_runs.size() / 2);
+        call.stride = (uint32_t)stride;
+        call.points = (uint32_t)count;
+        call.zero_amplitude = c.zero_runs.data();
+        call.zero_amplitude_runs = (uint32_t)(c.zero_runs.size() / 2);
+        call.flags = PACMAN_RECURRENCE_FORMS_ON_DEVICE | (host_reduce ? 0 : PACMAN_RECURRENCE_KEEP_AMPLITUDES);
+        call.external_momenta = ext.data();
+        call.external_count = (uint32_t)n;
+        call.term_count = (uint32_t)c.term_slots.size();
+        call.form_terms = c.form_terms.data();
+        call.term_slots = c.term_slots.data();
+        call.term_coefficients = c.term_coefficients.data();
+        int st = rn.run(&call);
+        if (st != 0) throw std::runtime_error("the runner returned " + std::to_string(st));
+        if (host_reduce) {
+            for (size_t o = 0; o < h; o++)
+                for (uint32_t k = c.offsets[o]; k < c.offsets[o + 1]; k++) {
+                    const size_t plane = (size_t)c.planes[k] * stride;
+                    for (size_t q = 0; q < count; q++) {
+                        double re = are[plane + q], im = aim[plane + q];
+                        values[(start + q) * h + o] += c.weights[k] * fma(re, re, im * im);
+                    }
+                }
+        } else {
+            pacman_recurrence_reduce_resolved r;
+            memset(&r, 0, sizeof r);
+            r.abi_version = PACMAN_RECURRENCE_ABI;
+            r.struct_size = sizeof r;
+            r.planes = c.planes.data();
+            r.weights = c.weights.data();
+            r.offsets = c.offsets.data();
+            r.output = values.data() + start * h;
+            r.outputs = (uint32_t)h;
+            r.points = (uint32_t)count;
+            r.output_stride = 1;
+            r.point_stride = (uint32_t)h;
+            st = rn.reduce_resolved(&r);
+            if (st != 0) throw std::runtime_error("the resolved reduction returned " + std::to_string(st));
+        }
+    }
+    if (p.num != p.den)
+        for (double &v : values) v = v * (double)p.num / (double)p.den;
+    return values;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    const char *file = nullptr, *runner = nullptr, *kernel = "f64";
+    size_t tile = 0;
+    bool host_reduce = false, dump = false;
+    for (int k = 1; k < argc; k++) {
+        std::string a = argv[k];
+        if (a == "--runner" && k + 1 < argc) runner = argv[++k];
+        else if (a == "--kernel" && k + 1 < argc) kernel = argv[++k];
+        else if (a == "--tile" && k + 1 < argc) tile = strtoul(argv[++k], nullptr, 10);
+        else if (a == "--host-reduce") host_reduce = true;
+        else if (a == "--dump") dump = true;
+        else if (!file) file = argv[k];
+        else {
+            fprintf(stderr, "pacuda-check: unknown argument %s\n", argv[k]);
+            return 2;
+        }
+    }
+    if (!file || (!runner && !dump)) {
+        fprintf(stderr, "usage: pacuda-check FILE.pacuda --runner LIBRARY [--kernel f64|df64|mixed|mixed64] [--tile N] "
+                        "[--host-reduce] [--dump]\n");
+        return 2;
+    }
+    try {
+        FILE *fp = fopen(file, "rb");
+        if (!fp) throw std::runtime_error(std::string(file) + ": cannot open");
+        std::vector<uint8_t> bytes;
+        uint8_t buf[1 << 16];
+        size_t got;
+        while ((got = fread(buf, 1, sizeof buf, fp)) > 0) bytes.insert(bytes.end(), buf, buf + got);
+        fclose(fp);
+        File f = read_file(bytes);
+        if (dump) {
+            printf("kernel %u batch %u symjit %zu processes %zu\n", f.kernel, f.batch, f.sha256.size(), f.processes.size());
+            for (const Process &p : f.processes)
+                for (const Colour &c : p.colours)
+                    printf("%s %s legs %u helicities %u words %zu planes %u/%u/%u points %u\n", p.key.c_str(), c.id.c_str(),
+                           p.legs, p.helicities, c.words.size(), c.current_planes, c.amplitude_planes, c.momentum_planes,
+                           c.points);
+            if (!runner) return 0;
+        }
+        void *lib = dlopen(runner, RTLD_NOW);
+        if (!lib) throw std::runtime_error(dlerror());
+        Runner rn;
+        rn.run = (int (*)(const pacman_recurrence_call *))dlsym(lib, "pacman_recurrence_v1_run");
+        rn.reduce_resolved = (int (*)(const pacman_recurrence_reduce_resolved *))dlsym(lib, "pacman_recurrence_v1_reduce_resolved");
+        rn.configure = (int (*)(const pacman_recurrence_options *))dlsym(lib, "pacman_recurrence_v1_configure");
+        rn.symjit_kernel = (int (*)(const char *))dlsym(lib, "pacman_recurrence_v1_symjit_kernel");
+        if (!rn.run || !rn.reduce_resolved || !rn.configure) throw std::runtime_error("the library lacks the v1 entry points");
+        pacman_recurrence_options o;
+        memset(&o, 0, sizeof o);
+        o.abi_version = PACMAN_RECURRENCE_ABI;
+        o.struct_size = sizeof o;
+        std::string k = kernel;
+        o.kernel = k == "f64" ? PACMAN_RECURRENCE_KERNEL_F64 : k == "mixed" ? PACMAN_RECURRENCE_KERNEL_MIXED
+                 : k == "mixed64" ? PACMAN_RECURRENCE_KERNEL_MIXED64 : PACMAN_RECURRENCE_KERNEL_DF64;
+        if (rn.configure(&o) != 0) throw std::runtime_error("configure failed");
+        std::vector<uint32_t> map;
+        for (const std::string &s : f.sha256) {
+            int idx = rn.symjit_kernel ? rn.symjit_kernel(s.c_str()) : -1;
+            if (idx < 0) throw std::runtime_error("the runner has no symjit kernel " + s);
+            map.push_back((uint32_t)idx);
+        }
+        size_t total_differ = 0, total_values = 0;
+        for (const Process &p : f.processes)
+            for (const Colour &c : p.colours) {
+                size_t t = tile ? tile : (f.batch ? f.batch : c.tile);
+                std::vector<double> got = evaluate(rn, map, p, c, t, host_reduce);
+                size_t differ = 0;
+                double worst = 0, worst_total = 0;
+                const size_t h = std::max<uint32_t>(1, p.helicities);
+                for (size_t q = 0; q < c.points; q++) {
+                    double scale = 0, ta = 0, tb = 0;
+                    for (size_t o = 0; o < h; o++) {
+                        double a = got[q * h + o], b = c.values[q * h + o];
+                        if (memcmp(&a, &b, 8)) differ++;
+                        scale += fabs(b);
+                        ta += a;
+                        tb += b;
+                    }
+                    if (scale == 0) continue;
+                    for (size_t o = 0; o < h; o++) worst = fmax(worst, fabs(got[q * h + o] - c.values[q * h + o]) / scale);
+                    worst_total = fmax(worst_total, fabs(ta / tb - 1.0));
+                }
+                printf("%s %s: %zu points, %zu of %zu values differ in bits, worst %.3e of the point's total, worst "
+                       "total %.3e\n", p.key.c_str(), c.id.c_str(), (size_t)c.points, differ, got.size(), worst, worst_total);
+                total_differ += differ;
+                total_values += got.size();
+            }
+        if (k == "f64" && total_differ) {
+            fprintf(stderr, "pacuda-check: %zu of %zu f64 values differ from the recording\n", total_differ, total_values);
+            return 1;
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        fprintf(stderr, "pacuda-check: %s\n", e.what());
+        return 1;
+    }
+}

@@ -158,3 +158,16 @@ def test_v1_other_kernels_in_fresh_processes(gg_ggg, tmp_path):
         want, _ = refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), ext[:, :points], points,
                                   kernel=abi.KERNELS[kernel])
         assert bits_equal(got, want), f"{kernel}: max |diff| {np.max(np.abs(got - want))}"
+
+
+def test_recording_replays_bit_exact_on_gpu():
+    """tests/golden/gg_ggg_qqbar.pacuda through pacuda-check on the HIP library: every f64 value
+    equals the recorded CPU value (device and host reductions)"""
+    from pacuda_amd import build
+    golden = os.path.join(os.path.dirname(__file__), "golden", "gg_ggg_qqbar.pacuda")
+    for args in [(), ("--host-reduce", "--tile", "3")]:
+        print("pacuda-check", args, flush=True)
+        p = subprocess.run([build.CHECK_TOOL, golden, "--runner", abi.LIBRARY, *args], capture_output=True, text=True,
+                           timeout=120)
+        assert p.returncode == 0, p.stdout + p.stderr
+        assert "0 of 256 values differ" in p.stdout and "0 of 128 values differ" in p.stdout
