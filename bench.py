@@ -39,11 +39,15 @@ def log(msg):
 def points_on_device(dev, plan, n, seed):
     """n cut-passing RAMBO points of stream `seed`, generated and compacted on the GPU"""
     gen = Generator(dev, plan)
-    draw = int(n * 1.2) + 4096
-    ext, feat, ok = gen.draw(seed, 0, draw)
-    idx = np.nonzero(ok)[0]
-    if len(idx) < n:
-        raise RuntimeError("too few points passed the cuts")
+    draw = int(n * 1.1) + 4096
+    while True:
+        ext, feat, ok = gen.draw(seed, 0, draw)
+        idx = np.nonzero(ok)[0]
+        if len(idx) >= n:
+            break
+        ext.free()
+        feat.free()
+        draw = int(draw * 1.2 * n / max(len(idx), 1)) + 4096
     out, _ = gen.gather(ext, None, draw, idx[:n])
     ext.free()
     feat.free()

@@ -104,6 +104,19 @@ typedef struct {
  */
 int pacuda_evaluate(const pacuda_item *items, uint32_t count, const pacuda_launch *launch);
 
+/*
+ * A sweep: the same batch of items evaluated again and again (a Monte Carlo loop that rewrites
+ * the momenta in place between iterations). pacuda_sweep_create uploads the item table once,
+ * gives the sweep its own counter and scratch, and captures the launch into a HIP graph;
+ * pacuda_sweep_run replays the graph on `stream` (NULL: the library's stream), asynchronously
+ * unless kernel_ms is set (then it waits and stores the graph's time). The items' buffers must
+ * outlive the sweep. Two sweeps never share state; one sweep's runs on one stream are ordered.
+ */
+typedef struct pacuda_sweep pacuda_sweep;
+int pacuda_sweep_create(const pacuda_item *items, uint32_t count, const pacuda_launch *launch, pacuda_sweep **sweep);
+int pacuda_sweep_run(pacuda_sweep *sweep, void *stream, float *kernel_ms);
+void pacuda_sweep_destroy(pacuda_sweep *sweep);
+
 /* device memory for callers without a GPU framework (NULL / nonzero on failure) */
 void *pacuda_device_alloc(size_t bytes);
 void pacuda_device_free(void *p);

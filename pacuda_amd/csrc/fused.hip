@@ -135,7 +135,9 @@ __global__ void __launch_bounds__(256) fused_kernel(const ItemDev *__restrict__ 
                 str(mom, at, narrow<D>(mom[at]));
             }
         }
-        a.sre = pl.cdf; a.sim = pl.cdf + S; a.fre = pl.cdf + 2 * S; a.fim = pl.cdf + 2 * S + X;
+        // the program's constants: df64 pairs for every kernel but f64 (as the v1 path converts them)
+        const double *cons = DF ? pl.cdf : pl.c64;
+        a.sre = cons; a.sim = cons + S; a.fre = cons + 2 * S; a.fim = cons + 2 * S + X;
         interpret<K>(pl.prog + pl.sources, pl.count - pl.sources, a, lane);
         // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
         double t = 0.0;
@@ -324,40 +326,86 @@ extern "C" int pacuda_plan_info(const pacuda_plan *p, uint32_t *instructions, ui
     return PACMAN_RECURRENCE_OK;
 }
 
-extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const pacuda_launch *l) {
+namespace {
+
+// A launch's schedule: the item table in queue order, the units' first indices, the grid.
+// Items go into the queue longest first (instructions x units, stable), so the units of long
+// plans are handed out before the short ones and the launch's tail is short.
+struct Schedule {
+    std::vector<ItemDev> items;
+    std::vector<uint32_t> starts;
+    uint64_t slot = 0, units = 0, waves = 0;
+    uint32_t grid = 0, block = 0;
+};
+
+int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, Schedule &sc) {
     if (!l || l->abi_version != PACMAN_RECURRENCE_ABI || l->struct_size != sizeof(pacuda_launch) ||
         l->kernel > PACMAN_RECURRENCE_KERNEL_MIXED64 || (l->block % LANES) != 0 || l->block > 1024 ||
         l->waves_per_cu > 64 || (count && !items))
         return PACMAN_RECURRENCE_INVALID;
-    if (count == 0) return PACMAN_RECURRENCE_OK;
-    std::lock_guard<std::mutex> guard(fs.lock);
-    std::vector<ItemDev> host(count);
-    std::vector<uint32_t> starts(count + 1, 0);
-    uint64_t slot = 0, units = 0;
+    std::vector<uint32_t> order(count);
+    std::vector<uint64_t> cost(count);
     for (uint32_t k = 0; k < count; k++) {
         const pacuda_item &it = items[k];
         if (!it.plan || (it.points && (!it.external || it.ext_stride < it.points))) return PACMAN_RECURRENCE_INVALID;
-        host[k].plan = it.plan->device;
-        host[k].external = it.external;
-        host[k].resolved = it.resolved;
-        host[k].total = it.total;
-        host[k].points = it.points;
-        host[k].ext_stride = it.ext_stride;
-        starts[k] = (uint32_t)units;
-        units += (it.points + LANES - 1) / LANES;
-        slot = std::max<uint64_t>(slot, it.plan->slot_doubles);
+        order[k] = k;
+        cost[k] = (uint64_t)it.plan->host.count * ((it.points + LANES - 1) / LANES);
     }
-    if (units >= 0xffffffffull) return PACMAN_RECURRENCE_INVALID;
-    starts[count] = (uint32_t)units;
-    if (units == 0) return PACMAN_RECURRENCE_OK;
-    const uint32_t block = l->block ? l->block : 256;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+    sc.items.resize(count);
+    sc.starts.assign(count + 1, 0);
+    for (uint32_t j = 0; j < count; j++) {
+        const pacuda_item &it = items[order[j]];
+        ItemDev &d = sc.items[j];
+        d.plan = it.plan->device;
+        d.external = it.external;
+        d.resolved = it.resolved;
+        d.total = it.total;
+        d.points = it.points;
+        d.ext_stride = it.ext_stride;
+        sc.starts[j] = (uint32_t)sc.units;
+        sc.units += (it.points + LANES - 1) / LANES;
+        sc.slot = std::max<uint64_t>(sc.slot, it.plan->slot_doubles);
+    }
+    if (sc.units >= 0xffffffffull) return PACMAN_RECURRENCE_INVALID;
+    sc.starts[count] = (uint32_t)sc.units;
+    sc.block = l->block ? l->block : 256;
     const uint32_t per_cu = l->waves_per_cu ? l->waves_per_cu : 4;
-    const uint32_t waves_per_block = block / LANES;
-    uint64_t waves = (uint64_t)compute_units() * per_cu;
-    waves = std::min<uint64_t>(waves, units);  // no more wavefronts than units
-    const uint32_t grid = (uint32_t)((waves + waves_per_block - 1) / waves_per_block);
-    waves = (uint64_t)grid * waves_per_block;
-    const size_t need = (size_t)waves * slot;
+    const uint32_t waves_per_block = sc.block / LANES;
+    uint64_t waves = std::min<uint64_t>((uint64_t)compute_units() * per_cu, std::max<uint64_t>(sc.units, 1));
+    sc.grid = (uint32_t)((waves + waves_per_block - 1) / waves_per_block);
+    sc.waves = (uint64_t)sc.grid * waves_per_block;
+    return PACMAN_RECURRENCE_OK;
+}
+
+void launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
+                  uint32_t count, uint32_t *counter, double *scratch) {
+    reset_counter<<<1, 64, 0, s>>>(counter);
+    switch (kernel) {
+    case PACMAN_RECURRENCE_KERNEL_F64:
+        fused_kernel<KF64><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot);
+        break;
+    case PACMAN_RECURRENCE_KERNEL_MIXED:
+        fused_kernel<KMixed><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot);
+        break;
+    case PACMAN_RECURRENCE_KERNEL_MIXED64:
+        fused_kernel<KMixed64><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot);
+        break;
+    default:
+        fused_kernel<KDF64><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot);
+        break;
+    }
+}
+
+}  // namespace
+
+extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const pacuda_launch *l) {
+    Schedule sc;
+    int st = schedule(items, count, l, sc);
+    if (st) return st;
+    if (count == 0 || sc.units == 0) return PACMAN_RECURRENCE_OK;
+    std::lock_guard<std::mutex> guard(fs.lock);
+    const size_t need = (size_t)sc.waves * sc.slot;
     hipStream_t s = l->stream ? (hipStream_t)l->stream : runner_stream();
     if (!s) return PACMAN_RECURRENCE_DEVICE_ERROR;
     if (need > fs.scratch_doubles) {
@@ -388,25 +436,11 @@ extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const p
     // the item table: no earlier launch on the stream may still read it, and the host copies go
     // out of scope on return, so wait for the stream and copy synchronously
     if (hip_failed(hipStreamSynchronize(s), "items") ||
-        hip_failed(hipMemcpy(fs.items, host.data(), count * sizeof(ItemDev), hipMemcpyHostToDevice), "items") ||
-        hip_failed(hipMemcpy(fs.unit_start, starts.data(), (count + 1) * sizeof(uint32_t), hipMemcpyHostToDevice), "items"))
+        hip_failed(hipMemcpy(fs.items, sc.items.data(), count * sizeof(ItemDev), hipMemcpyHostToDevice), "items") ||
+        hip_failed(hipMemcpy(fs.unit_start, sc.starts.data(), (count + 1) * sizeof(uint32_t), hipMemcpyHostToDevice), "items"))
         return PACMAN_RECURRENCE_DEVICE_ERROR;
-    reset_counter<<<1, 64, 0, s>>>(fs.counter);
     if (l->kernel_ms) (void)hipEventRecord(fs.start, s);
-    switch (l->kernel) {
-    case PACMAN_RECURRENCE_KERNEL_F64:
-        fused_kernel<KF64><<<grid, block, 0, s>>>(fs.items, fs.unit_start, count, fs.counter, fs.scratch, slot);
-        break;
-    case PACMAN_RECURRENCE_KERNEL_MIXED:
-        fused_kernel<KMixed><<<grid, block, 0, s>>>(fs.items, fs.unit_start, count, fs.counter, fs.scratch, slot);
-        break;
-    case PACMAN_RECURRENCE_KERNEL_MIXED64:
-        fused_kernel<KMixed64><<<grid, block, 0, s>>>(fs.items, fs.unit_start, count, fs.counter, fs.scratch, slot);
-        break;
-    default:
-        fused_kernel<KDF64><<<grid, block, 0, s>>>(fs.items, fs.unit_start, count, fs.counter, fs.scratch, slot);
-        break;
-    }
+    launch_fused(l->kernel, sc, s, fs.items, fs.unit_start, count, fs.counter, fs.scratch);
     if (hip_failed(hipGetLastError(), "fused launch")) return PACMAN_RECURRENCE_DEVICE_ERROR;
     if (l->kernel_ms) {
         if (hip_failed(hipEventRecord(fs.stop, s), "event") || hip_failed(hipEventSynchronize(fs.stop), "kernel"))
@@ -414,6 +448,84 @@ extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const p
         float ms = 0;
         if (hip_failed(hipEventElapsedTime(&ms, fs.start, fs.stop), "event")) return PACMAN_RECURRENCE_DEVICE_ERROR;
         *l->kernel_ms = ms;
+    }
+    return PACMAN_RECURRENCE_OK;
+}
+
+// A sweep: one batch of items with its own item table, counter and scratch, captured once into a
+// HIP graph (the counter reset kernel and the fused kernel) and replayed by pacuda_sweep_run.
+struct pacuda_sweep {
+    Schedule sc;
+    uint32_t kernel = 0, count = 0;
+    ItemDev *items = nullptr;
+    uint32_t *starts = nullptr, *counter = nullptr;
+    double *scratch = nullptr;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+extern "C" void pacuda_sweep_destroy(pacuda_sweep *w) {
+    if (!w) return;
+    (void)hipDeviceSynchronize();
+    if (w->exec) (void)hipGraphExecDestroy(w->exec);
+    if (w->graph) (void)hipGraphDestroy(w->graph);
+    if (w->capture) (void)hipStreamDestroy(w->capture);
+    if (w->start) (void)hipEventDestroy(w->start);
+    if (w->stop) (void)hipEventDestroy(w->stop);
+    for (void *p : {(void *)w->items, (void *)w->starts, (void *)w->counter, (void *)w->scratch})
+        if (p) (void)hipFree(p);
+    delete w;
+}
+
+extern "C" int pacuda_sweep_create(const pacuda_item *items, uint32_t count, const pacuda_launch *l, pacuda_sweep **out) {
+    if (!out) return PACMAN_RECURRENCE_INVALID;
+    *out = nullptr;
+    pacuda_sweep *w = new pacuda_sweep();
+    int st = schedule(items, count, l, w->sc);
+    if (st || count == 0 || w->sc.units == 0) {
+        delete w;
+        return st ? st : PACMAN_RECURRENCE_INVALID;
+    }
+    w->kernel = l->kernel;
+    w->count = count;
+    const size_t need = (size_t)w->sc.waves * w->sc.slot;
+    bool ok = !hip_failed(hipMalloc((void **)&w->items, count * sizeof(ItemDev)), "sweep") &&
+              !hip_failed(hipMalloc((void **)&w->starts, (count + 1) * sizeof(uint32_t)), "sweep") &&
+              !hip_failed(hipMalloc((void **)&w->counter, 64), "sweep") &&
+              !hip_failed(hipMalloc((void **)&w->scratch, std::max<size_t>(need, 1) * sizeof(double)), "sweep") &&
+              !hip_failed(hipMemcpy(w->items, w->sc.items.data(), count * sizeof(ItemDev), hipMemcpyHostToDevice), "sweep") &&
+              !hip_failed(hipMemcpy(w->starts, w->sc.starts.data(), (count + 1) * sizeof(uint32_t), hipMemcpyHostToDevice),
+                          "sweep") &&
+              !hip_failed(hipEventCreate(&w->start), "sweep") && !hip_failed(hipEventCreate(&w->stop), "sweep") &&
+              !hip_failed(hipStreamCreateWithFlags(&w->capture, hipStreamNonBlocking), "sweep") &&
+              !hip_failed(hipStreamBeginCapture(w->capture, hipStreamCaptureModeThreadLocal), "sweep capture");
+    if (ok) {
+        launch_fused(w->kernel, w->sc, w->capture, w->items, w->starts, count, w->counter, w->scratch);
+        ok = !hip_failed(hipStreamEndCapture(w->capture, &w->graph), "sweep capture") &&
+             !hip_failed(hipGraphInstantiate(&w->exec, w->graph, nullptr, nullptr, 0), "sweep graph");
+    }
+    if (!ok) {
+        pacuda_sweep_destroy(w);
+        return PACMAN_RECURRENCE_DEVICE_ERROR;
+    }
+    *out = w;
+    return PACMAN_RECURRENCE_OK;
+}
+
+extern "C" int pacuda_sweep_run(pacuda_sweep *w, void *stream, float *kernel_ms) {
+    if (!w) return PACMAN_RECURRENCE_INVALID;
+    hipStream_t s = stream ? (hipStream_t)stream : runner_stream();
+    if (!s) return PACMAN_RECURRENCE_DEVICE_ERROR;
+    if (kernel_ms) (void)hipEventRecord(w->start, s);
+    if (hip_failed(hipGraphLaunch(w->exec, s), "sweep launch")) return PACMAN_RECURRENCE_DEVICE_ERROR;
+    if (kernel_ms) {
+        float ms = 0;
+        if (hip_failed(hipEventRecord(w->stop, s), "event") || hip_failed(hipEventSynchronize(w->stop), "sweep") ||
+            hip_failed(hipEventElapsedTime(&ms, w->start, w->stop), "event"))
+            return PACMAN_RECURRENCE_DEVICE_ERROR;
+        *kernel_ms = ms;
     }
     return PACMAN_RECURRENCE_OK;
 }

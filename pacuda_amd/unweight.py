@@ -105,11 +105,15 @@ class Unweighting:
     # ---- training ---------------------------------------------------------------------------
     def training_set(self, points, seed=3):
         """`points` cut-passing points of stream `seed` with their exact log|M|^2 (f64 GPU)"""
-        draw = int(points * 1.25) + 1024
-        ext, feat, ok = self.gen.draw(seed, 0, draw)
-        idx = np.nonzero(ok)[0][:points]
-        if len(idx) < points:
-            raise RuntimeError("too few points passed the cuts")
+        draw = int(points * 1.1) + 1024
+        while True:
+            ext, feat, ok = self.gen.draw(seed, 0, draw)
+            idx = np.nonzero(ok)[0][:points]
+            if len(idx) == points:
+                break
+            ext.free()
+            feat.free()
+            draw = int(draw * 1.2 * points / max(len(idx), 1)) + 1024
         e2, f2 = self.gen.gather(ext, feat, draw, idx)
         ext.free()
         feat.free()
