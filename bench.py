@@ -92,6 +92,23 @@ def mode_errors(got, ref):
             "max_err_rel_to_max_f64": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))}
 
 
+def copy_calibration(rows=20, count=1 << 22, repeats=3):
+    """pacuda_gather with the identity index copies rows x count doubles: it reads
+    rows*count*8 + count*4 bytes and writes rows*count*8 (gather_kernel, one dispatch each)"""
+    dev = Device()
+    lib = dev.lib
+    src, dst = dev.empty(rows * count), dev.empty(rows * count)
+    idx = dev.array(np.arange(count, dtype=np.uint32))
+    for _ in range(repeats):
+        if lib.pacuda_gather(src.ptr, count, rows, idx.ptr, count, dst.ptr, None, 0, None) != 0:
+            raise RuntimeError("pacuda_gather failed")
+    dev.synchronize()
+    print(json.dumps({"copy_kernel": "gather_kernel", "bytes_read": rows * count * 8 + count * 4,
+                      "bytes_written": rows * count * 8, "dispatches": repeats}), flush=True)
+    for a in (src, dst, idx):
+        a.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,7 +123,14 @@ def main():
     ap.add_argument("--candidates", type=int, default=1 << 24)
     ap.add_argument("--train-points", type=int, default=1 << 20)
     ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--train-batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only a plain copy of known size (pacuda_gather, identity index), to scale HBM counters")
     args = ap.parse_args()
+    if args.copy_only:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
               "points with no exchange during evaluation)", file=sys.stderr)
@@ -207,18 +231,25 @@ def main():
 
         # surrogate vs plain unweighting of g g > g g g g
         t = time.perf_counter()
-        uw = Unweighting(legs=6, hidden=256, layers=4, batch=4096, lib=lib)
-        x, logm2 = uw.training_set(args.train_points, seed=3)
-        hist = uw.train(x, logm2, epochs=args.epochs, lr=1e-3)
+        uw = Unweighting(legs=6, hidden=256, layers=4, batch=args.train_batch, prior=True, lib=lib)
+        x, logm2, logp = uw.training_set(args.train_points, seed=3)
+        label_s = time.perf_counter() - t
+        t = time.perf_counter()
+        hist = uw.train(x, logm2, logp, epochs=args.epochs, lr=args.lr)
         train_s = time.perf_counter() - t
-        log(f"surrogate trained in {train_s:.1f} s, loss {hist[0]:.3f} -> {hist[-1]:.3f}")
+        log(f"surrogate: {args.train_points} labels in {label_s:.1f} s, trained in {train_s:.1f} s, "
+            f"loss {hist[0]:.3f} -> {hist[-1]:.3f}")
         res = uw.run(args.candidates, seed=4)
-        res["training"] = {"points": args.train_points, "epochs": args.epochs, "seconds": train_s,
+        res["training"] = {"points": args.train_points, "seed": 3, "epochs": args.epochs, "batch": args.train_batch,
+                           "label_seconds": label_s, "train_seconds": train_s,
                            "loss_first_last": [hist[0], hist[-1]],
+                           "rms_log_residual_before": uw.std,
                            "rms_log_error": float(np.sqrt(hist[-1]) * uw.std)}
         extra["unweighting"] = res
-        log(f"unweighting: plain {res['plain']['events_per_s']:.4g} ev/s, two-stage "
-            f"{res['two_stage']['events_per_s']:.4g} ev/s, gain {res['gain']:.2f}, pull {res['sigma_pull']:.2f}")
+        for k in ("two_stage", "two_stage_prior_only"):
+            r = res[k]
+            log(f"unweighting {k}: plain {res['plain']['events_per_s']:.4g} ev/s, {r['events_per_s']:.4g} ev/s, "
+                f"gain {r['gain']:.2f}, pull {r['sigma_pull']:.2f}")
         uw.destroy()
 
     cpu = None
@@ -245,7 +276,10 @@ def main():
         "modes": modes,
         "unweighted_events_per_s": ({"plain": extra["unweighting"]["plain"]["events_per_s"],
                                      "surrogate": extra["unweighting"]["two_stage"]["events_per_s"],
-                                     "gain": extra["unweighting"]["gain"]} if "unweighting" in extra else None),
+                                     "gain": extra["unweighting"]["two_stage"]["gain"],
+                                     "sigma_pull": extra["unweighting"]["two_stage"]["sigma_pull"],
+                                     "prior_only_gain": extra["unweighting"]["two_stage_prior_only"]["gain"]}
+                                    if "unweighting" in extra else None),
         "extra": extra,
     }
     print(json.dumps(line), flush=True)

@@ -89,7 +89,7 @@ typedef struct {
     uint32_t abi_version;   /* PACMAN_RECURRENCE_ABI */
     uint32_t struct_size;   /* sizeof(pacuda_launch) */
     uint32_t kernel;        /* PACMAN_RECURRENCE_KERNEL_* */
-    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 4 */
+    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 12 */
     uint32_t block;         /* threads per workgroup, a multiple of 64 up to 1024, 0: 256 */
     uint32_t reserved;
     void *stream;           /* hipStream_t to queue on, or NULL: the library's stream */
@@ -155,16 +155,42 @@ int pacuda_mlp_adam(pacuda_mlp *mlp, float lr, float beta1, float beta2, float e
  * 17-52 on a counter-based stream: point `first + i` of stream `seed`), written as the runner's
  * external momenta external[(l * 4 + k) * stride + i] = crossing_sign[l] * p[crossing_src[l]][k];
  * pass[i] = 1 when the outgoing partons pass pT > pt_min, |eta| < eta_max, Delta R > dr_min;
- * features (or NULL): [points][3 (legs - 2) + 2] floats in [0, 1], the surrogate's inputs.
+ * features (or NULL): [points][3 (legs - 2) + 2] floats in [0, 1], the surrogate's inputs;
+ * log_prior (or NULL): log(sum_{a<b} s_ab^4 / prod_l s_{l,l+1}), s_ab = |2 p_a.p_b| over the
+ * legs in crossing order (cyclic): the pure-gluon MHV Parke-Taylor structure of the ordering.
  * All pointers but the crossing tables are device pointers.
  */
 int pacuda_rambo(uint64_t seed, uint64_t first, uint32_t points, uint32_t legs, double ecm,
                  const uint32_t *crossing_src, const double *crossing_sign, double *external, uint32_t stride,
-                 float *features, uint8_t *pass, double pt_min, double eta_max, double dr_min);
+                 float *features, uint8_t *pass, double pt_min, double eta_max, double dr_min, double *log_prior);
 /* out[r * count + j] = external[r * stride + index[j]] for rows r < rows; features likewise
  * ([j][width] <- [index[j]][width]) when features is not NULL (device pointers) */
 int pacuda_gather(const double *external, uint32_t stride, uint32_t rows, const uint32_t *index, uint32_t count,
                   double *out, const float *features, uint32_t width, float *features_out);
+
+/*
+ * Unweighting on the device (no reference counterpart; all pointers are device pointers,
+ * asynchronous on the library's stream).
+ * pacuda_surrogate_weight: s[i] = exp(mean + scale * y[i] + log_prior[i]) where pass[i] (or
+ *   every i when pass is NULL), else 0; y or log_prior may be NULL (a zero term).
+ * pacuda_weight_stats: adds the count of each w[i] > 0 to hist[bin of log w[i]] (bins over
+ *   [log_lo, log_hi), clamped at both ends, bins <= PACUDA_MAX_BINS; hist may be NULL) and
+ *   writes stats[3 b + {0, 1, 2}] = max, sum, sum of squares of w over the b-th of
+ *   PACUDA_STATS_BLOCKS contiguous ranges (reduce them in order on the host).
+ * pacuda_select: the indices i < count (ascending) that pass: flags[i] != 0 (when flags is not
+ *   NULL) and, when w is not NULL, w[i] > 0 and u * threshold < w[i], u the uniform number
+ *   (id + 1) of the counter-based stream `seed`, id = first + (ids ? ids[i] : i). Writes
+ *   index[0..*selected) and, when values is not NULL, values[k] = w[index[k]]. index and
+ *   values must hold count entries.
+ */
+#define PACUDA_STATS_BLOCKS 256
+#define PACUDA_MAX_BINS 8192
+int pacuda_surrogate_weight(const float *y, const double *log_prior, const uint8_t *pass, uint32_t count, double mean,
+                            double scale, double *s);
+int pacuda_weight_stats(const double *w, uint32_t count, double log_lo, double log_hi, uint32_t bins, uint32_t *hist,
+                        double *stats);
+int pacuda_select(const double *w, const uint8_t *flags, const uint32_t *ids, uint64_t first, uint32_t count,
+                  double threshold, uint64_t seed, uint32_t *index, double *values, uint32_t *selected);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_DIR = os.path.join(HERE, "lib")
 LIBRARY = os.path.join(LIB_DIR, "librecurrence.so")
-SOURCES = ["runner.hip", "fused.hip", "surrogate.hip", "phase.hip"]
+SOURCES = ["runner.hip", "fused.hip", "surrogate.hip", "phase.hip", "unweight.hip"]
 ARCH = os.environ.get("PACUDA_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

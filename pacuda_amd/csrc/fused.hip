@@ -69,7 +69,7 @@ __global__ void reset_counter(uint32_t *counter) {
 }
 
 template <class K>
-__global__ void __launch_bounds__(256) fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start,
                                                     uint32_t item_count, uint32_t *counter, double *scratch,
                                                     size_t slot_doubles) {
     typedef typename K::real R;
@@ -212,6 +212,130 @@ bool reads_after_writes(const std::vector<Instr> &ops, uint32_t planes) {
     return true;
 }
 
+// Arena compaction: currents are objects (a plane range written as a unit: a source, a
+// contribution destination, a clear); an object lives from the row that first writes it to the
+// last row that touches it. Objects get physical plane ranges first-fit from a free list as the
+// program runs, so the slot holds only what is live at once. Values are unchanged (only plane
+// indices are rewritten). Every access must name a whole object; otherwise the program keeps
+// its planes. Returns the compacted current plane count (0: not compacted).
+uint32_t compact_planes(std::vector<Instr> &ops, uint32_t planes) {
+    struct Obj { uint32_t base, len, first, last, phys; };
+    std::vector<int> owner(planes, -1);  // plane -> object (by its base)
+    std::vector<Obj> objs;
+    auto object_at = [&](uint32_t base, uint32_t len, uint32_t row, bool write) -> int {
+        if (base + len > planes || len == 0) return -2;
+        int o = owner[base];
+        if (o >= 0) {
+            if (objs[o].base != base || objs[o].len != len) return -2;
+            objs[o].last = row;
+            return o;
+        }
+        if (!write) return -2;
+        for (uint32_t k = 0; k < len; k++)
+            if (owner[base + k] >= 0) return -2;
+        objs.push_back({base, len, row, row, 0});
+        for (uint32_t k = 0; k < len; k++) owner[base + k] = (int)objs.size() - 1;
+        return (int)objs.size() - 1;
+    };
+    // accesses per row: (operand slot index in w[], length, write?)
+    struct Acc { int word; uint32_t len; bool write; };
+    std::vector<std::vector<Acc>> access(ops.size());
+    for (size_t n = 0; n < ops.size(); n++) {
+        const Instr &o = ops[n];
+        const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff, flags = o.w[0] >> 16;
+        auto &a = access[n];
+        if (kind == K_SOURCE) {
+            uint32_t family = o.w[5] & 7;
+            a.push_back({1, family == 0 ? 1u : family == 1 ? 2u : 4u, true});
+        } else if (kind == K_CLEAR) {
+            a.push_back({1, o.w[2], true});
+        } else if (kind == K_CONTRIB) {
+            int c0, c1, cd;
+            contribution_shape(op, &c0, &c1, &cd);
+            a.push_back({2, (uint32_t)c0, false});
+            a.push_back({3, (uint32_t)c1, false});
+            a.push_back({4, (uint32_t)cd, (flags & F_FIRST) && (flags & F_INIT)});
+        } else if (kind == K_PROP) {
+            a.push_back({1, op == 9 ? 4u : 2u, false});
+        } else if (kind == K_IDENTITY) {
+            a.push_back({1, o.w[2], false});
+        } else if (kind == K_CLOSURE) {
+            a.push_back({1, o.w[6], false});
+            a.push_back({2, o.w[6], false});
+        } else {
+            return 0;
+        }
+        for (const Acc &x : a)
+            if (x.len && object_at(o.w[x.word], x.len, (uint32_t)n, x.write) < 0) return 0;
+    }
+    // first-fit allocation in row order
+    std::vector<std::pair<uint32_t, uint32_t>> free_list;  // (start, length), sorted by start
+    uint32_t top = 0;
+    std::vector<std::vector<int>> born(ops.size()), dies(ops.size());
+    for (size_t k = 0; k < objs.size(); k++) {
+        born[objs[k].first].push_back((int)k);
+        dies[objs[k].last].push_back((int)k);
+    }
+    auto release = [&](uint32_t start, uint32_t len) {
+        free_list.push_back({start, len});
+        std::sort(free_list.begin(), free_list.end());
+        std::vector<std::pair<uint32_t, uint32_t>> merged;
+        for (auto &f : free_list) {
+            if (!merged.empty() && merged.back().first + merged.back().second == f.first) merged.back().second += f.second;
+            else merged.push_back(f);
+        }
+        free_list.swap(merged);
+    };
+    for (size_t n = 0; n < ops.size(); n++) {
+        if (n > 0)
+            for (int k : dies[n - 1]) release(objs[k].phys, objs[k].len);
+        for (int k : born[n]) {
+            uint32_t len = objs[k].len;
+            bool placed = false;
+            for (auto &f : free_list)
+                if (f.second >= len) {
+                    objs[k].phys = f.first;
+                    f.first += len;
+                    f.second -= len;
+                    placed = true;
+                    break;
+                }
+            if (!placed) {
+                objs[k].phys = top;
+                top += len;
+            }
+            free_list.erase(std::remove_if(free_list.begin(), free_list.end(),
+                                           [](const std::pair<uint32_t, uint32_t> &f) { return f.second == 0; }),
+                            free_list.end());
+        }
+    }
+    // replay the placement: a read must find its own object in every physical plane, and a new
+    // object may only take planes whose previous holder is dead
+    std::vector<int> held(top, -1);
+    for (size_t n = 0; n < ops.size(); n++) {
+        for (int pass = 0; pass < 2; pass++)
+            for (const Acc &x : access[n]) {
+                if (!x.len) continue;
+                const int k = owner[ops[n].w[x.word]];
+                const bool birth = objs[k].first == n;
+                if (birth != (pass == 1)) continue;
+                for (uint32_t j = 0; j < x.len; j++) {
+                    int &h = held[objs[k].phys + j];
+                    if (birth) {
+                        if (h >= 0 && h != k && objs[h].last >= n) return 0;
+                        h = k;
+                    } else if (h != k) {
+                        return 0;
+                    }
+                }
+            }
+    }
+    for (size_t n = 0; n < ops.size(); n++)
+        for (const Acc &x : access[n])
+            if (x.len) ops[n].w[x.word] = objs[owner[ops[n].w[x.word]]].phys;
+    return top;
+}
+
 struct FusedState {
     std::mutex lock;
     double *scratch = nullptr;
@@ -267,6 +391,9 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
         fprintf(stderr, "pacman recurrence runner: the program reads a current plane before writing it\n");
         return PACMAN_RECURRENCE_INVALID;
     }
+    // the slot keeps only the live currents
+    uint32_t planes = d->current_planes;
+    if (const uint32_t compacted = compact_planes(hp.ops, d->current_planes)) planes = compacted;
     pacuda_plan *p = new pacuda_plan();
     const uint32_t S = d->scale_count, X = d->factor_count;
     std::vector<double> c64(2 * (size_t)(S + X)), cdf;
@@ -289,7 +416,7 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     h.cdf = upload(p, cdf.data(), cdf.size(), &ok);
     h.scale_count = S;
     h.factor_count = X;
-    h.current_planes = d->current_planes;
+    h.current_planes = planes;
     h.amplitude_planes = d->amplitude_planes;
     h.form_count = d->form_count;
     h.outputs = d->outputs;
@@ -303,7 +430,7 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     h.denominator = d->denominator;
     h.factor = d->numerator != d->denominator;
     p->device = upload(p, &h, 1, &ok);
-    p->slot_doubles = (uint64_t)LANES * (2ull * d->current_planes + 2ull * d->amplitude_planes + 4ull * d->form_count);
+    p->slot_doubles = (uint64_t)LANES * (2ull * planes + 2ull * d->amplitude_planes + 4ull * d->form_count);
     if (!ok) {
         pacuda_plan_destroy(p);
         return PACMAN_RECURRENCE_DEVICE_ERROR;
@@ -370,7 +497,7 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
     if (sc.units >= 0xffffffffull) return PACMAN_RECURRENCE_INVALID;
     sc.starts[count] = (uint32_t)sc.units;
     sc.block = l->block ? l->block : 256;
-    const uint32_t per_cu = l->waves_per_cu ? l->waves_per_cu : 4;
+    const uint32_t per_cu = l->waves_per_cu ? l->waves_per_cu : 12;
     const uint32_t waves_per_block = sc.block / LANES;
     uint64_t waves = std::min<uint64_t>((uint64_t)compute_units() * per_cu, std::max<uint64_t>(sc.units, 1));
     sc.grid = (uint32_t)((waves + waves_per_block - 1) / waves_per_block);

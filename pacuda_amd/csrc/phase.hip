@@ -42,7 +42,7 @@ struct Crossing {
 
 __global__ void rambo_kernel(uint64_t state, uint64_t first, uint32_t points, uint32_t legs, double ecm, Crossing cr,
                              double *external, uint32_t stride, float *features, uint8_t *pass, double pt_min,
-                             double eta_max, double dr_min) {
+                             double eta_max, double dr_min, double *log_prior) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= points) return;
     const uint32_t m = legs - 2;
@@ -90,6 +90,22 @@ __global__ void rambo_kernel(uint64_t state, uint64_t first, uint32_t points, ui
     if (pass) pass[i] = ok ? 1 : 0;
     for (uint32_t l = 0; l < legs; l++)
         for (int k = 0; k < 4; k++) external[((size_t)l * 4 + k) * stride + i] = cr.sign[l] * p[cr.src[l]][k];
+    if (log_prior) {
+        // log(sum_{a<b} s_ab^4 / prod_l s_{l,l+1}) over the plan's legs in order (cyclic),
+        // s_ab = |2 p_a.p_b|: the helicity-summed MHV Parke-Taylor structure of the ordering
+        double num = 0.0, log_den = 0.0;
+        for (uint32_t a = 0; a < legs; a++) {
+            const double *pa = p[cr.src[a]];
+            for (uint32_t b = a + 1; b < legs; b++) {
+                const double *pb = p[cr.src[b]];
+                const double sab = fabs(2.0 * (pa[0] * pb[0] - pa[1] * pb[1] - pa[2] * pb[2] - pa[3] * pb[3]));
+                const double s2 = sab * sab;
+                num += s2 * s2;
+                if (b == a + 1 || (a == 0 && b == legs - 1)) log_den += log(sab);
+            }
+        }
+        log_prior[i] = log(num) - log_den;
+    }
     if (features) {
         // per outgoing parton: the uniforms of its direction and the CDF of its energy product
         // (t - t ln t for t = u u'), then the smallest pT over sqrt(s)/2 and the smallest Delta R / 4
@@ -132,7 +148,8 @@ uint64_t seed_state(uint64_t seed) {
 
 extern "C" int pacuda_rambo(uint64_t seed, uint64_t first, uint32_t points, uint32_t legs, double ecm,
                             const uint32_t *crossing_src, const double *crossing_sign, double *external, uint32_t stride,
-                            float *features, uint8_t *pass, double pt_min, double eta_max, double dr_min) {
+                            float *features, uint8_t *pass, double pt_min, double eta_max, double dr_min,
+                            double *log_prior) {
     if (legs < 3 || legs > MAX_LEGS || !external || !crossing_src || !crossing_sign || stride < points)
         return PACMAN_RECURRENCE_INVALID;
     Crossing cr;
@@ -143,7 +160,8 @@ extern "C" int pacuda_rambo(uint64_t seed, uint64_t first, uint32_t points, uint
     }
     if (points == 0) return PACMAN_RECURRENCE_OK;
     rambo_kernel<<<(points + 255) / 256, 256, 0, runner_stream()>>>(seed_state(seed), first, points, legs, ecm, cr, external,
-                                                                    stride, features, pass, pt_min, eta_max, dr_min);
+                                                                    stride, features, pass, pt_min, eta_max, dr_min,
+                                                                    log_prior);
     return hip_failed(hipGetLastError(), "rambo") ? PACMAN_RECURRENCE_DEVICE_ERROR : PACMAN_RECURRENCE_OK;
 }
 

@@ -42,6 +42,28 @@ def uniforms(seed, count, offset=0):
     return ((z >> np.uint64(11)).astype(np.float64) + 0.5) / float(1 << 53)
 
 
+def uniform_at(seed, ids):
+    """the uniforms number ids + 1 of the stream `seed` (the device's hit-or-miss numbers)"""
+    with np.errstate(over="ignore"):
+        k = np.asarray(ids, dtype=np.uint64) + np.uint64(1)
+        z = _mix(seed_state(seed) + k * GAMMA)
+    return ((z >> np.uint64(11)).astype(np.float64) + 0.5) / float(1 << 53)
+
+
+def log_prior(mom, crossing):
+    """log(sum_{a<b} s_ab^4 / prod_l s_{l,l+1}), s_ab = |2 p_a.p_b| over the plan's legs in
+    crossing order (cyclic); mom [points][legs][4] physical momenta"""
+    legs = len(crossing)
+    p = np.stack([mom[:, c[0]] for c in crossing], axis=1)
+
+    def s(a, b):
+        x, y = p[:, a], p[:, b]
+        return np.abs(2 * (x[:, 0] * y[:, 0] - x[:, 1] * y[:, 1] - x[:, 2] * y[:, 2] - x[:, 3] * y[:, 3]))
+    num = sum(s(a, b) ** 4 for a in range(legs) for b in range(a + 1, legs))
+    den = sum(np.log(s(a, (a + 1) % legs)) for a in range(legs))
+    return np.log(num) - den
+
+
 def _rambo_from(u, n, ecm):
     """RAMBO from uniforms u[point, particle, 4] (c, phi, e1, e2), as record.rs builds them"""
     m = n - 2

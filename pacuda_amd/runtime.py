@@ -139,3 +139,36 @@ def evaluate(lib, jobs, kernel, waves_per_cu=0, block=0, timed=False):
         if lib.pacuda_synchronize() != 0:
             raise RuntimeError("device synchronisation failed")
     return ms.value
+
+
+class Sweep:
+    """A batch of items captured once into a HIP graph (pacuda_sweep_create) and replayed"""
+
+    def __init__(self, lib, jobs, kernel, waves_per_cu=0, block=0):
+        self.lib = lib
+        items = (abi.Item * len(jobs))()
+        for k, (fp, ext, points, stride, res, tot) in enumerate(jobs):
+            items[k].plan = fp.handle
+            items[k].external = ext.ptr
+            items[k].points = points
+            items[k].ext_stride = stride
+            items[k].resolved = res.ptr if res is not None else None
+            items[k].total = tot.ptr if tot is not None else None
+        l = abi.Launch(abi.ABI, C.sizeof(abi.Launch), kernel, waves_per_cu, block, 0, None, None)
+        h = C.c_void_p()
+        st = lib.pacuda_sweep_create(items, len(jobs), C.byref(l), C.byref(h))
+        if st != 0:
+            raise RuntimeError(f"pacuda_sweep_create: status {st}")
+        self.handle = h
+
+    def run(self, timed=False):
+        ms = C.c_float(0)
+        st = self.lib.pacuda_sweep_run(self.handle, None, C.pointer(ms) if timed else None)
+        if st != 0:
+            raise RuntimeError(f"pacuda_sweep_run: status {st}")
+        return ms.value
+
+    def destroy(self):
+        if self.handle:
+            self.lib.pacuda_sweep_destroy(self.handle)
+            self.handle = None

@@ -1,34 +1,46 @@
 # SPDX-License-Identifier: 0BSD
-"""Plain and surrogate two-stage unweighting on one GPU.
+"""Plain and surrogate two-stage unweighting, on one GPU.
 
-The integrand is w(x) = |M|^2(x) * cut(x) * w_RAMBO for the plan's colour ordering, summed over
-helicities (g = 1), with x the RAMBO point of the unit hypercube. Every phase-space point and
-every exact |M|^2 is computed on the GPU (pacuda_rambo, pacuda_evaluate in f64); the surrogate
-exp(NN(features)) of log|M|^2 is the library's BF16 MLP.
+The integrand is w(x) = |M|^2(x) * cut(x) for the plan's colour ordering, summed over
+helicities, with x the RAMBO point of the unit hypercube (the flat RAMBO weight and the flux are
+constant factors applied to the cross sections). Phase-space points, exact |M|^2 (the fused f64
+recurrence), the surrogate, the weight statistics and the hit-or-miss selections all run on the
+GPU; the host reads counts and the few accepted values.
 
-plain:      accept candidate k with probability w_k / w_max (w_max the sample maximum); the
-            exact recurrence runs on every candidate that passes the cuts.
-two-stage:  stage 1 accepts with probability s_k / s_max (s the surrogate weight, every candidate);
-            stage 2 runs the exact recurrence on the stage-1 survivors only and accepts with
-            probability min(1, r_k / r_max), r = w / s; an event with r > r_max is kept with the
-            correction weight r / r_max. The accepted sample, with its weights, is unbiased:
-            E[s_max r_max a_k c_k] = E[w] (a the acceptance, c the correction weight).
-Cross sections: sigma = N_candidates^-1 sum of the per-candidate estimator, with its standard
-error; plain w_max * accepted / N, two-stage s_max r_max * sum(weights) / N."""
+Surrogate: s(x) = exp(mu + sigma * NN(features(x)) + log P(x)), NN the library's BF16 MLP on the
+14 unit-hypercube features, P the pure-gluon MHV Parke-Taylor structure of the ordering
+(sum s_ab^4 / prod s_{l,l+1}); the network learns the normalised residual log(|M|^2 / P).
+Without the prior the network learns log|M|^2 itself; without the network s = exp(mu) P.
+
+Thresholds are max reductions: a threshold is the q-quantile of the weights it divides (read off
+a device log histogram), and an event above it is kept with the weight w / threshold > 1, so
+every estimate below stays unbiased; plain and two-stage use the same q.
+
+plain:      accept candidate k when u_k * w_thr < w_k; the exact recurrence runs on every
+            candidate that passes the cuts. Event weight max(1, w / w_thr).
+two-stage:  stage 1 accepts when u1_k * s_thr < s_k (s on every candidate); stage 2 runs the
+            exact recurrence on the stage-1 survivors only and accepts when u2_k * r_thr < r_k,
+            r = w / s, r_thr the q2-quantile of r over the survivors. Event weight
+            max(1, s / s_thr) * max(1, r / r_thr); E[s_thr r_thr a c] = w.
+Cross sections: sigma = mean over all candidates of the per-candidate estimator (threshold
+product times the event weight for accepted candidates, 0 otherwise), with its standard error."""
 import math
 import time
 
 import numpy as np
 
 from . import abi, plans, phase_space as ps
-from .runtime import Device, DeviceArray, FusedPlan, evaluate
+from .runtime import Device, FusedPlan, evaluate
 from .surrogate import MLP, he_init
 
 GEV2_TO_PB = 0.3893793721e9
+LOG_RANGE = (-230.0, 230.0)  # log-weight histogram range (clamped beyond)
+BINS = 8192
+CUTS = (20.0, 2.5, 0.4)
 
 
 class Generator:
-    """RAMBO points of a plan's process on the GPU, with cuts and surrogate features"""
+    """RAMBO points of a plan's process on the GPU, with cuts, surrogate features and prior"""
 
     def __init__(self, dev, plan, ecm=ps.ECM):
         self.dev, self.plan, self.ecm = dev, plan, ecm
@@ -37,42 +49,102 @@ class Generator:
         self.src = np.array([c[0] for c in plan.crossing], dtype=np.uint32)
         self.sign = np.array([c[1] for c in plan.crossing], dtype=np.float64)
 
-    def draw(self, seed, first, count):
-        """(external DeviceArray [legs*4][count], features DeviceArray [count][width], pass bool)"""
-        lib = self.dev.lib
-        ext = self.dev.empty(self.legs * 4 * count)
-        feat = self.dev.empty(count * self.width, np.float32)
-        passed = self.dev.empty(count, np.uint8)
+    def draw_device(self, seed, first, count, features=True, prior=False):
+        """dict of device arrays: ext [legs*4][count], pass [count] u8, feat [count][width] f32,
+        logp [count] f64 (the last two when asked for)"""
+        lib, dev = self.dev.lib, self.dev
+        out = {"ext": dev.empty(self.legs * 4 * count), "pass": dev.empty(count, np.uint8)}
+        out["feat"] = dev.empty(count * self.width, np.float32) if features else None
+        out["logp"] = dev.empty(count) if prior else None
         st = lib.pacuda_rambo(seed, first, count, self.legs, self.ecm, abi.ptr(self.src, abi.u32p), abi.ptr(self.sign),
-                              ext.ptr, count, feat.ptr, passed.ptr, 20.0, 2.5, 0.4)
+                              out["ext"].ptr, count, out["feat"].ptr if features else None, out["pass"].ptr, *CUTS,
+                              out["logp"].ptr if prior else None)
         if st != 0:
             raise RuntimeError(f"pacuda_rambo: status {st}")
-        ok = passed.download(count, np.uint8).astype(bool)
-        passed.free()
-        return ext, feat, ok
+        return out
 
-    def gather(self, ext, feat, stride, index):
+    def draw(self, seed, first, count):
+        """(external DeviceArray [legs*4][count], features DeviceArray [count][width], pass bool)"""
+        d = self.draw_device(seed, first, count)
+        ok = d["pass"].download(count, np.uint8).astype(bool)
+        d["pass"].free()
+        return d["ext"], d["feat"], ok
+
+    def gather(self, ext, feat, stride, index, rows=None):
+        """rows of ext (columns index, a host array or a (DeviceArray, count) pair) and feature rows"""
         lib = self.dev.lib
-        index = np.ascontiguousarray(index, dtype=np.uint32)
-        d_idx = self.dev.array(index)
-        n = len(index)
-        out = self.dev.empty(self.legs * 4 * max(n, 1))
+        if isinstance(index, tuple):
+            d_idx, n, own = index[0], index[1], False
+        else:
+            d_idx = self.dev.array(np.ascontiguousarray(index, dtype=np.uint32))
+            n, own = len(index), True
+        rows = self.legs * 4 if rows is None else rows
+        out = self.dev.empty(rows * max(n, 1))
         fout = self.dev.empty(max(n, 1) * self.width, np.float32) if feat is not None else None
-        st = lib.pacuda_gather(ext.ptr, stride, self.legs * 4, d_idx.ptr, n, out.ptr,
+        st = lib.pacuda_gather(ext.ptr, stride, rows, d_idx.ptr, n, out.ptr,
                                feat.ptr if feat is not None else None, self.width, fout.ptr if fout else None)
-        d_idx.free()
+        if own:
+            d_idx.free()
         if st != 0:
             raise RuntimeError(f"pacuda_gather: status {st}")
         return out, fout
 
 
+class Selector:
+    """pacuda_select / pacuda_weight_stats with their device outputs"""
+
+    def __init__(self, dev):
+        self.dev = dev
+        self.d_count = dev.empty(1, np.uint32)
+        self.d_stats = dev.empty(3 * abi.STATS_BLOCKS)
+
+    def select(self, count, w=None, flags=None, ids=None, first=0, threshold=1.0, seed=0, values=False):
+        """(index DeviceArray, values DeviceArray or None, selected count)"""
+        lib = self.dev.lib
+        idx = self.dev.empty(max(count, 1), np.uint32)
+        val = self.dev.empty(max(count, 1)) if values else None
+        st = lib.pacuda_select(w.ptr if w is not None else None, flags.ptr if flags is not None else None,
+                               ids.ptr if ids is not None else None, first, count, threshold, seed, idx.ptr,
+                               val.ptr if val is not None else None, self.d_count.ptr)
+        if st != 0:
+            raise RuntimeError(f"pacuda_select: status {st}")
+        n = int(self.d_count.download(1, np.uint32)[0])
+        return idx, val, n
+
+    def stats(self, w, count, hist=None):
+        """(max, sum, sum of squares) of w[0..count), reduced in a fixed order; hist accumulates"""
+        st = self.dev.lib.pacuda_weight_stats(w.ptr, count, LOG_RANGE[0], LOG_RANGE[1], BINS,
+                                              hist.ptr if hist is not None else None, self.d_stats.ptr)
+        if st != 0:
+            raise RuntimeError(f"pacuda_weight_stats: status {st}")
+        s = self.d_stats.download((abi.STATS_BLOCKS, 3))
+        return float(np.max(s[:, 0])), float(np.sum(s[:, 1])), float(np.sum(s[:, 2]))
+
+    def free(self):
+        self.d_count.free()
+        self.d_stats.free()
+
+
+def quantile_threshold(hist, q):
+    """the upper edge of the histogram bin holding the q-quantile of the counted weights"""
+    h = np.asarray(hist, dtype=np.int64)
+    total = int(h.sum())
+    above = int(math.floor((1.0 - q) * total))
+    tail = np.cumsum(h[::-1])
+    b = BINS - 1 - int(np.searchsorted(tail, above + 1))
+    lo, hi = LOG_RANGE
+    return math.exp(lo + (b + 1) * (hi - lo) / BINS)
+
+
 class Unweighting:
-    def __init__(self, legs=6, hidden=256, layers=4, batch=4096, lib=None):
+    def __init__(self, legs=6, hidden=256, layers=4, batch=4096, prior=True, lib=None):
         self.dev = Device(lib)
         self.plan = plans.gluon_plan(legs)
         self.fused = FusedPlan(self.dev.lib, self.plan)
         self.gen = Generator(self.dev, self.plan)
+        self.sel = Selector(self.dev)
         self.batch = batch
+        self.prior = prior
         self.mlp = MLP(self.gen.width, hidden, layers, max_batch=batch, lib=self.dev.lib)
         self.mean, self.std = 0.0, 1.0
         self.w_ps = ps.rambo_weight(legs)
@@ -85,49 +157,58 @@ class Unweighting:
             evaluate(self.dev.lib, [(self.fused, ext, count, count, None, tot)], kernel)
         return tot
 
-    def surrogate(self, feat, count):
-        """exp of the network, in |M|^2 units (padding to whole tiles of 64)"""
-        pad = (count + 63) // 64 * 64
-        y = self.dev.empty(pad, np.float32)
-        if pad != count:
-            # pad the feature rows with a copy of the last ones
-            host = feat.download((count, self.gen.width), np.float32)
-            host = np.concatenate([host, np.repeat(host[-1:], pad - count, axis=0)])
-            f2 = self.dev.array(host)
-            self.mlp.forward(f2, pad, y)
-            f2.free()
-        else:
-            self.mlp.forward(feat, pad, y)
-        out = y.download(pad, np.float32)[:count].astype(np.float64)
-        y.free()
-        return np.exp(self.mean + self.std * out)
+    def surrogate_weights(self, d, count, network=True):
+        """device s[count] of a draw (0 where the cuts fail)"""
+        lib = self.dev.lib
+        y = None
+        if network:
+            y = self.dev.empty((count + 63) // 64 * 64, np.float32)
+            if count % 64:
+                raise ValueError("candidate chunks are whole tiles of 64 points")
+            self.mlp.forward(d["feat"], count, y)
+        s = self.dev.empty(max(count, 1))
+        logp = d["logp"] if self.prior else None
+        st = lib.pacuda_surrogate_weight(y.ptr if y else None, logp.ptr if logp else None, d["pass"].ptr, count,
+                                         self.mean, self.std if network else 0.0, s.ptr)
+        if y:
+            y.free()
+        if st != 0:
+            raise RuntimeError(f"pacuda_surrogate_weight: status {st}")
+        return s
 
     # ---- training ---------------------------------------------------------------------------
     def training_set(self, points, seed=3):
-        """`points` cut-passing points of stream `seed` with their exact log|M|^2 (f64 GPU)"""
-        draw = int(points * 1.1) + 1024
+        """`points` cut-passing points of stream `seed`: features, exact log|M|^2 (f64 GPU) and
+        log P (host arrays)"""
+        draw = int(points * 1.25) + 1024
         while True:
-            ext, feat, ok = self.gen.draw(seed, 0, draw)
-            idx = np.nonzero(ok)[0][:points]
-            if len(idx) == points:
+            d = self.gen.draw_device(seed, 0, draw, features=True, prior=True)
+            idx, _, n = self.sel.select(draw, flags=d["pass"])
+            if n >= points:
                 break
-            ext.free()
-            feat.free()
-            draw = int(draw * 1.2 * points / max(len(idx), 1)) + 1024
-        e2, f2 = self.gen.gather(ext, feat, draw, idx)
-        ext.free()
-        feat.free()
+            for a in d.values():
+                a.free()
+            idx.free()
+            draw = int(draw * 1.2 * points / max(n, 1)) + 1024
+        e2, f2 = self.gen.gather(d["ext"], d["feat"], draw, (idx, points))
+        lp, _ = self.gen.gather(d["logp"], None, draw, (idx, points), rows=1)
+        for a in d.values():
+            a.free()
+        idx.free()
         tot = self.exact(e2, points)
         m2 = tot.download(points)
-        tot.free()
-        e2.free()
+        logp = lp.download(points)
         x = f2.download((points, self.gen.width), np.float32)
-        f2.free()
-        return x, np.log(m2)
+        for a in (tot, e2, f2, lp):
+            a.free()
+        return x, np.log(m2), logp
 
-    def train(self, x, logm2, epochs=30, lr=1e-3, seed=0, init_seed=5, log=None):
-        self.mean, self.std = float(np.mean(logm2)), float(np.std(logm2))
-        t = ((logm2 - self.mean) / self.std).astype(np.float32)
+    def train(self, x, logm2, logp=None, epochs=30, lr=1e-3, seed=0, init_seed=5, log=None):
+        """fit the network to the normalised residual log(|M|^2 / P) (or log|M|^2 without the
+        prior) with Adam and a cosine learning-rate decay; returns the mean loss per epoch"""
+        target = logm2 - logp if (self.prior and logp is not None) else logm2
+        self.mean, self.std = float(np.mean(target)), float(np.std(target))
+        t = ((target - self.mean) / self.std).astype(np.float32)
         self.mlp.set_params(he_init(self.gen.width, self.mlp.hidden, self.mlp.layers, init_seed))
         n = len(t) // self.batch * self.batch
         rng = np.random.default_rng(seed)
@@ -146,7 +227,6 @@ class Unweighting:
                 d_t.upload(t[sel])
                 self.mlp.forward(d_x, self.batch, d_y, train=True)
                 losses.append(self.mlp.backward(d_y, d_t, self.batch))
-                # cosine learning-rate decay
                 self.mlp.adam(lr * 0.5 * (1 + math.cos(math.pi * step / steps)))
                 step += 1
             history.append(float(np.mean(losses)))
@@ -157,107 +237,130 @@ class Unweighting:
         return history
 
     # ---- unweighting ---------------------------------------------------------------------------
-    def run(self, candidates, seed=4, chunk=1 << 22, r_quantile=0.999, rng_seed=11):
-        """both unweightings on the same candidates; dict of counts, times, cross sections"""
-        rng = np.random.default_rng(rng_seed)
-        sync = self.dev.synchronize
-        t_gen = t_nn = t_exact_plain = t_exact_stage2 = t_select = 0.0
-        s_all, w_all, ok_all = [], [], []
-        stage = []  # per chunk: (ext, feat, count) kept for stage 2
+    def _timed(self, clock, key, fn, *args, **kw):
+        self.dev.synchronize()
+        t = time.perf_counter()
+        out = fn(*args, **kw)
+        self.dev.synchronize()
+        clock[key] = clock.get(key, 0.0) + time.perf_counter() - t
+        return out
+
+    def plain(self, candidates, seed=4, chunk=1 << 22, q=0.99999, accept_seed=21):
+        """plain unweighting of `candidates` points: dict of counts, times and cross sections"""
+        clock, hist = {}, self.dev.array(np.zeros(BINS, dtype=np.uint32))
+        kept, sums = [], [0.0, 0.0, 0.0]  # per chunk (w, ids, n, first); sum w, sum w^2, max w
+        passed = 0
         for start in range(0, candidates, chunk):
             count = min(chunk, candidates - start)
-            sync()
-            t = time.perf_counter()
-            ext, feat, ok = self.gen.draw(seed, start, count)
-            sync()
-            t_gen += time.perf_counter() - t
-            t = time.perf_counter()
-            s = self.surrogate(feat, count) * ok
-            sync()
-            t_nn += time.perf_counter() - t
-            # plain: the exact integrand on every candidate that passes the cuts
-            t = time.perf_counter()
-            idx = np.nonzero(ok)[0]
-            e2, _ = self.gen.gather(ext, None, count, idx)
-            tot = self.exact(e2, len(idx))
-            w = np.zeros(count)
-            w[idx] = tot.download(len(idx))
-            sync()
-            t_exact_plain += time.perf_counter() - t
-            tot.free()
+            d = self._timed(clock, "generate", self.gen.draw_device, seed, start, count, features=False)
+            idx, _, n = self._timed(clock, "select", self.sel.select, count, flags=d["pass"])
+            e2, _ = self._timed(clock, "gather", self.gen.gather, d["ext"], None, count, (idx, n))
+            for a in d.values():
+                if a is not None:
+                    a.free()
+            w = self._timed(clock, "exact", self.exact, e2, n)
             e2.free()
-            s_all.append(s)
-            w_all.append(w)
-            ok_all.append(ok)
-            stage.append((ext, feat, count))
-        s = np.concatenate(s_all)
-        w = np.concatenate(w_all)
-        N = candidates
-        # plain unweighting
-        w_max = float(np.max(w))
-        u = rng.uniform(size=N)
-        plain_acc = u < w / w_max
-        n_plain = int(plain_acc.sum())
-        x_plain = w_max * plain_acc
-        # two-stage: stage 1 on the surrogate
-        t = time.perf_counter()
-        s_max = float(np.max(s))
-        u1 = rng.uniform(size=N)
-        acc1 = u1 < s / s_max
-        t_select += time.perf_counter() - t
-        # stage 2: the exact integrand on the stage-1 survivors only (gathered per chunk)
-        w_stage = np.zeros(N)
-        at = 0
-        for ext, feat, count in stage:
-            sync()
+            mx, s1, s2 = self._timed(clock, "stats", self.sel.stats, w, n, hist)
+            sums[0] += s1
+            sums[1] += s2
+            sums[2] = max(sums[2], mx)
+            kept.append((w, idx, n, start))
+            passed += n
+        h = hist.download(BINS, np.uint32)
+        hist.free()
+        w_thr = quantile_threshold(h, q)
+        acc_w = []
+        for w, idx, n, start in kept:
+            a_idx, a_val, m = self._timed(clock, "accept", self.sel.select, n, w=w, ids=idx, first=start,
+                                          threshold=w_thr, seed=accept_seed, values=True)
+            acc_w.append(a_val.download(m))
+            for a in (a_idx, a_val, w, idx):
+                a.free()
+        acc_w = np.concatenate(acc_w)
+        x = w_thr * np.maximum(1.0, acc_w / w_thr)
+        return {"candidates": candidates, "passed_cuts": passed, "exact_evaluations": passed,
+                "events": len(acc_w), "overweight_events": int(np.sum(acc_w > w_thr)), "threshold": w_thr,
+                "estimate": self._estimate(x, candidates), "weighted": self._weighted(sums, candidates),
+                "seconds": sum(clock.values()), "times_s": clock}
+
+    def two_stage(self, candidates, seed=4, chunk=1 << 22, q=0.99999, q2=0.999, network=True, accept_seed=22,
+                  stage2_seed=23):
+        """two-stage unweighting of `candidates` points (the same points as plain() for one seed)"""
+        clock, hist = {}, self.dev.array(np.zeros(BINS, dtype=np.uint32))
+        staged = []
+        for start in range(0, candidates, chunk):
+            count = min(chunk, candidates - start)
+            d = self._timed(clock, "generate", self.gen.draw_device, seed, start, count, features=network,
+                            prior=self.prior)
+            s = self._timed(clock, "surrogate", self.surrogate_weights, d, count, network)
+            self._timed(clock, "stats", self.sel.stats, s, count, hist)
+            for k in ("pass", "feat", "logp"):
+                if d[k] is not None:
+                    d[k].free()
+            staged.append((d["ext"], s, count, start))
+        h = hist.download(BINS, np.uint32)
+        hist.free()
+        s_thr = quantile_threshold(h, q)
+        ids, s1, w1 = [], [], []
+        for ext, s, count, start in staged:
+            a_idx, a_val, m = self._timed(clock, "stage1", self.sel.select, count, w=s, first=start, threshold=s_thr,
+                                          seed=accept_seed, values=True)
+            e2, _ = self._timed(clock, "gather", self.gen.gather, ext, None, count, (a_idx, m))
+            w = self._timed(clock, "exact", self.exact, e2, m)
             t = time.perf_counter()
-            idx = np.nonzero(acc1[at:at + count])[0]
-            e2, _ = self.gen.gather(ext, None, count, idx)
-            tot = self.exact(e2, len(idx))
-            w_stage[at + idx] = tot.download(len(idx))
-            sync()
-            t_exact_stage2 += time.perf_counter() - t
-            tot.free()
-            e2.free()
-            ext.free()
-            feat.free()
-            at += count
+            ids.append(a_idx.download(m, np.uint32).astype(np.int64) + start)
+            s1.append(a_val.download(m))
+            w1.append(w.download(m))
+            clock["stage2"] = clock.get("stage2", 0.0) + time.perf_counter() - t
+            for a in (a_idx, a_val, e2, w, ext, s):
+                a.free()
         t = time.perf_counter()
-        r = np.zeros(N)
-        r[acc1] = w_stage[acc1] / s[acc1]
-        r_max = float(np.quantile(r[acc1], r_quantile)) if acc1.any() else 1.0
-        u2 = rng.uniform(size=N)
-        acc2 = acc1 & (u2 < r / r_max)
-        weight = np.where(acc2, np.maximum(1.0, r / r_max), 0.0)
-        t_select += time.perf_counter() - t
-        n_two = int(acc2.sum())
-        x_two = s_max * r_max * weight
-        scale = self.w_ps * self.flux * GEV2_TO_PB
+        ids, s1, w1 = np.concatenate(ids), np.concatenate(s1), np.concatenate(w1)
+        r = w1 / s1
+        r_thr = float(np.quantile(r, q2)) if len(r) else 1.0
+        acc = ps.uniform_at(stage2_seed, ids) * r_thr < r
+        clock["stage2"] = clock.get("stage2", 0.0) + time.perf_counter() - t
+        x = s_thr * r_thr * np.maximum(1.0, s1[acc] / s_thr) * np.maximum(1.0, r[acc] / r_thr)
+        return {"candidates": candidates, "stage1_accepted": len(ids), "exact_evaluations": len(ids),
+                "events": int(acc.sum()), "overweight_events": int(np.sum((s1[acc] > s_thr) | (r[acc] > r_thr))),
+                "threshold": s_thr, "r_threshold": r_thr, "stage2_efficiency": float(acc.mean()) if len(r) else 0.0,
+                "estimate": self._estimate(x, candidates), "seconds": sum(clock.values()), "times_s": clock}
 
-        def est(xs):
-            return float(np.mean(xs) * scale), float(np.std(xs) / math.sqrt(N) * scale)
+    def _scale(self):
+        return self.w_ps * self.flux * GEV2_TO_PB
 
-        sig_plain, err_plain = est(x_plain)
-        sig_two, err_two = est(x_two)
-        sig_w, err_w = est(w)
-        time_plain = t_gen + t_exact_plain
-        time_two = t_gen + t_nn + t_exact_stage2 + t_select
-        return {
-            "candidates": N, "passed_cuts": int(np.concatenate(ok_all).sum()),
-            "plain": {"events": n_plain, "seconds": time_plain, "events_per_s": n_plain / time_plain,
-                      "efficiency": n_plain / N, "exact_evaluations": int(np.concatenate(ok_all).sum()),
-                      "sigma_pb": sig_plain, "sigma_err_pb": err_plain},
-            "two_stage": {"events": n_two, "seconds": time_two, "events_per_s": n_two / time_two,
-                          "stage1_accepted": int(acc1.sum()), "exact_evaluations": int(acc1.sum()),
-                          "overweight_events": int((weight > 1).sum()), "r_max": r_max,
-                          "sigma_pb": sig_two, "sigma_err_pb": err_two},
-            "weighted_mc_sigma_pb": sig_w, "weighted_mc_err_pb": err_w,
-            "gain": (n_two / time_two) / (n_plain / time_plain) if n_plain else None,
-            "sigma_pull": (sig_two - sig_plain) / math.hypot(err_plain, err_two),
-            "times_s": {"generate": t_gen, "surrogate": t_nn, "exact_plain": t_exact_plain,
-                        "exact_stage2": t_exact_stage2, "select": t_select},
-        }
+    def _estimate(self, x, n):
+        """(sigma, error) in pb of the per-candidate estimator with values x on accepted events"""
+        mean = np.sum(x) / n
+        var = max(np.sum(x * x) / n - mean * mean, 0.0)
+        return float(mean * self._scale()), float(math.sqrt(var / n) * self._scale())
+
+    def _weighted(self, sums, n):
+        mean = sums[0] / n
+        var = max(sums[1] / n - mean * mean, 0.0)
+        return float(mean * self._scale()), float(math.sqrt(var / n) * self._scale())
+
+    def run(self, candidates, seed=4, chunk=1 << 22, q=0.99999, q2=0.999, variants=("network", "prior")):
+        """plain and two-stage unweighting on the same candidates; `variants` of the surrogate:
+        "network" (the trained network, with the prior when the object has one) and "prior"
+        (the prior alone, no network)"""
+        out = {"plain": self.plain(candidates, seed, chunk, q)}
+        p = out["plain"]
+        rate_p = p["events"] / p["seconds"]
+        p["events_per_s"] = rate_p
+        sig_p, err_p = p["estimate"]
+        for v in variants:
+            if v == "prior" and not self.prior:
+                continue
+            t = self.two_stage(candidates, seed, chunk, q, q2, network=(v == "network"))
+            t["events_per_s"] = t["events"] / t["seconds"]
+            t["gain"] = t["events_per_s"] / rate_p if rate_p else None
+            sig, err = t["estimate"]
+            t["sigma_pull"] = (sig - sig_p) / math.hypot(err, err_p)
+            out["two_stage" if v == "network" else "two_stage_prior_only"] = t
+        return out
 
     def destroy(self):
+        self.sel.free()
         self.mlp.destroy()
         self.fused.destroy()

@@ -214,3 +214,23 @@ def test_pp_jjj_subprocesses_closed_forms():
                 else:
                     want = np.zeros(8)
             assert np.allclose(res[:, o], want, rtol=1e-9, atol=1e-18 * np.max(tot)), (sp.name, hel)
+
+
+@pytest.mark.parametrize("n", [4, 5])
+def test_surrogate_prior_is_exact_below_six_gluons(n):
+    """every 4- and 5-gluon helicity amplitude is MHV or anti-MHV, so the helicity-summed
+    leading-colour |M|^2 is a constant times sum_{a<b} s_ab^4 / prod s_{l,l+1}: the oracle's plan
+    run divided by exp(phase_space.log_prior) is the same number at every point"""
+    plan = plans.gluon_plan(n)
+    mom, _ = ps.generate(n, 300, 17)
+    ext = ps.external_planes(mom, plan.crossing)
+    _, tot = refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), ext, len(mom), kernel=1, resolved=False)
+    ratio = tot / np.exp(ps.log_prior(mom, plan.crossing))
+    assert np.std(ratio) < 1e-9 * np.mean(ratio)
+    # and at six gluons (NMHV configurations) it is not
+    plan6 = plans.gluon_plan(6)
+    mom6, _ = ps.generate(6, 300, 17)
+    _, tot6 = refcpu.evaluate(plan6.arrays(), harness.plan_counts(plan6), ps.external_planes(mom6, plan6.crossing),
+                              len(mom6), kernel=1, resolved=False)
+    r6 = np.log(tot6) - ps.log_prior(mom6, plan6.crossing)
+    assert 0.05 < np.std(r6) < 0.5
