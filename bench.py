@@ -14,6 +14,7 @@ f64; g g > g g g g (2^21 points, f64 and mixed, the persistent grid tuned by a s
 p p > j j j batch (every subprocess with at most one quark line in one persistent launch, Zipf
 point counts summing to 2^22); and surrogate vs plain unweighting of g g > g g g g."""
 import argparse
+import itertools
 import json
 import os
 import sys
@@ -25,7 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from pacuda_amd import abi, plans, processes, phase_space as ps  # noqa: E402
-from pacuda_amd.runtime import Device, FusedPlan, evaluate  # noqa: E402
+from pacuda_amd.runtime import Device, FusedPlan, autotune, evaluate  # noqa: E402
 from pacuda_amd.unweight import Generator, Unweighting  # noqa: E402
 
 METRIC = "|M|^2 points/s (f64 bit-exact) + unweighted events/s, surrogate vs plain"
@@ -64,7 +65,9 @@ def timed_steps(lib, jobs, kernel, steps, waves_per_cu=0, block=0):
     return time.perf_counter() - t, float(np.mean(ms))
 
 
-def cpu_baseline(plan, ext_host, gpu_total, sample):
+def cpu_baseline(plan, ext_host, gpu_total, sample, runs=3):
+    """the oracle's OpenMP plan interpreter on the host cores: median of `runs` timed runs over
+    `sample` points (all threads) and over 2^14 points (one thread), after a warm-up"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refcpu
     counts = dict(current_planes=plan.current_planes, amplitude_planes=plan.amplitude_planes,
@@ -72,18 +75,27 @@ def cpu_baseline(plan, ext_host, gpu_total, sample):
     arrays = plan.arrays()
     threads = max(1, min(16, os.cpu_count() or 1))
     refcpu.evaluate(arrays, counts, ext_host[:, :4096].copy(), 4096, kernel=1, threads=threads, resolved=False)
-    t = time.perf_counter()
-    _, tot = refcpu.evaluate(arrays, counts, ext_host, sample, kernel=1, threads=threads, resolved=False)
-    dt_all = time.perf_counter() - t
-    one = min(sample, 1 << 15)
-    t = time.perf_counter()
-    refcpu.evaluate(arrays, counts, np.ascontiguousarray(ext_host[:, :one]), one, kernel=1, threads=1, resolved=False)
-    dt_one = time.perf_counter() - t
+    times = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        _, tot = refcpu.evaluate(arrays, counts, ext_host, sample, kernel=1, threads=threads, resolved=False)
+        times.append(time.perf_counter() - t)
+    dt_all = float(np.median(times))
+    one = min(sample, 1 << 14)
+    ext_one = np.ascontiguousarray(ext_host[:, :one])
+    times1 = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        refcpu.evaluate(arrays, counts, ext_one, one, kernel=1, threads=1, resolved=False)
+        times1.append(time.perf_counter() - t)
+    dt_one = float(np.median(times1))
     exact = bool(np.array_equal(tot.view(np.uint64), gpu_total[:sample].view(np.uint64)))
     return {"value": sample / dt_all, "unit": "points/s", "cores": threads, "kind": "port",
             "sample": f"the first {sample} of the benchmark's points, f64, oracle/refcpu.cpp plan interpreter "
-                      f"(OpenMP, {threads} threads, {dt_all:.1f} s); single thread {one / dt_one:.4g} points/s",
-            "single_thread_points_per_s": one / dt_one, "f64_bitexact_on_sample": exact}
+                      f"(OpenMP, {threads} threads, median of {runs} runs, {dt_all:.2f} s each); single thread "
+                      f"{one / dt_one:.4g} points/s on {one} points",
+            "single_thread_points_per_s": one / dt_one, "host_cpus_visible": os.cpu_count(),
+            "f64_bitexact_on_sample": exact}
 
 
 def mode_errors(got, ref):
@@ -116,7 +128,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--points", type=int, default=1 << 22)
     ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 20)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-modes", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
@@ -167,6 +179,30 @@ def main():
 
     extra = {}
     if not args.no_extra:
+        # the whole leading-colour sum of g g > g g g: the 12 orderings (0, s1..s4) of the legs up
+        # to reflection, each the same plan on its permutation of the external legs, in one launch
+        orders = [(0,) + p for p in itertools.permutations(range(1, plan.legs)) if p[0] < p[-1]]
+        ident = dev.array(np.arange(n, dtype=np.uint32))
+        lc_jobs, lc_hold = [], []
+        for order in orders:
+            e = dev.empty(plan.legs * 4 * n)
+            for leg, src in enumerate(order):
+                st = lib.pacuda_gather(d_ext.ptr + src * 4 * n * 8, n, 4, ident.ptr, n, e.ptr + leg * 4 * n * 8,
+                                       None, 0, None)
+                assert st == 0
+            t = dev.empty(n)
+            lc_jobs.append((fp, e, n, n, None, t))
+            lc_hold += [e, t]
+        ident.free()
+        (lw, lb), _ = autotune(lib, lc_jobs, k64, key="lc5")
+        _, ms_lc = timed_steps(lib, lc_jobs, k64, 3, lw, lb)
+        extra["gg_ggg_lc_sum"] = {"orderings": len(orders), "points": n, "tuned": f"{lw}x{lb}",
+                                  "kernel_ms": ms_lc, "points_per_s": n / (ms_lc * 1e-3),
+                                  "plan_runs_per_s": len(orders) * n / (ms_lc * 1e-3)}
+        log(f"gg>ggg LC sum over {len(orders)} orderings: {n / (ms_lc * 1e-3):.4g} points/s")
+        for a in lc_hold:
+            a.free()
+
         # g g > g g g g: 2^21 points, the persistent grid swept (waves per CU, block)
         p6 = plans.gluon_plan(6)
         n6 = 1 << 21
@@ -174,14 +210,8 @@ def main():
         f6 = FusedPlan(lib, p6)
         t6 = dev.empty(n6)
         j6 = [(f6, e6, n6, n6, None, t6)]
-        sweep = {}
-        for wpc in (1, 2, 4, 8):
-            for block in (64, 256):
-                evaluate(lib, j6, k64, waves_per_cu=wpc, block=block, timed=True)
-                _, ms = timed_steps(lib, j6, k64, 2, wpc, block)
-                sweep[f"{wpc}x{block}"] = ms
-        best = min(sweep, key=sweep.get)
-        wpc, block = (int(v) for v in best.split("x"))
+        (wpc, block), sweep = autotune(lib, j6, k64)
+        best = f"{wpc}x{block}"
         _, ms64 = timed_steps(lib, j6, k64, 3, wpc, block)
         ref6 = t6.download(n6)
         f6flops = plans.plan_flops(p6)[0]
@@ -190,9 +220,10 @@ def main():
                 "f64_tflops": f6flops * n6 / (ms64 * 1e-3) / 1e12}
         for name in ("mixed", "mixed64", "df64"):
             k = abi.KERNELS[name]
-            evaluate(lib, j6, k, waves_per_cu=wpc, block=block, timed=True)
-            _, ms = timed_steps(lib, j6, k, 3, wpc, block)
-            res6[name] = {"points_per_s": n6 / (ms * 1e-3), **mode_errors(t6.download(n6), ref6)}
+            (kw, kb), _ = autotune(lib, j6, k)
+            _, ms = timed_steps(lib, j6, k, 3, kw, kb)
+            res6[name] = {"points_per_s": n6 / (ms * 1e-3), "tuned": f"{kw}x{kb}",
+                          **mode_errors(t6.download(n6), ref6)}
         extra["gg_gggg"] = res6
         log(f"gg>gggg f64 {res6['f64_points_per_s']:.4g} points/s tuned {best}")
         for d in (e6, t6):

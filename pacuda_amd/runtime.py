@@ -172,3 +172,27 @@ class Sweep:
         if self.handle:
             self.lib.pacuda_sweep_destroy(self.handle)
             self.handle = None
+
+
+# (waves per CU, block) pairs the autotuner tries: 2 to 6 persistent waves per SIMD, blocks of
+# one or four waves
+TUNE_SHAPES = ((8, 256), (12, 256), (16, 256), (24, 256), (12, 64), (16, 64))
+_tuned = {}
+
+
+def autotune(lib, jobs, kernel, shapes=TUNE_SHAPES, repeats=2, key=None):
+    """the (waves_per_cu, block) of `shapes` with the shortest persistent launch of `jobs` (the
+    minimum of `repeats` timed launches after one warm-up each); cached under `key` when given.
+    Returns ((waves_per_cu, block), {"WxB": ms})."""
+    if key is not None and key in _tuned:
+        return _tuned[key]
+    table = {}
+    for wpc, block in shapes:
+        evaluate(lib, jobs, kernel, waves_per_cu=wpc, block=block, timed=True)
+        table[f"{wpc}x{block}"] = min(evaluate(lib, jobs, kernel, waves_per_cu=wpc, block=block, timed=True)
+                                      for _ in range(repeats))
+    best = min(table, key=table.get)
+    out = (tuple(int(v) for v in best.split("x")), table)
+    if key is not None:
+        _tuned[key] = out
+    return out

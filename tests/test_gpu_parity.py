@@ -171,3 +171,32 @@ def test_recording_replays_bit_exact_on_gpu():
                            timeout=120)
         assert p.returncode == 0, p.stdout + p.stderr
         assert "0 of 256 values differ" in p.stdout and "0 of 128 values differ" in p.stdout
+
+
+def test_fused_colour_orderings_match_oracle(lib, gg_ggg):
+    """the 12 orderings of g g > g g g (legs 1..4 permuted, up to reflection) as 12 items of one
+    plan on permuted external legs in one launch == the oracle's plan run per ordering"""
+    import itertools
+    plan, ext = gg_ggg
+    n = 700
+    fp = FusedPlan(lib, plan)
+    orders = [(0,) + p for p in itertools.permutations(range(1, 5)) if p[0] < p[-1]]
+    assert len(orders) == 12
+    jobs, devs, wants = [], [], []
+    for order in orders:
+        rows = np.concatenate([ext[4 * s:4 * s + 4, :n] for s in order])
+        d_e = DeviceArray(lib, rows.nbytes)
+        d_e.upload(rows)
+        d_t = DeviceArray(lib, n * 8)
+        jobs.append((fp, d_e, n, n, None, d_t))
+        devs += [d_e, d_t]
+        wants.append(refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), rows, n, kernel=1)[1])
+    print("orderings launch", flush=True)
+    fused_evaluate(lib, jobs, abi.KERNEL_F64)
+    for order, job, want in zip(orders, jobs, wants):
+        assert bits_equal(job[5].download(n), want), order
+    # the orderings differ (a leading-colour sum is not 12 copies of one)
+    assert not np.allclose(wants[0], wants[1])
+    for d in devs:
+        d.free()
+    fp.destroy()
