@@ -132,8 +132,13 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
     typedef Cx<R> C;
     const size_t ld = a.ld;
     C acc[6];
+    if (count == 0) return;
+    // the next instruction is fetched while this one runs (its scalar load overlaps the row's
+    // vector loads instead of preceding them)
+    Instr next = cnst(prog)[0];
     for (uint32_t n = 0; n < count; n++) {
-        const Instr o = cnst(prog)[n];
+        const Instr o = next;
+        next = cnst(prog)[n + 1 < count ? n + 1 : n];
         const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff, flags = o.w[0] >> 16;
         if (kind == K_CONTRIB) {
             switch (op) {

@@ -160,6 +160,134 @@ __global__ void __launch_bounds__(256) mlp_forward(FwdArgs a) {
     if (part == 0) a.y[row0 + r] = s + a.params[a.bout_off];
 }
 
+// ---- inference at hidden = 256 -----------------------------------------------------------------
+//
+// 128 points per workgroup, every layer in one pass. The products are computed transposed,
+// Z^T = W H^T (A = W: 16 output units per block, B = H: 16 points per block), so that a lane's
+// four results are four consecutive units of one point: bias, SiLU and the bf16 rounding happen
+// in registers and one 8-byte LDS store writes them. Wavefront w owns units 64w .. 64w+63 of all
+// 128 points: its weight slice (64 x 256 bf16) sits in registers for the whole layer and the
+// activations stream from LDS (two 128 x 256 bf16 buffers, 16-byte granules XOR-swizzled by
+// row). Layers >= 1 use v_mfma_f32_16x16x32_bf16 (lane l: A row / B column l % 16, k = 8 (l / 16)
+// .. +7, eight bf16 each); the first layer (K = in_pad = 16) uses the 16x16x16 form.
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr uint32_t ITILE = 128;  // points per inference workgroup
+constexpr uint32_t IH = 256;     // hidden units handled by the inference kernel
+constexpr uint32_t MAX_LAYERS = 16;
+
+struct InferArgs {
+    const float *x;
+    float *y;
+    uint32_t batch, inputs, layers;
+    const bf16 *wb;
+    const float *params;
+    uint64_t wb_off[MAX_LAYERS], b_off[MAX_LAYERS];
+    uint64_t wout_off, bout_off;
+};
+
+__device__ __forceinline__ uint32_t swz(uint32_t r, uint32_t c) {
+    return r * IH + ((((c >> 3) ^ (r & 15))) << 3) + (c & 7);
+}
+
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// bias, SiLU and bf16 of acc[j][i] (units 64w + 16j + 4(l/16) + e, point 16i + l%16) into `out`
+__device__ __forceinline__ void infer_epilogue(f32x4 (&acc)[4][8], const float *bias, bf16 *out, uint32_t wave,
+                                               uint32_t lane) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t c = 64 * wave + 16 * j + 4 * (lane >> 4);
+        const float4 b = *reinterpret_cast<const float4 *>(bias + c);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t r = 16 * i + (lane & 15);
+            const uint32_t h0 = to_bf16(silu(acc[j][i][0] + b.x)), h1 = to_bf16(silu(acc[j][i][1] + b.y));
+            const uint32_t h2 = to_bf16(silu(acc[j][i][2] + b.z)), h3 = to_bf16(silu(acc[j][i][3] + b.w));
+            uint2 v;
+            v.x = h0 | (h1 << 16);
+            v.y = h2 | (h3 << 16);
+            *reinterpret_cast<uint2 *>(&out[swz(r, c)]) = v;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256, 1) mlp_infer(InferArgs a) {
+    extern __shared__ bf16 lds[];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t row0 = blockIdx.x * ITILE;
+    bf16 *const base = lds;
+    auto act = [base](uint32_t k) { return base + k * (ITILE * IH); };
+    // the input tile in bf16, columns 0..15 (zero beyond the inputs and beyond the batch)
+    for (uint32_t idx = tid; idx < ITILE * 16; idx += blockDim.x) {
+        const uint32_t r = idx >> 4, c = idx & 15;
+        const float v = (c < a.inputs && row0 + r < a.batch) ? a.x[(size_t)(row0 + r) * a.inputs + c] : 0.0f;
+        act(0)[swz(r, c)] = to_bf16(v);
+    }
+    __syncthreads();
+    f32x4 acc[4][8];
+    {  // layer 0: K = 16
+        const bf16 *W = a.wb + a.wb_off[0];
+        bf16x4 w0[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            w0[j] = *reinterpret_cast<const bf16x4 *>(&W[(64 * wave + 16 * j + (lane & 15)) * 16 + 4 * (lane >> 4)]);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const bf16x4 h = *reinterpret_cast<const bf16x4 *>(&act(0)[swz(16 * i + (lane & 15), 4 * (lane >> 4))]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[j][i] = mfma(w0[j], h, acc[j][i]);
+        }
+    }
+    bf16x8 wf[4][8];
+    auto load_weights = [&](uint32_t l) {
+        const bf16 *W = a.wb + a.wb_off[l];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int s = 0; s < 8; s++)
+                wf[j][s] = *reinterpret_cast<const bf16x8 *>(&W[(size_t)(64 * wave + 16 * j + (lane & 15)) * IH + 32 * s +
+                                                               8 * (lane >> 4)]);
+    };
+    if (a.layers > 1) load_weights(1);
+    infer_epilogue(acc, a.params + a.b_off[0], act(1), wave, lane);
+    __syncthreads();
+    uint32_t cur = 1;
+    for (uint32_t l = 1; l < a.layers; l++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 8; s++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const bf16x8 h =
+                    *reinterpret_cast<const bf16x8 *>(&act(cur)[swz(16 * i + (lane & 15), 32 * s + 8 * (lane >> 4))]);
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[j][i] = mfma32(wf[j][s], h, acc[j][i]);
+            }
+        if (l + 1 < a.layers) load_weights(l + 1);
+        infer_epilogue(acc, a.params + a.b_off[l], act(cur ^ 1), wave, lane);
+        __syncthreads();
+        cur ^= 1;
+    }
+    // output unit: two threads per point, 128 units each, then one shuffle
+    const uint32_t r = tid >> 1, part = tid & 1;
+    const float *wout = a.params + a.wout_off;
+    float s = 0.0f;
+    for (uint32_t k = part * 128; k < part * 128 + 128; k++) s += from_bf16(act(cur)[swz(r, k)]) * wout[k];
+    s += __shfl_xor(s, 1);
+    if (part == 0 && row0 + r < a.batch) a.y[row0 + r] = s + a.params[a.bout_off];
+}
+
 // ---- backward --------------------------------------------------------------------------------
 
 // delta = 2 (y - t) / batch; loss += (y - t)^2 / batch; dZ_last = delta w_out silu'(Z_last)
@@ -398,6 +526,23 @@ extern "C" int pacuda_mlp_forward(pacuda_mlp *m, const float *x, uint32_t batch,
     a.z = n.z; a.ht = n.ht; a.train = train ? 1 : 0;
     const size_t lds = 2 * (size_t)TILE * n.hidden * sizeof(bf16);
     if (n.in_pad > n.hidden) return PACMAN_RECURRENCE_INVALID;
+    if (!train && n.hidden == IH && n.in_pad == 16 && n.layers <= MAX_LAYERS) {
+        InferArgs ia;
+        ia.x = x; ia.y = y; ia.batch = batch; ia.inputs = n.inputs; ia.layers = n.layers;
+        ia.wb = n.wb; ia.params = n.params;
+        for (uint32_t l = 0; l < n.layers; l++) {
+            ia.wb_off[l] = n.wb_off[l];
+            ia.b_off[l] = n.b_off[l];
+        }
+        ia.wout_off = n.wout_off; ia.bout_off = n.bout_off;
+        static const bool lds_ok = !hip_failed(
+            hipFuncSetAttribute(reinterpret_cast<const void *>(mlp_infer), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                2 * ITILE * IH * sizeof(bf16)),
+            "mlp inference");
+        if (!lds_ok) return PACMAN_RECURRENCE_DEVICE_ERROR;
+        mlp_infer<<<(batch + ITILE - 1) / ITILE, 256, 2 * ITILE * IH * sizeof(bf16), mlp_stream()>>>(ia);
+        return hip_failed(hipGetLastError(), "mlp inference") ? PACMAN_RECURRENCE_DEVICE_ERROR : PACMAN_RECURRENCE_OK;
+    }
     mlp_forward<<<batch / TILE, 256, lds, mlp_stream()>>>(a);
     if (hip_failed(hipGetLastError(), "mlp forward")) return PACMAN_RECURRENCE_DEVICE_ERROR;
     return PACMAN_RECURRENCE_OK;

@@ -78,3 +78,29 @@ def test_training_reduces_loss():
             net.adam(1e-3)
     assert np.mean(losses[-8:]) < 0.1 * np.mean(losses[:8]), (losses[:8], losses[-8:])
     net.destroy()
+
+
+@pytest.mark.parametrize("layers,batch", [(1, 64), (2, 192), (4, 4096 + 64)])
+def test_inference_kernel(layers, batch):
+    """the inference kernel (hidden 256, forward without train) against the oracle and against
+    the training forward, on whole and ragged 128-point tiles"""
+    inputs, hidden = 14, 256
+    print("mlp inference", layers, batch, flush=True)
+    rng = np.random.default_rng(layers * 7 + batch)
+    dev = Device()
+    net = surrogate.MLP(inputs, hidden, layers, max_batch=batch, lib=dev.lib)
+    p = mlp_ref.init(inputs, hidden, layers, seed=5)
+    p = (p + rng.normal(0, 0.02, p.size)).astype(np.float32)
+    net.set_params(p)
+    x = rng.uniform(0, 1, (batch, inputs)).astype(np.float32)
+    d_x, d_y, d_y2 = dev.array(x), dev.empty(batch, np.float32), dev.empty(batch, np.float32)
+    net.forward(d_x, batch, d_y, train=False)
+    net.forward(d_x, batch, d_y2, train=True)
+    y, y2 = d_y.download(batch, np.float32), d_y2.download(batch, np.float32)
+    want_y, _, _ = mlp_ref.forward(p, x, inputs, hidden, layers)
+    scale = max(1.0, np.max(np.abs(want_y)))
+    assert np.max(np.abs(y - want_y)) <= 2e-3 * scale
+    assert np.max(np.abs(y - y2)) <= 2e-3 * scale
+    for d in (d_x, d_y, d_y2):
+        d.free()
+    net.destroy()
