@@ -104,6 +104,19 @@ def mode_errors(got, ref):
             "max_err_rel_to_max_f64": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))}
 
 
+def unweighting_summary(u):
+    """the unweighting figures of the JSON line (None when the unweighting did not run)"""
+    if u is None:
+        return None
+    t = u["two_stage"]
+    return {"plain": u["plain"]["events_per_s"], "surrogate": t["events_per_s"], "gain": t["gain"],
+            "sigma_pb": {"plain": u["plain"]["estimate"], "surrogate": t["estimate"],
+                         "weighted": u["plain"]["weighted"]},
+            "sigma_pull_vs_plain": t["sigma_pull"], "sigma_pull_vs_weighted": t["sigma_pull_vs_weighted"],
+            "gain_network_on_every_candidate": u["two_stage_dense"]["gain"],
+            "gain_prior_only": u["two_stage_prior_only"]["gain"]}
+
+
 def copy_calibration(rows=20, count=1 << 22, repeats=3):
     """pacuda_gather with the identity index copies rows x count doubles: it reads
     rows*count*8 + count*4 bytes and writes rows*count*8 (gather_kernel, one dispatch each)"""
@@ -132,6 +145,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-modes", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    ap.add_argument("--only-unweighting", action="store_true", help="of the extra workloads, only the unweighting")
     ap.add_argument("--candidates", type=int, default=1 << 24)
     ap.add_argument("--train-points", type=int, default=1 << 20)
     ap.add_argument("--epochs", type=int, default=20)
@@ -178,7 +192,7 @@ def main():
             log(f"{name} {modes[name]['points_per_s']:.4g} points/s")
 
     extra = {}
-    if not args.no_extra:
+    if not args.no_extra and not args.only_unweighting:
         # the whole leading-colour sum of g g > g g g: the 12 orderings (0, s1..s4) of the legs up
         # to reflection, each the same plan on its permutation of the external legs, in one launch
         orders = [(0,) + p for p in itertools.permutations(range(1, plan.legs)) if p[0] < p[-1]]
@@ -260,6 +274,7 @@ def main():
             t.free()
             f.destroy()
 
+    if not args.no_extra:
         # surrogate vs plain unweighting of g g > g g g g
         t = time.perf_counter()
         uw = Unweighting(legs=6, hidden=256, layers=4, batch=args.train_batch, prior=True, lib=lib)
@@ -270,17 +285,17 @@ def main():
         train_s = time.perf_counter() - t
         log(f"surrogate: {args.train_points} labels in {label_s:.1f} s, trained in {train_s:.1f} s, "
             f"loss {hist[0]:.3f} -> {hist[-1]:.3f}")
-        res = uw.run(args.candidates, seed=4)
+        res = uw.run(args.candidates, seed=4, variants=("network", "network_dense", "prior"))
         res["training"] = {"points": args.train_points, "seed": 3, "epochs": args.epochs, "batch": args.train_batch,
                            "label_seconds": label_s, "train_seconds": train_s,
                            "loss_first_last": [hist[0], hist[-1]],
                            "rms_log_residual_before": uw.std,
                            "rms_log_error": float(np.sqrt(hist[-1]) * uw.std)}
         extra["unweighting"] = res
-        for k in ("two_stage", "two_stage_prior_only"):
+        for k in ("two_stage", "two_stage_dense", "two_stage_prior_only"):
             r = res[k]
             log(f"unweighting {k}: plain {res['plain']['events_per_s']:.4g} ev/s, {r['events_per_s']:.4g} ev/s, "
-                f"gain {r['gain']:.2f}, pull {r['sigma_pull']:.2f}")
+                f"gain {r['gain']:.2f}, pull {r['sigma_pull']:.2f} (vs weighted {r['sigma_pull_vs_weighted']:.2f})")
         uw.destroy()
 
     cpu = None
@@ -305,12 +320,7 @@ def main():
                      "algorithmic_bytes_per_point": 20 * 8 + 8},
         "cpu_baseline": cpu,
         "modes": modes,
-        "unweighted_events_per_s": ({"plain": extra["unweighting"]["plain"]["events_per_s"],
-                                     "surrogate": extra["unweighting"]["two_stage"]["events_per_s"],
-                                     "gain": extra["unweighting"]["two_stage"]["gain"],
-                                     "sigma_pull": extra["unweighting"]["two_stage"]["sigma_pull"],
-                                     "prior_only_gain": extra["unweighting"]["two_stage_prior_only"]["gain"]}
-                                    if "unweighting" in extra else None),
+        "unweighted_events_per_s": unweighting_summary(extra.get("unweighting")),
         "extra": extra,
     }
     print(json.dumps(line), flush=True)

@@ -73,12 +73,14 @@ class Generator:
     def gather(self, ext, feat, stride, index, rows=None):
         """rows of ext (columns index, a host array or a (DeviceArray, count) pair) and feature rows"""
         lib = self.dev.lib
+        rows = self.legs * 4 if rows is None else rows
         if isinstance(index, tuple):
             d_idx, n, own = index[0], index[1], False
+        elif len(index) == 0:
+            return self.dev.empty(rows), (self.dev.empty(self.width, np.float32) if feat is not None else None)
         else:
             d_idx = self.dev.array(np.ascontiguousarray(index, dtype=np.uint32))
             n, own = len(index), True
-        rows = self.legs * 4 if rows is None else rows
         out = self.dev.empty(rows * max(n, 1))
         fout = self.dev.empty(max(n, 1) * self.width, np.float32) if feat is not None else None
         st = lib.pacuda_gather(ext.ptr, stride, rows, d_idx.ptr, n, out.ptr,
@@ -326,6 +328,83 @@ class Unweighting:
                 "threshold": s_thr, "r_threshold": r_thr, "stage2_efficiency": float(acc.mean()) if len(r) else 0.0,
                 "estimate": self._estimate(x, candidates), "seconds": sum(clock.values()), "times_s": clock}
 
+    def two_stage_envelope(self, candidates, seed=4, chunk=1 << 22, q=0.99999, q2=0.999, clamp=4.0, accept_seed=22,
+                           stage2_seed=23, keep_ids=False):
+        """two-stage unweighting with the network evaluated only where it can matter.
+
+        The surrogate is s = P e^(mu + sigma clip(NN, -clamp, clamp)): any positive function is a
+        valid surrogate, and the clip bounds it by the envelope E = P e^(mu + sigma clamp). Stage 1
+        keeps a candidate when u1 s_thr < s; since s <= E, every candidate with u1 s_thr >= E is
+        rejected from the prior alone, and the network runs only on the others (the same
+        decisions as evaluating it everywhere). s_thr is the q-quantile of P e^mu (the surrogate
+        without its network), so it is known before any network evaluation."""
+        if not self.prior:
+            raise ValueError("the envelope needs the prior")
+        clock, hist = {}, self.dev.array(np.zeros(BINS, dtype=np.uint32))
+        staged = []
+        for start in range(0, candidates, chunk):
+            count = min(chunk, candidates - start)
+            d = self._timed(clock, "generate", self.gen.draw_device, seed, start, count, features=True, prior=True)
+            s0 = self._timed(clock, "prior", self.surrogate_weights, d, count, False)
+            self._timed(clock, "stats", self.sel.stats, s0, count, hist)
+            for k in ("pass", "logp"):
+                d[k].free()
+            staged.append((d["ext"], d["feat"], s0, count, start))
+        h = hist.download(BINS, np.uint32)
+        hist.free()
+        s_thr = quantile_threshold(h, q)
+        lift = math.exp(self.std * clamp)
+        ids, s1, w1 = [], [], []
+        n_env = n_net = 0
+        for ext, feat, s0, count, start in staged:
+            e_idx, e_val, m = self._timed(clock, "envelope", self.sel.select, count, w=s0, first=start,
+                                          threshold=s_thr / lift, seed=accept_seed, values=True)
+            n_env += m
+            t = time.perf_counter()
+            idx = e_idx.download(m, np.uint32)
+            base = e_val.download(m)
+            keep = np.zeros(m, dtype=bool)
+            s = np.zeros(m)
+            if m:
+                pad = (m + 63) // 64 * 64
+                pidx = np.concatenate([idx, np.repeat(idx[-1:], pad - m)])
+                e1, f2 = self.gen.gather(ext, feat, count, pidx, rows=1)
+                y = self.dev.empty(pad, np.float32)
+                self.mlp.forward(f2, pad, y)
+                nn = y.download(pad, np.float32)[:m].astype(np.float64)
+                for a in (e1, f2, y):
+                    a.free()
+                n_net += pad
+                s = base * np.exp(self.std * np.clip(nn, -clamp, clamp))
+                keep = ps.uniform_at(accept_seed, start + idx.astype(np.uint64)) * s_thr < s
+            self.dev.synchronize()
+            clock["network"] = clock.get("network", 0.0) + time.perf_counter() - t
+            k_idx = idx[keep]
+            n = len(k_idx)
+            e2, _ = self._timed(clock, "gather", self.gen.gather, ext, None, count, k_idx)
+            w = self._timed(clock, "exact", self.exact, e2, n)
+            t = time.perf_counter()
+            ids.append(k_idx.astype(np.int64) + start)
+            s1.append(s[keep])
+            w1.append(w.download(n))
+            clock["stage2"] = clock.get("stage2", 0.0) + time.perf_counter() - t
+            for a in (e_idx, e_val, e2, w, ext, feat, s0):
+                a.free()
+        t = time.perf_counter()
+        ids, s1, w1 = np.concatenate(ids), np.concatenate(s1), np.concatenate(w1)
+        r = w1 / s1
+        r_thr = float(np.quantile(r, q2)) if len(r) else 1.0
+        acc = ps.uniform_at(stage2_seed, ids) * r_thr < r
+        clock["stage2"] = clock.get("stage2", 0.0) + time.perf_counter() - t
+        x = s_thr * r_thr * np.maximum(1.0, s1[acc] / s_thr) * np.maximum(1.0, r[acc] / r_thr)
+        extra = {"stage1_ids": ids, "stage1_s": s1} if keep_ids else {}
+        return {**extra, "candidates": candidates, "envelope_accepted": n_env, "network_evaluations": n_net,
+                "stage1_accepted": len(ids), "exact_evaluations": len(ids),
+                "events": int(acc.sum()), "overweight_events": int(np.sum((s1[acc] > s_thr) | (r[acc] > r_thr))),
+                "threshold": s_thr, "r_threshold": r_thr, "clamp": clamp,
+                "stage2_efficiency": float(acc.mean()) if len(r) else 0.0,
+                "estimate": self._estimate(x, candidates), "seconds": sum(clock.values()), "times_s": clock}
+
     def _scale(self):
         return self.w_ps * self.flux * GEV2_TO_PB
 
@@ -340,24 +419,37 @@ class Unweighting:
         var = max(sums[1] / n - mean * mean, 0.0)
         return float(mean * self._scale()), float(math.sqrt(var / n) * self._scale())
 
+    VARIANTS = {"network": "two_stage", "network_dense": "two_stage_dense", "prior": "two_stage_prior_only"}
+
     def run(self, candidates, seed=4, chunk=1 << 22, q=0.99999, q2=0.999, variants=("network", "prior")):
         """plain and two-stage unweighting on the same candidates; `variants` of the surrogate:
-        "network" (the trained network, with the prior when the object has one) and "prior"
-        (the prior alone, no network)"""
+        "network" (the trained network with the prior, evaluated inside the envelope only),
+        "network_dense" (the same network evaluated on every candidate) and "prior" (the prior
+        alone, no network). Pulls: against the plain estimate (errors added in quadrature) and
+        against the weighted Monte Carlo estimate of the same candidates (the method's error)."""
         out = {"plain": self.plain(candidates, seed, chunk, q)}
         p = out["plain"]
         rate_p = p["events"] / p["seconds"]
         p["events_per_s"] = rate_p
         sig_p, err_p = p["estimate"]
+        sig_w = p["weighted"][0]
+        p["sigma_pull_vs_weighted"] = (sig_p - sig_w) / err_p
         for v in variants:
-            if v == "prior" and not self.prior:
-                continue
-            t = self.two_stage(candidates, seed, chunk, q, q2, network=(v == "network"))
+            if v in ("prior", "network") and not self.prior:
+                if v == "network":
+                    v = "network_dense"
+                else:
+                    continue
+            if v == "network":
+                t = self.two_stage_envelope(candidates, seed, chunk, q, q2)
+            else:
+                t = self.two_stage(candidates, seed, chunk, q, q2, network=(v == "network_dense"))
             t["events_per_s"] = t["events"] / t["seconds"]
             t["gain"] = t["events_per_s"] / rate_p if rate_p else None
             sig, err = t["estimate"]
             t["sigma_pull"] = (sig - sig_p) / math.hypot(err, err_p)
-            out["two_stage" if v == "network" else "two_stage_prior_only"] = t
+            t["sigma_pull_vs_weighted"] = (sig - sig_w) / err
+            out[self.VARIANTS[v]] = t
         return out
 
     def destroy(self):

@@ -7,9 +7,11 @@
 - pacuda_select keeps exactly the indices the host's hit-or-miss test keeps (the same counter
   stream), in ascending order, on ragged sizes, with ids and flags; pacuda_weight_stats gives the
   numpy max / sum / sum of squares and histogram;
-- a small two-stage unweighting of g g > g g g g: the cross sections of the plain and both
-  surrogate unweightings agree with the weighted Monte Carlo estimate within 3 sigma, and the
-  two-stage methods evaluate the exact recurrence on fewer points than the plain one."""
+- a small two-stage unweighting of g g > g g g g: the cross sections of the plain and the three
+  surrogate unweightings (network inside the prior's envelope, network everywhere, prior alone)
+  agree with the weighted Monte Carlo estimate within 3 sigma, the two-stage methods evaluate the
+  exact recurrence on fewer points than the plain one, and the envelope's stage 1 keeps exactly
+  the candidates that the clipped network evaluated on every candidate keeps."""
 import numpy as np
 import pytest
 
@@ -94,14 +96,37 @@ def test_two_stage_unweighting_small():
     x, logm2, logp = uw.training_set(1 << 15, seed=3)
     hist = uw.train(x, logm2, logp, epochs=6, lr=2e-3)
     assert hist[-1] < hist[0]
-    res = uw.run(1 << 19, seed=4, chunk=1 << 17, q=0.9999, q2=0.99)
+    res = uw.run(1 << 19, seed=4, chunk=1 << 17, q=0.9999, q2=0.99, variants=("network", "network_dense", "prior"))
     p = res["plain"]
     ref, ref_err = p["weighted"]
-    for k in ("plain", "two_stage", "two_stage_prior_only"):
+    for k in ("plain", "two_stage", "two_stage_dense", "two_stage_prior_only"):
         sig, err = res[k]["estimate"]
         print(k, res[k]["events"], sig, err, ref, ref_err, flush=True)
         assert abs(sig - ref) < 3 * np.hypot(err, ref_err), k
-    for k in ("two_stage", "two_stage_prior_only"):
+    for k in ("two_stage", "two_stage_dense", "two_stage_prior_only"):
         assert res[k]["exact_evaluations"] < p["exact_evaluations"]
         assert res[k]["events"] > 0
+    assert res["two_stage"]["network_evaluations"] < (1 << 19) // 10
+
+    # the envelope's stage 1 keeps exactly what the clipped network evaluated everywhere keeps
+    n, chunk, clamp, seed1 = 1 << 17, 1 << 16, 4.0, 22
+    t = uw.two_stage_envelope(n, seed=4, chunk=chunk, q=0.9999, q2=0.99, clamp=clamp, accept_seed=seed1,
+                              keep_ids=True)
+    want_ids = []
+    for start in range(0, n, chunk):
+        d = uw.gen.draw_device(4, start, chunk, features=True, prior=True)
+        d_s0 = uw.surrogate_weights(d, chunk, False)
+        s0 = d_s0.download(chunk)
+        d_s0.free()
+        y = uw.dev.empty(chunk, np.float32)
+        uw.mlp.forward(d["feat"], chunk, y)
+        nn = y.download(chunk, np.float32).astype(np.float64)
+        s = s0 * np.exp(uw.std * np.clip(nn, -clamp, clamp))
+        u = ps.uniform_at(seed1, np.arange(start, start + chunk, dtype=np.uint64))
+        want_ids.append(np.nonzero(u * t["threshold"] < s)[0] + start)
+        for a in list(d.values()) + [y]:
+            if a is not None:
+                a.free()
+    want_ids = np.concatenate(want_ids)
+    assert np.array_equal(t["stage1_ids"], want_ids)
     uw.destroy()
