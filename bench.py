@@ -104,6 +104,19 @@ def mode_errors(got, ref):
             "max_err_rel_to_max_f64": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))}
 
 
+def measured_traffic(kernel, points, rows, waves_per_cu):
+    """HBM bytes per dispatch from the committed rocprofv3 counters (profiles/traffic.json),
+    scaled by the copy-kernel calibration, when they were taken on this workload; else None"""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not t or (t["points"], t["plan_rows"], t["waves_per_cu"]) != (points, rows, waves_per_cu):
+        return None
+    return 1024.0 * (t["fetch_size_kib"] * t["fetch_scale"] + t["write_size_kib"] * t["write_scale"])
+
+
 def unweighting_summary(u):
     """the unweighting figures of the JSON line (None when the unweighting did not run)"""
     if u is None:
@@ -315,7 +328,9 @@ def main():
                    "flops_per_point": flops, "divides_per_point": divs, "parallelism": "1 GPU, persistent fused kernel"},
         "roofline": {"bound": "mfma", "pipe": "fp64 VALU (MI355X FP64 vector peak = matrix peak)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic("fused_kernel<KF64>", n, len(plan.rows), args.waves_per_cu or 12),
+                     "traffic_unit": "bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "kernel": "fused_kernel<KF64>", "kernel_ms": kernel_ms,
                      "algorithmic_bytes_per_point": 20 * 8 + 8},
         "cpu_baseline": cpu,

@@ -130,10 +130,12 @@ struct Arena {
 
 // Precision policies: R the values, Den the propagator denominators and their forms, Amp the
 // amplitude accumulation (runner/recurrence_runner.cu:178-190).
-struct KF64 { typedef double real; typedef double den; typedef double amp; };
-struct KDF64 { typedef D real; typedef D den; typedef D amp; };
-struct KMixed { typedef float real; typedef D den; typedef D amp; };
-struct KMixed64 { typedef float real; typedef double den; typedef D amp; };
+// forwarding: whether the kernel honours F_HOLD / F_FROMACC (the df64 kernel does not: with them
+// its accumulators no longer stay in registers, so it runs the program without forwarding)
+struct KF64 { typedef double real; typedef double den; typedef double amp; static constexpr bool forwarding = true; };
+struct KDF64 { typedef D real; typedef D den; typedef D amp; static constexpr bool forwarding = false; };
+struct KMixed { typedef float real; typedef D den; typedef D amp; static constexpr bool forwarding = true; };
+struct KMixed64 { typedef float real; typedef double den; typedef D amp; static constexpr bool forwarding = true; };
 
 PCD float to_float(float x) { return x; }
 PCD float to_float(double x) { return (float)x; }

@@ -32,8 +32,9 @@ using namespace pacuda;
 namespace {
 
 struct PlanDev {
-    const Instr *prog;
-    uint32_t count, sources;
+    const Instr *prog;        // with forwarding (F_HOLD / F_FROMACC)
+    const Instr *prog_plain;  // the same program without it (kernels without forwarding)
+    uint32_t count, count_plain, sources;
     const double *c64;  // scales re, im, factors re, im (f64)
     const double *cdf;  // the same in the df64 layout
     uint32_t scale_count, factor_count;
@@ -138,7 +139,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
         // the program's constants: df64 pairs for every kernel but f64 (as the v1 path converts them)
         const double *cons = DF ? pl.cdf : pl.c64;
         a.sre = cons; a.sim = cons + S; a.fre = cons + 2 * S; a.fim = cons + 2 * S + X;
-        interpret<K>(pl.prog + pl.sources, pl.count - pl.sources, a, lane);
+        if constexpr (K::forwarding) interpret<K>(pl.prog + pl.sources, pl.count - pl.sources, a, lane);
+        else interpret<K>(pl.prog_plain + pl.sources, pl.count_plain - pl.sources, a, lane);
         // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
         double t = 0.0;
         for (uint32_t o = 0; o < pl.outputs; o++) {
@@ -336,6 +338,80 @@ uint32_t compact_planes(std::vector<Instr> &ops, uint32_t planes) {
     return top;
 }
 
+// The current-plane ranges row o touches: operand word, length, and whether the row creates the
+// range (a source, a clear, the first row of an initialising run). -1 for a kind not known here.
+struct Access { int word; uint32_t len; bool write; };
+int row_accesses(const Instr &o, Access *a) {
+    const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff, flags = o.w[0] >> 16;
+    if (kind == K_SOURCE) {
+        const uint32_t family = o.w[5] & 7;
+        a[0] = {1, family == 0 ? 1u : family == 1 ? 2u : 4u, true};
+        return 1;
+    }
+    if (kind == K_CLEAR) { a[0] = {1, o.w[2], true}; return 1; }
+    if (kind == K_CONTRIB) {
+        int c0, c1, cd;
+        contribution_shape(op, &c0, &c1, &cd);
+        a[0] = {2, (uint32_t)c0, false};
+        a[1] = {3, (uint32_t)c1, false};
+        a[2] = {4, (uint32_t)cd, (flags & F_FIRST) && (flags & F_INIT)};
+        return 3;
+    }
+    if (kind == K_PROP) { a[0] = {1, op == 9 ? 4u : 2u, false}; return 1; }
+    if (kind == K_IDENTITY) { a[0] = {1, o.w[2], false}; return 1; }
+    if (kind == K_CLOSURE) { a[0] = {1, o.w[6], false}; a[1] = {2, o.w[6], false}; return 2; }
+    return -1;
+}
+
+// Forwarding: an in-place row (a propagator, or an identity of at most 6 components) whose
+// planes were last touched by the final row of the contribution run that produces them (or by
+// an in-place row on the same planes) moves right after that row; the producer keeps its result
+// in the accumulators (F_HOLD) and the in-place row takes it from there (F_FROMACC), so the
+// store and the load of those values disappear. Moving is safe: no row in between touches the
+// planes. Bit-identical: the same operations on the same values.
+uint32_t forward_in_place(std::vector<Instr> &ops) {
+    uint32_t moved = 0;
+    for (size_t x = 0; x < ops.size(); x++) {
+        const Instr X = ops[x];
+        const int kind = X.w[0] & 0xff, op = (X.w[0] >> 8) & 0xff;
+        if (kind != K_PROP && kind != K_IDENTITY) continue;
+        const uint32_t base = X.w[1], m = kind == K_PROP ? (op == 9 ? 4u : 2u) : X.w[2];
+        if (m == 0 || m > 6) continue;
+        size_t y = x;
+        bool found = false, unknown = false;
+        while (y > 0 && !found) {
+            y--;
+            Access a[3];
+            const int na = row_accesses(ops[y], a);
+            if (na < 0) { unknown = true; break; }
+            for (int k = 0; k < na; k++) {
+                const uint32_t b = ops[y].w[a[k].word], l = a[k].len;
+                if (b < base + m && base < b + l) found = true;
+            }
+        }
+        if (!found || unknown) continue;
+        Instr &Y = ops[y];
+        const int yk = Y.w[0] & 0xff, yop = (Y.w[0] >> 8) & 0xff, yflags = Y.w[0] >> 16;
+        bool ok = false;
+        if (yk == K_CONTRIB) {
+            int c0, c1, cd;
+            contribution_shape(yop, &c0, &c1, &cd);
+            ok = (yflags & F_LAST) && Y.w[4] == base && (uint32_t)cd == m;
+        } else if (yk == K_PROP || yk == K_IDENTITY) {
+            const uint32_t ym = yk == K_PROP ? (yop == 9 ? 4u : 2u) : Y.w[2];
+            ok = Y.w[1] == base && ym == m;
+        }
+        if (!ok) continue;
+        Y.w[0] |= (uint32_t)F_HOLD << 16;
+        Instr Z = X;
+        Z.w[0] |= (uint32_t)F_FROMACC << 16;
+        ops.erase(ops.begin() + x);
+        ops.insert(ops.begin() + y + 1, Z);
+        moved++;
+    }
+    return moved;
+}
+
 struct FusedState {
     std::mutex lock;
     double *scratch = nullptr;
@@ -394,6 +470,8 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     // the slot keeps only the live currents
     uint32_t planes = d->current_planes;
     if (const uint32_t compacted = compact_planes(hp.ops, d->current_planes)) planes = compacted;
+    const std::vector<Instr> plain = hp.ops;
+    forward_in_place(hp.ops);
     pacuda_plan *p = new pacuda_plan();
     const uint32_t S = d->scale_count, X = d->factor_count;
     std::vector<double> c64(2 * (size_t)(S + X)), cdf;
@@ -411,6 +489,8 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     memset(&h, 0, sizeof h);
     h.prog = upload(p, hp.ops.data(), hp.ops.size(), &ok);
     h.count = (uint32_t)hp.ops.size();
+    h.prog_plain = upload(p, plain.data(), plain.size(), &ok);
+    h.count_plain = (uint32_t)plain.size();
     h.sources = hp.sources;
     h.c64 = upload(p, c64.data(), c64.size(), &ok);
     h.cdf = upload(p, cdf.data(), cdf.size(), &ok);

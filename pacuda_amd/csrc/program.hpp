@@ -21,7 +21,10 @@
 namespace pacuda {
 
 enum { K_CLEAR = 1, K_CONTRIB, K_PROP, K_IDENTITY, K_CLOSURE, K_SOURCE, K_SYMJIT };
-enum { F_INIT = 1, F_FIRST = 2, F_LAST = 4 };
+// F_HOLD: the row's result stays in the accumulators for the next row instead of being stored;
+// F_FROMACC: an in-place row (propagator, identity) takes its input from the accumulators
+// instead of loading it (set only by the fused path's forwarding pass, never on v1 programs)
+enum { F_INIT = 1, F_FIRST = 2, F_LAST = 4, F_HOLD = 8, F_FROMACC = 16 };
 
 // w[0] = kind | op << 8 | flags << 16; contributions: w[1] scale slot, w[2] parent 0, w[3]
 // parent 1, w[4] destination, w[5] form 0, w[6] form 1, w[7] factor; clear: w[1] base, w[2]
@@ -93,14 +96,14 @@ PCD void contribution(const Arena &a, const Instr &o, int flags, Cx<typename K::
 #pragma unroll
         for (int k = 0; k < cd; k++) acc[k] = add(Cur<R>::ld(a, (dst + k) * ld + i), mul(v[k], scale));
     }
-    if (flags & F_LAST) {
+    if ((flags & F_LAST) && !(flags & F_HOLD)) {
 #pragma unroll
         for (int k = 0; k < cd; k++) Cur<R>::st(a, (dst + k) * ld + i, acc[k]);
     }
 }
 
 template <class K, int OP>
-PCD void propagator(const Arena &a, const Instr &o, size_t i) {
+PCD void propagator(const Arena &a, const Instr &o, int flags, Cx<typename K::real> *acc, size_t i) {
     typedef typename K::real R;
     typedef typename K::den Den;
     typedef Cx<R> C;
@@ -117,11 +120,21 @@ PCD void propagator(const Arena &a, const Instr &o, size_t i) {
     }
     R reciprocal = ToReal<R, Den>::of(reciprocal_minkowski_norm(q[0], q[1], q[2], q[3]));
     C c[m], v[m];
+    if (flags & F_FROMACC) {
 #pragma unroll
-    for (int k = 0; k < m; k++) c[k] = Cur<R>::ld(a, (base + k) * ld + i);
+        for (int k = 0; k < m; k++) c[k] = acc[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < m; k++) c[k] = Cur<R>::ld(a, (base + k) * ld + i);
+    }
     propagator_values<OP>(c, p, reciprocal, cx(ldc<R>(a.fre, o.w[3]), ldc<R>(a.fim, o.w[3])), v);
+    if (flags & F_HOLD) {
 #pragma unroll
-    for (int k = 0; k < m; k++) Cur<R>::st(a, (base + k) * ld + i, v[k]);
+        for (int k = 0; k < m; k++) acc[k] = v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < m; k++) Cur<R>::st(a, (base + k) * ld + i, v[k]);
+    }
 }
 
 // One lane runs `count` instructions for its point i (runner/recurrence_runner.cu:787-838).
@@ -139,7 +152,8 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
     for (uint32_t n = 0; n < count; n++) {
         const Instr o = next;
         next = cnst(prog)[n + 1 < count ? n + 1 : n];
-        const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff, flags = o.w[0] >> 16;
+        const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff;
+        const int flags = K::forwarding ? (int)(o.w[0] >> 16) : (int)(o.w[0] >> 16) & (F_INIT | F_FIRST | F_LAST);
         if (kind == K_CONTRIB) {
             switch (op) {
             case 6: contribution<K, 6>(a, o, flags, acc, i); break;
@@ -149,9 +163,9 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
             default: contribution<K, 3>(a, o, flags, acc, i); break;
             }
         } else if (kind == K_PROP) {
-            if (op == 9) propagator<K, 9>(a, o, i);
-            else if (op == 7) propagator<K, 7>(a, o, i);
-            else propagator<K, 8>(a, o, i);
+            if (op == 9) propagator<K, 9>(a, o, flags, acc, i);
+            else if (op == 7) propagator<K, 7>(a, o, flags, acc, i);
+            else propagator<K, 8>(a, o, flags, acc, i);
         } else if (kind == K_CLOSURE) {
             R rfr = ldc<R>(a.fre, o.w[4]), rfi = ldc<R>(a.fim, o.w[4]);
             size_t at = o.w[3] * ld + i;
@@ -166,9 +180,25 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
             str(a.aim, at, di);
         } else if (kind == K_IDENTITY) {
             R fr = ldc<R>(a.fre, o.w[3]), fi = ldc<R>(a.fim, o.w[3]);
-            for (uint32_t k = 0; k < o.w[2]; k++) {
-                size_t at = (o.w[1] + k) * ld + i;
-                Cur<R>::st(a, at, identity_value(fr, fi, Cur<R>::ld(a, at)));
+            if (flags & (F_FROMACC | F_HOLD)) {  // at most 6 components (the accumulators)
+                const uint32_t count = o.w[2];
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    if ((uint32_t)k < count) {
+                        const size_t at = (o.w[1] + k) * ld + i;
+                        C u;
+                        if (flags & F_FROMACC) u = acc[k];
+                        else u = Cur<R>::ld(a, at);
+                        const C v = identity_value(fr, fi, u);
+                        if (flags & F_HOLD) acc[k] = v;
+                        else Cur<R>::st(a, at, v);
+                    }
+                }
+            } else {
+                for (uint32_t k = 0; k < o.w[2]; k++) {
+                    size_t at = (o.w[1] + k) * ld + i;
+                    Cur<R>::st(a, at, identity_value(fr, fi, Cur<R>::ld(a, at)));
+                }
             }
         } else if (kind == K_CLEAR) {
             for (uint32_t k = 0; k < o.w[2]; k++) Cur<R>::st(a, (o.w[1] + k) * ld + i, cx(R(0.0), R(0.0)));

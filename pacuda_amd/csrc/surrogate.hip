@@ -176,6 +176,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr uint32_t ITILE = 128;  // points per inference workgroup
 constexpr uint32_t IH = 256;     // hidden units handled by the inference kernel
 constexpr uint32_t MAX_LAYERS = 16;
+constexpr uint32_t MAX_SPLIT = 16;  // split-K ways of the weight-gradient products
 
 struct InferArgs {
     const float *x;
@@ -333,15 +334,20 @@ __global__ void __launch_bounds__(256) row_dot(const T *a, const float *w, uint3
 // C[m][n] = sum_k A[m][k] B[n][k], one wavefront per 64x64 tile of C.
 // mode 0: C fp32 to out (ld = N).
 // mode 1: the SiLU derivative: v = C * silu'(z[m][n]) -> dz (bf16 [m][n]) and dzt ([n][m]).
+// blockIdx.z takes k in [z * k_split, (z + 1) * k_split) (a multiple of 16); mode 0 writes its
+// partial product to out + z M N (summed in order by sum_splits)
 __global__ void __launch_bounds__(64) gemm_nt(const bf16 *A, const bf16 *B, uint32_t M, uint32_t N, uint32_t K,
-                                              uint32_t mode, float *out, const float *z, bf16 *dz, bf16 *dzt) {
+                                              uint32_t k_split, uint32_t mode, float *out, const float *z, bf16 *dz,
+                                              bf16 *dzt) {
     const uint32_t lane = threadIdx.x;
     const uint32_t tm = blockIdx.x * TILE, tn = blockIdx.y * TILE;
     const uint32_t nb = std::min<uint32_t>(4, (N - tn) / 16);
+    const uint32_t k_begin = blockIdx.z * k_split, k_end = std::min(K, k_begin + k_split);
+    if (mode == 0) out += (size_t)blockIdx.z * M * N;
     f32x4 acc[4][4];
     for (int i = 0; i < 4; i++)
         for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (uint32_t k0 = 0; k0 < K; k0 += 16) {
+    for (uint32_t k0 = k_begin; k0 < k_end; k0 += 16) {
         const uint32_t kk = k0 + 4 * (lane >> 4);
         bf16x4 av[4], bv[4];
         for (int i = 0; i < 4; i++) av[i] = *reinterpret_cast<const bf16x4 *>(&A[(size_t)(tm + 16 * i + (lane & 15)) * K + kk]);
@@ -362,6 +368,15 @@ __global__ void __launch_bounds__(64) gemm_nt(const bf16 *A, const bf16 *B, uint
                     dzt[(size_t)c * M + r] = h;
                 }
             }
+}
+
+// out[i] = sum over z = 0 .. splits-1 of partial[z][i], in that order
+__global__ void sum_splits(const float *partial, uint32_t splits, uint32_t count, float *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float v = partial[i];
+    for (uint32_t z = 1; z < splits; z++) v += partial[(size_t)z * count + i];
+    out[i] = v;
 }
 
 // the first layer's weight gradient drops the padded input columns: dst[n][c] = src[n][c], c < inputs
@@ -400,6 +415,7 @@ struct pacuda_mlp {
     Net n;
     uint64_t *d_wb_off = nullptr, *d_b_off = nullptr;
     float *scratch_w = nullptr;  // the first layer's padded weight gradient
+    float *partial = nullptr;    // split-K partial weight gradients, [MAX_SPLIT][hidden][fan_in_pad]
 };
 
 namespace {
@@ -465,6 +481,8 @@ extern "C" int pacuda_mlp_create(uint32_t inputs, uint32_t hidden, uint32_t laye
               !hip_failed(hipMalloc((void **)&m->d_wb_off, layers * 8), "mlp") &&
               !hip_failed(hipMalloc((void **)&m->d_b_off, layers * 8), "mlp") &&
               !hip_failed(hipMalloc((void **)&m->scratch_w, (size_t)hidden * n.in_pad * 4), "mlp") &&
+              !hip_failed(hipMalloc((void **)&m->partial, (size_t)MAX_SPLIT * hidden * std::max(hidden, n.in_pad) * 4),
+                          "mlp") &&
               !hip_failed(hipMemcpy(m->d_wb_off, n.wb_off.data(), layers * 8, hipMemcpyHostToDevice), "mlp") &&
               !hip_failed(hipMemcpy(m->d_b_off, n.b_off.data(), layers * 8, hipMemcpyHostToDevice), "mlp") &&
               !hip_failed(hipMemset(n.params, 0, off * 4), "mlp") && !hip_failed(hipMemset(n.m, 0, off * 4), "mlp") &&
@@ -483,7 +501,7 @@ extern "C" void pacuda_mlp_destroy(pacuda_mlp *m) {
     Net &n = m->n;
     for (void *p : {(void *)n.params, (void *)n.grads, (void *)n.m, (void *)n.v, (void *)n.wb, (void *)n.wtb, (void *)n.z,
                     (void *)n.ht, (void *)n.dz[0], (void *)n.dz[1], (void *)n.dzt, (void *)n.delta, (void *)n.loss, (void *)m->d_wb_off,
-                    (void *)m->d_b_off, (void *)m->scratch_w})
+                    (void *)m->d_b_off, (void *)m->scratch_w, (void *)m->partial})
         if (p) (void)hipFree(p);
     delete m;
 }
@@ -566,8 +584,13 @@ extern "C" int pacuda_mlp_backward(pacuda_mlp *m, const float *y, const float *t
         const uint32_t K_in = l == 0 ? n.in_pad : H;
         // dW_l[n][k] = sum_b dZ_l^T[n][b] H_(l-1)^T[k][b]
         float *dw = l == 0 ? m->scratch_w : n.grads + n.w_off[l];
-        dim3 g1(H / TILE, (K_in + TILE - 1) / TILE);
-        gemm_nt<<<g1, 64, 0, s>>>(n.dzt, htrow(l - 1), H, K_in, batch, 0, dw, nullptr, nullptr, nullptr);
+        // split K (= batch) so that the few output tiles still fill the GPU; partials summed in order
+        const uint32_t splits = std::max(1u, std::min(MAX_SPLIT, batch / 256));
+        const uint32_t k_split = (batch / splits + 15) / 16 * 16;
+        const uint32_t used = (batch + k_split - 1) / k_split;
+        dim3 g1(H / TILE, (K_in + TILE - 1) / TILE, used);
+        gemm_nt<<<g1, 64, 0, s>>>(n.dzt, htrow(l - 1), H, K_in, batch, k_split, 0, m->partial, nullptr, nullptr, nullptr);
+        sum_splits<<<(H * K_in + 255) / 256, 256, 0, s>>>(m->partial, used, H * K_in, dw);
         if (l == 0) {
             const uint32_t total = H * n.inputs;
             unpad_columns<<<(total + 255) / 256, 256, 0, s>>>(m->scratch_w, H, n.in_pad, n.inputs, n.grads + n.w_off[0]);
@@ -577,8 +600,8 @@ extern "C" int pacuda_mlp_backward(pacuda_mlp *m, const float *y, const float *t
         if (l > 0) {
             // dZ_(l-1) = (dZ_l W_l) * silu'(Z_(l-1)): A = dZ_l [b][n], B = W_l^T [k][n]
             dim3 g2(batch / TILE, H / TILE);
-            gemm_nt<<<g2, 64, 0, s>>>(n.dz[cur], n.wtb + n.wtb_off[l], batch, H, H, 1, nullptr, n.z + (size_t)(l - 1) * B * H,
-                                      n.dz[cur ^ 1], n.dzt);
+            gemm_nt<<<g2, 64, 0, s>>>(n.dz[cur], n.wtb + n.wtb_off[l], batch, H, H, H, 1, nullptr,
+                                      n.z + (size_t)(l - 1) * B * H, n.dz[cur ^ 1], n.dzt);
             cur ^= 1;
         }
     }
