@@ -247,7 +247,7 @@ class Unweighting:
         clock[key] = clock.get(key, 0.0) + time.perf_counter() - t
         return out
 
-    def plain(self, candidates, seed=4, chunk=1 << 22, q=0.99999, accept_seed=21):
+    def plain(self, candidates, seed=4, chunk=1 << 24, q=0.99999, accept_seed=21):
         """plain unweighting of `candidates` points: dict of counts, times and cross sections"""
         clock, hist = {}, self.dev.array(np.zeros(BINS, dtype=np.uint32))
         kept, sums = [], [0.0, 0.0, 0.0]  # per chunk (w, ids, n, first); sum w, sum w^2, max w
@@ -285,7 +285,7 @@ class Unweighting:
                 "estimate": self._estimate(x, candidates), "weighted": self._weighted(sums, candidates),
                 "seconds": sum(clock.values()), "times_s": clock}
 
-    def two_stage(self, candidates, seed=4, chunk=1 << 22, q=0.99999, q2=0.999, network=True, accept_seed=22,
+    def two_stage(self, candidates, seed=4, chunk=1 << 24, q=0.99999, q2=0.999, network=True, accept_seed=22,
                   stage2_seed=23):
         """two-stage unweighting of `candidates` points (the same points as plain() for one seed)"""
         clock, hist = {}, self.dev.array(np.zeros(BINS, dtype=np.uint32))
@@ -328,7 +328,7 @@ class Unweighting:
                 "threshold": s_thr, "r_threshold": r_thr, "stage2_efficiency": float(acc.mean()) if len(r) else 0.0,
                 "estimate": self._estimate(x, candidates), "seconds": sum(clock.values()), "times_s": clock}
 
-    def two_stage_envelope(self, candidates, seed=4, chunk=1 << 22, q=0.99999, q2=0.999, clamp=4.0, accept_seed=22,
+    def two_stage_envelope(self, candidates, seed=4, chunk=1 << 24, q=0.99999, q2=0.999, clamp=4.0, accept_seed=22,
                            stage2_seed=23, keep_ids=False):
         """two-stage unweighting with the network evaluated only where it can matter.
 
@@ -421,7 +421,7 @@ class Unweighting:
 
     VARIANTS = {"network": "two_stage", "network_dense": "two_stage_dense", "prior": "two_stage_prior_only"}
 
-    def run(self, candidates, seed=4, chunk=1 << 22, q=0.99999, q2=0.999, variants=("network", "prior")):
+    def run(self, candidates, seed=4, chunk=1 << 24, q=0.99999, q2=0.999, variants=("network", "prior")):
         """plain and two-stage unweighting on the same candidates; `variants` of the surrogate:
         "network" (the trained network with the prior, evaluated inside the envelope only),
         "network_dense" (the same network evaluated on every candidate) and "prior" (the prior
