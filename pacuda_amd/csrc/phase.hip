@@ -40,16 +40,22 @@ struct Crossing {
     double sign[MAX_LEGS];
 };
 
-__global__ void rambo_kernel(uint64_t state, uint64_t first, uint32_t points, uint32_t legs, double ecm, Crossing cr,
-                             double *external, uint32_t stride, float *features, uint8_t *pass, double pt_min,
-                             double eta_max, double dr_min, double *log_prior) {
+// LEGS is a compile-time count so that every per-parton array stays in registers (the leg
+// permutation of the crossing is applied by unrolled selects, not by indexing)
+template <int LEGS>
+__global__ void __launch_bounds__(256) rambo_kernel(uint64_t state, uint64_t first, uint32_t points, double ecm,
+                                                    Crossing cr, double *external, uint32_t stride, float *features,
+                                                    uint8_t *pass, double pt_min, double eta_max, double dr_min,
+                                                    double *log_prior) {
+    constexpr int M = LEGS - 2;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= points) return;
-    const uint32_t m = legs - 2;
-    const uint64_t base = (first + i) * (uint64_t)m * 4;
-    double q[MAX_LEGS][4], u[MAX_LEGS][4];
+    const uint64_t base = (first + i) * (uint64_t)M * 4;
+    double q[M][4], u[M][4];
     double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
-    for (uint32_t j = 0; j < m; j++) {
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+#pragma unroll
         for (int r = 0; r < 4; r++) u[j][r] = uniform(state, base + 4 * j + r);
         const double c = 2.0 * u[j][0] - 1.0, s = sqrt(1.0 - c * c), f = 2.0 * M_PI * u[j][1];
         const double e = -log(u[j][2] * u[j][3]);
@@ -59,18 +65,21 @@ __global__ void rambo_kernel(uint64_t state, uint64_t first, uint32_t points, ui
     const double mass = sqrt(t0 * t0 - t1 * t1 - t2 * t2 - t3 * t3);
     const double b[3] = {-t1 / mass, -t2 / mass, -t3 / mass};
     const double x = ecm / mass, g = t0 / mass, a = 1.0 / (1.0 + g);
-    double p[MAX_LEGS][4];
+    double p[LEGS][4];
     p[0][0] = ecm / 2; p[0][1] = 0; p[0][2] = 0; p[0][3] = ecm / 2;
     p[1][0] = ecm / 2; p[1][1] = 0; p[1][2] = 0; p[1][3] = -ecm / 2;
-    for (uint32_t j = 0; j < m; j++) {
+#pragma unroll
+    for (int j = 0; j < M; j++) {
         const double bq = b[0] * q[j][1] + b[1] * q[j][2] + b[2] * q[j][3];
         p[j + 2][0] = x * (g * q[j][0] + bq);
+#pragma unroll
         for (int k = 0; k < 3; k++) p[j + 2][k + 1] = x * (q[j][k + 1] + b[k] * (q[j][0] + a * bq));
     }
     // cuts on the outgoing partons
-    double eta[MAX_LEGS], phi[MAX_LEGS], pt_frac_min = 1.0;
+    double eta[LEGS], phi[LEGS], pt_frac_min = 1.0;
     bool ok = true;
-    for (uint32_t j = 2; j < legs; j++) {
+#pragma unroll
+    for (int j = 2; j < LEGS; j++) {
         const double px = p[j][1], py = p[j][2], pz = p[j][3];
         const double pt = sqrt(px * px + py * py), pp = sqrt(px * px + py * py + pz * pz);
         eta[j] = 0.5 * log((pp + pz) / (pp - pz));
@@ -79,8 +88,10 @@ __global__ void rambo_kernel(uint64_t state, uint64_t first, uint32_t points, ui
         pt_frac_min = fmin(pt_frac_min, pt / (ecm / 2));
     }
     double dr2_min = 1e30;
-    for (uint32_t j = 2; j < legs; j++)
-        for (uint32_t k = j + 1; k < legs; k++) {
+#pragma unroll
+    for (int j = 2; j < LEGS; j++)
+#pragma unroll
+        for (int k = j + 1; k < LEGS; k++) {
             double dphi = fabs(phi[j] - phi[k]);
             if (dphi > M_PI) dphi = 2 * M_PI - dphi;
             const double deta = eta[j] - eta[k], dr2 = deta * deta + dphi * dphi;
@@ -88,36 +99,51 @@ __global__ void rambo_kernel(uint64_t state, uint64_t first, uint32_t points, ui
             dr2_min = fmin(dr2_min, dr2);
         }
     if (pass) pass[i] = ok ? 1 : 0;
-    for (uint32_t l = 0; l < legs; l++)
-        for (int k = 0; k < 4; k++) external[((size_t)l * 4 + k) * stride + i] = cr.sign[l] * p[cr.src[l]][k];
+    // the plan's legs: xl[l] = p[src[l]] (unrolled selects on the uniform crossing table)
+    double xl[LEGS][4];
+#pragma unroll
+    for (int l = 0; l < LEGS; l++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) xl[l][k] = p[0][k];
+#pragma unroll
+        for (int s = 1; s < LEGS; s++)
+            if (cr.src[l] == (uint32_t)s)
+#pragma unroll
+                for (int k = 0; k < 4; k++) xl[l][k] = p[s][k];
+    }
+#pragma unroll
+    for (int l = 0; l < LEGS; l++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) external[((size_t)l * 4 + k) * stride + i] = cr.sign[l] * xl[l][k];
     if (log_prior) {
         // log(sum_{a<b} s_ab^4 / prod_l s_{l,l+1}) over the plan's legs in order (cyclic),
         // s_ab = |2 p_a.p_b|: the helicity-summed MHV Parke-Taylor structure of the ordering
         double num = 0.0, log_den = 0.0;
-        for (uint32_t a = 0; a < legs; a++) {
-            const double *pa = p[cr.src[a]];
-            for (uint32_t b = a + 1; b < legs; b++) {
-                const double *pb = p[cr.src[b]];
-                const double sab = fabs(2.0 * (pa[0] * pb[0] - pa[1] * pb[1] - pa[2] * pb[2] - pa[3] * pb[3]));
+#pragma unroll
+        for (int a = 0; a < LEGS; a++)
+#pragma unroll
+            for (int c = a + 1; c < LEGS; c++) {
+                const double sab = fabs(2.0 * (xl[a][0] * xl[c][0] - xl[a][1] * xl[c][1] - xl[a][2] * xl[c][2] -
+                                               xl[a][3] * xl[c][3]));
                 const double s2 = sab * sab;
                 num += s2 * s2;
-                if (b == a + 1 || (a == 0 && b == legs - 1)) log_den += log(sab);
+                if (c == a + 1 || (a == 0 && c == LEGS - 1)) log_den += log(sab);
             }
-        }
         log_prior[i] = log(num) - log_den;
     }
     if (features) {
         // per outgoing parton: the uniforms of its direction and the CDF of its energy product
         // (t - t ln t for t = u u'), then the smallest pT over sqrt(s)/2 and the smallest Delta R / 4
-        float *f = features + (size_t)i * (3 * m + 2);
-        for (uint32_t j = 0; j < m; j++) {
+        float *f = features + (size_t)i * (3 * M + 2);
+#pragma unroll
+        for (int j = 0; j < M; j++) {
             const double t = u[j][2] * u[j][3];
             f[3 * j] = (float)u[j][0];
             f[3 * j + 1] = (float)u[j][1];
             f[3 * j + 2] = (float)(t - t * log(t));
         }
-        f[3 * m] = (float)pt_frac_min;
-        f[3 * m + 1] = (float)fmin(1.0, sqrt(dr2_min) / 4.0);
+        f[3 * M] = (float)pt_frac_min;
+        f[3 * M + 1] = (float)fmin(1.0, sqrt(dr2_min) / 4.0);
     }
 }
 
@@ -159,9 +185,19 @@ extern "C" int pacuda_rambo(uint64_t seed, uint64_t first, uint32_t points, uint
         cr.sign[l] = crossing_sign[l];
     }
     if (points == 0) return PACMAN_RECURRENCE_OK;
-    rambo_kernel<<<(points + 255) / 256, 256, 0, runner_stream()>>>(seed_state(seed), first, points, legs, ecm, cr, external,
-                                                                    stride, features, pass, pt_min, eta_max, dr_min,
-                                                                    log_prior);
+    const dim3 grid((points + 255) / 256);
+    const uint64_t st = seed_state(seed);
+    hipStream_t s = runner_stream();
+    switch (legs) {
+#define PACUDA_RAMBO(L)                                                                                                   \
+    case L:                                                                                                               \
+        rambo_kernel<L><<<grid, 256, 0, s>>>(st, first, points, ecm, cr, external, stride, features, pass, pt_min, eta_max, \
+                                             dr_min, log_prior);                                                          \
+        break;
+        PACUDA_RAMBO(3) PACUDA_RAMBO(4) PACUDA_RAMBO(5) PACUDA_RAMBO(6) PACUDA_RAMBO(7) PACUDA_RAMBO(8)
+#undef PACUDA_RAMBO
+    default: return PACMAN_RECURRENCE_INVALID;
+    }
     return hip_failed(hipGetLastError(), "rambo") ? PACMAN_RECURRENCE_DEVICE_ERROR : PACMAN_RECURRENCE_OK;
 }
 
