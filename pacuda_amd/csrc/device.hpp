@@ -126,7 +126,13 @@ struct Arena {
     const double *mom;
     const double *sre, *sim, *fre, *fim;
     size_t ld;
+    uint32_t lds_base;  // the wavefront's LDS region, in elements of the current type (fused path)
 };
+
+// Current planes at or above LDS_PLANE live in the wavefront's LDS region (slot plane - LDS_PLANE:
+// its real part at lds_base + 2 s ld, its imaginary part ld further); only the fused path's LDS
+// pass writes such indices, so v1 programs never take that branch.
+constexpr uint32_t LDS_PLANE = 1u << 24;
 
 // Precision policies: R the values, Den the propagator denominators and their forms, Amp the
 // amplitude accumulation (runner/recurrence_runner.cu:178-190).
@@ -163,10 +169,47 @@ template <> PCD D narrow<D>(double x) {
 template <class R> struct Cur {
     static PCD Cx<R> ld(const Arena &a, size_t at) { return cx(ldr<R>(a.cre, at), ldr<R>(a.cim, at)); }
     static PCD void st(const Arena &a, size_t at, Cx<R> v) { str(a.cre, at, v.re); str(a.cim, at, v.im); }
+    // by plane index (slot or LDS) for lane i
+    static PCD Cx<R> ldp(const Arena &a, uint32_t plane, size_t i) {
+        if (plane >= LDS_PLANE) {
+            extern __shared__ double lds_currents[];
+            const size_t at = a.lds_base + (size_t)(plane - LDS_PLANE) * 2 * a.ld + i;
+            return cx(ldr<R>(lds_currents, at), ldr<R>(lds_currents, at + a.ld));
+        }
+        return ld(a, (size_t)plane * a.ld + i);
+    }
+    static PCD void stp(const Arena &a, uint32_t plane, size_t i, Cx<R> v) {
+        if (plane >= LDS_PLANE) {
+            extern __shared__ double lds_currents[];
+            const size_t at = a.lds_base + (size_t)(plane - LDS_PLANE) * 2 * a.ld + i;
+            str(lds_currents, at, v.re);
+            str(lds_currents, at + a.ld, v.im);
+            return;
+        }
+        st(a, (size_t)plane * a.ld + i, v);
+    }
 };
 template <> struct Cur<float> {
     static PCD Cx<float> ld(const Arena &a, size_t at) { return cx(a.fcre[at], a.fcim[at]); }
     static PCD void st(const Arena &a, size_t at, Cx<float> v) { a.fcre[at] = v.re; a.fcim[at] = v.im; }
+    static PCD Cx<float> ldp(const Arena &a, uint32_t plane, size_t i) {
+        if (plane >= LDS_PLANE) {
+            extern __shared__ float lds_currents_f32[];
+            const size_t at = a.lds_base + (size_t)(plane - LDS_PLANE) * 2 * a.ld + i;
+            return cx(lds_currents_f32[at], lds_currents_f32[at + a.ld]);
+        }
+        return ld(a, (size_t)plane * a.ld + i);
+    }
+    static PCD void stp(const Arena &a, uint32_t plane, size_t i, Cx<float> v) {
+        if (plane >= LDS_PLANE) {
+            extern __shared__ float lds_currents_f32[];
+            const size_t at = a.lds_base + (size_t)(plane - LDS_PLANE) * 2 * a.ld + i;
+            lds_currents_f32[at] = v.re;
+            lds_currents_f32[at + a.ld] = v.im;
+            return;
+        }
+        st(a, (size_t)plane * a.ld + i, v);
+    }
 };
 
 template <class R> PCD R reciprocal_minkowski_norm(R p0, R p1, R p2, R p3) {

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <array>
 #include <vector>
 
 #include "pacman_recurrence.h"
@@ -32,9 +33,11 @@ using namespace pacuda;
 namespace {
 
 struct PlanDev {
-    const Instr *prog;        // with forwarding (F_HOLD / F_FROMACC)
-    const Instr *prog_plain;  // the same program without it (kernels without forwarding)
+    const Instr *prog;        // with forwarding (F_HOLD / F_FROMACC) and LDS-resident currents
+    const Instr *prog_fwd;    // with forwarding only (launches that give the wavefronts no LDS)
+    const Instr *prog_plain;  // neither (kernels without forwarding)
     uint32_t count, count_plain, sources;
+    uint32_t lds_planes;      // LDS-resident current planes of `prog` per wavefront
     const double *c64;  // scales re, im, factors re, im (f64)
     const double *cdf;  // the same in the df64 layout
     uint32_t scale_count, factor_count;
@@ -55,6 +58,8 @@ struct ItemDev {
 };
 
 constexpr int LANES = 64;
+constexpr uint32_t LDS_CAPACITY = 20;        // LDS-resident current planes per wavefront (a plan's maximum)
+constexpr uint32_t LDS_BYTES = 160 * 1024;   // LDS per CU (gfx950), the most one workgroup may take
 
 template <bool DF> PCD double amplitude_value(const double *p, size_t at) {
     if constexpr (DF) {
@@ -72,7 +77,7 @@ __global__ void reset_counter(uint32_t *counter) {
 template <class K>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start,
                                                     uint32_t item_count, uint32_t *counter, double *scratch,
-                                                    size_t slot_doubles) {
+                                                    size_t slot_doubles, uint32_t lds_per_wave) {
     typedef typename K::real R;
     constexpr bool DF = !__is_same(R, double) || !__is_same(typename K::den, double);
     constexpr bool DF_FORMS = __is_same(typename K::den, D);
@@ -108,6 +113,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
         double *mom = a.aim + (size_t)A * LANES;
         a.mom = mom;
         a.ld = LANES;
+        a.lds_base = (threadIdx.x / LANES) * lds_per_wave;
         // the momentum forms in f64 (runner/recurrence_runner.cu:745-764)
         for (uint32_t f = 0; f < F; f++) {
             const uint32_t first = cnst(pl.form_terms)[2 * f], n = cnst(pl.form_terms)[2 * f + 1];
@@ -129,7 +135,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
         // sources in f64 from the f64 constants, then the program in the kernel's layout
         const uint32_t S = pl.scale_count, X = pl.factor_count;
         a.sre = pl.c64; a.sim = pl.c64 + S; a.fre = pl.c64 + 2 * S; a.fim = pl.c64 + 2 * S + X;
-        run_sources_point<R>(pl.prog, pl.sources, a, lane);
+        run_sources_point<R>(lds_per_wave ? pl.prog : pl.prog_fwd, pl.sources, a, lane);
         if constexpr (DF_FORMS) {
             for (uint32_t q = 0; q < 4 * F; q++) {
                 size_t at = q * LANES + lane;
@@ -139,8 +145,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
         // the program's constants: df64 pairs for every kernel but f64 (as the v1 path converts them)
         const double *cons = DF ? pl.cdf : pl.c64;
         a.sre = cons; a.sim = cons + S; a.fre = cons + 2 * S; a.fim = cons + 2 * S + X;
-        if constexpr (K::forwarding) interpret<K>(pl.prog + pl.sources, pl.count - pl.sources, a, lane);
-        else interpret<K>(pl.prog_plain + pl.sources, pl.count_plain - pl.sources, a, lane);
+        if constexpr (K::forwarding)
+            interpret<K>((lds_per_wave ? pl.prog : pl.prog_fwd) + pl.sources, pl.count - pl.sources, a, lane);
+        else
+            interpret<K>(pl.prog_plain + pl.sources, pl.count_plain - pl.sources, a, lane);
         // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
         double t = 0.0;
         for (uint32_t o = 0; o < pl.outputs; o++) {
@@ -412,6 +420,100 @@ uint32_t forward_in_place(std::vector<Instr> &ops) {
     return moved;
 }
 
+// LDS residency: short-lived, often-touched currents move from the slot to the wavefront's LDS
+// region. Objects are rebuilt from the final program (a creating write starts one at its base;
+// every other access must name a whole live object), ranked by accesses per plane per row of
+// lifetime, and placed first-fit in [0, capacity) planes over their lifetimes; a replay checks the
+// placement (a new object only takes slots no live object holds, every access finds its own) and
+// the program is rewritten to planes LDS_PLANE + slot. Returns the LDS planes used (0: unchanged).
+uint32_t assign_lds(std::vector<Instr> &ops, uint32_t capacity) {
+    struct Obj { uint32_t base, len, first, last, touches, slot; bool lds; };
+    std::vector<Obj> objs;
+    std::vector<int> live;  // plane -> object living there
+    std::vector<std::array<int, 3>> obj_of(ops.size());
+    for (size_t n = 0; n < ops.size(); n++) {
+        Access acc[3];
+        const int na = row_accesses(ops[n], acc);
+        if (na < 0) return 0;
+        obj_of[n] = {-1, -1, -1};
+        for (int k = 0; k < na; k++) {
+            const uint32_t base = ops[n].w[acc[k].word], len = acc[k].len;
+            if (len == 0) continue;
+            if (base >= LDS_PLANE || base + len > LDS_PLANE) return 0;
+            if (live.size() < base + len) live.resize(base + len, -1);
+            int o = live[base];
+            if (acc[k].write) {
+                objs.push_back({base, len, (uint32_t)n, (uint32_t)n, 0, 0, false});
+                o = (int)objs.size() - 1;
+                for (uint32_t j = 0; j < len; j++) live[base + j] = o;
+            } else if (o < 0 || objs[o].base != base || objs[o].len != len) {
+                return 0;
+            }
+            objs[o].last = (uint32_t)n;
+            objs[o].touches += len;
+            obj_of[n][k] = o;
+        }
+    }
+    std::vector<uint32_t> order(objs.size());
+    for (uint32_t k = 0; k < order.size(); k++) order[k] = k;
+    auto density = [&](uint32_t k) {
+        const Obj &o = objs[k];
+        return (double)o.touches / ((double)o.len * (o.last - o.first + 1));
+    };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return density(x) > density(y); });
+    std::vector<uint8_t> busy((size_t)ops.size() * capacity, 0);  // [row][slot]
+    uint32_t used = 0;
+    for (uint32_t k : order) {
+        Obj &o = objs[k];
+        if (o.len > capacity) continue;
+        for (uint32_t s0 = 0; s0 + o.len <= capacity && !o.lds; s0++) {
+            bool fits = true;
+            for (uint32_t r = o.first; r <= o.last && fits; r++)
+                for (uint32_t j = 0; j < o.len; j++)
+                    if (busy[(size_t)r * capacity + s0 + j]) { fits = false; break; }
+            if (!fits) continue;
+            for (uint32_t r = o.first; r <= o.last; r++)
+                for (uint32_t j = 0; j < o.len; j++) busy[(size_t)r * capacity + s0 + j] = 1;
+            o.slot = s0;
+            o.lds = true;
+            used = std::max(used, s0 + o.len);
+        }
+    }
+    if (!used) return 0;
+    // replay the LDS placement
+    std::vector<int> held(capacity, -1);
+    for (size_t n = 0; n < ops.size(); n++) {
+        Access acc[3];
+        const int na = row_accesses(ops[n], acc);
+        for (int pass = 0; pass < 2; pass++)
+            for (int k = 0; k < na; k++) {
+                const int oi = obj_of[n][k];
+                if (oi < 0 || !objs[oi].lds) continue;
+                const Obj &o = objs[oi];
+                const bool birth = acc[k].write;
+                if (birth != (pass == 1)) continue;
+                for (uint32_t j = 0; j < o.len; j++) {
+                    int &h = held[o.slot + j];
+                    if (birth) {
+                        if (h >= 0 && h != oi && objs[h].last >= n) return 0;
+                        h = oi;
+                    } else if (h != oi) {
+                        return 0;
+                    }
+                }
+            }
+    }
+    for (size_t n = 0; n < ops.size(); n++) {
+        Access acc[3];
+        const int na = row_accesses(ops[n], acc);
+        for (int k = 0; k < na; k++) {
+            const int oi = obj_of[n][k];
+            if (oi >= 0 && objs[oi].lds) ops[n].w[acc[k].word] = LDS_PLANE + objs[oi].slot;
+        }
+    }
+    return used;
+}
+
 struct FusedState {
     std::mutex lock;
     double *scratch = nullptr;
@@ -472,6 +574,8 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     if (const uint32_t compacted = compact_planes(hp.ops, d->current_planes)) planes = compacted;
     const std::vector<Instr> plain = hp.ops;
     forward_in_place(hp.ops);
+    const std::vector<Instr> fwd = hp.ops;
+    const uint32_t lds_planes = assign_lds(hp.ops, LDS_CAPACITY);
     pacuda_plan *p = new pacuda_plan();
     const uint32_t S = d->scale_count, X = d->factor_count;
     std::vector<double> c64(2 * (size_t)(S + X)), cdf;
@@ -491,6 +595,8 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     h.count = (uint32_t)hp.ops.size();
     h.prog_plain = upload(p, plain.data(), plain.size(), &ok);
     h.count_plain = (uint32_t)plain.size();
+    h.prog_fwd = upload(p, fwd.data(), fwd.size(), &ok);
+    h.lds_planes = lds_planes;
     h.sources = hp.sources;
     h.c64 = upload(p, c64.data(), c64.size(), &ok);
     h.cdf = upload(p, cdf.data(), cdf.size(), &ok);
@@ -543,7 +649,26 @@ struct Schedule {
     std::vector<uint32_t> starts;
     uint64_t slot = 0, units = 0, waves = 0;
     uint32_t grid = 0, block = 0;
+    uint32_t lds_planes = 0;  // the most LDS-resident planes of any item's plan
 };
+
+// LDS elements per wavefront and bytes per workgroup of a kernel for a schedule: the f64 kernel
+// keeps double currents, the mixed kernels float ones, the df64 kernel none (it runs the plain
+// program); launches whose workgroups would need more than LDS_BYTES run without LDS residency
+void lds_shape(uint32_t kernel, const Schedule &sc, uint32_t *per_wave, size_t *bytes) {
+    size_t elem = kernel == PACMAN_RECURRENCE_KERNEL_F64 ? 8
+                  : (kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) ? 4
+                                                                                                          : 0;
+    const size_t elems = 2ull * sc.lds_planes * LANES;
+    size_t b = elem * elems * (sc.block / LANES);
+    if (!elem || b > LDS_BYTES) {
+        *per_wave = 0;
+        *bytes = 0;
+        return;
+    }
+    *per_wave = (uint32_t)elems;
+    *bytes = b;
+}
 
 int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, Schedule &sc) {
     if (!l || l->abi_version != PACMAN_RECURRENCE_ABI || l->struct_size != sizeof(pacuda_launch) ||
@@ -573,11 +698,16 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
         sc.starts[j] = (uint32_t)sc.units;
         sc.units += (it.points + LANES - 1) / LANES;
         sc.slot = std::max<uint64_t>(sc.slot, it.plan->slot_doubles);
+        sc.lds_planes = std::max(sc.lds_planes, it.plan->host.lds_planes);
     }
     if (sc.units >= 0xffffffffull) return PACMAN_RECURRENCE_INVALID;
     sc.starts[count] = (uint32_t)sc.units;
     sc.block = l->block ? l->block : 256;
-    const uint32_t per_cu = l->waves_per_cu ? l->waves_per_cu : 12;
+    // default occupancy per kernel (measured on g g > g g g and g g > g g g g): the f64 kernel is
+    // bound by arena traffic and holds 20 KB of LDS per wavefront, the others gain from more waves
+    const uint32_t per_cu = l->waves_per_cu ? l->waves_per_cu
+                            : l->kernel == PACMAN_RECURRENCE_KERNEL_F64 ? 8
+                            : l->kernel == PACMAN_RECURRENCE_KERNEL_DF64 ? 16 : 12;
     const uint32_t waves_per_block = sc.block / LANES;
     uint64_t waves = std::min<uint64_t>((uint64_t)compute_units() * per_cu, std::max<uint64_t>(sc.units, 1));
     sc.grid = (uint32_t)((waves + waves_per_block - 1) / waves_per_block);
@@ -585,23 +715,49 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
     return PACMAN_RECURRENCE_OK;
 }
 
-void launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
+template <class K> bool allow_lds() {
+    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(fused_kernel<K>),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES),
+                                       "fused kernel LDS");
+    return ok;
+}
+
+// the LDS attribute of the kernel a launch of `sc` needs, set once (before any graph capture)
+bool lds_ready(uint32_t kernel, const Schedule &sc) {
+    uint32_t per_wave;
+    size_t bytes;
+    lds_shape(kernel, sc, &per_wave, &bytes);
+    if (!bytes) return true;
+    if (kernel == PACMAN_RECURRENCE_KERNEL_F64) return allow_lds<KF64>();
+    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED) return allow_lds<KMixed>();
+    return allow_lds<KMixed64>();
+}
+
+bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
                   uint32_t count, uint32_t *counter, double *scratch) {
+    uint32_t per_wave;
+    size_t bytes;
+    lds_shape(kernel, sc, &per_wave, &bytes);
     reset_counter<<<1, 64, 0, s>>>(counter);
     switch (kernel) {
     case PACMAN_RECURRENCE_KERNEL_F64:
-        fused_kernel<KF64><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot);
+        if (bytes && !allow_lds<KF64>()) return false;
+        fused_kernel<KF64><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave);
         break;
     case PACMAN_RECURRENCE_KERNEL_MIXED:
-        fused_kernel<KMixed><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot);
+        if (bytes && !allow_lds<KMixed>()) return false;
+        fused_kernel<KMixed><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave);
         break;
     case PACMAN_RECURRENCE_KERNEL_MIXED64:
-        fused_kernel<KMixed64><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot);
+        if (bytes && !allow_lds<KMixed64>()) return false;
+        fused_kernel<KMixed64><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot,
+                                                                  per_wave);
         break;
     default:
-        fused_kernel<KDF64><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot);
+        fused_kernel<KDF64><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot, 0);
         break;
     }
+    return true;
 }
 
 }  // namespace
@@ -647,8 +803,9 @@ extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const p
         hip_failed(hipMemcpy(fs.unit_start, sc.starts.data(), (count + 1) * sizeof(uint32_t), hipMemcpyHostToDevice), "items"))
         return PACMAN_RECURRENCE_DEVICE_ERROR;
     if (l->kernel_ms) (void)hipEventRecord(fs.start, s);
-    launch_fused(l->kernel, sc, s, fs.items, fs.unit_start, count, fs.counter, fs.scratch);
-    if (hip_failed(hipGetLastError(), "fused launch")) return PACMAN_RECURRENCE_DEVICE_ERROR;
+    if (!launch_fused(l->kernel, sc, s, fs.items, fs.unit_start, count, fs.counter, fs.scratch) ||
+        hip_failed(hipGetLastError(), "fused launch"))
+        return PACMAN_RECURRENCE_DEVICE_ERROR;
     if (l->kernel_ms) {
         if (hip_failed(hipEventRecord(fs.stop, s), "event") || hip_failed(hipEventSynchronize(fs.stop), "kernel"))
             return PACMAN_RECURRENCE_DEVICE_ERROR;
@@ -707,10 +864,12 @@ extern "C" int pacuda_sweep_create(const pacuda_item *items, uint32_t count, con
                           "sweep") &&
               !hip_failed(hipEventCreate(&w->start), "sweep") && !hip_failed(hipEventCreate(&w->stop), "sweep") &&
               !hip_failed(hipStreamCreateWithFlags(&w->capture, hipStreamNonBlocking), "sweep") &&
+              lds_ready(w->kernel, w->sc) &&
               !hip_failed(hipStreamBeginCapture(w->capture, hipStreamCaptureModeThreadLocal), "sweep capture");
     if (ok) {
-        launch_fused(w->kernel, w->sc, w->capture, w->items, w->starts, count, w->counter, w->scratch);
-        ok = !hip_failed(hipStreamEndCapture(w->capture, &w->graph), "sweep capture") &&
+        const bool launched = launch_fused(w->kernel, w->sc, w->capture, w->items, w->starts, count, w->counter,
+                                           w->scratch);
+        ok = !hip_failed(hipStreamEndCapture(w->capture, &w->graph), "sweep capture") && launched &&
              !hip_failed(hipGraphInstantiate(&w->exec, w->graph, nullptr, nullptr, 0), "sweep graph");
     }
     if (!ok) {

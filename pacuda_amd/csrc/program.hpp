@@ -75,9 +75,9 @@ PCD void contribution(const Arena &a, const Instr &o, int flags, Cx<typename K::
     C l[c0], r[c1], v[cd];
     R q0[4], q1[4];
 #pragma unroll
-    for (int k = 0; k < c0; k++) l[k] = Cur<R>::ld(a, (p0 + k) * ld + i);
+    for (int k = 0; k < c0; k++) l[k] = Cur<R>::ldp(a, p0 + k, i);
 #pragma unroll
-    for (int k = 0; k < c1; k++) r[k] = Cur<R>::ld(a, (p1 + k) * ld + i);
+    for (int k = 0; k < c1; k++) r[k] = Cur<R>::ldp(a, p1 + k, i);
     if constexpr (OP == 6) {
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -94,11 +94,11 @@ PCD void contribution(const Arena &a, const Instr &o, int flags, Cx<typename K::
         for (int k = 0; k < cd; k++) acc[k] = mul(v[k], scale);
     } else {
 #pragma unroll
-        for (int k = 0; k < cd; k++) acc[k] = add(Cur<R>::ld(a, (dst + k) * ld + i), mul(v[k], scale));
+        for (int k = 0; k < cd; k++) acc[k] = add(Cur<R>::ldp(a, dst + k, i), mul(v[k], scale));
     }
     if ((flags & F_LAST) && !(flags & F_HOLD)) {
 #pragma unroll
-        for (int k = 0; k < cd; k++) Cur<R>::st(a, (dst + k) * ld + i, acc[k]);
+        for (int k = 0; k < cd; k++) Cur<R>::stp(a, dst + k, i, acc[k]);
     }
 }
 
@@ -125,7 +125,7 @@ PCD void propagator(const Arena &a, const Instr &o, int flags, Cx<typename K::re
         for (int k = 0; k < m; k++) c[k] = acc[k];
     } else {
 #pragma unroll
-        for (int k = 0; k < m; k++) c[k] = Cur<R>::ld(a, (base + k) * ld + i);
+        for (int k = 0; k < m; k++) c[k] = Cur<R>::ldp(a, base + k, i);
     }
     propagator_values<OP>(c, p, reciprocal, cx(ldc<R>(a.fre, o.w[3]), ldc<R>(a.fim, o.w[3])), v);
     if (flags & F_HOLD) {
@@ -133,7 +133,7 @@ PCD void propagator(const Arena &a, const Instr &o, int flags, Cx<typename K::re
         for (int k = 0; k < m; k++) acc[k] = v[k];
     } else {
 #pragma unroll
-        for (int k = 0; k < m; k++) Cur<R>::st(a, (base + k) * ld + i, v[k]);
+        for (int k = 0; k < m; k++) Cur<R>::stp(a, base + k, i, v[k]);
     }
 }
 
@@ -171,7 +171,7 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
             size_t at = o.w[3] * ld + i;
             Amp dr = ldr<Amp>(a.are, at), di = ldr<Amp>(a.aim, at);
             for (uint32_t k = 0; k < o.w[6]; k++) {
-                C v = closure_increment(Cur<R>::ld(a, (o.w[1] + k) * ld + i), Cur<R>::ld(a, (o.w[2] + k) * ld + i),
+                C v = closure_increment(Cur<R>::ldp(a, o.w[1] + k, i), Cur<R>::ldp(a, o.w[2] + k, i),
                                         ldc<R>(a.fre, o.w[5] + k), ldc<R>(a.fim, o.w[5] + k), rfr, rfi);
                 dr = dr + widen(v.re);
                 di = di + widen(v.im);
@@ -185,23 +185,21 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
 #pragma unroll
                 for (int k = 0; k < 6; k++) {
                     if ((uint32_t)k < count) {
-                        const size_t at = (o.w[1] + k) * ld + i;
                         C u;
                         if (flags & F_FROMACC) u = acc[k];
-                        else u = Cur<R>::ld(a, at);
+                        else u = Cur<R>::ldp(a, o.w[1] + k, i);
                         const C v = identity_value(fr, fi, u);
                         if (flags & F_HOLD) acc[k] = v;
-                        else Cur<R>::st(a, at, v);
+                        else Cur<R>::stp(a, o.w[1] + k, i, v);
                     }
                 }
             } else {
                 for (uint32_t k = 0; k < o.w[2]; k++) {
-                    size_t at = (o.w[1] + k) * ld + i;
-                    Cur<R>::st(a, at, identity_value(fr, fi, Cur<R>::ld(a, at)));
+                    Cur<R>::stp(a, o.w[1] + k, i, identity_value(fr, fi, Cur<R>::ldp(a, o.w[1] + k, i)));
                 }
             }
         } else if (kind == K_CLEAR) {
-            for (uint32_t k = 0; k < o.w[2]; k++) Cur<R>::st(a, (o.w[1] + k) * ld + i, cx(R(0.0), R(0.0)));
+            for (uint32_t k = 0; k < o.w[2]; k++) Cur<R>::stp(a, o.w[1] + k, i, cx(R(0.0), R(0.0)));
         }
     }
 }
@@ -218,10 +216,13 @@ __device__ inline void run_sources_point(const Instr *__restrict__ prog, uint32_
         for (int k = 0; k < 4; k++) p[k] = a.mom[(o.w[2] * 4 + k) * ld + i];
         double fr = ldc<double>(a.fre, o.w[3]), fi = ldc<double>(a.fim, o.w[3]), mass = ldc<double>(a.sre, o.w[4]);
         W v[4];
-        int c = source_values(o.w[5], mass, p, v);
-        for (int k = 0; k < c; k++) {
-            W s = cx(fr * v[k].re - fi * v[k].im, fr * v[k].im + fi * v[k].re);
-            Cur<R>::st(a, (o.w[1] + k) * ld + i, cx(narrow<R>(s.re), narrow<R>(s.im)));
+        const int c = source_values(o.w[5], mass, p, v);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {  // c <= 4: a constant bound keeps v in registers
+            if (k < c) {
+                W s = cx(fr * v[k].re - fi * v[k].im, fr * v[k].im + fi * v[k].re);
+                Cur<R>::stp(a, o.w[1] + k, i, cx(narrow<R>(s.re), narrow<R>(s.im)));
+            }
         }
     }
 }

@@ -656,6 +656,7 @@ int run_locked(const pacman_recurrence_call *in) {
     a.cre = st.cre.p; a.cim = st.cim.p; a.are = st.are.p; a.aim = st.aim.p;
     a.fcre = st.fcre.p; a.fcim = st.fcim.p; a.mom = st.mom.p;
     a.sre = dsre; a.sim = dsim; a.fre = dfre; a.fim = dfim; a.ld = stride;
+    a.lds_base = 0;
     const Program *prepared = prepare(call);
     if (!prepared) return PACMAN_RECURRENCE_DEVICE_ERROR;
     // a program indexing past this call's tables would read or write outside the arena
