@@ -32,12 +32,14 @@ using namespace pacuda;
 
 namespace {
 
+constexpr int LDS_VARIANTS = 3;
+
 struct PlanDev {
-    const Instr *prog;        // with forwarding (F_HOLD / F_FROMACC) and LDS-resident currents
-    const Instr *prog_fwd;    // with forwarding only (launches that give the wavefronts no LDS)
-    const Instr *prog_plain;  // neither (kernels without forwarding)
+    const Instr *prog_lds[LDS_VARIANTS];  // with forwarding and LDS-resident currents (LDS_CAPS[v] planes)
+    const Instr *prog_fwd;                // with forwarding only (launches that give the wavefronts no LDS)
+    const Instr *prog_plain;              // neither (kernels without forwarding)
     uint32_t count, count_plain, sources;
-    uint32_t lds_planes;      // LDS-resident current planes of `prog` per wavefront
+    uint32_t lds_planes[LDS_VARIANTS];    // LDS planes each variant uses per wavefront (0: no variant)
     const double *c64;  // scales re, im, factors re, im (f64)
     const double *cdf;  // the same in the df64 layout
     uint32_t scale_count, factor_count;
@@ -58,8 +60,10 @@ struct ItemDev {
 };
 
 constexpr int LANES = 64;
-constexpr uint32_t LDS_CAPACITY = 20;        // LDS-resident current planes per wavefront (a plan's maximum)
-constexpr uint32_t LDS_BYTES = 160 * 1024;   // LDS per CU (gfx950), the most one workgroup may take
+// LDS-resident current planes per wavefront of the program variants: a launch takes the largest
+// whose LDS fits the wavefronts it asks per CU (20 at 8 f64 wavefronts, 13 at 12, 10 at 16)
+constexpr uint32_t LDS_CAPS[3] = {10, 13, 20};
+constexpr uint32_t LDS_BYTES = 160 * 1024;  // LDS per CU (gfx950), the most one workgroup may take
 
 template <bool DF> PCD double amplitude_value(const double *p, size_t at) {
     if constexpr (DF) {
@@ -77,7 +81,8 @@ __global__ void reset_counter(uint32_t *counter) {
 template <class K>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start,
                                                     uint32_t item_count, uint32_t *counter, double *scratch,
-                                                    size_t slot_doubles, uint32_t lds_per_wave) {
+                                                    size_t slot_doubles, uint32_t lds_per_wave,
+                                                    uint32_t lds_variant) {
     typedef typename K::real R;
     constexpr bool DF = !__is_same(R, double) || !__is_same(typename K::den, double);
     constexpr bool DF_FORMS = __is_same(typename K::den, D);
@@ -135,7 +140,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
         // sources in f64 from the f64 constants, then the program in the kernel's layout
         const uint32_t S = pl.scale_count, X = pl.factor_count;
         a.sre = pl.c64; a.sim = pl.c64 + S; a.fre = pl.c64 + 2 * S; a.fim = pl.c64 + 2 * S + X;
-        run_sources_point<R>(lds_per_wave ? pl.prog : pl.prog_fwd, pl.sources, a, lane);
+        // (selects, not an index: a runtime index into the local copy of the plan would put it in scratch)
+        const Instr *prog = !lds_per_wave       ? pl.prog_fwd
+                            : lds_variant == 0 ? pl.prog_lds[0]
+                            : lds_variant == 1 ? pl.prog_lds[1]
+                                               : pl.prog_lds[2];
+        run_sources_point<R>(prog, pl.sources, a, lane);
         if constexpr (DF_FORMS) {
             for (uint32_t q = 0; q < 4 * F; q++) {
                 size_t at = q * LANES + lane;
@@ -146,7 +156,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
         const double *cons = DF ? pl.cdf : pl.c64;
         a.sre = cons; a.sim = cons + S; a.fre = cons + 2 * S; a.fim = cons + 2 * S + X;
         if constexpr (K::forwarding)
-            interpret<K>((lds_per_wave ? pl.prog : pl.prog_fwd) + pl.sources, pl.count - pl.sources, a, lane);
+            interpret<K>(prog + pl.sources, pl.count - pl.sources, a, lane);
         else
             interpret<K>(pl.prog_plain + pl.sources, pl.count_plain - pl.sources, a, lane);
         // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
@@ -575,7 +585,12 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     const std::vector<Instr> plain = hp.ops;
     forward_in_place(hp.ops);
     const std::vector<Instr> fwd = hp.ops;
-    const uint32_t lds_planes = assign_lds(hp.ops, LDS_CAPACITY);
+    std::vector<Instr> lds_ops[LDS_VARIANTS];
+    uint32_t lds_used[LDS_VARIANTS];
+    for (int v = 0; v < LDS_VARIANTS; v++) {
+        lds_ops[v] = fwd;
+        lds_used[v] = assign_lds(lds_ops[v], LDS_CAPS[v]);
+    }
     pacuda_plan *p = new pacuda_plan();
     const uint32_t S = d->scale_count, X = d->factor_count;
     std::vector<double> c64(2 * (size_t)(S + X)), cdf;
@@ -591,12 +606,14 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     bool ok = true;
     PlanDev &h = p->host;
     memset(&h, 0, sizeof h);
-    h.prog = upload(p, hp.ops.data(), hp.ops.size(), &ok);
-    h.count = (uint32_t)hp.ops.size();
+    for (int v = 0; v < LDS_VARIANTS; v++) {
+        h.prog_lds[v] = upload(p, lds_ops[v].data(), lds_ops[v].size(), &ok);
+        h.lds_planes[v] = lds_used[v];
+    }
+    h.count = (uint32_t)fwd.size();
     h.prog_plain = upload(p, plain.data(), plain.size(), &ok);
     h.count_plain = (uint32_t)plain.size();
     h.prog_fwd = upload(p, fwd.data(), fwd.size(), &ok);
-    h.lds_planes = lds_planes;
     h.sources = hp.sources;
     h.c64 = upload(p, c64.data(), c64.size(), &ok);
     h.cdf = upload(p, cdf.data(), cdf.size(), &ok);
@@ -648,26 +665,35 @@ struct Schedule {
     std::vector<ItemDev> items;
     std::vector<uint32_t> starts;
     uint64_t slot = 0, units = 0, waves = 0;
-    uint32_t grid = 0, block = 0;
-    uint32_t lds_planes = 0;  // the most LDS-resident planes of any item's plan
+    uint32_t grid = 0, block = 0, per_cu = 0;
+    uint32_t lds_planes[LDS_VARIANTS] = {0, 0, 0};  // per variant, the most any item's plan uses
+    bool lds_all[LDS_VARIANTS] = {true, true, true};  // every plan has the variant
 };
 
-// LDS elements per wavefront and bytes per workgroup of a kernel for a schedule: the f64 kernel
-// keeps double currents, the mixed kernels float ones, the df64 kernel none (it runs the plain
-// program); launches whose workgroups would need more than LDS_BYTES run without LDS residency
-void lds_shape(uint32_t kernel, const Schedule &sc, uint32_t *per_wave, size_t *bytes) {
-    size_t elem = kernel == PACMAN_RECURRENCE_KERNEL_F64 ? 8
-                  : (kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) ? 4
-                                                                                                          : 0;
-    const size_t elems = 2ull * sc.lds_planes * LANES;
-    size_t b = elem * elems * (sc.block / LANES);
-    if (!elem || b > LDS_BYTES) {
-        *per_wave = 0;
-        *bytes = 0;
+// The LDS variant of a launch, its LDS elements per wavefront and bytes per workgroup: the f64
+// kernel keeps double currents, the mixed kernels float ones, the df64 kernel none (it runs the
+// plain program). A variant qualifies when every plan has it and the workgroups a CU must hold
+// for the requested wavefronts fit in its LDS; the largest one is taken, else none (per_wave 0).
+void lds_shape(uint32_t kernel, const Schedule &sc, uint32_t *per_wave, size_t *bytes, uint32_t *variant) {
+    *per_wave = 0;
+    *bytes = 0;
+    *variant = 0;
+    const size_t elem = kernel == PACMAN_RECURRENCE_KERNEL_F64 ? 8
+                        : (kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) ? 4
+                                                                                                                : 0;
+    if (!elem) return;
+    const uint32_t waves_per_block = sc.block / LANES;
+    const uint32_t blocks_per_cu = (sc.per_cu + waves_per_block - 1) / waves_per_block;
+    for (int v = LDS_VARIANTS - 1; v >= 0; v--) {
+        if (!sc.lds_all[v] || !sc.lds_planes[v]) continue;
+        const size_t elems = 2ull * sc.lds_planes[v] * LANES;
+        const size_t b = elem * elems * waves_per_block;
+        if (b * blocks_per_cu > LDS_BYTES) continue;
+        *per_wave = (uint32_t)elems;
+        *bytes = b;
+        *variant = (uint32_t)v;
         return;
     }
-    *per_wave = (uint32_t)elems;
-    *bytes = b;
 }
 
 int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, Schedule &sc) {
@@ -698,16 +724,17 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
         sc.starts[j] = (uint32_t)sc.units;
         sc.units += (it.points + LANES - 1) / LANES;
         sc.slot = std::max<uint64_t>(sc.slot, it.plan->slot_doubles);
-        sc.lds_planes = std::max(sc.lds_planes, it.plan->host.lds_planes);
+        for (int v = 0; v < LDS_VARIANTS; v++) {
+            sc.lds_planes[v] = std::max(sc.lds_planes[v], it.plan->host.lds_planes[v]);
+            sc.lds_all[v] = sc.lds_all[v] && it.plan->host.lds_planes[v] > 0;
+        }
     }
     if (sc.units >= 0xffffffffull) return PACMAN_RECURRENCE_INVALID;
     sc.starts[count] = (uint32_t)sc.units;
     sc.block = l->block ? l->block : 256;
-    // default occupancy per kernel (measured on g g > g g g and g g > g g g g): the f64 kernel is
-    // bound by arena traffic and holds 20 KB of LDS per wavefront, the others gain from more waves
-    const uint32_t per_cu = l->waves_per_cu ? l->waves_per_cu
-                            : l->kernel == PACMAN_RECURRENCE_KERNEL_F64 ? 8
-                            : l->kernel == PACMAN_RECURRENCE_KERNEL_DF64 ? 16 : 12;
+    // default occupancy per kernel (measured on g g > g g g and g g > g g g g)
+    const uint32_t per_cu = l->waves_per_cu ? l->waves_per_cu : l->kernel == PACMAN_RECURRENCE_KERNEL_F64 ? 12 : 16;
+    sc.per_cu = per_cu;
     const uint32_t waves_per_block = sc.block / LANES;
     uint64_t waves = std::min<uint64_t>((uint64_t)compute_units() * per_cu, std::max<uint64_t>(sc.units, 1));
     sc.grid = (uint32_t)((waves + waves_per_block - 1) / waves_per_block);
@@ -724,9 +751,9 @@ template <class K> bool allow_lds() {
 
 // the LDS attribute of the kernel a launch of `sc` needs, set once (before any graph capture)
 bool lds_ready(uint32_t kernel, const Schedule &sc) {
-    uint32_t per_wave;
+    uint32_t per_wave, variant;
     size_t bytes;
-    lds_shape(kernel, sc, &per_wave, &bytes);
+    lds_shape(kernel, sc, &per_wave, &bytes, &variant);
     if (!bytes) return true;
     if (kernel == PACMAN_RECURRENCE_KERNEL_F64) return allow_lds<KF64>();
     if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED) return allow_lds<KMixed>();
@@ -735,26 +762,28 @@ bool lds_ready(uint32_t kernel, const Schedule &sc) {
 
 bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
                   uint32_t count, uint32_t *counter, double *scratch) {
-    uint32_t per_wave;
+    uint32_t per_wave, variant;
     size_t bytes;
-    lds_shape(kernel, sc, &per_wave, &bytes);
+    lds_shape(kernel, sc, &per_wave, &bytes, &variant);
     reset_counter<<<1, 64, 0, s>>>(counter);
     switch (kernel) {
     case PACMAN_RECURRENCE_KERNEL_F64:
         if (bytes && !allow_lds<KF64>()) return false;
-        fused_kernel<KF64><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave);
+        fused_kernel<KF64><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
+                                                              variant);
         break;
     case PACMAN_RECURRENCE_KERNEL_MIXED:
         if (bytes && !allow_lds<KMixed>()) return false;
-        fused_kernel<KMixed><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave);
+        fused_kernel<KMixed><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
+                                                              variant);
         break;
     case PACMAN_RECURRENCE_KERNEL_MIXED64:
         if (bytes && !allow_lds<KMixed64>()) return false;
         fused_kernel<KMixed64><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot,
-                                                                  per_wave);
+                                                                  per_wave, variant);
         break;
     default:
-        fused_kernel<KDF64><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot, 0);
+        fused_kernel<KDF64><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot, 0, 0);
         break;
     }
     return true;
