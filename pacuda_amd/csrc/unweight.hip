@@ -177,7 +177,10 @@ size_t work_len = 0;
 
 uint32_t *select_work(size_t n) {
     if (n <= work_len) return work_buf;
-    if (work_buf) hip_failed(hipFree(work_buf), "select");
+    if (work_buf) {  // an earlier selection's kernels may still read it
+        hip_failed(hipStreamSynchronize(runner_stream()), "select");
+        hip_failed(hipFree(work_buf), "select");
+    }
     work_buf = nullptr;
     work_len = 0;
     if (hip_failed(hipMalloc((void **)&work_buf, n * 4), "select")) return nullptr;
