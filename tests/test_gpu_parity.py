@@ -222,3 +222,35 @@ def test_fused_launch_shapes_bitwise(lib, gg_ggg, kernel):
     d_ext.free()
     d_tot.free()
     fp.destroy()
+
+
+def test_fused_pp_jjj_subprocesses_bitwise(lib):
+    """every p p > j j j subprocess plan (quark lines: ops 2, 3, 7, 8) in one persistent launch,
+    uneven point counts, f64 and mixed, equals the oracle's plan run bit for bit"""
+    from pacuda_amd import processes
+    subs = processes.subprocesses()
+    jobs, devs, fps, exts = [], [], [], []
+    for k, sp in enumerate(subs):
+        points = 64 * (k % 5) + 17 * k + 1
+        mom, _ = ps.generate(sp.plan.legs, points, 300 + k)
+        ext = ps.external_planes(mom, sp.plan.crossing)
+        fp = FusedPlan(lib, sp.plan)
+        d_ext = DeviceArray(lib, ext.nbytes)
+        d_ext.upload(ext)
+        d_tot = DeviceArray(lib, points * 8)
+        jobs.append((fp, d_ext, points, ext.shape[1], None, d_tot))
+        devs += [d_ext, d_tot]
+        fps.append(fp)
+        exts.append(ext)
+    for kernel in ("f64", "mixed"):
+        k = abi.KERNELS[kernel]
+        print("pp>jjj launch", kernel, flush=True)
+        fused_evaluate(lib, jobs, k)
+        for sp, (fp, _, points, _, _, d_tot), ext in zip(subs, jobs, exts):
+            _, want = refcpu.evaluate(sp.plan.arrays(), harness.plan_counts(sp.plan), ext, points, kernel=k,
+                                      resolved=False)
+            assert bits_equal(d_tot.download(points), want), f"{kernel} {sp.name}"
+    for d in devs:
+        d.free()
+    for fp in fps:
+        fp.destroy()
