@@ -89,8 +89,8 @@ typedef struct {
     uint32_t abi_version;   /* PACMAN_RECURRENCE_ABI */
     uint32_t struct_size;   /* sizeof(pacuda_launch) */
     uint32_t kernel;        /* PACMAN_RECURRENCE_KERNEL_* */
-    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 12 (f64), 16 (others) */
-    uint32_t block;         /* threads per workgroup, a multiple of 64 up to 1024, 0: 256 */
+    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 12 (f64), 24 (mixed64), 16 (others) */
+    uint32_t block;         /* threads per workgroup, a multiple of 64 up to 256, 0: 256 */
     uint32_t reserved;
     void *stream;           /* hipStream_t to queue on, or NULL: the library's stream */
     float *kernel_ms;       /* not NULL: synchronise and store the fused kernel's time (HIP events) */
@@ -152,8 +152,9 @@ int pacuda_mlp_adam(pacuda_mlp *mlp, float lr, float beta1, float beta2, float e
 
 /*
  * RAMBO phase space of 2 -> legs-2 massless partons in f64 on the device (convert/src/record.rs:
- * 17-52 on a counter-based stream: point `first + i` of stream `seed`), written as the runner's
- * external momenta external[(l * 4 + k) * stride + i] = crossing_sign[l] * p[crossing_src[l]][k];
+ * 17-52 on a counter-based stream: point `first + index[i]` of stream `seed`, or `first + i` when
+ * index is NULL), written as the runner's external momenta (when external is not NULL)
+ * external[(l * 4 + k) * stride + i] = crossing_sign[l] * p[crossing_src[l]][k];
  * pass[i] = 1 when the outgoing partons pass pT > pt_min, |eta| < eta_max, Delta R > dr_min;
  * features (or NULL): [points][3 (legs - 2) + 2] floats in [0, 1], the surrogate's inputs;
  * log_prior (or NULL): log(sum_{a<b} s_ab^4 / prod_l s_{l,l+1}), s_ab = |2 p_a.p_b| over the
@@ -162,7 +163,8 @@ int pacuda_mlp_adam(pacuda_mlp *mlp, float lr, float beta1, float beta2, float e
  */
 int pacuda_rambo(uint64_t seed, uint64_t first, uint32_t points, uint32_t legs, double ecm,
                  const uint32_t *crossing_src, const double *crossing_sign, double *external, uint32_t stride,
-                 float *features, uint8_t *pass, double pt_min, double eta_max, double dr_min, double *log_prior);
+                 float *features, uint8_t *pass, double pt_min, double eta_max, double dr_min, double *log_prior,
+                 const uint32_t *index);
 /* out[r * count + j] = external[r * stride + index[j]] for rows r < rows; features likewise
  * ([j][width] <- [index[j]][width]) when features is not NULL (device pointers) */
 int pacuda_gather(const double *external, uint32_t stride, uint32_t rows, const uint32_t *index, uint32_t count,

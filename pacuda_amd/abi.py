@@ -129,7 +129,8 @@ EXTENSION_SYMBOLS = [
     ("pacuda_sweep_run", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     ("pacuda_sweep_destroy", None, [C.c_void_p]),
     ("pacuda_rambo", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, u32p, f64p, C.c_void_p,
-                               C.c_uint32, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_void_p]),
+                               C.c_uint32, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_void_p,
+                               C.c_void_p]),
     ("pacuda_gather", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                 C.c_uint32, C.c_void_p]),
     ("pacuda_surrogate_weight", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double, C.c_double,

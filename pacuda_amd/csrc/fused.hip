@@ -698,7 +698,7 @@ void lds_shape(uint32_t kernel, const Schedule &sc, uint32_t *per_wave, size_t *
 
 int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, Schedule &sc) {
     if (!l || l->abi_version != PACMAN_RECURRENCE_ABI || l->struct_size != sizeof(pacuda_launch) ||
-        l->kernel > PACMAN_RECURRENCE_KERNEL_MIXED64 || (l->block % LANES) != 0 || l->block > 1024 ||
+        l->kernel > PACMAN_RECURRENCE_KERNEL_MIXED64 || (l->block % LANES) != 0 || l->block > 256 ||
         l->waves_per_cu > 64 || (count && !items))
         return PACMAN_RECURRENCE_INVALID;
     std::vector<uint32_t> order(count);
@@ -733,7 +733,10 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
     sc.starts[count] = (uint32_t)sc.units;
     sc.block = l->block ? l->block : 256;
     // default occupancy per kernel (measured on g g > g g g and g g > g g g g)
-    const uint32_t per_cu = l->waves_per_cu ? l->waves_per_cu : l->kernel == PACMAN_RECURRENCE_KERNEL_F64 ? 12 : 16;
+    const uint32_t per_cu = l->waves_per_cu                                  ? l->waves_per_cu
+                            : l->kernel == PACMAN_RECURRENCE_KERNEL_F64     ? 12
+                            : l->kernel == PACMAN_RECURRENCE_KERNEL_MIXED64 ? 24
+                                                                            : 16;
     sc.per_cu = per_cu;
     const uint32_t waves_per_block = sc.block / LANES;
     uint64_t waves = std::min<uint64_t>((uint64_t)compute_units() * per_cu, std::max<uint64_t>(sc.units, 1));

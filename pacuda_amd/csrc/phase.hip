@@ -46,11 +46,11 @@ template <int LEGS>
 __global__ void __launch_bounds__(256) rambo_kernel(uint64_t state, uint64_t first, uint32_t points, double ecm,
                                                     Crossing cr, double *external, uint32_t stride, float *features,
                                                     uint8_t *pass, double pt_min, double eta_max, double dr_min,
-                                                    double *log_prior) {
+                                                    double *log_prior, const uint32_t *index) {
     constexpr int M = LEGS - 2;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= points) return;
-    const uint64_t base = (first + i) * (uint64_t)M * 4;
+    const uint64_t base = (first + (index ? index[i] : i)) * (uint64_t)M * 4;
     double q[M][4], u[M][4];
     double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
 #pragma unroll
@@ -111,10 +111,11 @@ __global__ void __launch_bounds__(256) rambo_kernel(uint64_t state, uint64_t fir
 #pragma unroll
                 for (int k = 0; k < 4; k++) xl[l][k] = p[s][k];
     }
+    if (external)
 #pragma unroll
-    for (int l = 0; l < LEGS; l++)
+        for (int l = 0; l < LEGS; l++)
 #pragma unroll
-        for (int k = 0; k < 4; k++) external[((size_t)l * 4 + k) * stride + i] = cr.sign[l] * xl[l][k];
+            for (int k = 0; k < 4; k++) external[((size_t)l * 4 + k) * stride + i] = cr.sign[l] * xl[l][k];
     if (log_prior) {
         // log(sum_{a<b} s_ab^4 / prod_l s_{l,l+1}) over the plan's legs in order (cyclic),
         // s_ab = |2 p_a.p_b|: the helicity-summed MHV Parke-Taylor structure of the ordering
@@ -175,8 +176,8 @@ uint64_t seed_state(uint64_t seed) {
 extern "C" int pacuda_rambo(uint64_t seed, uint64_t first, uint32_t points, uint32_t legs, double ecm,
                             const uint32_t *crossing_src, const double *crossing_sign, double *external, uint32_t stride,
                             float *features, uint8_t *pass, double pt_min, double eta_max, double dr_min,
-                            double *log_prior) {
-    if (legs < 3 || legs > MAX_LEGS || !external || !crossing_src || !crossing_sign || stride < points)
+                            double *log_prior, const uint32_t *index) {
+    if (legs < 3 || legs > MAX_LEGS || !crossing_src || !crossing_sign || (external && stride < points))
         return PACMAN_RECURRENCE_INVALID;
     Crossing cr;
     for (uint32_t l = 0; l < legs; l++) {
@@ -192,7 +193,7 @@ extern "C" int pacuda_rambo(uint64_t seed, uint64_t first, uint32_t points, uint
 #define PACUDA_RAMBO(L)                                                                                                   \
     case L:                                                                                                               \
         rambo_kernel<L><<<grid, 256, 0, s>>>(st, first, points, ecm, cr, external, stride, features, pass, pt_min, eta_max, \
-                                             dr_min, log_prior);                                                          \
+                                             dr_min, log_prior, index);                                                   \
         break;
         PACUDA_RAMBO(3) PACUDA_RAMBO(4) PACUDA_RAMBO(5) PACUDA_RAMBO(6) PACUDA_RAMBO(7) PACUDA_RAMBO(8)
 #undef PACUDA_RAMBO

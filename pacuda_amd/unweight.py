@@ -49,16 +49,18 @@ class Generator:
         self.src = np.array([c[0] for c in plan.crossing], dtype=np.uint32)
         self.sign = np.array([c[1] for c in plan.crossing], dtype=np.float64)
 
-    def draw_device(self, seed, first, count, features=True, prior=False):
-        """dict of device arrays: ext [legs*4][count], pass [count] u8, feat [count][width] f32,
-        logp [count] f64 (the last two when asked for)"""
+    def draw_device(self, seed, first, count, features=True, prior=False, momenta=True, index=None):
+        """dict of device arrays: ext [legs*4][count] (when momenta), pass [count] u8, feat
+        [count][width] f32, logp [count] f64 (the last two when asked for); points first + i, or
+        first + index[i] for a device index array"""
         lib, dev = self.dev.lib, self.dev
-        out = {"ext": dev.empty(self.legs * 4 * count), "pass": dev.empty(count, np.uint8)}
+        out = {"ext": dev.empty(self.legs * 4 * count) if momenta else None, "pass": dev.empty(count, np.uint8)}
         out["feat"] = dev.empty(count * self.width, np.float32) if features else None
         out["logp"] = dev.empty(count) if prior else None
         st = lib.pacuda_rambo(seed, first, count, self.legs, self.ecm, abi.ptr(self.src, abi.u32p), abi.ptr(self.sign),
-                              out["ext"].ptr, count, out["feat"].ptr if features else None, out["pass"].ptr, *CUTS,
-                              out["logp"].ptr if prior else None)
+                              out["ext"].ptr if momenta else None, count, out["feat"].ptr if features else None,
+                              out["pass"].ptr, *CUTS, out["logp"].ptr if prior else None,
+                              index.ptr if index is not None else None)
         if st != 0:
             raise RuntimeError(f"pacuda_rambo: status {st}")
         return out
@@ -329,7 +331,7 @@ class Unweighting:
                 "estimate": self._estimate(x, candidates), "seconds": sum(clock.values()), "times_s": clock}
 
     def two_stage_envelope(self, candidates, seed=4, chunk=1 << 24, q=0.99999, q2=0.999, clamp=4.0, accept_seed=22,
-                           stage2_seed=23, keep_ids=False):
+                           stage2_seed=23, keep_ids=False, network=True):
         """two-stage unweighting with the network evaluated only where it can matter.
 
         The surrogate is s = P e^(mu + sigma clip(NN, -clamp, clamp)): any positive function is a
@@ -344,19 +346,21 @@ class Unweighting:
         staged = []
         for start in range(0, candidates, chunk):
             count = min(chunk, candidates - start)
-            d = self._timed(clock, "generate", self.gen.draw_device, seed, start, count, features=True, prior=True)
+            # cuts and prior only: momenta and features are made again for the envelope's points
+            d = self._timed(clock, "generate", self.gen.draw_device, seed, start, count, features=False, prior=True,
+                            momenta=False)
             s0 = self._timed(clock, "prior", self.surrogate_weights, d, count, False)
             self._timed(clock, "stats", self.sel.stats, s0, count, hist)
             for k in ("pass", "logp"):
                 d[k].free()
-            staged.append((d["ext"], d["feat"], s0, count, start))
+            staged.append((s0, count, start))
         h = hist.download(BINS, np.uint32)
         hist.free()
         s_thr = quantile_threshold(h, q)
-        lift = math.exp(self.std * clamp)
+        lift = math.exp(self.std * clamp) if network else 1.0  # without the network, s = P e^mu exactly
         ids, s1, w1 = [], [], []
         n_env = n_net = 0
-        for ext, feat, s0, count, start in staged:
+        for s0, count, start in staged:
             e_idx, e_val, m = self._timed(clock, "envelope", self.sel.select, count, w=s0, first=start,
                                           threshold=s_thr / lift, seed=accept_seed, values=True)
             n_env += m
@@ -365,30 +369,39 @@ class Unweighting:
             base = e_val.download(m)
             keep = np.zeros(m, dtype=bool)
             s = np.zeros(m)
+            ext = None
             if m:
+                # the envelope's points again, with momenta and features (padded to whole tiles)
                 pad = (m + 63) // 64 * 64
-                pidx = np.concatenate([idx, np.repeat(idx[-1:], pad - m)])
-                e1, f2 = self.gen.gather(ext, feat, count, pidx, rows=1)
-                y = self.dev.empty(pad, np.float32)
-                self.mlp.forward(f2, pad, y)
-                nn = y.download(pad, np.float32)[:m].astype(np.float64)
-                for a in (e1, f2, y):
+                pidx = self.dev.array(np.concatenate([idx, np.repeat(idx[-1:], pad - m)]))
+                d = self.gen.draw_device(seed, start, pad, features=network, prior=False, index=pidx)
+                ext = d["ext"]
+                if network:
+                    y = self.dev.empty(pad, np.float32)
+                    self.mlp.forward(d["feat"], pad, y)
+                    nn = y.download(pad, np.float32)[:m].astype(np.float64)
+                    for a in (d["feat"], y):
+                        a.free()
+                    n_net += pad
+                    s = base * np.exp(self.std * np.clip(nn, -clamp, clamp))
+                    keep = ps.uniform_at(accept_seed, start + idx.astype(np.uint64)) * s_thr < s
+                else:
+                    s, keep = base, np.ones(m, dtype=bool)
+                for a in (pidx, d["pass"]):
                     a.free()
-                n_net += pad
-                s = base * np.exp(self.std * np.clip(nn, -clamp, clamp))
-                keep = ps.uniform_at(accept_seed, start + idx.astype(np.uint64)) * s_thr < s
             self.dev.synchronize()
             clock["network"] = clock.get("network", 0.0) + time.perf_counter() - t
-            k_idx = idx[keep]
-            n = len(k_idx)
-            e2, _ = self._timed(clock, "gather", self.gen.gather, ext, None, count, k_idx)
+            k_pos = np.nonzero(keep)[0]
+            n = len(k_pos)
+            e2, _ = self._timed(clock, "gather", self.gen.gather, ext, None, pad if m else 1, k_pos) if ext is not None \
+                else (self.dev.empty(self.gen.legs * 4), None)
             w = self._timed(clock, "exact", self.exact, e2, n)
             t = time.perf_counter()
-            ids.append(k_idx.astype(np.int64) + start)
+            ids.append(idx[keep].astype(np.int64) + start)
             s1.append(s[keep])
             w1.append(w.download(n))
             clock["stage2"] = clock.get("stage2", 0.0) + time.perf_counter() - t
-            for a in (e_idx, e_val, e2, w, ext, feat, s0):
+            for a in (e_idx, e_val, e2, w, s0) + ((ext,) if ext is not None else ()):
                 a.free()
         t = time.perf_counter()
         ids, s1, w1 = np.concatenate(ids), np.concatenate(s1), np.concatenate(w1)
@@ -398,7 +411,9 @@ class Unweighting:
         clock["stage2"] = clock.get("stage2", 0.0) + time.perf_counter() - t
         x = s_thr * r_thr * np.maximum(1.0, s1[acc] / s_thr) * np.maximum(1.0, r[acc] / r_thr)
         extra = {"stage1_ids": ids, "stage1_s": s1} if keep_ids else {}
-        return {**extra, "candidates": candidates, "envelope_accepted": n_env, "network_evaluations": n_net,
+        if network:
+            extra["envelope_accepted"] = n_env
+        return {**extra, "candidates": candidates, "network_evaluations": n_net,
                 "stage1_accepted": len(ids), "exact_evaluations": len(ids),
                 "events": int(acc.sum()), "overweight_events": int(np.sum((s1[acc] > s_thr) | (r[acc] > r_thr))),
                 "threshold": s_thr, "r_threshold": r_thr, "clamp": clamp,
@@ -440,10 +455,10 @@ class Unweighting:
                     v = "network_dense"
                 else:
                     continue
-            if v == "network":
-                t = self.two_stage_envelope(candidates, seed, chunk, q, q2)
+            if v in ("network", "prior"):
+                t = self.two_stage_envelope(candidates, seed, chunk, q, q2, network=(v == "network"))
             else:
-                t = self.two_stage(candidates, seed, chunk, q, q2, network=(v == "network_dense"))
+                t = self.two_stage(candidates, seed, chunk, q, q2, network=True)
             t["events_per_s"] = t["events"] / t["seconds"]
             t["gain"] = t["events_per_s"] / rate_p if rate_p else None
             sig, err = t["estimate"]

@@ -214,7 +214,7 @@ def test_fused_launch_shapes_bitwise(lib, gg_ggg, kernel):
     d_ext.upload(ext)
     d_tot = DeviceArray(lib, points * 8)
     _, want = refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), ext[:, :points], points, kernel=k, resolved=False)
-    for wpc, block in [(8, 256), (12, 256), (16, 256), (16, 64), (24, 64), (32, 1024)]:
+    for wpc, block in [(8, 256), (12, 256), (16, 256), (16, 64), (24, 64), (32, 256)]:
         print("fused", kernel, "waves per CU", wpc, "block", block, flush=True)
         fused_evaluate(lib, [(fp, d_ext, points, ext.shape[1], None, d_tot)], k, waves_per_cu=wpc, block=block)
         got = d_tot.download(points)
