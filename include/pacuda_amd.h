@@ -64,6 +64,19 @@ typedef struct {
     uint32_t no_sort;                /* 1: the contribution rows in plan order */
 } pacuda_plan_desc;
 
+/*
+ * The instruction stream the fused path runs for a plan (host only: validation, the program and
+ * the fused path's passes, no device work). variant 0: the plain program (arena compaction only;
+ * the df64 kernel runs it), 1: with forwarding through the accumulators, 2 + v: with LDS variant
+ * v (10, 13, 20 planes; plane indices >= 2^24 are LDS slots). Each instruction is 8 u32: w[0] =
+ * kind | op << 8 | flags << 16 (kinds 1 clear, 2 contribution, 3 propagator, 4 identity, 5
+ * closure, 6 source; flags 1 init, 2 first, 4 last, 8 hold, 16 from the accumulators). out (may
+ * be NULL) receives up to `capacity` instructions; *count their number, *sources the leading
+ * source rows, *planes the slot's current planes, *lds_planes the LDS slots used. For tests
+ * and inspection; no reference counterpart.
+ */
+int pacuda_plan_program(const pacuda_plan_desc *desc, uint32_t variant, uint32_t *out, uint32_t capacity,
+                        uint32_t *count, uint32_t *sources, uint32_t *planes, uint32_t *lds_planes);
 /* 0 OK, 2 invalid description, 3 device error */
 int pacuda_plan_create(const pacuda_plan_desc *desc, pacuda_plan **plan);
 void pacuda_plan_destroy(pacuda_plan *plan);

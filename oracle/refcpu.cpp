@@ -480,7 +480,10 @@ int source_values(uint32_t code, double mass, const double *p, W *o) {
 // ---- rows: decode, sort, program (runner/recurrence_runner.cu:914-1054) -------------------
 
 enum { K_CLEAR = 1, K_CONTRIB, K_PROP, K_IDENTITY, K_CLOSURE, K_SOURCE, K_SYMJIT };
-enum { F_INIT = 1, F_FIRST = 2, F_LAST = 4 };
+// F_HOLD / F_FROMACC: forwarding through the accumulators, as the library's fused path emits it
+// (a run keeps its result for the next in-place row, which takes it instead of loading); the
+// reference's programs never carry them. Used by refcpu_evaluate_program.
+enum { F_INIT = 1, F_FIRST = 2, F_LAST = 4, F_HOLD = 8, F_FROMACC = 16 };
 
 struct Row {
     int op;
@@ -672,7 +675,7 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
                     else if (o.flags & F_INIT) ac[k] = c;
                     else ac[k] = add(ld_cur<R>(a, (dst + k) * a.ld + i), c);
                 }
-                if (o.flags & F_LAST)
+                if ((o.flags & F_LAST) && !(o.flags & F_HOLD))
                     for (int k = 0; k < cd; k++) st_cur<R>(a, (dst + k) * a.ld + i, ac[k]);
             }
         } else if (o.kind == K_CLEAR) {
@@ -692,16 +695,23 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
                 }
                 R reciprocal = ToReal<R, Den>::of(reciprocal_minkowski_norm(q[0], q[1], q[2], q[3]));
                 C c[4], v[4];
-                for (int k = 0; k < m; k++) c[k] = ld_cur<R>(a, (base + k) * a.ld + i);
+                C *ac = &acc[6 * (i - i0)];
+                for (int k = 0; k < m; k++) c[k] = (o.flags & F_FROMACC) ? ac[k] : ld_cur<R>(a, (base + k) * a.ld + i);
                 propagator_values(o.op, c, p, reciprocal, rf, v);
-                for (int k = 0; k < m; k++) st_cur<R>(a, (base + k) * a.ld + i, v[k]);
+                for (int k = 0; k < m; k++) {
+                    if (o.flags & F_HOLD) ac[k] = v[k];
+                    else st_cur<R>(a, (base + k) * a.ld + i, v[k]);
+                }
             }
         } else if (o.kind == K_IDENTITY) {
             R fr = ldr<R>(a.fre, o.f[2]), fi = ldr<R>(a.fim, o.f[2]);
             for (uint32_t k = 0; k < o.f[1]; k++)
                 for (size_t i = i0; i < i1; i++) {
                     size_t at = (o.f[0] + k) * a.ld + i;
-                    st_cur<R>(a, at, identity_value(fr, fi, ld_cur<R>(a, at)));
+                    C *ac = &acc[6 * (i - i0)];
+                    C v = identity_value(fr, fi, (o.flags & F_FROMACC) ? ac[k] : ld_cur<R>(a, at));
+                    if (o.flags & F_HOLD) ac[k] = v;
+                    else st_cur<R>(a, at, v);
                 }
         } else if (o.kind == K_CLOSURE) {
             uint32_t p0 = o.f[0], p1 = o.f[1], amp = o.f[2], x = o.f[3], first = o.f[4], cnt = o.f[5];
@@ -1121,22 +1131,17 @@ int refcpu_test_sources(const uint32_t *codes, const double *masses, const doubl
 // v[1] + ... . `resolved` ([point][output]) and `total` (per point) may each be NULL. Every
 // plane the program reads is written by it first (sources, clears, initialising rows); amplitude
 // planes start at zero. Points are processed in blocks of `block`, `threads` OpenMP threads.
-int refcpu_evaluate(uint32_t kernel, const uint64_t *words, uint32_t word_count, const uint8_t *width,
-                    uint32_t current_planes, uint32_t amplitude_planes, uint32_t form_count,
-                    const double *scale_re, const double *scale_im, uint32_t scale_count,
-                    const double *factor_re, const double *factor_im, uint32_t factor_count,
-                    uint32_t external_count, const uint32_t *form_terms, const uint32_t *term_slots,
-                    const double *term_coefficients, uint32_t outputs, const uint32_t *offsets,
-                    const uint32_t *planes, const double *weights, double numerator, double denominator,
-                    const double *external, uint32_t points, double *resolved, double *total,
-                    uint32_t block, int threads) {
-    (void)external_count;
-    if (kernel > KERNEL_MIXED64 || block == 0) return PACMAN_RECURRENCE_INVALID;
-    std::vector<Row> rows = decode(words, word_count, width);
-    sort_contribution_runs(rows);
-    std::vector<Op> ops;
-    uint32_t sources = 0;
-    if (!program(rows, ops, &sources)) return PACMAN_RECURRENCE_DEVICE_ERROR;
+namespace {
+
+// The whole plan run of an executable program per point (forms, sources, program, reduction,
+// helicity sum), in blocks of `block` points, OpenMP over blocks.
+int evaluate_ops(uint32_t kernel, const std::vector<Op> &ops, uint32_t sources, uint32_t current_planes,
+                 uint32_t amplitude_planes, uint32_t form_count, const double *scale_re, const double *scale_im,
+                 uint32_t scale_count, const double *factor_re, const double *factor_im, uint32_t factor_count,
+                 const uint32_t *form_terms, const uint32_t *term_slots, const double *term_coefficients,
+                 uint32_t outputs, const uint32_t *offsets, const uint32_t *planes, const double *weights,
+                 double numerator, double denominator, const double *external, uint32_t points, double *resolved,
+                 double *total, uint32_t block, int threads) {
     bool df = kernel != KERNEL_F64, df_forms = kernel == KERNEL_DF64 || kernel == KERNEL_MIXED;
     bool mixed = kernel == KERNEL_MIXED || kernel == KERNEL_MIXED64;
     // constants: f64 for the sources, df64 layout for the program when df
@@ -1202,6 +1207,77 @@ int refcpu_evaluate(uint32_t kernel, const uint64_t *words, uint32_t word_count,
         }
     }
     return PACMAN_RECURRENCE_OK;
+}
+
+}  // namespace
+
+int refcpu_evaluate(uint32_t kernel, const uint64_t *words, uint32_t word_count, const uint8_t *width,
+                    uint32_t current_planes, uint32_t amplitude_planes, uint32_t form_count,
+                    const double *scale_re, const double *scale_im, uint32_t scale_count,
+                    const double *factor_re, const double *factor_im, uint32_t factor_count,
+                    uint32_t external_count, const uint32_t *form_terms, const uint32_t *term_slots,
+                    const double *term_coefficients, uint32_t outputs, const uint32_t *offsets,
+                    const uint32_t *planes, const double *weights, double numerator, double denominator,
+                    const double *external, uint32_t points, double *resolved, double *total,
+                    uint32_t block, int threads) {
+    (void)external_count;
+    if (kernel > KERNEL_MIXED64 || block == 0) return PACMAN_RECURRENCE_INVALID;
+    std::vector<Row> rows = decode(words, word_count, width);
+    sort_contribution_runs(rows);
+    std::vector<Op> ops;
+    uint32_t sources = 0;
+    if (!program(rows, ops, &sources)) return PACMAN_RECURRENCE_DEVICE_ERROR;
+    return evaluate_ops(kernel, ops, sources, current_planes, amplitude_planes, form_count, scale_re, scale_im,
+                        scale_count, factor_re, factor_im, factor_count, form_terms, term_slots, term_coefficients,
+                        outputs, offsets, planes, weights, numerator, denominator, external, points, resolved, total,
+                        block, threads);
+}
+
+// The same plan run from an instruction stream of the library's fused path (pacuda_plan_program:
+// 8 u32 per instruction, w[0] = kind | op << 8 | flags << 16, the operands in w[1..7]); current
+// plane indices >= 2^24 (LDS slots) are mapped after the slot's `slot_planes` planes. Lets the
+// tests check, on the CPU, that the library's program passes leave every value unchanged.
+int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t count, uint32_t sources,
+                            uint32_t slot_planes, uint32_t lds_planes, uint32_t amplitude_planes, uint32_t form_count,
+                            const double *scale_re, const double *scale_im, uint32_t scale_count,
+                            const double *factor_re, const double *factor_im, uint32_t factor_count,
+                            const uint32_t *form_terms, const uint32_t *term_slots, const double *term_coefficients,
+                            uint32_t outputs, const uint32_t *offsets, const uint32_t *planes, const double *weights,
+                            double numerator, double denominator, const double *external, uint32_t points,
+                            double *resolved, double *total, uint32_t block, int threads) {
+    if (kernel > KERNEL_MIXED64 || block == 0 || !instr) return PACMAN_RECURRENCE_INVALID;
+    const uint32_t LDS = 1u << 24;
+    auto plane = [&](uint32_t p) { return p >= LDS ? slot_planes + (p - LDS) : p; };
+    std::vector<Op> ops(count);
+    for (uint32_t n = 0; n < count; n++) {
+        const uint32_t *w = instr + 8 * (size_t)n;
+        Op &o = ops[n];
+        memset(&o, 0, sizeof o);
+        o.kind = (int)(w[0] & 0xff);
+        o.op = (int)((w[0] >> 8) & 0xff);
+        o.flags = (int)(w[0] >> 16);
+        switch (o.kind) {
+        case K_CONTRIB:
+            o.f[0] = (o.flags & F_INIT) ? 1 : 0;
+            o.f[1] = w[1];
+            o.f[2] = plane(w[2]); o.f[3] = plane(w[3]); o.f[4] = plane(w[4]);
+            o.f[5] = w[5]; o.f[6] = w[6]; o.f[7] = w[7];
+            break;
+        case K_CLEAR: o.f[0] = plane(w[1]); o.f[1] = w[2]; break;
+        case K_PROP: o.f[0] = plane(w[1]); o.f[1] = w[2]; o.f[2] = w[3]; break;
+        case K_IDENTITY: o.f[0] = plane(w[1]); o.f[1] = w[2]; o.f[2] = w[3]; break;
+        case K_CLOSURE:
+            o.f[0] = plane(w[1]); o.f[1] = plane(w[2]);
+            o.f[2] = w[3]; o.f[3] = w[4]; o.f[4] = w[5]; o.f[5] = w[6];
+            break;
+        case K_SOURCE: o.f[0] = plane(w[1]); o.f[1] = w[2]; o.f[2] = w[3]; o.f[3] = w[4]; o.f[4] = w[5]; break;
+        default: return PACMAN_RECURRENCE_INVALID;
+        }
+    }
+    return evaluate_ops(kernel, ops, sources, slot_planes + lds_planes, amplitude_planes, form_count, scale_re, scale_im,
+                        scale_count, factor_re, factor_im, factor_count, form_terms, term_slots, term_coefficients,
+                        outputs, offsets, planes, weights, numerator, denominator, external, points, resolved, total,
+                        block, threads);
 }
 
 #ifdef REFCPU_ABI_NAMES

@@ -32,6 +32,12 @@ def load():
             f64p, f64p, C.c_uint32, f64p, f64p, C.c_uint32,
             C.c_uint32, u32p, u32p, f64p, C.c_uint32, u32p, u32p, f64p, C.c_double, C.c_double,
             f64p, C.c_uint32, f64p, f64p, C.c_uint32, C.c_int]
+        lib.refcpu_evaluate_program.restype = C.c_int
+        lib.refcpu_evaluate_program.argtypes = [
+            C.c_uint32, u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+            f64p, f64p, C.c_uint32, f64p, f64p, C.c_uint32,
+            u32p, u32p, f64p, C.c_uint32, u32p, u32p, f64p, C.c_double, C.c_double,
+            f64p, C.c_uint32, f64p, f64p, C.c_uint32, C.c_int]
         lib.refcpu_test_sources.restype = C.c_int
         lib.refcpu_test_sources.argtypes = [u32p, f64p, f64p, C.c_uint32, f64p, f64p]
         lib.refcpu_reset.restype = None
@@ -65,6 +71,28 @@ def evaluate(plan_arrays, counts, external, points, kernel=1, block=64, threads=
         None if res is None else _p(res, f64p), None if tot is None else _p(tot, f64p), block, threads)
     if st != 0:
         raise RuntimeError(f"refcpu_evaluate: status {st}")
+    return res, tot
+
+
+def evaluate_program(instr, sources, slot_planes, lds_planes, plan_arrays, counts, external, points, kernel=1,
+                     block=64, threads=1, numerator=1.0, denominator=1.0):
+    """The whole plan run per point from an instruction stream of the library's fused path
+    (uint32 [count][8]): (resolved [points][outputs], total [points])."""
+    lib = load()
+    a = plan_arrays
+    outputs = len(a["offsets"]) - 1
+    res, tot = np.zeros((points, outputs)), np.zeros(points)
+    ins = np.ascontiguousarray(instr, dtype=np.uint32)
+    ext = np.ascontiguousarray(external, dtype=np.float64)
+    st = lib.refcpu_evaluate_program(
+        kernel, _p(ins, u32p), len(ins), sources, slot_planes, lds_planes, counts["amplitude_planes"],
+        counts["form_count"], _p(a["scale_re"], f64p), _p(a["scale_im"], f64p), len(a["scale_re"]),
+        _p(a["factor_re"], f64p), _p(a["factor_im"], f64p), len(a["factor_re"]), _p(a["form_terms"], u32p),
+        _p(a["term_slots"], u32p), _p(a["term_coefficients"], f64p), outputs, _p(a["offsets"], u32p),
+        _p(a["planes"], u32p), _p(a["weights"], f64p), numerator, denominator, _p(ext, f64p), points,
+        _p(res, f64p), _p(tot, f64p), block, threads)
+    if st != 0:
+        raise RuntimeError(f"refcpu_evaluate_program: status {st}")
     return res, tot
 
 

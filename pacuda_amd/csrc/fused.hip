@@ -548,9 +548,16 @@ int compute_units() {
 
 }  // namespace
 
-extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) {
-    if (!out) return PACMAN_RECURRENCE_INVALID;
-    *out = nullptr;
+namespace {
+
+// The host half of plan creation: validation, the program, and the fused path's passes.
+struct Compiled {
+    uint32_t sources = 0, planes = 0;
+    std::vector<Instr> plain, fwd, lds[LDS_VARIANTS];
+    uint32_t lds_used[LDS_VARIANTS] = {0, 0, 0};
+};
+
+int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
     if (!d || d->abi_version != PACMAN_RECURRENCE_ABI || d->struct_size != sizeof(pacuda_plan_desc) || !d->words ||
         !d->offsets || (d->form_count && !d->form_terms) || (d->term_count && (!d->term_slots || !d->term_coefficients)) ||
         (d->scale_count && (!d->scale_re || !d->scale_im)) || (d->factor_count && (!d->factor_re || !d->factor_im)))
@@ -580,17 +587,46 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
         return PACMAN_RECURRENCE_INVALID;
     }
     // the slot keeps only the live currents
-    uint32_t planes = d->current_planes;
-    if (const uint32_t compacted = compact_planes(hp.ops, d->current_planes)) planes = compacted;
-    const std::vector<Instr> plain = hp.ops;
+    c.planes = d->current_planes;
+    if (const uint32_t compacted = compact_planes(hp.ops, d->current_planes)) c.planes = compacted;
+    c.sources = hp.sources;
+    c.plain = hp.ops;
     forward_in_place(hp.ops);
-    const std::vector<Instr> fwd = hp.ops;
-    std::vector<Instr> lds_ops[LDS_VARIANTS];
-    uint32_t lds_used[LDS_VARIANTS];
+    c.fwd = hp.ops;
     for (int v = 0; v < LDS_VARIANTS; v++) {
-        lds_ops[v] = fwd;
-        lds_used[v] = assign_lds(lds_ops[v], LDS_CAPS[v]);
+        c.lds[v] = c.fwd;
+        c.lds_used[v] = assign_lds(c.lds[v], LDS_CAPS[v]);
     }
+    return PACMAN_RECURRENCE_OK;
+}
+
+}  // namespace
+
+extern "C" int pacuda_plan_program(const pacuda_plan_desc *d, uint32_t variant, uint32_t *out, uint32_t capacity,
+                                   uint32_t *count, uint32_t *sources, uint32_t *planes, uint32_t *lds_planes) {
+    if (!count || variant >= 2 + LDS_VARIANTS) return PACMAN_RECURRENCE_INVALID;
+    Compiled c;
+    const int st = compile_plan(d, c);
+    if (st) return st;
+    const std::vector<Instr> &ops = variant == 0 ? c.plain : variant == 1 ? c.fwd : c.lds[variant - 2];
+    *count = (uint32_t)ops.size();
+    if (sources) *sources = c.sources;
+    if (planes) *planes = c.planes;
+    if (lds_planes) *lds_planes = variant >= 2 ? c.lds_used[variant - 2] : 0;
+    if (out) {
+        if (capacity < ops.size()) return PACMAN_RECURRENCE_INVALID;
+        memcpy(out, ops.data(), ops.size() * sizeof(Instr));
+    }
+    return PACMAN_RECURRENCE_OK;
+}
+
+extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) {
+    if (!out) return PACMAN_RECURRENCE_INVALID;
+    *out = nullptr;
+    Compiled cp;
+    if (const int st = compile_plan(d, cp)) return st;
+    const uint32_t count = d->offsets[d->outputs];
+    const uint32_t planes = cp.planes;
     pacuda_plan *p = new pacuda_plan();
     const uint32_t S = d->scale_count, X = d->factor_count;
     std::vector<double> c64(2 * (size_t)(S + X)), cdf;
@@ -607,14 +643,14 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     PlanDev &h = p->host;
     memset(&h, 0, sizeof h);
     for (int v = 0; v < LDS_VARIANTS; v++) {
-        h.prog_lds[v] = upload(p, lds_ops[v].data(), lds_ops[v].size(), &ok);
-        h.lds_planes[v] = lds_used[v];
+        h.prog_lds[v] = upload(p, cp.lds[v].data(), cp.lds[v].size(), &ok);
+        h.lds_planes[v] = cp.lds_used[v];
     }
-    h.count = (uint32_t)fwd.size();
-    h.prog_plain = upload(p, plain.data(), plain.size(), &ok);
-    h.count_plain = (uint32_t)plain.size();
-    h.prog_fwd = upload(p, fwd.data(), fwd.size(), &ok);
-    h.sources = hp.sources;
+    h.count = (uint32_t)cp.fwd.size();
+    h.prog_plain = upload(p, cp.plain.data(), cp.plain.size(), &ok);
+    h.count_plain = (uint32_t)cp.plain.size();
+    h.prog_fwd = upload(p, cp.fwd.data(), cp.fwd.size(), &ok);
+    h.sources = cp.sources;
     h.c64 = upload(p, c64.data(), c64.size(), &ok);
     h.cdf = upload(p, cdf.data(), cdf.size(), &ok);
     h.scale_count = S;

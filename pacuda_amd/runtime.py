@@ -72,6 +72,49 @@ class Device:
             raise RuntimeError("device synchronisation failed")
 
 
+def plan_desc(plan, arrays, no_sort=0):
+    """the pacuda_plan_desc of a plan (pacuda_amd.plans.Plan) over its arrays (kept by the caller)"""
+    a = arrays
+    d = abi.PlanDesc()
+    d.abi_version = abi.ABI
+    d.struct_size = C.sizeof(abi.PlanDesc)
+    d.words = ptr(a["words"], u64p)
+    d.word_count = len(a["words"])
+    for k in range(8):
+        d.width[k] = int(a["width"][k])
+    d.current_planes = plan.current_planes
+    d.amplitude_planes = plan.amplitude_planes
+    d.form_count = len(plan.forms)
+    d.scale_re, d.scale_im, d.scale_count = ptr(a["scale_re"]), ptr(a["scale_im"]), len(a["scale_re"])
+    d.factor_re, d.factor_im, d.factor_count = ptr(a["factor_re"]), ptr(a["factor_im"]), len(a["factor_re"])
+    d.external_count = plan.legs
+    d.term_count = len(a["term_slots"])
+    d.form_terms, d.term_slots = ptr(a["form_terms"], u32p), ptr(a["term_slots"], u32p)
+    d.term_coefficients = ptr(a["term_coefficients"])
+    d.outputs = len(a["offsets"]) - 1
+    d.offsets, d.planes, d.weights = ptr(a["offsets"], u32p), ptr(a["planes"], u32p), ptr(a["weights"])
+    d.numerator, d.denominator = float(plan.numerator), float(plan.denominator)
+    d.no_sort = no_sort
+    return d
+
+
+def plan_program(lib, plan, variant):
+    """the fused path's instruction stream of a plan (pacuda_plan_program, host only): (uint32
+    [count][8], leading sources, slot planes, LDS planes)"""
+    a = plan.arrays()
+    d = plan_desc(plan, a)
+    n, src, planes, lds = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+    st = lib.pacuda_plan_program(C.byref(d), variant, None, 0, C.byref(n), None, None, None)
+    if st != 0:
+        raise RuntimeError(f"pacuda_plan_program: status {st}")
+    out = np.zeros((n.value, 8), dtype=np.uint32)
+    st = lib.pacuda_plan_program(C.byref(d), variant, out.ctypes.data_as(u32p), n.value, C.byref(n), C.byref(src),
+                                 C.byref(planes), C.byref(lds))
+    if st != 0:
+        raise RuntimeError(f"pacuda_plan_program: status {st}")
+    return out, src.value, planes.value, lds.value
+
+
 class FusedPlan:
     """A plan (pacuda_amd.plans.Plan) compiled into the library (pacuda_plan_create)."""
 
@@ -80,26 +123,7 @@ class FusedPlan:
         self.plan = plan
         a = plan.arrays()
         self.arrays = a
-        d = abi.PlanDesc()
-        d.abi_version = abi.ABI
-        d.struct_size = C.sizeof(abi.PlanDesc)
-        d.words = ptr(a["words"], u64p)
-        d.word_count = len(a["words"])
-        for k in range(8):
-            d.width[k] = int(a["width"][k])
-        d.current_planes = plan.current_planes
-        d.amplitude_planes = plan.amplitude_planes
-        d.form_count = len(plan.forms)
-        d.scale_re, d.scale_im, d.scale_count = ptr(a["scale_re"]), ptr(a["scale_im"]), len(a["scale_re"])
-        d.factor_re, d.factor_im, d.factor_count = ptr(a["factor_re"]), ptr(a["factor_im"]), len(a["factor_re"])
-        d.external_count = plan.legs
-        d.term_count = len(a["term_slots"])
-        d.form_terms, d.term_slots = ptr(a["form_terms"], u32p), ptr(a["term_slots"], u32p)
-        d.term_coefficients = ptr(a["term_coefficients"])
-        d.outputs = len(a["offsets"]) - 1
-        d.offsets, d.planes, d.weights = ptr(a["offsets"], u32p), ptr(a["planes"], u32p), ptr(a["weights"])
-        d.numerator, d.denominator = float(plan.numerator), float(plan.denominator)
-        d.no_sort = no_sort
+        d = plan_desc(plan, a, no_sort)
         h = C.c_void_p()
         st = lib.pacuda_plan_create(C.byref(d), C.byref(h))
         if st != 0:

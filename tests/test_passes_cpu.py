@@ -1,0 +1,74 @@
+# SPDX-License-Identifier: 0BSD
+"""The fused path's program passes, checked on the CPU (no GPU): the library compiles a plan
+host-side (pacuda_plan_program) into the programs its fused kernels run — arena compaction only
+(the df64 kernel), with forwarding through the accumulators, and with each LDS-residency variant
+— and the oracle runs each instruction stream (refcpu_evaluate_program, LDS slots mapped to
+extra planes) beside its own plan run from the words. Every value must be equal bit for bit, in
+the f64 and the mixed policy, for 4–7 gluon plans, quark-line plans and every p p > j j j
+subprocess plan; and the passes must have done their work (fewer planes, forwarded rows, LDS
+slots)."""
+import numpy as np
+import pytest
+
+from pacuda_amd import abi, plans, processes, phase_space as ps
+from pacuda_amd.runtime import plan_program
+import harness
+import refcpu
+
+LDS_PLANE = 1 << 24
+F_HOLD, F_FROMACC = 8, 16
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return abi.load()
+
+
+def _plans():
+    out = [(f"{n} gluons", plans.gluon_plan(n)) for n in (4, 5, 6, 7)]
+    out += [(f"quark line, {n} legs", plans.quark_line_plan(n)) for n in (4, 5, 6)]
+    out += [(sp.name, sp.plan) for sp in processes.subprocesses()]
+    return out
+
+
+def _check(lib, name, plan, points, seed, kernels):
+    mom, _ = ps.generate(plan.legs, points, seed)
+    ext = ps.external_planes(mom, plan.crossing)
+    counts = harness.plan_counts(plan)
+    arrays = plan.arrays()
+    for kernel in kernels:
+        want_res, want_tot = refcpu.evaluate(arrays, counts, ext, points, kernel=kernel)
+        for variant in range(5):
+            instr, sources, planes, lds = plan_program(lib, plan, variant)
+            if variant == 0:
+                assert not np.any((instr[:, 0] >> 16) & (F_HOLD | F_FROMACC)), name
+            if variant < 2:
+                assert lds == 0
+            res, tot = refcpu.evaluate_program(instr, sources, planes, lds, arrays, counts, ext, points,
+                                               kernel=kernel)
+            assert harness.bits_equal(res, want_res), f"{name}: variant {variant}, kernel {kernel}"
+            assert harness.bits_equal(tot, want_tot), f"{name}: variant {variant}, kernel {kernel}"
+
+
+@pytest.mark.parametrize("name,plan", _plans()[:7], ids=[n for n, _ in _plans()[:7]])
+def test_passes_keep_every_value(lib, name, plan):
+    _check(lib, name, plan, 96 if plan.legs < 7 else 40, 21, (abi.KERNEL_F64, abi.KERNEL_MIXED))
+
+
+def test_passes_keep_every_value_pp_jjj(lib):
+    for k, (name, plan) in enumerate(_plans()[7:]):
+        _check(lib, name, plan, 40, 40 + k, (abi.KERNEL_F64,))
+
+
+def test_passes_do_their_work(lib):
+    plan = plans.gluon_plan(5)
+    plain, _, planes, _ = plan_program(lib, plan, 0)
+    assert planes < plan.current_planes  # compaction: 384 -> 278
+    fwd, _, _, _ = plan_program(lib, plan, 1)
+    assert len(fwd) == len(plain)
+    moved = np.count_nonzero((fwd[:, 0] >> 16) & F_FROMACC)
+    assert moved == np.count_nonzero((fwd[:, 0] >> 16) & F_HOLD) > 0
+    for variant, cap in zip((2, 3, 4), (10, 13, 20)):
+        prog, _, _, lds = plan_program(lib, plan, variant)
+        assert 0 < lds <= cap
+        assert np.any(prog[:, 1:5] >= LDS_PLANE)
