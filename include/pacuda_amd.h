@@ -66,8 +66,8 @@ typedef struct {
 
 /*
  * The instruction stream the fused path runs for a plan (host only: validation, the program and
- * the fused path's passes, no device work). variant 0: the plain program (arena compaction only;
- * the df64 kernel runs it), 1: with forwarding through the accumulators, 2 + v: with LDS variant
+ * the fused path's passes, no device work). variant 0: the program with arena compaction only,
+ * 1: with forwarding through the accumulators, 2 + v: with LDS variant
  * v (10, 13, 20 planes; plane indices >= 2^24 are LDS slots). Each instruction is 8 u32: w[0] =
  * kind | op << 8 | flags << 16 (kinds 1 clear, 2 contribution, 3 propagator, 4 identity, 5
  * closure, 6 source; flags 1 init, 2 first, 4 last, 8 hold, 16 from the accumulators). out (may

@@ -25,7 +25,7 @@ namespace pacuda {
 // (runner/recurrence_runner.cu:72-126).
 struct D {
     float hi, lo;
-    PCD D() {}
+    D() = default;
     PCD D(float h, float l) : hi(h), lo(l) {}
     PCD explicit D(double x) : hi((float)x), lo(0.0f) {}  // exact constants only (0, +-1, 2)
 };
@@ -136,10 +136,10 @@ constexpr uint32_t LDS_PLANE = 1u << 24;
 
 // Precision policies: R the values, Den the propagator denominators and their forms, Amp the
 // amplitude accumulation (runner/recurrence_runner.cu:178-190).
-// forwarding: whether the kernel honours F_HOLD / F_FROMACC (the df64 kernel does not: with them
-// its accumulators no longer stay in registers, so it runs the program without forwarding)
+// forwarding: whether the kernel honours F_HOLD / F_FROMACC (every kernel does; a kernel without
+// it would run the plan's program without forwarding and LDS residency)
 struct KF64 { typedef double real; typedef double den; typedef double amp; static constexpr bool forwarding = true; };
-struct KDF64 { typedef D real; typedef D den; typedef D amp; static constexpr bool forwarding = false; };
+struct KDF64 { typedef D real; typedef D den; typedef D amp; static constexpr bool forwarding = true; };
 struct KMixed { typedef float real; typedef D den; typedef D amp; static constexpr bool forwarding = true; };
 struct KMixed64 { typedef float real; typedef double den; typedef D amp; static constexpr bool forwarding = true; };
 

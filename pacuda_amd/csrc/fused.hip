@@ -706,18 +706,15 @@ struct Schedule {
     bool lds_all[LDS_VARIANTS] = {true, true, true};  // every plan has the variant
 };
 
-// The LDS variant of a launch, its LDS elements per wavefront and bytes per workgroup: the f64
-// kernel keeps double currents, the mixed kernels float ones, the df64 kernel none (it runs the
-// plain program). A variant qualifies when every plan has it and the workgroups a CU must hold
-// for the requested wavefronts fit in its LDS; the largest one is taken, else none (per_wave 0).
+// The LDS variant of a launch, its LDS elements per wavefront and bytes per workgroup. A variant
+// qualifies when every plan has it and the workgroups a CU must hold for the requested
+// wavefronts fit in its LDS; the largest one is taken, else none (per_wave 0).
 void lds_shape(uint32_t kernel, const Schedule &sc, uint32_t *per_wave, size_t *bytes, uint32_t *variant) {
     *per_wave = 0;
     *bytes = 0;
     *variant = 0;
-    const size_t elem = kernel == PACMAN_RECURRENCE_KERNEL_F64 ? 8
-                        : (kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) ? 4
-                                                                                                                : 0;
-    if (!elem) return;
+    // current bytes per plane and lane: f64 and df64 (a float pair) 8, the mixed kernels' f32 4
+    const size_t elem = (kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) ? 4 : 8;
     const uint32_t waves_per_block = sc.block / LANES;
     const uint32_t blocks_per_cu = (sc.per_cu + waves_per_block - 1) / waves_per_block;
     for (int v = LDS_VARIANTS - 1; v >= 0; v--) {
@@ -796,7 +793,8 @@ bool lds_ready(uint32_t kernel, const Schedule &sc) {
     if (!bytes) return true;
     if (kernel == PACMAN_RECURRENCE_KERNEL_F64) return allow_lds<KF64>();
     if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED) return allow_lds<KMixed>();
-    return allow_lds<KMixed64>();
+    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) return allow_lds<KMixed64>();
+    return allow_lds<KDF64>();
 }
 
 bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
@@ -822,7 +820,9 @@ bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const Item
                                                                   per_wave, variant);
         break;
     default:
-        fused_kernel<KDF64><<<sc.grid, sc.block, 0, s>>>(items, starts, count, counter, scratch, sc.slot, 0, 0);
+        if (bytes && !allow_lds<KDF64>()) return false;
+        fused_kernel<KDF64><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
+                                                               variant);
         break;
     }
     return true;

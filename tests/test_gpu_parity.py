@@ -202,7 +202,7 @@ def test_fused_colour_orderings_match_oracle(lib, gg_ggg):
     fp.destroy()
 
 
-@pytest.mark.parametrize("kernel", ["f64", "mixed"])
+@pytest.mark.parametrize("kernel", ["f64", "df64", "mixed"])
 def test_fused_launch_shapes_bitwise(lib, gg_ggg, kernel):
     """every LDS variant (20, 13 and 10 LDS-resident planes per wavefront, picked from the
     requested occupancy and block) and the launch without LDS residency give the oracle's bits"""

@@ -1,10 +1,10 @@
 # SPDX-License-Identifier: 0BSD
 """The fused path's program passes, checked on the CPU (no GPU): the library compiles a plan
-host-side (pacuda_plan_program) into the programs its fused kernels run — arena compaction only
-(the df64 kernel), with forwarding through the accumulators, and with each LDS-residency variant
+host-side (pacuda_plan_program) into its programs — arena compaction only, with forwarding
+through the accumulators, and with each LDS-residency variant (the ones the kernels run)
 — and the oracle runs each instruction stream (refcpu_evaluate_program, LDS slots mapped to
 extra planes) beside its own plan run from the words. Every value must be equal bit for bit, in
-the f64 and the mixed policy, for 4–7 gluon plans, quark-line plans and every p p > j j j
+the f64, df64 and mixed policies, for 4–7 gluon plans, quark-line plans and every p p > j j j
 subprocess plan; and the passes must have done their work (fewer planes, forwarded rows, LDS
 slots)."""
 import numpy as np
@@ -52,7 +52,8 @@ def _check(lib, name, plan, points, seed, kernels):
 
 @pytest.mark.parametrize("name,plan", _plans()[:7], ids=[n for n, _ in _plans()[:7]])
 def test_passes_keep_every_value(lib, name, plan):
-    _check(lib, name, plan, 96 if plan.legs < 7 else 40, 21, (abi.KERNEL_F64, abi.KERNEL_MIXED))
+    kernels = (abi.KERNEL_F64, abi.KERNEL_MIXED) + ((abi.KERNEL_DF64,) if plan.legs < 7 else ())
+    _check(lib, name, plan, 96 if plan.legs < 7 else 40, 21, kernels)
 
 
 def test_passes_keep_every_value_pp_jjj(lib):
