@@ -153,3 +153,55 @@ def test_fused_rejects_invalid_plans_and_launches(lib):
     assert lib.pacuda_evaluate(items, 1, C.byref(l)) == abi.INVALID  # no plan
     assert lib.pacuda_evaluate(items, 0, C.byref(l)) == abi.OK       # nothing to do
     del a
+
+
+def test_plan_compilation_rejects_bad_descriptions(lib):
+    """pacuda_plan_program (host only) refuses what pacuda_plan_create refuses, without a GPU:
+    out-of-range tables, decreasing offsets, reads of unwritten planes, symjit rows, a block of
+    1024 (the fused kernel's limit is 256), and a too-small output buffer"""
+    from pacuda_amd import plans
+    from pacuda_amd.runtime import plan_desc
+    plan = plans.gluon_plan(4)
+    n = C.c_uint32()
+
+    def compile_with(**changes):
+        a = {k: np.array(v, copy=True) for k, v in plan.arrays().items()}
+        for k, v in changes.items():
+            if k in a:
+                a[k] = v
+        d = plan_desc(plan, a)
+        for k, v in changes.items():
+            if k not in a:
+                setattr(d, k, v)
+        return lib.pacuda_plan_program(C.byref(d), 1, None, 0, C.byref(n), None, None, None), a
+
+    st, _ = compile_with()
+    assert st == abi.OK and n.value > 0
+    assert compile_with(current_planes=plan.current_planes - 1)[0] == abi.INVALID  # indexes past the planes
+    a = plan.arrays()
+    bad = np.array(a["planes"], copy=True)
+    bad[0] = plan.amplitude_planes
+    assert compile_with(planes=bad)[0] == abi.INVALID  # an output plane past the amplitudes
+    off = np.array(a["offsets"], copy=True)
+    off[1], off[2] = off[2], off[1]
+    assert compile_with(offsets=off)[0] == abi.INVALID  # decreasing offsets
+    slots = np.array(a["term_slots"], copy=True)
+    slots[0] = plan.legs
+    assert compile_with(term_slots=slots)[0] == abi.INVALID  # a form term past the legs
+    # a program that reads a plane before anything writes it: drop the source rows
+    words = np.array(a["words"], copy=True)
+    ops = words & np.uint64(15)
+    assert compile_with(words=words[ops != 12])[0] == abi.INVALID
+    # symjit rows are refused (no generated kernels in this build)
+    sym = np.concatenate([words[:1] & ~np.uint64(15) | np.uint64(14), words])
+    assert compile_with(words=sym)[0] == abi.INVALID
+    # variant out of range, missing count, too small a buffer
+    d = plan_desc(plan, a)
+    assert lib.pacuda_plan_program(C.byref(d), 5, None, 0, C.byref(n), None, None, None) == abi.INVALID
+    assert lib.pacuda_plan_program(C.byref(d), 1, None, 0, None, None, None, None) == abi.INVALID
+    small = np.zeros((1, 8), dtype=np.uint32)
+    assert lib.pacuda_plan_program(C.byref(d), 1, small.ctypes.data_as(abi.u32p), 1, C.byref(n), None, None,
+                                   None) == abi.INVALID
+    l = abi.Launch(abi.ABI, C.sizeof(abi.Launch), abi.KERNEL_F64, 0, 1024, 0, None, None)
+    items = (abi.Item * 1)()
+    assert lib.pacuda_evaluate(items, 1, C.byref(l)) == abi.INVALID
