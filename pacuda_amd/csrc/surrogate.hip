@@ -121,22 +121,35 @@ __global__ void __launch_bounds__(256) mlp_forward(FwdArgs a) {
         const bf16 *W = a.wb + a.wb_off[l];
         const float *bias = a.params + a.b_off[l];
         const uint32_t colbase = wave * cols_per_wave;
+        // constant loop bounds with predicates: the accumulators stay in registers
         f32x4 acc[4][4];
+#pragma unroll
         for (int i = 0; i < 4; i++)
+#pragma unroll
             for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
         for (uint32_t k0 = 0; k0 < K; k0 += 16) {
             const uint32_t kk = k0 + 4 * (lane >> 4);
             bf16x4 av[4], bv[4];
+#pragma unroll
             for (int i = 0; i < 4; i++)
                 av[i] = *reinterpret_cast<const bf16x4 *>(&buf[cur][lds_at(16 * i + (lane & 15), kk, stride)]);
-            for (uint32_t j = 0; j < col_blocks; j++)
-                bv[j] = *reinterpret_cast<const bf16x4 *>(&W[(size_t)(colbase + 16 * j + (lane & 15)) * K + kk]);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if ((uint32_t)j < col_blocks)
+                    bv[j] = *reinterpret_cast<const bf16x4 *>(&W[(size_t)(colbase + 16 * j + (lane & 15)) * K + kk]);
+#pragma unroll
             for (int i = 0; i < 4; i++)
-                for (uint32_t j = 0; j < col_blocks; j++) acc[i][j] = mfma(av[i], bv[j], acc[i][j]);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if ((uint32_t)j < col_blocks) acc[i][j] = mfma(av[i], bv[j], acc[i][j]);
         }
         const uint32_t nxt = cur ^ 1;
+#pragma unroll
         for (int i = 0; i < 4; i++)
-            for (uint32_t j = 0; j < col_blocks; j++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if ((uint32_t)j < col_blocks)
+#pragma unroll
                 for (int e = 0; e < 4; e++) {
                     const uint32_t r = 16 * i + 4 * (lane >> 4) + e, c = colbase + 16 * j + (lane & 15);
                     const float zv = acc[i][j][e] + bias[c];
@@ -344,19 +357,55 @@ __global__ void __launch_bounds__(64) gemm_nt(const bf16 *A, const bf16 *B, uint
     const uint32_t nb = std::min<uint32_t>(4, (N - tn) / 16);
     const uint32_t k_begin = blockIdx.z * k_split, k_end = std::min(K, k_begin + k_split);
     if (mode == 0) out += (size_t)blockIdx.z * M * N;
+    // constant loop bounds with predicates (the accumulators stay in registers); the operands of
+    // four k-steps are loaded before their products when the range allows
     f32x4 acc[4][4];
+#pragma unroll
     for (int i = 0; i < 4; i++)
+#pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (uint32_t k0 = k_begin; k0 < k_end; k0 += 16) {
+    uint32_t k0 = k_begin;
+    for (; k0 + 64 <= k_end; k0 += 64) {
+        bf16x4 av[4][4], bv[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t kk = k0 + 16 * q + 4 * (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                av[q][i] = *reinterpret_cast<const bf16x4 *>(&A[(size_t)(tm + 16 * i + (lane & 15)) * K + kk]);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if ((uint32_t)j < nb)
+                    bv[q][j] = *reinterpret_cast<const bf16x4 *>(&B[(size_t)(tn + 16 * j + (lane & 15)) * K + kk]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if ((uint32_t)j < nb) acc[i][j] = mfma(av[q][i], bv[q][j], acc[i][j]);
+    }
+    for (; k0 < k_end; k0 += 16) {
         const uint32_t kk = k0 + 4 * (lane >> 4);
         bf16x4 av[4], bv[4];
+#pragma unroll
         for (int i = 0; i < 4; i++) av[i] = *reinterpret_cast<const bf16x4 *>(&A[(size_t)(tm + 16 * i + (lane & 15)) * K + kk]);
-        for (uint32_t j = 0; j < nb; j++) bv[j] = *reinterpret_cast<const bf16x4 *>(&B[(size_t)(tn + 16 * j + (lane & 15)) * K + kk]);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if ((uint32_t)j < nb) bv[j] = *reinterpret_cast<const bf16x4 *>(&B[(size_t)(tn + 16 * j + (lane & 15)) * K + kk]);
+#pragma unroll
         for (int i = 0; i < 4; i++)
-            for (uint32_t j = 0; j < nb; j++) acc[i][j] = mfma(av[i], bv[j], acc[i][j]);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if ((uint32_t)j < nb) acc[i][j] = mfma(av[i], bv[j], acc[i][j]);
     }
+#pragma unroll
     for (int i = 0; i < 4; i++)
-        for (uint32_t j = 0; j < nb; j++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if ((uint32_t)j < nb)
+#pragma unroll
             for (int e = 0; e < 4; e++) {
                 const uint32_t r = tm + 16 * i + 4 * (lane >> 4) + e, c = tn + 16 * j + (lane & 15);
                 const float v = acc[i][j][e];
