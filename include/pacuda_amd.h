@@ -68,7 +68,7 @@ typedef struct {
  * The instruction stream the fused path runs for a plan (host only: validation, the program and
  * the fused path's passes, no device work). variant 0: the program with arena compaction only,
  * 1: with forwarding through the accumulators, 2 + v: with LDS variant
- * v (10, 13, 20 planes; plane indices >= 2^24 are LDS slots). Each instruction is 8 u32: w[0] =
+ * v (10, 13, 20, 26, 40 planes; plane indices >= 2^24 are LDS slots). Each instruction is 8 u32: w[0] =
  * kind | op << 8 | flags << 16 (kinds 1 clear, 2 contribution, 3 propagator, 4 identity, 5
  * closure, 6 source; flags 1 init, 2 first, 4 last, 8 hold, 16 from the accumulators). out (may
  * be NULL) receives up to `capacity` instructions; *count their number, *sources the leading
@@ -102,7 +102,7 @@ typedef struct {
     uint32_t abi_version;   /* PACMAN_RECURRENCE_ABI */
     uint32_t struct_size;   /* sizeof(pacuda_launch) */
     uint32_t kernel;        /* PACMAN_RECURRENCE_KERNEL_* */
-    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 12 (f64), 24 (mixed64), 16 (others) */
+    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 8 (f64), 16 (others) */
     uint32_t block;         /* threads per workgroup, a multiple of 64 up to 256, 0: 256 */
     uint32_t reserved;
     void *stream;           /* hipStream_t to queue on, or NULL: the library's stream */

@@ -6,8 +6,8 @@
 // Every function keeps the reference's operation order (runner/recurrence_runner.cu:72-542) so
 // the f64 kernel is bit-identical to RustiCol's arithmetic; the build passes -ffp-contract=off
 // and the pragma below repeats it, so no product or sum is fused unless an fma is written.
-// Each value of a row is a register of the lane that owns the point; nothing here touches LDS
-// or crosses lanes.
+// Each value of a row is a register of the lane that owns the point; nothing here crosses
+// lanes (the fused path's arena keeps some currents in the wavefront's LDS region).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -117,22 +117,21 @@ PCD void str(double *p, size_t at, double v) { p[at] = v; }
 PCD void str(double *p, size_t at, D v) { reinterpret_cast<float2 *>(p)[at] = make_float2(v.hi, v.lo); }
 PCD void str(double *p, size_t at, float v) { reinterpret_cast<float2 *>(p)[at] = make_float2(v, 0.0f); }
 
-// The arena as one lane sees it: value (plane q, point i) at q * ld + i of each array. The v1
-// path passes the caller's layout (ld = stride); the fused path passes one wave's slice of its
-// scratch (ld = 64, i = lane).
-struct Arena {
-    double *cre, *cim, *are, *aim;
-    float *fcre, *fcim;  // mixed kernels: the currents as f32 planes
-    const double *mom;
-    const double *sre, *sim, *fre, *fim;
-    size_t ld;
-    uint32_t lds_base;  // the wavefront's LDS region, in elements of the current type (fused path)
-};
+// ---- raw 8-byte values: a double, or a df64 pair / f32 value in a double's 8 bytes -----------
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef float flt2 __attribute__((ext_vector_type(2)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
-// Current planes at or above LDS_PLANE live in the wavefront's LDS region (slot plane - LDS_PLANE:
-// its real part at lds_base + 2 s ld, its imaginary part ld further); only the fused path's LDS
-// pass writes such indices, so v1 programs never take that branch.
-constexpr uint32_t LDS_PLANE = 1u << 24;
+template <class R> PCD R from_raw(double raw);
+template <> PCD double from_raw<double>(double raw) { return raw; }
+template <> PCD D from_raw<D>(double raw) {
+    const flt2 v = __builtin_bit_cast(flt2, raw);
+    return D(v.x, v.y);
+}
+template <> PCD float from_raw<float>(double raw) { return __builtin_bit_cast(flt2, raw).x; }
+PCD double to_raw(double v) { return v; }
+PCD double to_raw(D v) { return __builtin_bit_cast(double, flt2{v.hi, v.lo}); }
+PCD double to_raw(float v) { return __builtin_bit_cast(double, flt2{v, 0.0f}); }
 
 // Precision policies: R the values, Den the propagator denominators and their forms, Amp the
 // amplitude accumulation (runner/recurrence_runner.cu:178-190).
@@ -147,9 +146,11 @@ PCD float to_float(float x) { return x; }
 PCD float to_float(double x) { return (float)x; }
 PCD float to_float(D x) { return x.hi + x.lo; }
 
-template <class R, class Den> struct Form { static PCD R ld(const double *m, size_t at) { return ldr<R>(m, at); } };
-template <> struct Form<float, double> { static PCD float ld(const double *m, size_t at) { return (float)m[at]; } };
-template <> struct Form<float, D> { static PCD float ld(const double *m, size_t at) { return ldr<float>(m, at); } };
+// a form component as the kernel's R, from its raw value (f64, or df64 after the df64 kernels'
+// in-place conversion)
+template <class R, class Den> struct Form { static PCD R of(double raw) { return from_raw<R>(raw); } };
+template <> struct Form<float, double> { static PCD float of(double raw) { return (float)raw; } };
+template <> struct Form<float, D> { static PCD float of(double raw) { return from_raw<float>(raw); } };
 
 template <class R, class S> struct ToReal { static PCD R of(S x) { return x; } };
 template <class S> struct ToReal<float, S> { static PCD float of(S x) { return to_float(x); } };
@@ -166,50 +167,186 @@ template <> PCD D narrow<D>(double x) {
     return D(hi, (float)(x - (double)hi));
 }
 
+// ---- arenas ------------------------------------------------------------------------------------
+// The interpreter reaches its values through an arena: cur / curs / put / puts (current
+// components by plane), form (the four raw components of a momentum form), amp / put_amp
+// (amplitude planes), and the constant tables sre, sim, fre, fim.
+
+// The v1 path's arena: the reference's layout (runner/recurrence_runner.cu:850-876, 1083-1104),
+// value (plane q, point i) at q * ld + i of each array, ld = the caller's stride; the mixed
+// kernels' currents in separate f32 planes.
+struct Arena {
+    double *cre, *cim, *are, *aim;
+    float *fcre, *fcim;  // mixed kernels: the currents as f32 planes
+    const double *mom;
+    const double *sre, *sim, *fre, *fim;
+    size_t ld;
+};
+
 template <class R> struct Cur {
     static PCD Cx<R> ld(const Arena &a, size_t at) { return cx(ldr<R>(a.cre, at), ldr<R>(a.cim, at)); }
     static PCD void st(const Arena &a, size_t at, Cx<R> v) { str(a.cre, at, v.re); str(a.cim, at, v.im); }
-    // by plane index (slot or LDS) for lane i
-    static PCD Cx<R> ldp(const Arena &a, uint32_t plane, size_t i) {
-        if (plane >= LDS_PLANE) {
-            extern __shared__ double lds_currents[];
-            const size_t at = a.lds_base + (size_t)(plane - LDS_PLANE) * 2 * a.ld + i;
-            return cx(ldr<R>(lds_currents, at), ldr<R>(lds_currents, at + a.ld));
-        }
-        return ld(a, (size_t)plane * a.ld + i);
-    }
-    static PCD void stp(const Arena &a, uint32_t plane, size_t i, Cx<R> v) {
-        if (plane >= LDS_PLANE) {
-            extern __shared__ double lds_currents[];
-            const size_t at = a.lds_base + (size_t)(plane - LDS_PLANE) * 2 * a.ld + i;
-            str(lds_currents, at, v.re);
-            str(lds_currents, at + a.ld, v.im);
-            return;
-        }
-        st(a, (size_t)plane * a.ld + i, v);
-    }
 };
 template <> struct Cur<float> {
     static PCD Cx<float> ld(const Arena &a, size_t at) { return cx(a.fcre[at], a.fcim[at]); }
     static PCD void st(const Arena &a, size_t at, Cx<float> v) { a.fcre[at] = v.re; a.fcim[at] = v.im; }
-    static PCD Cx<float> ldp(const Arena &a, uint32_t plane, size_t i) {
-        if (plane >= LDS_PLANE) {
-            extern __shared__ float lds_currents_f32[];
-            const size_t at = a.lds_base + (size_t)(plane - LDS_PLANE) * 2 * a.ld + i;
-            return cx(lds_currents_f32[at], lds_currents_f32[at + a.ld]);
-        }
-        return ld(a, (size_t)plane * a.ld + i);
+};
+
+// one lane's view of the v1 arena: point i (v1 programs carry no forwarding flags)
+struct PlaneArena {
+    static constexpr bool forwarding = false;
+    Arena a;
+    size_t i;
+    template <class R> PCD Cx<R> cur(uint32_t p) const { return Cur<R>::ld(a, (size_t)p * a.ld + i); }
+    template <class R> PCD void put(uint32_t p, Cx<R> v) const { Cur<R>::st(a, (size_t)p * a.ld + i, v); }
+    template <class R, int N> PCD void curs(uint32_t p, Cx<R> *v) const {
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = cur<R>(p + k);
     }
-    static PCD void stp(const Arena &a, uint32_t plane, size_t i, Cx<float> v) {
-        if (plane >= LDS_PLANE) {
-            extern __shared__ float lds_currents_f32[];
-            const size_t at = a.lds_base + (size_t)(plane - LDS_PLANE) * 2 * a.ld + i;
-            lds_currents_f32[at] = v.re;
-            lds_currents_f32[at + a.ld] = v.im;
-            return;
-        }
-        st(a, (size_t)plane * a.ld + i, v);
+    template <class R, int N> PCD void puts(uint32_t p, const Cx<R> *v) const {
+#pragma unroll
+        for (int k = 0; k < N; k++) put<R>(p + k, v[k]);
     }
+    PCD void form(uint32_t f, double *raw) const {
+#pragma unroll
+        for (int k = 0; k < 4; k++) raw[k] = a.mom[((size_t)f * 4 + k) * a.ld + i];
+    }
+    template <class T> PCD Cx<T> amp(uint32_t q) const {
+        return cx(ldr<T>(a.are, (size_t)q * a.ld + i), ldr<T>(a.aim, (size_t)q * a.ld + i));
+    }
+    template <class T> PCD void put_amp(uint32_t q, Cx<T> v) const {
+        str(a.are, (size_t)q * a.ld + i, v.re);
+        str(a.aim, (size_t)q * a.ld + i, v.im);
+    }
+    PCD const double *sre() const { return a.sre; }
+    PCD const double *sim() const { return a.sim; }
+    PCD const double *fre() const { return a.fre; }
+    PCD const double *fim() const { return a.fim; }
+};
+
+// A complex current component as one vector: (re, im) doubles, (re.hi, re.lo, im.hi, im.lo) for
+// df64, (re, im) floats for the mixed kernels -- 16, 16 and 8 bytes per lane.
+template <class R> struct Packed;
+template <> struct Packed<double> {
+    typedef dbl2 T;
+    static PCD Cx<double> get(T v) { return cx(v.x, v.y); }
+    static PCD T set(Cx<double> c) { return T{c.re, c.im}; }
+};
+template <> struct Packed<D> {
+    typedef flt4 T;
+    static PCD Cx<D> get(T v) { return cx(D(v.x, v.y), D(v.z, v.w)); }
+    static PCD T set(Cx<D> c) { return T{c.re.hi, c.re.lo, c.im.hi, c.im.lo}; }
+};
+template <> struct Packed<float> {
+    typedef flt2 T;
+    static PCD Cx<float> get(T v) { return cx(v.x, v.y); }
+    static PCD T set(Cx<float> c) { return T{c.re, c.im}; }
+};
+
+// Current planes at or above LDS_PLANE live in the wavefront's LDS region (slot plane -
+// LDS_PLANE); only the fused path's LDS pass writes such indices.
+constexpr uint32_t LDS_PLANE = 1u << 24;
+
+// Raw buffer accesses: a wavefront-uniform resource (base, size), a per-lane byte offset
+// (VGPR) and a uniform byte offset (SGPR), so a lane's access needs no address arithmetic of
+// its own. Word 3 of the resource is the gfx9-family raw-buffer format (32-bit data format).
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int BUFFER_FORMAT = 0x00020000;
+template <int BYTES> struct Buffer;
+template <> struct Buffer<16> {
+    template <class T> static PCD T load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+    }
+    template <class T> static PCD void store(T x, __amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, x), r, v, s, 0);
+    }
+};
+template <> struct Buffer<8> {
+    template <class T> static PCD T load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+    }
+    template <class T> static PCD void store(T x, __amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, s, 0);
+    }
+};
+
+// The fused path's arena: one wavefront's scratch slot for 64 points, each plane 64 lanes of one
+// vector: currents as Packed<R> (PLANE bytes per plane), then amplitudes as (re, im) raw pairs
+// (1 KB per plane), then forms as four raw doubles (2 KB per form); LDS-resident currents as
+// Packed<R> in the wavefront's LDS region. A lane's access is one buffer load or store at
+// (lane offset, uniform plane offset). Every operand of a row is one whole object of the LDS
+// pass, so one residency test serves all its components.
+template <class R> struct SlotArena {
+    typedef typename Packed<R>::T V;
+    typedef Buffer<sizeof(V)> B;
+    static constexpr bool forwarding = true;
+    static constexpr uint32_t PLANE = 64 * sizeof(V);  // bytes per current plane
+    __amdgpu_buffer_rsrc_t slot;
+    uint32_t amp_base, form_base;  // byte offsets of the amplitude and form regions in the slot
+    const double *sre_, *sim_, *fre_, *fim_;
+    uint32_t idx;    // the lane (0-63)
+    uint32_t lds;    // byte address of the lane's value in LDS plane 0 of the wavefront's region
+
+    static PCD V *lds_at(uint32_t off) {
+        extern __shared__ __attribute__((aligned(16))) char pacuda_lds[];
+        return reinterpret_cast<V *>(pacuda_lds + off);
+    }
+    PCD V mem_ld(uint32_t p) const { return B::template load<V>(slot, idx * (uint32_t)sizeof(V), p * PLANE); }
+    PCD void mem_st(uint32_t p, V v) const { B::store(v, slot, idx * (uint32_t)sizeof(V), p * PLANE); }
+
+    template <class Q> PCD Cx<R> cur(uint32_t p) const {
+        static_assert(__is_same(Q, R), "one value type per slot");
+        if (p >= LDS_PLANE) return Packed<R>::get(*lds_at(lds + (p - LDS_PLANE) * PLANE));
+        return Packed<R>::get(mem_ld(p));
+    }
+    template <class Q> PCD void put(uint32_t p, Cx<R> v) const {
+        static_assert(__is_same(Q, R), "one value type per slot");
+        if (p >= LDS_PLANE) *lds_at(lds + (p - LDS_PLANE) * PLANE) = Packed<R>::set(v);
+        else mem_st(p, Packed<R>::set(v));
+    }
+    template <class Q, int N> PCD void curs(uint32_t p, Cx<R> *v) const {
+        static_assert(__is_same(Q, R), "one value type per slot");
+        if (p >= LDS_PLANE) {
+            const uint32_t o = lds + (p - LDS_PLANE) * PLANE;
+#pragma unroll
+            for (int k = 0; k < N; k++) v[k] = Packed<R>::get(*lds_at(o + k * PLANE));
+        } else {
+#pragma unroll
+            for (int k = 0; k < N; k++) v[k] = Packed<R>::get(mem_ld(p + k));
+        }
+    }
+    template <class Q, int N> PCD void puts(uint32_t p, const Cx<R> *v) const {
+        static_assert(__is_same(Q, R), "one value type per slot");
+        if (p >= LDS_PLANE) {
+            const uint32_t o = lds + (p - LDS_PLANE) * PLANE;
+#pragma unroll
+            for (int k = 0; k < N; k++) *lds_at(o + k * PLANE) = Packed<R>::set(v[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < N; k++) mem_st(p + k, Packed<R>::set(v[k]));
+        }
+    }
+    PCD void form(uint32_t f, double *raw) const {
+        const dbl2 x = Buffer<16>::load<dbl2>(slot, idx * 32u, form_base + f * 2048u);
+        const dbl2 y = Buffer<16>::load<dbl2>(slot, idx * 32u + 16u, form_base + f * 2048u);
+        raw[0] = x.x; raw[1] = x.y; raw[2] = y.x; raw[3] = y.y;
+    }
+    PCD void put_form(uint32_t f, const double *raw) const {
+        Buffer<16>::store(dbl2{raw[0], raw[1]}, slot, idx * 32u, form_base + f * 2048u);
+        Buffer<16>::store(dbl2{raw[2], raw[3]}, slot, idx * 32u + 16u, form_base + f * 2048u);
+    }
+    template <class T> PCD Cx<T> amp(uint32_t q) const {
+        const dbl2 v = Buffer<16>::load<dbl2>(slot, idx * 16u, amp_base + q * 1024u);
+        return cx(from_raw<T>(v.x), from_raw<T>(v.y));
+    }
+    template <class T> PCD void put_amp(uint32_t q, Cx<T> v) const {
+        Buffer<16>::store(dbl2{to_raw(v.re), to_raw(v.im)}, slot, idx * 16u, amp_base + q * 1024u);
+    }
+    PCD const double *sre() const { return sre_; }
+    PCD const double *sim() const { return sim_; }
+    PCD const double *fre() const { return fre_; }
+    PCD const double *fim() const { return fim_; }
 };
 
 template <class R> PCD R reciprocal_minkowski_norm(R p0, R p1, R p2, R p3) {

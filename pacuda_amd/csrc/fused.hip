@@ -5,10 +5,12 @@
 //
 // Layout in HBM: the external momenta stay where the caller put them (SoA planes of points);
 // each resident wavefront owns a scratch slot holding its arena for 64 points, plane-major with
-// 64 doubles (512 bytes) per plane: currents re/im, amplitudes re/im, momentum forms. A slot is
-// reused for every 64-point unit the wavefront takes, so the arena's footprint is
-// (resident wavefronts x slot size), not (points x plan size): it stays in the L2 of the XCD
-// the wavefront runs on and in the MALL, and HBM carries the momenta in and |M|^2 out.
+// one vector per lane and plane: current planes (a complex value, 16 bytes per lane; 8 in the
+// mixed kernels), amplitude planes (16 bytes), momentum forms (four doubles, 32 bytes). A lane's
+// access is one load at a uniform plane base plus its lane offset. A slot is reused for every
+// 64-point unit the wavefront takes, so the arena's footprint is (resident wavefronts x slot
+// size), not (points x plan size); HBM carries the momenta in and |M|^2 out, plus whatever of
+// the slots the L2 and the MALL do not hold.
 //
 // Work queue: the items' 64-point units are numbered item by item; a wavefront takes the next
 // unit with one atomic add of lane 0, so plans of uneven length and items of uneven size
@@ -32,7 +34,7 @@ using namespace pacuda;
 
 namespace {
 
-constexpr int LDS_VARIANTS = 3;
+constexpr int LDS_VARIANTS = 5;
 
 struct PlanDev {
     const Instr *prog_lds[LDS_VARIANTS];  // with forwarding and LDS-resident currents (LDS_CAPS[v] planes)
@@ -61,16 +63,18 @@ struct ItemDev {
 
 constexpr int LANES = 64;
 // LDS-resident current planes per wavefront of the program variants: a launch takes the largest
-// whose LDS fits the wavefronts it asks per CU (20 at 8 f64 wavefronts, 13 at 12, 10 at 16)
-constexpr uint32_t LDS_CAPS[3] = {10, 13, 20};
+// whose LDS fits the wavefronts it asks per CU (f64, 256-thread blocks: 40 at 4 wavefronts, 20 at
+// 8, 13 at 12, 10 at 16; 64-thread blocks: 26 at 6)
+constexpr uint32_t LDS_CAPS[LDS_VARIANTS] = {10, 13, 20, 26, 40};
 constexpr uint32_t LDS_BYTES = 160 * 1024;  // LDS per CU (gfx950), the most one workgroup may take
 
-template <bool DF> PCD double amplitude_value(const double *p, size_t at) {
+// an amplitude's raw value as a double: the df64 kernels' pairs are summed in f64
+template <bool DF> PCD double amplitude_value(double raw) {
     if constexpr (DF) {
-        float2 v = reinterpret_cast<const float2 *>(p)[at];
+        const flt2 v = __builtin_bit_cast(flt2, raw);
         return (double)v.x + (double)v.y;
     } else {
-        return p[at];
+        return raw;
     }
 }
 
@@ -108,64 +112,63 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
         const bool active = i < it.points;
         const size_t ie = active ? i : it.points - 1;  // inactive lanes repeat the last point
         const uint32_t C = pl.current_planes, A = pl.amplitude_planes, F = pl.form_count;
-        Arena a;
-        a.cre = slot;
-        a.cim = slot + (size_t)C * LANES;
-        a.fcre = reinterpret_cast<float *>(a.cre);
-        a.fcim = reinterpret_cast<float *>(a.cim);
-        a.are = a.cim + (size_t)C * LANES;
-        a.aim = a.are + (size_t)A * LANES;
-        double *mom = a.aim + (size_t)A * LANES;
-        a.mom = mom;
-        a.ld = LANES;
-        a.lds_base = (threadIdx.x / LANES) * lds_per_wave;
-        // the momentum forms in f64 (runner/recurrence_runner.cu:745-764)
+        typedef typename Packed<R>::T V;
+        SlotArena<R> a;
+        a.slot = __builtin_amdgcn_make_buffer_rsrc(slot, (short)0, (int)(slot_doubles * sizeof(double)), BUFFER_FORMAT);
+        a.amp_base = C * 1024u;
+        a.form_base = a.amp_base + A * 1024u;
+        a.idx = lane;
+        a.lds = (threadIdx.x / LANES) * lds_per_wave * (uint32_t)(sizeof(V) / 2) + lane * (uint32_t)sizeof(V);
+        // the momentum forms in f64 (runner/recurrence_runner.cu:745-764), four per lane
         for (uint32_t f = 0; f < F; f++) {
             const uint32_t first = cnst(pl.form_terms)[2 * f], n = cnst(pl.form_terms)[2 * f + 1];
+            double v[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                double v = 0.0;
+                v[k] = 0.0;
                 for (uint32_t t = 0; t < n; t++) {
                     double term = cnst(pl.coefficients)[first + t] *
                                   it.external[((size_t)cnst(pl.term_slots)[first + t] * 4 + k) * it.ext_stride + ie];
-                    v = t == 0 ? 0.0 + term : v + term;
+                    v[k] = t == 0 ? 0.0 + term : v[k] + term;
                 }
-                mom[(f * 4 + k) * LANES + lane] = v;
             }
+            a.put_form(f, v);
         }
-        for (uint32_t q = 0; q < A; q++) {
-            a.are[q * LANES + lane] = 0.0;
-            a.aim[q * LANES + lane] = 0.0;
-        }
+        for (uint32_t q = 0; q < A; q++) a.template put_amp<double>(q, cx(0.0, 0.0));
         // sources in f64 from the f64 constants, then the program in the kernel's layout
         const uint32_t S = pl.scale_count, X = pl.factor_count;
-        a.sre = pl.c64; a.sim = pl.c64 + S; a.fre = pl.c64 + 2 * S; a.fim = pl.c64 + 2 * S + X;
+        a.sre_ = pl.c64; a.sim_ = pl.c64 + S; a.fre_ = pl.c64 + 2 * S; a.fim_ = pl.c64 + 2 * S + X;
         // (selects, not an index: a runtime index into the local copy of the plan would put it in scratch)
         const Instr *prog = !lds_per_wave       ? pl.prog_fwd
                             : lds_variant == 0 ? pl.prog_lds[0]
                             : lds_variant == 1 ? pl.prog_lds[1]
-                                               : pl.prog_lds[2];
-        run_sources_point<R>(prog, pl.sources, a, lane);
+                            : lds_variant == 2 ? pl.prog_lds[2]
+                            : lds_variant == 3 ? pl.prog_lds[3]
+                                               : pl.prog_lds[4];
+        run_sources_point<R>(prog, pl.sources, a);
         if constexpr (DF_FORMS) {
-            for (uint32_t q = 0; q < 4 * F; q++) {
-                size_t at = q * LANES + lane;
-                str(mom, at, narrow<D>(mom[at]));
+            for (uint32_t f = 0; f < F; f++) {
+                double raw[4];
+                a.form(f, raw);
+#pragma unroll
+                for (int k = 0; k < 4; k++) raw[k] = to_raw(narrow<D>(raw[k]));
+                a.put_form(f, raw);
             }
         }
         // the program's constants: df64 pairs for every kernel but f64 (as the v1 path converts them)
         const double *cons = DF ? pl.cdf : pl.c64;
-        a.sre = cons; a.sim = cons + S; a.fre = cons + 2 * S; a.fim = cons + 2 * S + X;
+        a.sre_ = cons; a.sim_ = cons + S; a.fre_ = cons + 2 * S; a.fim_ = cons + 2 * S + X;
         if constexpr (K::forwarding)
-            interpret<K>(prog + pl.sources, pl.count - pl.sources, a, lane);
+            interpret<K>(prog + pl.sources, pl.count - pl.sources, a);
         else
-            interpret<K>(pl.prog_plain + pl.sources, pl.count_plain - pl.sources, a, lane);
+            interpret<K>(pl.prog_plain + pl.sources, pl.count_plain - pl.sources, a);
         // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
         double t = 0.0;
         for (uint32_t o = 0; o < pl.outputs; o++) {
             double acc = 0.0;
             for (uint32_t k = cnst(pl.offsets)[o]; k < cnst(pl.offsets)[o + 1]; k++) {
-                size_t at = (size_t)cnst(pl.planes)[k] * LANES + lane;
-                double re = amplitude_value<DF>(a.are, at), im = amplitude_value<DF>(a.aim, at);
+                const Cx<double> amp = a.template amp<double>(cnst(pl.planes)[k]);
+                double re = amplitude_value<DF>(amp.re), im = amplitude_value<DF>(amp.im);
                 acc = acc + cnst(pl.weights)[k] * __builtin_fma(re, re, im * im);
             }
             if (it.resolved && active) it.resolved[(size_t)i * pl.outputs + o] = acc;
@@ -554,7 +557,7 @@ namespace {
 struct Compiled {
     uint32_t sources = 0, planes = 0;
     std::vector<Instr> plain, fwd, lds[LDS_VARIANTS];
-    uint32_t lds_used[LDS_VARIANTS] = {0, 0, 0};
+    uint32_t lds_used[LDS_VARIANTS] = {};
 };
 
 int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
@@ -702,8 +705,8 @@ struct Schedule {
     std::vector<uint32_t> starts;
     uint64_t slot = 0, units = 0, waves = 0;
     uint32_t grid = 0, block = 0, per_cu = 0;
-    uint32_t lds_planes[LDS_VARIANTS] = {0, 0, 0};  // per variant, the most any item's plan uses
-    bool lds_all[LDS_VARIANTS] = {true, true, true};  // every plan has the variant
+    uint32_t lds_planes[LDS_VARIANTS] = {};  // per variant, the most any item's plan uses
+    bool lds_all[LDS_VARIANTS] = {true, true, true, true, true};  // every plan has the variant
 };
 
 // The LDS variant of a launch, its LDS elements per wavefront and bytes per workgroup. A variant
@@ -762,13 +765,13 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
             sc.lds_all[v] = sc.lds_all[v] && it.plan->host.lds_planes[v] > 0;
         }
     }
-    if (sc.units >= 0xffffffffull) return PACMAN_RECURRENCE_INVALID;
+    // (a slot is addressed by 32-bit buffer offsets)
+    if (sc.units >= 0xffffffffull || sc.slot * sizeof(double) > 0x7fffffffull) return PACMAN_RECURRENCE_INVALID;
     sc.starts[count] = (uint32_t)sc.units;
     sc.block = l->block ? l->block : 256;
     // default occupancy per kernel (measured on g g > g g g and g g > g g g g)
     const uint32_t per_cu = l->waves_per_cu                                  ? l->waves_per_cu
-                            : l->kernel == PACMAN_RECURRENCE_KERNEL_F64     ? 12
-                            : l->kernel == PACMAN_RECURRENCE_KERNEL_MIXED64 ? 24
+                            : l->kernel == PACMAN_RECURRENCE_KERNEL_F64     ? 8
                                                                             : 16;
     sc.per_cu = per_cu;
     const uint32_t waves_per_block = sc.block / LANES;

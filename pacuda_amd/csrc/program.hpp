@@ -62,27 +62,28 @@ HostProgram build_program(const uint64_t *words, uint32_t count, const uint8_t *
 void contribution_shape(int op, int *c0, int *c1, int *cd);
 
 // ---- the interpreter (device) ----------------------------------------------------------------
+// A is the arena (PlaneArena for the v1 path, SlotArena for the fused path).
 
-template <class K, int OP>
-PCD void contribution(const Arena &a, const Instr &o, int flags, Cx<typename K::real> *acc, size_t i) {
+template <class K, int OP, class A>
+PCD void contribution(const A &a, const Instr &o, int flags, Cx<typename K::real> *acc) {
     typedef typename K::real R;
     typedef typename K::den Den;
     typedef Cx<R> C;
     constexpr int c0 = Shape<OP>::c0, c1 = Shape<OP>::c1, cd = Shape<OP>::cd;
-    const size_t ld = a.ld;
-    const uint32_t p0 = o.w[2], p1 = o.w[3], dst = o.w[4];
-    C scale = mul(cx(ldc<R>(a.sre, o.w[1]), ldc<R>(a.sim, o.w[1])), cx(ldc<R>(a.fre, o.w[7]), ldc<R>(a.fim, o.w[7])));
+    const uint32_t dst = o.w[4];
+    C scale = mul(cx(ldc<R>(a.sre(), o.w[1]), ldc<R>(a.sim(), o.w[1])), cx(ldc<R>(a.fre(), o.w[7]), ldc<R>(a.fim(), o.w[7])));
     C l[c0], r[c1], v[cd];
     R q0[4], q1[4];
-#pragma unroll
-    for (int k = 0; k < c0; k++) l[k] = Cur<R>::ldp(a, p0 + k, i);
-#pragma unroll
-    for (int k = 0; k < c1; k++) r[k] = Cur<R>::ldp(a, p1 + k, i);
+    a.template curs<R, c0>(o.w[2], l);
+    a.template curs<R, c1>(o.w[3], r);
     if constexpr (OP == 6) {
+        double f0[4], f1[4];
+        a.form(o.w[5], f0);
+        a.form(o.w[6], f1);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            q0[k] = Form<R, Den>::ld(a.mom, (o.w[5] * 4 + k) * ld + i);
-            q1[k] = Form<R, Den>::ld(a.mom, (o.w[6] * 4 + k) * ld + i);
+            q0[k] = Form<R, Den>::of(f0[k]);
+            q1[k] = Form<R, Den>::of(f1[k]);
         }
     }
     contribution_values<OP>(l, r, q0, q1, v);
@@ -93,30 +94,29 @@ PCD void contribution(const Arena &a, const Instr &o, int flags, Cx<typename K::
 #pragma unroll
         for (int k = 0; k < cd; k++) acc[k] = mul(v[k], scale);
     } else {
+        C old[cd];
+        a.template curs<R, cd>(dst, old);
 #pragma unroll
-        for (int k = 0; k < cd; k++) acc[k] = add(Cur<R>::ldp(a, dst + k, i), mul(v[k], scale));
+        for (int k = 0; k < cd; k++) acc[k] = add(old[k], mul(v[k], scale));
     }
-    if ((flags & F_LAST) && !(flags & F_HOLD)) {
-#pragma unroll
-        for (int k = 0; k < cd; k++) Cur<R>::stp(a, dst + k, i, acc[k]);
-    }
+    if ((flags & F_LAST) && !(flags & F_HOLD)) a.template puts<R, cd>(dst, acc);
 }
 
-template <class K, int OP>
-PCD void propagator(const Arena &a, const Instr &o, int flags, Cx<typename K::real> *acc, size_t i) {
+template <class K, int OP, class A>
+PCD void propagator(const A &a, const Instr &o, int flags, Cx<typename K::real> *acc) {
     typedef typename K::real R;
     typedef typename K::den Den;
     typedef Cx<R> C;
     constexpr int m = OP == 9 ? 4 : 2;
-    const size_t ld = a.ld;
     const uint32_t base = o.w[1];
     R p[4];
     Den q[4];
+    double f[4];
+    a.form(o.w[2], f);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        size_t at = (o.w[2] * 4 + k) * ld + i;
-        q[k] = ldr<Den>(a.mom, at);
-        p[k] = Form<R, Den>::ld(a.mom, at);
+        q[k] = from_raw<Den>(f[k]);
+        p[k] = Form<R, Den>::of(f[k]);
     }
     R reciprocal = ToReal<R, Den>::of(reciprocal_minkowski_norm(q[0], q[1], q[2], q[3]));
     C c[m], v[m];
@@ -124,26 +124,23 @@ PCD void propagator(const Arena &a, const Instr &o, int flags, Cx<typename K::re
 #pragma unroll
         for (int k = 0; k < m; k++) c[k] = acc[k];
     } else {
-#pragma unroll
-        for (int k = 0; k < m; k++) c[k] = Cur<R>::ldp(a, base + k, i);
+        a.template curs<R, m>(base, c);
     }
-    propagator_values<OP>(c, p, reciprocal, cx(ldc<R>(a.fre, o.w[3]), ldc<R>(a.fim, o.w[3])), v);
+    propagator_values<OP>(c, p, reciprocal, cx(ldc<R>(a.fre(), o.w[3]), ldc<R>(a.fim(), o.w[3])), v);
     if (flags & F_HOLD) {
 #pragma unroll
         for (int k = 0; k < m; k++) acc[k] = v[k];
     } else {
-#pragma unroll
-        for (int k = 0; k < m; k++) Cur<R>::stp(a, base + k, i, v[k]);
+        a.template puts<R, m>(base, v);
     }
 }
 
-// One lane runs `count` instructions for its point i (runner/recurrence_runner.cu:787-838).
-template <class K>
-__device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count, const Arena &a, size_t i) {
+// One lane runs `count` instructions for its point (runner/recurrence_runner.cu:787-838).
+template <class K, class A>
+__device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count, const A &a) {
     typedef typename K::real R;
     typedef typename K::amp Amp;
     typedef Cx<R> C;
-    const size_t ld = a.ld;
     C acc[6];
     if (count == 0) return;
     // the next instruction is fetched while this one runs (its scalar load overlaps the row's
@@ -153,33 +150,32 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
         const Instr o = next;
         next = cnst(prog)[n + 1 < count ? n + 1 : n];
         const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff;
-        const int flags = K::forwarding ? (int)(o.w[0] >> 16) : (int)(o.w[0] >> 16) & (F_INIT | F_FIRST | F_LAST);
+        const int flags = K::forwarding && A::forwarding ? (int)(o.w[0] >> 16) : (int)(o.w[0] >> 16) & (F_INIT | F_FIRST | F_LAST);
         if (kind == K_CONTRIB) {
             switch (op) {
-            case 6: contribution<K, 6>(a, o, flags, acc, i); break;
-            case 4: contribution<K, 4>(a, o, flags, acc, i); break;
-            case 5: contribution<K, 5>(a, o, flags, acc, i); break;
-            case 2: contribution<K, 2>(a, o, flags, acc, i); break;
-            default: contribution<K, 3>(a, o, flags, acc, i); break;
+            case 6: contribution<K, 6>(a, o, flags, acc); break;
+            case 4: contribution<K, 4>(a, o, flags, acc); break;
+            case 5: contribution<K, 5>(a, o, flags, acc); break;
+            case 2: contribution<K, 2>(a, o, flags, acc); break;
+            default: contribution<K, 3>(a, o, flags, acc); break;
             }
         } else if (kind == K_PROP) {
-            if (op == 9) propagator<K, 9>(a, o, flags, acc, i);
-            else if (op == 7) propagator<K, 7>(a, o, flags, acc, i);
-            else propagator<K, 8>(a, o, flags, acc, i);
+            if (op == 9) propagator<K, 9>(a, o, flags, acc);
+            else if (op == 7) propagator<K, 7>(a, o, flags, acc);
+            else propagator<K, 8>(a, o, flags, acc);
         } else if (kind == K_CLOSURE) {
-            R rfr = ldc<R>(a.fre, o.w[4]), rfi = ldc<R>(a.fim, o.w[4]);
-            size_t at = o.w[3] * ld + i;
-            Amp dr = ldr<Amp>(a.are, at), di = ldr<Amp>(a.aim, at);
+            R rfr = ldc<R>(a.fre(), o.w[4]), rfi = ldc<R>(a.fim(), o.w[4]);
+            const Cx<Amp> d = a.template amp<Amp>(o.w[3]);
+            Amp dr = d.re, di = d.im;
             for (uint32_t k = 0; k < o.w[6]; k++) {
-                C v = closure_increment(Cur<R>::ldp(a, o.w[1] + k, i), Cur<R>::ldp(a, o.w[2] + k, i),
-                                        ldc<R>(a.fre, o.w[5] + k), ldc<R>(a.fim, o.w[5] + k), rfr, rfi);
+                C v = closure_increment(a.template cur<R>(o.w[1] + k), a.template cur<R>(o.w[2] + k),
+                                        ldc<R>(a.fre(), o.w[5] + k), ldc<R>(a.fim(), o.w[5] + k), rfr, rfi);
                 dr = dr + widen(v.re);
                 di = di + widen(v.im);
             }
-            str(a.are, at, dr);
-            str(a.aim, at, di);
+            a.template put_amp<Amp>(o.w[3], cx(dr, di));
         } else if (kind == K_IDENTITY) {
-            R fr = ldc<R>(a.fre, o.w[3]), fi = ldc<R>(a.fim, o.w[3]);
+            R fr = ldc<R>(a.fre(), o.w[3]), fi = ldc<R>(a.fim(), o.w[3]);
             if (flags & (F_FROMACC | F_HOLD)) {  // at most 6 components (the accumulators)
                 const uint32_t count = o.w[2];
 #pragma unroll
@@ -187,41 +183,38 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
                     if ((uint32_t)k < count) {
                         C u;
                         if (flags & F_FROMACC) u = acc[k];
-                        else u = Cur<R>::ldp(a, o.w[1] + k, i);
+                        else u = a.template cur<R>(o.w[1] + k);
                         const C v = identity_value(fr, fi, u);
                         if (flags & F_HOLD) acc[k] = v;
-                        else Cur<R>::stp(a, o.w[1] + k, i, v);
+                        else a.template put<R>(o.w[1] + k, v);
                     }
                 }
             } else {
-                for (uint32_t k = 0; k < o.w[2]; k++) {
-                    Cur<R>::stp(a, o.w[1] + k, i, identity_value(fr, fi, Cur<R>::ldp(a, o.w[1] + k, i)));
-                }
+                for (uint32_t k = 0; k < o.w[2]; k++)
+                    a.template put<R>(o.w[1] + k, identity_value(fr, fi, a.template cur<R>(o.w[1] + k)));
             }
         } else if (kind == K_CLEAR) {
-            for (uint32_t k = 0; k < o.w[2]; k++) Cur<R>::stp(a, o.w[1] + k, i, cx(R(0.0), R(0.0)));
+            for (uint32_t k = 0; k < o.w[2]; k++) a.template put<R>(o.w[1] + k, cx(R(0.0), R(0.0)));
         }
     }
 }
 
-// The leading source instructions for point i, in f64 from f64 forms, scales and factors, then
-// narrowed to the kernel's current layout (runner/recurrence_runner.cu:766-785).
-template <class R>
-__device__ inline void run_sources_point(const Instr *__restrict__ prog, uint32_t count, const Arena &a, size_t i) {
-    const size_t ld = a.ld;
+// The leading source instructions, in f64 from f64 forms, scales and factors, then narrowed to
+// the kernel's current layout (runner/recurrence_runner.cu:766-785).
+template <class R, class A>
+__device__ inline void run_sources_point(const Instr *__restrict__ prog, uint32_t count, const A &a) {
     for (uint32_t n = 0; n < count; n++) {
         const Instr o = cnst(prog)[n];
         double p[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) p[k] = a.mom[(o.w[2] * 4 + k) * ld + i];
-        double fr = ldc<double>(a.fre, o.w[3]), fi = ldc<double>(a.fim, o.w[3]), mass = ldc<double>(a.sre, o.w[4]);
+        a.form(o.w[2], p);
+        double fr = ldc<double>(a.fre(), o.w[3]), fi = ldc<double>(a.fim(), o.w[3]), mass = ldc<double>(a.sre(), o.w[4]);
         W v[4];
         const int c = source_values(o.w[5], mass, p, v);
 #pragma unroll
         for (int k = 0; k < 4; k++) {  // c <= 4: a constant bound keeps v in registers
             if (k < c) {
                 W s = cx(fr * v[k].re - fi * v[k].im, fr * v[k].im + fi * v[k].re);
-                Cur<R>::stp(a, o.w[1] + k, i, cx(narrow<R>(s.re), narrow<R>(s.im)));
+                a.template put<R>(o.w[1] + k, cx(narrow<R>(s.re), narrow<R>(s.im)));
             }
         }
     }

@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) sources_kernel(const Instr *__restrict__ 
                                                       uint32_t points) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= points) return;
-    run_sources_point<R>(prog, count, a, i);
+    run_sources_point<R>(prog, count, PlaneArena{a, i});
 }
 
 template <class K>
@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(1024) program_kernel(const Instr *__restrict__
                                                        uint32_t points) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= points) return;
-    interpret<K>(prog, count, a, i);
+    interpret<K>(prog, count, PlaneArena{a, i});
 }
 
 // in-place f64 <-> df64 (runner/recurrence_runner.cu:840-880)
@@ -656,7 +656,6 @@ int run_locked(const pacman_recurrence_call *in) {
     a.cre = st.cre.p; a.cim = st.cim.p; a.are = st.are.p; a.aim = st.aim.p;
     a.fcre = st.fcre.p; a.fcim = st.fcim.p; a.mom = st.mom.p;
     a.sre = dsre; a.sim = dsim; a.fre = dfre; a.fim = dfim; a.ld = stride;
-    a.lds_base = 0;
     const Program *prepared = prepare(call);
     if (!prepared) return PACMAN_RECURRENCE_DEVICE_ERROR;
     // a program indexing past this call's tables would read or write outside the arena

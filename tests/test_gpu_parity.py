@@ -204,8 +204,9 @@ def test_fused_colour_orderings_match_oracle(lib, gg_ggg):
 
 @pytest.mark.parametrize("kernel", ["f64", "df64", "mixed"])
 def test_fused_launch_shapes_bitwise(lib, gg_ggg, kernel):
-    """every LDS variant (20, 13 and 10 LDS-resident planes per wavefront, picked from the
-    requested occupancy and block) and the launch without LDS residency give the oracle's bits"""
+    """every LDS variant (40, 26, 20, 13 and 10 LDS-resident planes per wavefront, picked from
+    the requested occupancy and block) and the launch without LDS residency give the oracle's
+    bits"""
     plan, ext = gg_ggg
     k = abi.KERNELS[kernel]
     fp = FusedPlan(lib, plan)
@@ -214,7 +215,7 @@ def test_fused_launch_shapes_bitwise(lib, gg_ggg, kernel):
     d_ext.upload(ext)
     d_tot = DeviceArray(lib, points * 8)
     _, want = refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), ext[:, :points], points, kernel=k, resolved=False)
-    for wpc, block in [(8, 256), (12, 256), (16, 256), (16, 64), (24, 64), (32, 256)]:
+    for wpc, block in [(4, 256), (6, 64), (8, 256), (12, 256), (16, 256), (16, 64), (24, 64), (32, 256)]:
         print("fused", kernel, "waves per CU", wpc, "block", block, flush=True)
         fused_evaluate(lib, [(fp, d_ext, points, ext.shape[1], None, d_tot)], k, waves_per_cu=wpc, block=block)
         got = d_tot.download(points)

@@ -38,7 +38,7 @@ def _check(lib, name, plan, points, seed, kernels):
     arrays = plan.arrays()
     for kernel in kernels:
         want_res, want_tot = refcpu.evaluate(arrays, counts, ext, points, kernel=kernel)
-        for variant in range(5):
+        for variant in range(7):
             instr, sources, planes, lds = plan_program(lib, plan, variant)
             if variant == 0:
                 assert not np.any((instr[:, 0] >> 16) & (F_HOLD | F_FROMACC)), name
@@ -69,7 +69,7 @@ def test_passes_do_their_work(lib):
     assert len(fwd) == len(plain)
     moved = np.count_nonzero((fwd[:, 0] >> 16) & F_FROMACC)
     assert moved == np.count_nonzero((fwd[:, 0] >> 16) & F_HOLD) > 0
-    for variant, cap in zip((2, 3, 4), (10, 13, 20)):
+    for variant, cap in zip((2, 3, 4, 5, 6), (10, 13, 20, 26, 40)):
         prog, _, _, lds = plan_program(lib, plan, variant)
         assert 0 < lds <= cap
         assert np.any(prog[:, 1:5] >= LDS_PLANE)
