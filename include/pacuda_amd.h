@@ -68,11 +68,13 @@ typedef struct {
  * The instruction stream the fused path runs for a plan (host only: validation, the program and
  * the fused path's passes, no device work). variant 0: the program with arena compaction only,
  * 1: with forwarding through the accumulators, 2 + v: with LDS variant
- * v (10, 13, 20, 26, 40 planes; plane indices >= 2^24 are LDS slots). Each instruction is 8 u32: w[0] =
+ * v (10, 13, 20, 26, 40 planes; plane indices >= 2^24 are on-chip slots), 7 + v: LDS variant v
+ * with a register tier of 16 planes (whole four-plane currents at slots from the variant's LDS
+ * capacity on). Each instruction is 8 u32: w[0] =
  * kind | op << 8 | flags << 16 (kinds 1 clear, 2 contribution, 3 propagator, 4 identity, 5
  * closure, 6 source; flags 1 init, 2 first, 4 last, 8 hold, 16 from the accumulators). out (may
  * be NULL) receives up to `capacity` instructions; *count their number, *sources the leading
- * source rows, *planes the slot's current planes, *lds_planes the LDS slots used. For tests
+ * source rows, *planes the slot's current planes, *lds_planes the on-chip slots named. For tests
  * and inspection; no reference counterpart.
  */
 int pacuda_plan_program(const pacuda_plan_desc *desc, uint32_t variant, uint32_t *out, uint32_t capacity,

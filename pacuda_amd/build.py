@@ -20,6 +20,10 @@ HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
     "-ffp-contract=off", "-fno-fast-math", "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no sinking of code common to the arms of a branch into their join: it turns the register
+    # tier's per-unit copies into one access through a chosen address, which puts the tier in
+    # scratch memory
+    "-mllvm", "-simplifycfg-sink-common=false",
     "-Wno-unused-command-line-argument",
 ]
 

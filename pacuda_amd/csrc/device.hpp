@@ -243,9 +243,12 @@ template <> struct Packed<float> {
     static PCD T set(Cx<float> c) { return T{c.re, c.im}; }
 };
 
-// Current planes at or above LDS_PLANE live in the wavefront's LDS region (slot plane -
-// LDS_PLANE); only the fused path's LDS pass writes such indices.
+// Current planes at or above LDS_PLANE are on chip, at slot s = plane - LDS_PLANE: in the
+// wavefront's LDS region for s below the launch's LDS capacity, else (s - capacity) in the lane's
+// register tier; only the fused path's passes write such indices.
 constexpr uint32_t LDS_PLANE = 1u << 24;
+// the register tier: REG_PLANES planes in units of one four-plane current
+constexpr uint32_t REG_PLANES = 16;
 
 // Raw buffer accesses: a wavefront-uniform resource (base, size), a per-lane byte offset
 // (VGPR) and a uniform byte offset (SGPR), so a lane's access needs no address arithmetic of
@@ -277,7 +280,32 @@ template <> struct Buffer<8> {
 // Packed<R> in the wavefront's LDS region. A lane's access is one buffer load or store at
 // (lane offset, uniform plane offset). Every operand of a row is one whole object of the LDS
 // pass, so one residency test serves all its components.
-template <class R> struct SlotArena {
+// The register tier of one lane: REG_PLANES / 4 four-plane currents, selected by a uniform
+// switch (the unit index is a scalar) so every element keeps a fixed register.
+template <class R> struct RegFile {
+    Cx<R> u0[4], u1[4], u2[4], u3[4];
+    PCD void get(uint32_t unit, Cx<R> *v) const {
+        switch (unit) {
+        case 0: for (int k = 0; k < 4; k++) v[k] = u0[k]; break;
+        case 1: for (int k = 0; k < 4; k++) v[k] = u1[k]; break;
+        case 2: for (int k = 0; k < 4; k++) v[k] = u2[k]; break;
+        default: for (int k = 0; k < 4; k++) v[k] = u3[k]; break;
+        }
+    }
+    PCD void set(uint32_t unit, const Cx<R> *v) {
+        switch (unit) {
+        case 0: for (int k = 0; k < 4; k++) u0[k] = v[k]; break;
+        case 1: for (int k = 0; k < 4; k++) u1[k] = v[k]; break;
+        case 2: for (int k = 0; k < 4; k++) u2[k] = v[k]; break;
+        default: for (int k = 0; k < 4; k++) u3[k] = v[k]; break;
+        }
+    }
+};
+static_assert(REG_PLANES == 16, "RegFile holds four units");
+
+// REG: the lane has a register tier; an on-chip slot at or above lds_cap is in it (only whole
+// four-plane currents go there, so only the four-plane accessors look)
+template <class R, bool REG = false> struct SlotArena {
     typedef typename Packed<R>::T V;
     typedef Buffer<sizeof(V)> B;
     static constexpr bool forwarding = true;
@@ -287,6 +315,8 @@ template <class R> struct SlotArena {
     const double *sre_, *sim_, *fre_, *fim_;
     uint32_t idx;    // the lane (0-63)
     uint32_t lds;    // byte address of the lane's value in LDS plane 0 of the wavefront's region
+    uint32_t lds_cap;     // on-chip slots below this are LDS planes
+    RegFile<R> *regs;     // the register tier (REG only)
 
     static PCD V *lds_at(uint32_t off) {
         extern __shared__ __attribute__((aligned(16))) char pacuda_lds[];
@@ -308,6 +338,12 @@ template <class R> struct SlotArena {
     template <class Q, int N> PCD void curs(uint32_t p, Cx<R> *v) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) {
+            if constexpr (REG && N == 4) {
+                if (p - LDS_PLANE >= lds_cap) {
+                    regs->get((p - LDS_PLANE - lds_cap) / 4, v);
+                    return;
+                }
+            }
             const uint32_t o = lds + (p - LDS_PLANE) * PLANE;
 #pragma unroll
             for (int k = 0; k < N; k++) v[k] = Packed<R>::get(*lds_at(o + k * PLANE));
@@ -319,6 +355,12 @@ template <class R> struct SlotArena {
     template <class Q, int N> PCD void puts(uint32_t p, const Cx<R> *v) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) {
+            if constexpr (REG && N == 4) {
+                if (p - LDS_PLANE >= lds_cap) {
+                    regs->set((p - LDS_PLANE - lds_cap) / 4, v);
+                    return;
+                }
+            }
             const uint32_t o = lds + (p - LDS_PLANE) * PLANE;
 #pragma unroll
             for (int k = 0; k < N; k++) *lds_at(o + k * PLANE) = Packed<R>::set(v[k]);

@@ -38,6 +38,7 @@ constexpr int LDS_VARIANTS = 5;
 
 struct PlanDev {
     const Instr *prog_lds[LDS_VARIANTS];  // with forwarding and LDS-resident currents (LDS_CAPS[v] planes)
+    const Instr *prog_reg[LDS_VARIANTS];  // the same with a register tier of REG_PLANES planes after them
     const Instr *prog_fwd;                // with forwarding only (launches that give the wavefronts no LDS)
     const Instr *prog_plain;              // neither (kernels without forwarding)
     uint32_t count, count_plain, sources;
@@ -82,11 +83,17 @@ __global__ void reset_counter(uint32_t *counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
 }
 
-template <class K>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start,
-                                                    uint32_t item_count, uint32_t *counter, double *scratch,
-                                                    size_t slot_doubles, uint32_t lds_per_wave,
-                                                    uint32_t lds_variant) {
+// Wavefronts per SIMD a kernel is built for: the register-tier kernels of the f64 and df64
+// policies take two (up to 256 VGPRs), every other kernel four (128 VGPRs).
+template <class K, bool REG> constexpr int waves_per_simd() {
+    return REG && !__is_same(typename K::real, float) ? 2 : 4;
+}
+
+template <class K, bool REG>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_simd<K, REG>())))
+fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start, uint32_t item_count,
+             uint32_t *counter, double *scratch, size_t slot_doubles, uint32_t lds_per_wave, uint32_t lds_variant,
+             uint32_t lds_cap) {
     typedef typename K::real R;
     constexpr bool DF = !__is_same(R, double) || !__is_same(typename K::den, double);
     constexpr bool DF_FORMS = __is_same(typename K::den, D);
@@ -113,7 +120,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
         const size_t ie = active ? i : it.points - 1;  // inactive lanes repeat the last point
         const uint32_t C = pl.current_planes, A = pl.amplitude_planes, F = pl.form_count;
         typedef typename Packed<R>::T V;
-        SlotArena<R> a;
+        SlotArena<R, REG> a;
+        RegFile<R> regs;
+        a.regs = &regs;
+        a.lds_cap = lds_cap;
         a.slot = __builtin_amdgcn_make_buffer_rsrc(slot, (short)0, (int)(slot_doubles * sizeof(double)), BUFFER_FORMAT);
         a.amp_base = C * 1024u;
         a.form_base = a.amp_base + A * 1024u;
@@ -139,12 +149,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
         const uint32_t S = pl.scale_count, X = pl.factor_count;
         a.sre_ = pl.c64; a.sim_ = pl.c64 + S; a.fre_ = pl.c64 + 2 * S; a.fim_ = pl.c64 + 2 * S + X;
         // (selects, not an index: a runtime index into the local copy of the plan would put it in scratch)
-        const Instr *prog = !lds_per_wave       ? pl.prog_fwd
-                            : lds_variant == 0 ? pl.prog_lds[0]
-                            : lds_variant == 1 ? pl.prog_lds[1]
-                            : lds_variant == 2 ? pl.prog_lds[2]
-                            : lds_variant == 3 ? pl.prog_lds[3]
-                                               : pl.prog_lds[4];
+        const Instr *prog;
+        if constexpr (REG)
+            prog = !lds_per_wave       ? pl.prog_fwd
+                   : lds_variant == 0 ? pl.prog_reg[0]
+                   : lds_variant == 1 ? pl.prog_reg[1]
+                   : lds_variant == 2 ? pl.prog_reg[2]
+                   : lds_variant == 3 ? pl.prog_reg[3]
+                                      : pl.prog_reg[4];
+        else
+            prog = !lds_per_wave       ? pl.prog_fwd
+                   : lds_variant == 0 ? pl.prog_lds[0]
+                   : lds_variant == 1 ? pl.prog_lds[1]
+                   : lds_variant == 2 ? pl.prog_lds[2]
+                   : lds_variant == 3 ? pl.prog_lds[3]
+                                      : pl.prog_lds[4];
         run_sources_point<R>(prog, pl.sources, a);
         if constexpr (DF_FORMS) {
             for (uint32_t f = 0; f < F; f++) {
@@ -438,8 +457,11 @@ uint32_t forward_in_place(std::vector<Instr> &ops) {
 // every other access must name a whole live object), ranked by accesses per plane per row of
 // lifetime, and placed first-fit in [0, capacity) planes over their lifetimes; a replay checks the
 // placement (a new object only takes slots no live object holds, every access finds its own) and
-// the program is rewritten to planes LDS_PLANE + slot. Returns the LDS planes used (0: unchanged).
-uint32_t assign_lds(std::vector<Instr> &ops, uint32_t capacity) {
+// the program is rewritten to planes LDS_PLANE + base + slot. Returns the planes used (0:
+// unchanged). With `only` set, only objects of `only` planes are placed, at slots that are
+// multiples of it (the register tier: whole four-plane currents); objects already on chip
+// (planes >= LDS_PLANE) are left where they are.
+uint32_t assign_lds(std::vector<Instr> &ops, uint32_t capacity, uint32_t base = 0, uint32_t only = 0) {
     struct Obj { uint32_t base, len, first, last, touches, slot; bool lds; };
     std::vector<Obj> objs;
     std::vector<int> live;  // plane -> object living there
@@ -450,16 +472,17 @@ uint32_t assign_lds(std::vector<Instr> &ops, uint32_t capacity) {
         if (na < 0) return 0;
         obj_of[n] = {-1, -1, -1};
         for (int k = 0; k < na; k++) {
-            const uint32_t base = ops[n].w[acc[k].word], len = acc[k].len;
+            const uint32_t at = ops[n].w[acc[k].word], len = acc[k].len;
             if (len == 0) continue;
-            if (base >= LDS_PLANE || base + len > LDS_PLANE) return 0;
-            if (live.size() < base + len) live.resize(base + len, -1);
-            int o = live[base];
+            if (at >= LDS_PLANE && at - LDS_PLANE + len <= LDS_PLANE) continue;  // on chip already
+            if (at >= LDS_PLANE || at + len > LDS_PLANE) return 0;
+            if (live.size() < at + len) live.resize(at + len, -1);
+            int o = live[at];
             if (acc[k].write) {
-                objs.push_back({base, len, (uint32_t)n, (uint32_t)n, 0, 0, false});
+                objs.push_back({at, len, (uint32_t)n, (uint32_t)n, 0, 0, false});
                 o = (int)objs.size() - 1;
-                for (uint32_t j = 0; j < len; j++) live[base + j] = o;
-            } else if (o < 0 || objs[o].base != base || objs[o].len != len) {
+                for (uint32_t j = 0; j < len; j++) live[at + j] = o;
+            } else if (o < 0 || objs[o].base != at || objs[o].len != len) {
                 return 0;
             }
             objs[o].last = (uint32_t)n;
@@ -478,8 +501,8 @@ uint32_t assign_lds(std::vector<Instr> &ops, uint32_t capacity) {
     uint32_t used = 0;
     for (uint32_t k : order) {
         Obj &o = objs[k];
-        if (o.len > capacity) continue;
-        for (uint32_t s0 = 0; s0 + o.len <= capacity && !o.lds; s0++) {
+        if (o.len > capacity || (only && o.len != only)) continue;
+        for (uint32_t s0 = 0; s0 + o.len <= capacity && !o.lds; s0 += only ? only : 1) {
             bool fits = true;
             for (uint32_t r = o.first; r <= o.last && fits; r++)
                 for (uint32_t j = 0; j < o.len; j++)
@@ -521,7 +544,7 @@ uint32_t assign_lds(std::vector<Instr> &ops, uint32_t capacity) {
         const int na = row_accesses(ops[n], acc);
         for (int k = 0; k < na; k++) {
             const int oi = obj_of[n][k];
-            if (oi >= 0 && objs[oi].lds) ops[n].w[acc[k].word] = LDS_PLANE + objs[oi].slot;
+            if (oi >= 0 && objs[oi].lds) ops[n].w[acc[k].word] = LDS_PLANE + base + objs[oi].slot;
         }
     }
     return used;
@@ -556,8 +579,8 @@ namespace {
 // The host half of plan creation: validation, the program, and the fused path's passes.
 struct Compiled {
     uint32_t sources = 0, planes = 0;
-    std::vector<Instr> plain, fwd, lds[LDS_VARIANTS];
-    uint32_t lds_used[LDS_VARIANTS] = {};
+    std::vector<Instr> plain, fwd, lds[LDS_VARIANTS], reg[LDS_VARIANTS];
+    uint32_t lds_used[LDS_VARIANTS] = {}, reg_used[LDS_VARIANTS] = {};
 };
 
 int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
@@ -599,6 +622,8 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
     for (int v = 0; v < LDS_VARIANTS; v++) {
         c.lds[v] = c.fwd;
         c.lds_used[v] = assign_lds(c.lds[v], LDS_CAPS[v]);
+        c.reg[v] = c.lds[v];
+        c.reg_used[v] = assign_lds(c.reg[v], REG_PLANES, LDS_CAPS[v], 4);
     }
     return PACMAN_RECURRENCE_OK;
 }
@@ -607,15 +632,18 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
 
 extern "C" int pacuda_plan_program(const pacuda_plan_desc *d, uint32_t variant, uint32_t *out, uint32_t capacity,
                                    uint32_t *count, uint32_t *sources, uint32_t *planes, uint32_t *lds_planes) {
-    if (!count || variant >= 2 + LDS_VARIANTS) return PACMAN_RECURRENCE_INVALID;
+    if (!count || variant >= 2 + 2 * LDS_VARIANTS) return PACMAN_RECURRENCE_INVALID;
     Compiled c;
     const int st = compile_plan(d, c);
     if (st) return st;
-    const std::vector<Instr> &ops = variant == 0 ? c.plain : variant == 1 ? c.fwd : c.lds[variant - 2];
+    const uint32_t v = variant >= 2 + LDS_VARIANTS ? variant - 2 - LDS_VARIANTS : variant - 2;
+    const bool reg = variant >= 2 + LDS_VARIANTS;
+    const std::vector<Instr> &ops = variant == 0 ? c.plain : variant == 1 ? c.fwd : reg ? c.reg[v] : c.lds[v];
     *count = (uint32_t)ops.size();
     if (sources) *sources = c.sources;
     if (planes) *planes = c.planes;
-    if (lds_planes) *lds_planes = variant >= 2 ? c.lds_used[variant - 2] : 0;
+    // on-chip slots the program names: LDS slots, then (register variants) the register tier
+    if (lds_planes) *lds_planes = variant < 2 ? 0 : reg && c.reg_used[v] ? LDS_CAPS[v] + c.reg_used[v] : c.lds_used[v];
     if (out) {
         if (capacity < ops.size()) return PACMAN_RECURRENCE_INVALID;
         memcpy(out, ops.data(), ops.size() * sizeof(Instr));
@@ -647,6 +675,7 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     memset(&h, 0, sizeof h);
     for (int v = 0; v < LDS_VARIANTS; v++) {
         h.prog_lds[v] = upload(p, cp.lds[v].data(), cp.lds[v].size(), &ok);
+        h.prog_reg[v] = upload(p, cp.reg[v].data(), cp.reg[v].size(), &ok);
         h.lds_planes[v] = cp.lds_used[v];
     }
     h.count = (uint32_t)cp.fwd.size();
@@ -781,11 +810,31 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
     return PACMAN_RECURRENCE_OK;
 }
 
-template <class K> bool allow_lds() {
-    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(fused_kernel<K>),
+template <class K, bool REG> bool allow_lds() {
+    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(fused_kernel<K, REG>),
                                                            hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES),
                                        "fused kernel LDS");
     return ok;
+}
+
+// whether a launch of `kernel` at `per_cu` wavefronts per CU runs the register-tier kernel: when
+// its wavefronts fit the CU's registers
+bool use_regs(uint32_t kernel, uint32_t per_cu) {
+    const bool mixed = kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64;
+    return per_cu <= 4u * (uint32_t)(mixed ? waves_per_simd<KMixed, true>() : waves_per_simd<KF64, true>());
+}
+
+template <class K> bool allow_lds(bool reg) { return reg ? allow_lds<K, true>() : allow_lds<K, false>(); }
+
+template <class K>
+void launch(const Schedule &sc, hipStream_t s, bool reg, size_t bytes, const ItemDev *items, const uint32_t *starts,
+            uint32_t count, uint32_t *counter, double *scratch, uint32_t per_wave, uint32_t variant) {
+    if (reg)
+        fused_kernel<K, true><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
+                                                                 variant, LDS_CAPS[variant]);
+    else
+        fused_kernel<K, false><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot,
+                                                                  per_wave, variant, LDS_CAPS[variant]);
 }
 
 // the LDS attribute of the kernel a launch of `sc` needs, set once (before any graph capture)
@@ -794,10 +843,11 @@ bool lds_ready(uint32_t kernel, const Schedule &sc) {
     size_t bytes;
     lds_shape(kernel, sc, &per_wave, &bytes, &variant);
     if (!bytes) return true;
-    if (kernel == PACMAN_RECURRENCE_KERNEL_F64) return allow_lds<KF64>();
-    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED) return allow_lds<KMixed>();
-    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) return allow_lds<KMixed64>();
-    return allow_lds<KDF64>();
+    const bool reg = use_regs(kernel, sc.per_cu);
+    if (kernel == PACMAN_RECURRENCE_KERNEL_F64) return allow_lds<KF64>(reg);
+    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED) return allow_lds<KMixed>(reg);
+    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) return allow_lds<KMixed64>(reg);
+    return allow_lds<KDF64>(reg);
 }
 
 bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
@@ -806,26 +856,23 @@ bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const Item
     size_t bytes;
     lds_shape(kernel, sc, &per_wave, &bytes, &variant);
     reset_counter<<<1, 64, 0, s>>>(counter);
+    const bool reg = use_regs(kernel, sc.per_cu);
     switch (kernel) {
     case PACMAN_RECURRENCE_KERNEL_F64:
-        if (bytes && !allow_lds<KF64>()) return false;
-        fused_kernel<KF64><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
-                                                              variant);
+        if (bytes && !allow_lds<KF64>(reg)) return false;
+        launch<KF64>(sc, s, reg, bytes, items, starts, count, counter, scratch, per_wave, variant);
         break;
     case PACMAN_RECURRENCE_KERNEL_MIXED:
-        if (bytes && !allow_lds<KMixed>()) return false;
-        fused_kernel<KMixed><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
-                                                              variant);
+        if (bytes && !allow_lds<KMixed>(reg)) return false;
+        launch<KMixed>(sc, s, reg, bytes, items, starts, count, counter, scratch, per_wave, variant);
         break;
     case PACMAN_RECURRENCE_KERNEL_MIXED64:
-        if (bytes && !allow_lds<KMixed64>()) return false;
-        fused_kernel<KMixed64><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot,
-                                                                  per_wave, variant);
+        if (bytes && !allow_lds<KMixed64>(reg)) return false;
+        launch<KMixed64>(sc, s, reg, bytes, items, starts, count, counter, scratch, per_wave, variant);
         break;
     default:
-        if (bytes && !allow_lds<KDF64>()) return false;
-        fused_kernel<KDF64><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
-                                                               variant);
+        if (bytes && !allow_lds<KDF64>(reg)) return false;
+        launch<KDF64>(sc, s, reg, bytes, items, starts, count, counter, scratch, per_wave, variant);
         break;
     }
     return true;

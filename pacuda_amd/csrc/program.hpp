@@ -167,17 +167,45 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
             R rfr = ldc<R>(a.fre(), o.w[4]), rfi = ldc<R>(a.fim(), o.w[4]);
             const Cx<Amp> d = a.template amp<Amp>(o.w[3]);
             Amp dr = d.re, di = d.im;
-            for (uint32_t k = 0; k < o.w[6]; k++) {
-                C v = closure_increment(a.template cur<R>(o.w[1] + k), a.template cur<R>(o.w[2] + k),
-                                        ldc<R>(a.fre(), o.w[5] + k), ldc<R>(a.fim(), o.w[5] + k), rfr, rfi);
-                dr = dr + widen(v.re);
-                di = di + widen(v.im);
+            if (o.w[6] == 4) {  // four-plane currents: whole-object accesses
+                C l[4], r[4];
+                a.template curs<R, 4>(o.w[1], l);
+                a.template curs<R, 4>(o.w[2], r);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    C v = closure_increment(l[k], r[k], ldc<R>(a.fre(), o.w[5] + k), ldc<R>(a.fim(), o.w[5] + k), rfr, rfi);
+                    dr = dr + widen(v.re);
+                    di = di + widen(v.im);
+                }
+            } else {
+                for (uint32_t k = 0; k < o.w[6]; k++) {
+                    C v = closure_increment(a.template cur<R>(o.w[1] + k), a.template cur<R>(o.w[2] + k),
+                                            ldc<R>(a.fre(), o.w[5] + k), ldc<R>(a.fim(), o.w[5] + k), rfr, rfi);
+                    dr = dr + widen(v.re);
+                    di = di + widen(v.im);
+                }
             }
             a.template put_amp<Amp>(o.w[3], cx(dr, di));
         } else if (kind == K_IDENTITY) {
             R fr = ldc<R>(a.fre(), o.w[3]), fi = ldc<R>(a.fim(), o.w[3]);
-            if (flags & (F_FROMACC | F_HOLD)) {  // at most 6 components (the accumulators)
-                const uint32_t count = o.w[2];
+            const uint32_t count = o.w[2];
+            if (count == 4) {  // four-plane currents: whole-object accesses
+                C u[4], v[4];
+                if (flags & F_FROMACC) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) v[k] = identity_value(fr, fi, acc[k]);
+                } else {
+                    a.template curs<R, 4>(o.w[1], u);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) v[k] = identity_value(fr, fi, u[k]);
+                }
+                if (flags & F_HOLD) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) acc[k] = v[k];
+                } else {
+                    a.template puts<R, 4>(o.w[1], v);
+                }
+            } else if (flags & (F_FROMACC | F_HOLD)) {  // at most 6 components (the accumulators)
 #pragma unroll
                 for (int k = 0; k < 6; k++) {
                     if ((uint32_t)k < count) {
@@ -190,11 +218,16 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
                     }
                 }
             } else {
-                for (uint32_t k = 0; k < o.w[2]; k++)
+                for (uint32_t k = 0; k < count; k++)
                     a.template put<R>(o.w[1] + k, identity_value(fr, fi, a.template cur<R>(o.w[1] + k)));
             }
         } else if (kind == K_CLEAR) {
-            for (uint32_t k = 0; k < o.w[2]; k++) a.template put<R>(o.w[1] + k, cx(R(0.0), R(0.0)));
+            if (o.w[2] == 4) {
+                const C z[4] = {cx(R(0.0), R(0.0)), cx(R(0.0), R(0.0)), cx(R(0.0), R(0.0)), cx(R(0.0), R(0.0))};
+                a.template puts<R, 4>(o.w[1], z);
+            } else {
+                for (uint32_t k = 0; k < o.w[2]; k++) a.template put<R>(o.w[1] + k, cx(R(0.0), R(0.0)));
+            }
         }
     }
 }
@@ -210,12 +243,20 @@ __device__ inline void run_sources_point(const Instr *__restrict__ prog, uint32_
         double fr = ldc<double>(a.fre(), o.w[3]), fi = ldc<double>(a.fim(), o.w[3]), mass = ldc<double>(a.sre(), o.w[4]);
         W v[4];
         const int c = source_values(o.w[5], mass, p, v);
+        Cx<R> out[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {  // c <= 4: a constant bound keeps v in registers
             if (k < c) {
                 W s = cx(fr * v[k].re - fi * v[k].im, fr * v[k].im + fi * v[k].re);
-                a.template put<R>(o.w[1] + k, cx(narrow<R>(s.re), narrow<R>(s.im)));
+                out[k] = cx(narrow<R>(s.re), narrow<R>(s.im));
             }
+        }
+        if (c == 4) {  // four-plane currents: a whole-object store
+            a.template puts<R, 4>(o.w[1], out);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (k < c) a.template put<R>(o.w[1] + k, out[k]);
         }
     }
 }

@@ -1,9 +1,10 @@
 # SPDX-License-Identifier: 0BSD
 """The fused path's program passes, checked on the CPU (no GPU): the library compiles a plan
 host-side (pacuda_plan_program) into its programs — arena compaction only, with forwarding
-through the accumulators, and with each LDS-residency variant (the ones the kernels run)
-— and the oracle runs each instruction stream (refcpu_evaluate_program, LDS slots mapped to
-extra planes) beside its own plan run from the words. Every value must be equal bit for bit, in
+through the accumulators, and with each LDS-residency variant, alone and with the register tier
+(the ones the kernels run) — and the oracle runs each instruction stream
+(refcpu_evaluate_program, on-chip slots mapped to extra planes) beside its own plan run from the
+words. Every value must be equal bit for bit, in
 the f64, df64 and mixed policies, for 4–7 gluon plans, quark-line plans and every p p > j j j
 subprocess plan; and the passes must have done their work (fewer planes, forwarded rows, LDS
 slots)."""
@@ -16,6 +17,7 @@ import harness
 import refcpu
 
 LDS_PLANE = 1 << 24
+REG_PLANES = 16
 F_HOLD, F_FROMACC = 8, 16
 
 
@@ -38,7 +40,7 @@ def _check(lib, name, plan, points, seed, kernels):
     arrays = plan.arrays()
     for kernel in kernels:
         want_res, want_tot = refcpu.evaluate(arrays, counts, ext, points, kernel=kernel)
-        for variant in range(7):
+        for variant in range(12):
             instr, sources, planes, lds = plan_program(lib, plan, variant)
             if variant == 0:
                 assert not np.any((instr[:, 0] >> 16) & (F_HOLD | F_FROMACC)), name
@@ -73,3 +75,8 @@ def test_passes_do_their_work(lib):
         prog, _, _, lds = plan_program(lib, plan, variant)
         assert 0 < lds <= cap
         assert np.any(prog[:, 1:5] >= LDS_PLANE)
+        # the register-tier variant: the same LDS slots, plus whole four-plane currents after them
+        reg, _, _, onchip = plan_program(lib, plan, variant + 5)
+        assert cap < onchip <= cap + REG_PLANES
+        tier = (reg[:, 1:5] >= LDS_PLANE + cap) & (reg[:, 1:5] < LDS_PLANE + cap + REG_PLANES)
+        assert np.any(tier) and np.all((reg[:, 1:5][tier] - LDS_PLANE - cap) % 4 == 0)
