@@ -274,12 +274,6 @@ template <> struct Buffer<8> {
     }
 };
 
-// The fused path's arena: one wavefront's scratch slot for 64 points, each plane 64 lanes of one
-// vector: currents as Packed<R> (PLANE bytes per plane), then amplitudes as (re, im) raw pairs
-// (1 KB per plane), then forms as four raw doubles (2 KB per form); LDS-resident currents as
-// Packed<R> in the wavefront's LDS region. A lane's access is one buffer load or store at
-// (lane offset, uniform plane offset). Every operand of a row is one whole object of the LDS
-// pass, so one residency test serves all its components.
 // The register tier of one lane: REG_PLANES / 4 four-plane currents, selected by a uniform
 // switch (the unit index is a scalar) so every element keeps a fixed register.
 template <class R> struct RegFile {
@@ -303,6 +297,12 @@ template <class R> struct RegFile {
 };
 static_assert(REG_PLANES == 16, "RegFile holds four units");
 
+// The fused path's arena: one wavefront's scratch slot for 64 points, each plane 64 lanes of one
+// vector: currents as Packed<R> (PLANE bytes per plane), then amplitudes as (re, im) raw pairs
+// (1 KB per plane), then forms as four raw doubles (2 KB per form); LDS-resident currents as
+// Packed<R> in the wavefront's LDS region. A lane's access is one buffer load or store at
+// (lane offset, uniform plane offset). Every operand of a row is one whole object of the LDS
+// pass, so one residency test serves all its components.
 // REG: the lane has a register tier; an on-chip slot at or above lds_cap is in it (only whole
 // four-plane currents go there, so only the four-plane accessors look)
 template <class R, bool REG = false> struct SlotArena {
