@@ -329,7 +329,7 @@ def main():
         "roofline": {"bound": "mfma", "pipe": "fp64 VALU (MI355X FP64 vector peak = matrix peak)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": measured_traffic("fused_kernel<KF64>", n, len(plan.rows), args.waves_per_cu or 12),
+                     "traffic": measured_traffic("fused_kernel<KF64>", n, len(plan.rows), args.waves_per_cu or 8),
                      "traffic_unit": "bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "kernel": "fused_kernel<KF64>", "kernel_ms": kernel_ms,
                      "algorithmic_bytes_per_point": 20 * 8 + 8},
