@@ -1235,8 +1235,10 @@ int refcpu_evaluate(uint32_t kernel, const uint64_t *words, uint32_t word_count,
 
 // The same plan run from an instruction stream of the library's fused path (pacuda_plan_program:
 // 8 u32 per instruction, w[0] = kind | op << 8 | flags << 16, the operands in w[1..7]); current
-// plane indices >= 2^24 (LDS slots) are mapped after the slot's `slot_planes` planes. Lets the
-// tests check, on the CPU, that the library's program passes leave every value unchanged.
+// plane indices >= 2^24 (on-chip slots: LDS, then the register tier) are mapped after the slot's
+// `slot_planes` planes, and form fields keep their low 24 bits (the bits above name the register
+// slot a fused kernel keeps the form in). Lets the tests check, on the CPU, that the library's
+// program passes leave every value unchanged.
 int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t count, uint32_t sources,
                             uint32_t slot_planes, uint32_t lds_planes, uint32_t amplitude_planes, uint32_t form_count,
                             const double *scale_re, const double *scale_im, uint32_t scale_count,
@@ -1248,6 +1250,7 @@ int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t cou
     if (kernel > KERNEL_MIXED64 || block == 0 || !instr) return PACMAN_RECURRENCE_INVALID;
     const uint32_t LDS = 1u << 24;
     auto plane = [&](uint32_t p) { return p >= LDS ? slot_planes + (p - LDS) : p; };
+    auto form = [](uint32_t f) { return f & 0xffffffu; };
     std::vector<Op> ops(count);
     for (uint32_t n = 0; n < count; n++) {
         const uint32_t *w = instr + 8 * (size_t)n;
@@ -1261,10 +1264,10 @@ int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t cou
             o.f[0] = (o.flags & F_INIT) ? 1 : 0;
             o.f[1] = w[1];
             o.f[2] = plane(w[2]); o.f[3] = plane(w[3]); o.f[4] = plane(w[4]);
-            o.f[5] = w[5]; o.f[6] = w[6]; o.f[7] = w[7];
+            o.f[5] = form(w[5]); o.f[6] = form(w[6]); o.f[7] = w[7];
             break;
         case K_CLEAR: o.f[0] = plane(w[1]); o.f[1] = w[2]; break;
-        case K_PROP: o.f[0] = plane(w[1]); o.f[1] = w[2]; o.f[2] = w[3]; break;
+        case K_PROP: o.f[0] = plane(w[1]); o.f[1] = form(w[2]); o.f[2] = w[3]; break;
         case K_IDENTITY: o.f[0] = plane(w[1]); o.f[1] = w[2]; o.f[2] = w[3]; break;
         case K_CLOSURE:
             o.f[0] = plane(w[1]); o.f[1] = plane(w[2]);

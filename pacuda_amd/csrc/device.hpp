@@ -249,6 +249,10 @@ template <> struct Packed<float> {
 constexpr uint32_t LDS_PLANE = 1u << 24;
 // the register tier: REG_PLANES planes in units of one four-plane current
 constexpr uint32_t REG_PLANES = 16;
+// ... and FORM_REGS momentum forms: a form field of an instruction with bits 24 and up set
+// names the register slot ((field >> 24) - 1) holding form (field & 0xffffff)
+constexpr uint32_t FORM_REGS = 4;
+constexpr uint32_t FORM_SLOT_SHIFT = 24;
 
 // Raw buffer accesses: a wavefront-uniform resource (base, size), a per-lane byte offset
 // (VGPR) and a uniform byte offset (SGPR), so a lane's access needs no address arithmetic of
@@ -297,6 +301,28 @@ template <class R> struct RegFile {
 };
 static_assert(REG_PLANES == 16, "RegFile holds four units");
 
+// The register-resident forms of one lane (four raw doubles each), selected like RegFile.
+struct FormRegs {
+    double f0[4], f1[4], f2[4], f3[4];
+    PCD void get(uint32_t s, double *raw) const {
+        switch (s) {
+        case 0: for (int k = 0; k < 4; k++) raw[k] = f0[k]; break;
+        case 1: for (int k = 0; k < 4; k++) raw[k] = f1[k]; break;
+        case 2: for (int k = 0; k < 4; k++) raw[k] = f2[k]; break;
+        default: for (int k = 0; k < 4; k++) raw[k] = f3[k]; break;
+        }
+    }
+    PCD void set(uint32_t s, const double *raw) {
+        switch (s) {
+        case 0: for (int k = 0; k < 4; k++) f0[k] = raw[k]; break;
+        case 1: for (int k = 0; k < 4; k++) f1[k] = raw[k]; break;
+        case 2: for (int k = 0; k < 4; k++) f2[k] = raw[k]; break;
+        default: for (int k = 0; k < 4; k++) f3[k] = raw[k]; break;
+        }
+    }
+};
+static_assert(FORM_REGS == 4, "FormRegs holds four forms");
+
 // The fused path's arena: one wavefront's scratch slot for 64 points, each plane 64 lanes of one
 // vector: currents as Packed<R> (PLANE bytes per plane), then amplitudes as (re, im) raw pairs
 // (1 KB per plane), then forms as four raw doubles (2 KB per form); LDS-resident currents as
@@ -304,8 +330,9 @@ static_assert(REG_PLANES == 16, "RegFile holds four units");
 // (lane offset, uniform plane offset). Every operand of a row is one whole object of the LDS
 // pass, so one residency test serves all its components.
 // REG: the lane has a register tier; an on-chip slot at or above lds_cap is in it (only whole
-// four-plane currents go there, so only the four-plane accessors look)
-template <class R, bool REG = false> struct SlotArena {
+// four-plane currents go there, so only the four-plane accessors look). FREG: the lane also holds
+// the register-resident forms (else a marked form field is read from the slot's copy).
+template <class R, bool REG = false, bool FREG = false> struct SlotArena {
     typedef typename Packed<R>::T V;
     typedef Buffer<sizeof(V)> B;
     static constexpr bool forwarding = true;
@@ -317,6 +344,7 @@ template <class R, bool REG = false> struct SlotArena {
     uint32_t lds;    // byte address of the lane's value in LDS plane 0 of the wavefront's region
     uint32_t lds_cap;     // on-chip slots below this are LDS planes
     RegFile<R> *regs;     // the register tier (REG only)
+    FormRegs *fregs;      // the register-resident forms (FREG only)
 
     static PCD V *lds_at(uint32_t off) {
         extern __shared__ __attribute__((aligned(16))) char pacuda_lds[];
@@ -369,7 +397,15 @@ template <class R, bool REG = false> struct SlotArena {
             for (int k = 0; k < N; k++) mem_st(p + k, Packed<R>::set(v[k]));
         }
     }
+    // form field f: a register slot when bits 24 and up are set (REG kernels), else a slot form
     PCD void form(uint32_t f, double *raw) const {
+        if constexpr (FREG) {
+            if (f >> FORM_SLOT_SHIFT) {
+                fregs->get((f >> FORM_SLOT_SHIFT) - 1, raw);
+                return;
+            }
+        }
+        f &= (1u << FORM_SLOT_SHIFT) - 1;
         const dbl2 x = Buffer<16>::load<dbl2>(slot, idx * 32u, form_base + f * 2048u);
         const dbl2 y = Buffer<16>::load<dbl2>(slot, idx * 32u + 16u, form_base + f * 2048u);
         raw[0] = x.x; raw[1] = x.y; raw[2] = y.x; raw[3] = y.y;

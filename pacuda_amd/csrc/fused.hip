@@ -39,6 +39,7 @@ constexpr int LDS_VARIANTS = 5;
 struct PlanDev {
     const Instr *prog_lds[LDS_VARIANTS];  // with forwarding and LDS-resident currents (LDS_CAPS[v] planes)
     const Instr *prog_reg[LDS_VARIANTS];  // the same with a register tier of REG_PLANES planes after them
+    uint32_t reg_forms[FORM_REGS];        // the form each register slot of prog_reg holds (UINT32_MAX: none)
     const Instr *prog_fwd;                // with forwarding only (launches that give the wavefronts no LDS)
     const Instr *prog_plain;              // neither (kernels without forwarding)
     uint32_t count, count_plain, sources;
@@ -84,7 +85,8 @@ __global__ void reset_counter(uint32_t *counter) {
 }
 
 // Wavefronts per SIMD a kernel is built for: the register-tier kernels of the f64 and df64
-// policies take two (up to 256 VGPRs), every other kernel four (128 VGPRs).
+// policies take two (up to 256 VGPRs; they also hold FORM_REGS forms), every other kernel four
+// (128 VGPRs).
 template <class K, bool REG> constexpr int waves_per_simd() {
     return REG && !__is_same(typename K::real, float) ? 2 : 4;
 }
@@ -120,9 +122,13 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
         const size_t ie = active ? i : it.points - 1;  // inactive lanes repeat the last point
         const uint32_t C = pl.current_planes, A = pl.amplitude_planes, F = pl.form_count;
         typedef typename Packed<R>::T V;
-        SlotArena<R, REG> a;
+        // forms in registers where the kernel has room for them (two wavefronts per SIMD)
+        constexpr bool FREG = REG && waves_per_simd<K, REG>() == 2;
+        SlotArena<R, REG, FREG> a;
         RegFile<R> regs;
+        FormRegs fregs;
         a.regs = &regs;
+        a.fregs = &fregs;
         a.lds_cap = lds_cap;
         a.slot = __builtin_amdgcn_make_buffer_rsrc(slot, (short)0, (int)(slot_doubles * sizeof(double)), BUFFER_FORMAT);
         a.amp_base = C * 1024u;
@@ -177,6 +183,16 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
         // the program's constants: df64 pairs for every kernel but f64 (as the v1 path converts them)
         const double *cons = DF ? pl.cdf : pl.c64;
         a.sre_ = cons; a.sim_ = cons + S; a.fre_ = cons + 2 * S; a.fim_ = cons + 2 * S + X;
+        if constexpr (FREG) {  // the register-resident forms, in the kernel's form layout
+#pragma unroll
+            for (uint32_t q = 0; q < FORM_REGS; q++) {
+                if (pl.reg_forms[q] < F) {
+                    double raw[4];
+                    a.form(pl.reg_forms[q], raw);
+                    fregs.set(q, raw);
+                }
+            }
+        }
         if constexpr (K::forwarding)
             interpret<K>(prog + pl.sources, pl.count - pl.sources, a);
         else
@@ -581,13 +597,57 @@ struct Compiled {
     uint32_t sources = 0, planes = 0;
     std::vector<Instr> plain, fwd, lds[LDS_VARIANTS], reg[LDS_VARIANTS];
     uint32_t lds_used[LDS_VARIANTS] = {}, reg_used[LDS_VARIANTS] = {};
+    uint32_t reg_forms[FORM_REGS];
 };
+
+// The register-resident forms: the FORM_REGS forms the program's rows (sources aside, which
+// take their forms from the slot in f64) read most often go to register slots 0.. in that order;
+// their form fields in the rows are rewritten to f | (slot + 1) << FORM_SLOT_SHIFT. The same
+// choice serves every program of a plan (they differ only in where currents live).
+void choose_form_regs(const std::vector<Instr> &ops, uint32_t form_count, uint32_t *reg_forms) {
+    std::vector<uint64_t> uses(form_count, 0);
+    for (const Instr &o : ops) {
+        const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff;
+        if (kind == K_CONTRIB && op == 6) {
+            if (o.w[5] < form_count) uses[o.w[5]]++;
+            if (o.w[6] < form_count) uses[o.w[6]]++;
+        } else if (kind == K_PROP && o.w[2] < form_count) {
+            uses[o.w[2]]++;
+        }
+    }
+    std::vector<uint32_t> order(form_count);
+    for (uint32_t f = 0; f < form_count; f++) order[f] = f;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return uses[a] > uses[b]; });
+    for (uint32_t s = 0; s < FORM_REGS; s++)
+        reg_forms[s] = s < form_count && uses[order[s]] ? order[s] : UINT32_MAX;
+}
+
+void mark_form_regs(std::vector<Instr> &ops, const uint32_t *reg_forms) {
+    auto mark = [&](uint32_t &w) {
+        for (uint32_t s = 0; s < FORM_REGS; s++)
+            if (reg_forms[s] == w) {
+                w |= (s + 1) << FORM_SLOT_SHIFT;
+                return;
+            }
+    };
+    for (Instr &o : ops) {
+        const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff;
+        if (kind == K_CONTRIB && op == 6) {
+            mark(o.w[5]);
+            mark(o.w[6]);
+        } else if (kind == K_PROP) {
+            mark(o.w[2]);
+        }
+    }
+}
 
 int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
     if (!d || d->abi_version != PACMAN_RECURRENCE_ABI || d->struct_size != sizeof(pacuda_plan_desc) || !d->words ||
         !d->offsets || (d->form_count && !d->form_terms) || (d->term_count && (!d->term_slots || !d->term_coefficients)) ||
         (d->scale_count && (!d->scale_re || !d->scale_im)) || (d->factor_count && (!d->factor_re || !d->factor_im)))
         return PACMAN_RECURRENCE_INVALID;
+    // (plane and form indices share their fields with the on-chip markers of the fused passes)
+    if (d->current_planes >= LDS_PLANE || d->form_count >= (1u << FORM_SLOT_SHIFT)) return PACMAN_RECURRENCE_INVALID;
     const uint32_t count = d->offsets[d->outputs];
     if (d->offsets[0] != 0 || (count && (!d->planes || !d->weights))) return PACMAN_RECURRENCE_INVALID;
     for (uint32_t o = 0; o < d->outputs; o++)
@@ -625,6 +685,8 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
         c.reg[v] = c.lds[v];
         c.reg_used[v] = assign_lds(c.reg[v], REG_PLANES, LDS_CAPS[v], 4);
     }
+    choose_form_regs(c.fwd, d->form_count, c.reg_forms);
+    for (int v = 0; v < LDS_VARIANTS; v++) mark_form_regs(c.reg[v], c.reg_forms);
     return PACMAN_RECURRENCE_OK;
 }
 
@@ -683,6 +745,7 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     h.count_plain = (uint32_t)cp.plain.size();
     h.prog_fwd = upload(p, cp.fwd.data(), cp.fwd.size(), &ok);
     h.sources = cp.sources;
+    memcpy(h.reg_forms, cp.reg_forms, sizeof h.reg_forms);
     h.c64 = upload(p, c64.data(), c64.size(), &ok);
     h.cdf = upload(p, cdf.data(), cdf.size(), &ok);
     h.scale_count = S;

@@ -80,3 +80,8 @@ def test_passes_do_their_work(lib):
         assert cap < onchip <= cap + REG_PLANES
         tier = (reg[:, 1:5] >= LDS_PLANE + cap) & (reg[:, 1:5] < LDS_PLANE + cap + REG_PLANES)
         assert np.any(tier) and np.all((reg[:, 1:5][tier] - LDS_PLANE - cap) % 4 == 0)
+        # ... and the most-read forms marked as register-resident (slot + 1 in bits 24 and up)
+        kind, op = reg[:, 0] & 0xff, (reg[:, 0] >> 8) & 0xff
+        fields = np.concatenate([reg[(kind == 2) & (op == 6)][:, 5:7].ravel(), reg[kind == 3][:, 2]])
+        slots = fields >> 24
+        assert set(np.unique(slots)) == {0, 1, 2, 3, 4}
