@@ -40,6 +40,7 @@ struct PlanDev {
     const Instr *prog_lds[LDS_VARIANTS];  // with forwarding and LDS-resident currents (LDS_CAPS[v] planes)
     const Instr *prog_reg[LDS_VARIANTS];  // the same with a register tier of REG_PLANES planes after them
     uint32_t reg_forms[FORM_REGS];        // the form each register slot of prog_reg holds (UINT32_MAX: none)
+    uint32_t zero_fill;                   // 1: some amplitude plane has no F_ZERO closure, so all start zeroed
     const Instr *prog_fwd;                // with forwarding only (launches that give the wavefronts no LDS)
     const Instr *prog_plain;              // neither (kernels without forwarding)
     uint32_t count, count_plain, sources;
@@ -150,7 +151,8 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
             }
             a.put_form(f, v);
         }
-        for (uint32_t q = 0; q < A; q++) a.template put_amp<double>(q, cx(0.0, 0.0));
+        if (pl.zero_fill || !K::forwarding)
+            for (uint32_t q = 0; q < A; q++) a.template put_amp<double>(q, cx(0.0, 0.0));
         // sources in f64 from the f64 constants, then the program in the kernel's layout
         const uint32_t S = pl.scale_count, X = pl.factor_count;
         a.sre_ = pl.c64; a.sim_ = pl.c64 + S; a.fre_ = pl.c64 + 2 * S; a.fim_ = pl.c64 + 2 * S + X;
@@ -598,7 +600,25 @@ struct Compiled {
     std::vector<Instr> plain, fwd, lds[LDS_VARIANTS], reg[LDS_VARIANTS];
     uint32_t lds_used[LDS_VARIANTS] = {}, reg_used[LDS_VARIANTS] = {};
     uint32_t reg_forms[FORM_REGS];
+    bool zero_fill = true;
 };
+
+// F_ZERO on the first closure of every amplitude plane (closures are the only rows that touch
+// amplitudes); returns whether some plane in [0, amplitude_planes) has none, i.e. whether the
+// kernel must still zero the amplitude planes before the program.
+bool mark_first_closures(std::vector<Instr> &ops, uint32_t amplitude_planes) {
+    std::vector<uint8_t> seen(amplitude_planes, 0);
+    uint32_t first = 0;
+    for (Instr &o : ops) {
+        if ((o.w[0] & 0xff) != K_CLOSURE || o.w[3] >= amplitude_planes) continue;
+        if (!seen[o.w[3]]) {
+            seen[o.w[3]] = 1;
+            o.w[0] |= (uint32_t)F_ZERO << 16;
+            first++;
+        }
+    }
+    return first < amplitude_planes;
+}
 
 // The register-resident forms: the FORM_REGS forms the program's rows (sources aside, which
 // take their forms from the slot in f64) read most often go to register slots 0.. in that order;
@@ -678,6 +698,7 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
     c.sources = hp.sources;
     c.plain = hp.ops;
     forward_in_place(hp.ops);
+    c.zero_fill = mark_first_closures(hp.ops, d->amplitude_planes);
     c.fwd = hp.ops;
     for (int v = 0; v < LDS_VARIANTS; v++) {
         c.lds[v] = c.fwd;
@@ -746,6 +767,7 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     h.prog_fwd = upload(p, cp.fwd.data(), cp.fwd.size(), &ok);
     h.sources = cp.sources;
     memcpy(h.reg_forms, cp.reg_forms, sizeof h.reg_forms);
+    h.zero_fill = cp.zero_fill ? 1 : 0;
     h.c64 = upload(p, c64.data(), c64.size(), &ok);
     h.cdf = upload(p, cdf.data(), cdf.size(), &ok);
     h.scale_count = S;

@@ -23,8 +23,10 @@ namespace pacuda {
 enum { K_CLEAR = 1, K_CONTRIB, K_PROP, K_IDENTITY, K_CLOSURE, K_SOURCE, K_SYMJIT };
 // F_HOLD: the row's result stays in the accumulators for the next row instead of being stored;
 // F_FROMACC: an in-place row (propagator, identity) takes its input from the accumulators
-// instead of loading it (set only by the fused path's forwarding pass, never on v1 programs)
-enum { F_INIT = 1, F_FIRST = 2, F_LAST = 4, F_HOLD = 8, F_FROMACC = 16 };
+// instead of loading it; F_ZERO: a closure that is the first to touch its amplitude starts from
+// zero instead of loading the zero the arena holds (all three set only by the fused path's
+// passes, never on v1 programs)
+enum { F_INIT = 1, F_FIRST = 2, F_LAST = 4, F_HOLD = 8, F_FROMACC = 16, F_ZERO = 32 };
 
 // w[0] = kind | op << 8 | flags << 16; contributions: w[1] scale slot, w[2] parent 0, w[3]
 // parent 1, w[4] destination, w[5] form 0, w[6] form 1, w[7] factor; clear: w[1] base, w[2]
@@ -165,8 +167,15 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
             else propagator<K, 8>(a, o, flags, acc);
         } else if (kind == K_CLOSURE) {
             R rfr = ldc<R>(a.fre(), o.w[4]), rfi = ldc<R>(a.fim(), o.w[4]);
-            const Cx<Amp> d = a.template amp<Amp>(o.w[3]);
-            Amp dr = d.re, di = d.im;
+            Amp dr, di;
+            if (flags & F_ZERO) {
+                dr = Amp(0.0);
+                di = Amp(0.0);
+            } else {
+                const Cx<Amp> d = a.template amp<Amp>(o.w[3]);
+                dr = d.re;
+                di = d.im;
+            }
             if (o.w[6] == 4) {  // four-plane currents: whole-object accesses
                 C l[4], r[4];
                 a.template curs<R, 4>(o.w[1], l);

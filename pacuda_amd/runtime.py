@@ -200,7 +200,7 @@ class Sweep:
 
 # (waves per CU, block) pairs the autotuner tries: 2 to 6 persistent waves per SIMD, blocks of
 # one or four waves
-TUNE_SHAPES = ((8, 256), (12, 256), (16, 256), (24, 256), (12, 64), (16, 64))
+TUNE_SHAPES = ((6, 64), (8, 128), (8, 256), (12, 256), (16, 256), (16, 64))
 _tuned = {}
 
 

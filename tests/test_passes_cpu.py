@@ -18,7 +18,7 @@ import refcpu
 
 LDS_PLANE = 1 << 24
 REG_PLANES = 16
-F_HOLD, F_FROMACC = 8, 16
+F_HOLD, F_FROMACC, F_ZERO = 8, 16, 32
 
 
 @pytest.fixture(scope="module")
@@ -71,6 +71,10 @@ def test_passes_do_their_work(lib):
     assert len(fwd) == len(plain)
     moved = np.count_nonzero((fwd[:, 0] >> 16) & F_FROMACC)
     assert moved == np.count_nonzero((fwd[:, 0] >> 16) & F_HOLD) > 0
+    # the first closure of each amplitude plane starts from zero (no load of the zeroed plane)
+    closures = fwd[(fwd[:, 0] & 0xff) == 5]
+    zero = closures[((closures[:, 0] >> 16) & F_ZERO) != 0]
+    assert sorted(zero[:, 3]) == sorted(set(closures[:, 3]))
     for variant, cap in zip((2, 3, 4, 5, 6), (10, 13, 20, 26, 40)):
         prog, _, _, lds = plan_program(lib, plan, variant)
         assert 0 < lds <= cap
