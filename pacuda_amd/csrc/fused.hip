@@ -85,11 +85,11 @@ __global__ void reset_counter(uint32_t *counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
 }
 
-// Wavefronts per SIMD a kernel is built for: the register-tier kernels of the f64 and df64
-// policies take two (up to 256 VGPRs; they also hold FORM_REGS forms), every other kernel four
-// (128 VGPRs).
+// Wavefronts per SIMD a kernel is built for: the register-tier kernels take two in the f64 and
+// df64 policies (up to 256 VGPRs) and three in the mixed ones (168 VGPRs), and also hold
+// FORM_REGS forms; the other kernels take four (128 VGPRs).
 template <class K, bool REG> constexpr int waves_per_simd() {
-    return REG && !__is_same(typename K::real, float) ? 2 : 4;
+    return REG ? (__is_same(typename K::real, float) ? 3 : 2) : 4;
 }
 
 template <class K, bool REG>
@@ -123,8 +123,8 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
         const size_t ie = active ? i : it.points - 1;  // inactive lanes repeat the last point
         const uint32_t C = pl.current_planes, A = pl.amplitude_planes, F = pl.form_count;
         typedef typename Packed<R>::T V;
-        // forms in registers where the kernel has room for them (two wavefronts per SIMD)
-        constexpr bool FREG = REG && waves_per_simd<K, REG>() == 2;
+        // forms in registers where the kernel has room for them (up to three wavefronts per SIMD)
+        constexpr bool FREG = REG && waves_per_simd<K, REG>() <= 3;
         SlotArena<R, REG, FREG> a;
         RegFile<R> regs;
         FormRegs fregs;
@@ -886,7 +886,8 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
     // default occupancy per kernel (measured on g g > g g g and g g > g g g g)
     const uint32_t per_cu = l->waves_per_cu                                  ? l->waves_per_cu
                             : l->kernel == PACMAN_RECURRENCE_KERNEL_F64     ? 8
-                                                                            : 16;
+                            : l->kernel == PACMAN_RECURRENCE_KERNEL_DF64    ? 16
+                                                                            : 12;
     sc.per_cu = per_cu;
     const uint32_t waves_per_block = sc.block / LANES;
     uint64_t waves = std::min<uint64_t>((uint64_t)compute_units() * per_cu, std::max<uint64_t>(sc.units, 1));
