@@ -127,7 +127,13 @@ def unweighting_summary(u):
                          "weighted": u["plain"]["weighted"]},
             "sigma_pull_vs_plain": t["sigma_pull"], "sigma_pull_vs_weighted": t["sigma_pull_vs_weighted"],
             "gain_network_on_every_candidate": u["two_stage_dense"]["gain"],
-            "gain_prior_only": u["two_stage_prior_only"]["gain"]}
+            "gain_prior_only": u["two_stage_prior_only"]["gain"],
+            "training_seconds": u["training"]["label_seconds"] + u["training"]["train_seconds"],
+            # events beyond which two-stage unweighting, its one-off labelling and training included,
+            # takes less time than plain unweighting
+            "break_even_events": (u["training"]["label_seconds"] + u["training"]["train_seconds"]) /
+                                 (1.0 / u["plain"]["events_per_s"] - 1.0 / t["events_per_s"])
+                                 if t["events_per_s"] > u["plain"]["events_per_s"] else None}
 
 
 def copy_calibration(rows=20, count=1 << 22, repeats=3):

@@ -98,14 +98,18 @@ __device__ __forceinline__ uint32_t lds_at(uint32_t r, uint32_t c, uint32_t stri
     return r * stride + ((((c >> 2) ^ (r & 15))) << 2) + (c & 3);
 }
 
+// ROWS blocks of 16 points per workgroup (FT = 16 ROWS points): small batches take ROWS = 1 so
+// that they still spread over the compute units
+template <int ROWS>
 __global__ void __launch_bounds__(256) mlp_forward(FwdArgs a) {
+    constexpr uint32_t FT = 16 * ROWS;
     extern __shared__ bf16 lds[];
-    const uint32_t H = a.hidden, stride = H;  // two 64 x H bf16 buffers: 64 KB at H = 256
-    bf16 *buf[2] = {lds, lds + TILE * stride};
+    const uint32_t H = a.hidden, stride = H;  // two FT x H bf16 buffers: 64 KB at FT = 64, H = 256
+    bf16 *buf[2] = {lds, lds + FT * stride};
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t row0 = blockIdx.x * TILE;
+    const uint32_t row0 = blockIdx.x * FT;
     // the input tile, bf16, padded with zeros to in_pad columns
-    for (uint32_t idx = tid; idx < TILE * a.in_pad; idx += blockDim.x) {
+    for (uint32_t idx = tid; idx < FT * a.in_pad; idx += blockDim.x) {
         uint32_t r = idx / a.in_pad, c = idx % a.in_pad;
         float v = c < a.inputs ? a.x[(size_t)(row0 + r) * a.inputs + c] : 0.0f;
         bf16 h = to_bf16(v);
@@ -122,30 +126,30 @@ __global__ void __launch_bounds__(256) mlp_forward(FwdArgs a) {
         const float *bias = a.params + a.b_off[l];
         const uint32_t colbase = wave * cols_per_wave;
         // constant loop bounds with predicates: the accumulators stay in registers
-        f32x4 acc[4][4];
+        f32x4 acc[ROWS][4];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < ROWS; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
         for (uint32_t k0 = 0; k0 < K; k0 += 16) {
             const uint32_t kk = k0 + 4 * (lane >> 4);
-            bf16x4 av[4], bv[4];
+            bf16x4 av[ROWS], bv[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < ROWS; i++)
                 av[i] = *reinterpret_cast<const bf16x4 *>(&buf[cur][lds_at(16 * i + (lane & 15), kk, stride)]);
 #pragma unroll
             for (int j = 0; j < 4; j++)
                 if ((uint32_t)j < col_blocks)
                     bv[j] = *reinterpret_cast<const bf16x4 *>(&W[(size_t)(colbase + 16 * j + (lane & 15)) * K + kk]);
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < ROWS; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     if ((uint32_t)j < col_blocks) acc[i][j] = mfma(av[i], bv[j], acc[i][j]);
         }
         const uint32_t nxt = cur ^ 1;
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < ROWS; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++)
                 if ((uint32_t)j < col_blocks)
@@ -163,13 +167,14 @@ __global__ void __launch_bounds__(256) mlp_forward(FwdArgs a) {
         __syncthreads();
         cur = nxt;
     }
-    // output unit: y = sum_k h[k] w_out[k] + b_out, four threads per point
-    const uint32_t r = tid >> 2, part = tid & 3, span = H / 4;
+    // output unit: y = sum_k h[k] w_out[k] + b_out, TPP threads per point
+    constexpr uint32_t TPP = 256 / FT;
+    const uint32_t r = tid / TPP, part = tid % TPP, span = H / TPP;
     const float *wout = a.params + a.wout_off;
     float s = 0.0f;
     for (uint32_t k = part * span; k < (part + 1) * span; k++) s += from_bf16(buf[cur][lds_at(r, k, stride)]) * wout[k];
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
+#pragma unroll
+    for (uint32_t m = 1; m < TPP; m *= 2) s += __shfl_xor(s, m);
     if (part == 0) a.y[row0 + r] = s + a.params[a.bout_off];
 }
 
@@ -304,21 +309,40 @@ __global__ void __launch_bounds__(256, 1) mlp_infer(InferArgs a) {
 
 // ---- backward --------------------------------------------------------------------------------
 
-// delta = 2 (y - t) / batch; loss += (y - t)^2 / batch; dZ_last = delta w_out silu'(Z_last)
-__global__ void mlp_output_grad(const float *y, const float *t, uint32_t batch, uint32_t hidden, const float *z_last,
-                                const float *wout, float *delta, float *loss, bf16 *dz, bf16 *dzt) {
-    const uint32_t b = blockIdx.x;
-    const float d = y[b] - t[b];
-    const float g = 2.0f * d / (float)batch;
-    if (threadIdx.x == 0) {
+// delta = 2 (y - t) / batch; loss += (y - t)^2 / batch; dZ_last = delta w_out silu'(Z_last).
+// One workgroup per TILE points: dZ rows are written along the units, and their transpose goes
+// through LDS so that dZ^T rows are written along the points; one atomic add of the tile's loss.
+__global__ void __launch_bounds__(256) mlp_output_grad(const float *y, const float *t, uint32_t batch, uint32_t hidden,
+                                                       const float *z_last, const float *wout, float *delta, float *loss,
+                                                       bf16 *dz, bf16 *dzt) {
+    __shared__ bf16 tile[TILE][256 + 2];  // hidden <= 256; the pad spreads a column over the banks
+    __shared__ float g_s[TILE], l_s[TILE];
+    const uint32_t b0 = blockIdx.x * TILE;
+    if (threadIdx.x < TILE) {
+        const uint32_t b = b0 + threadIdx.x;
+        const float d = y[b] - t[b];
+        const float g = 2.0f * d / (float)batch;
         delta[b] = g;
-        atomicAdd(loss, d * d / (float)batch);
+        g_s[threadIdx.x] = g;
+        l_s[threadIdx.x] = d * d / (float)batch;
     }
-    for (uint32_t n = threadIdx.x; n < hidden; n += blockDim.x) {
-        const float v = g * wout[n] * silu_grad(z_last[(size_t)b * hidden + n]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float sum = 0.0f;
+        for (uint32_t r = 0; r < TILE; r++) sum += l_s[r];
+        atomicAdd(loss, sum);
+    }
+    for (uint32_t idx = threadIdx.x; idx < TILE * hidden; idx += blockDim.x) {
+        const uint32_t r = idx / hidden, n = idx % hidden;
+        const float v = g_s[r] * wout[n] * silu_grad(z_last[(size_t)(b0 + r) * hidden + n]);
         const bf16 h = to_bf16(v);
-        dz[(size_t)b * hidden + n] = h;
-        dzt[(size_t)n * batch + b] = h;
+        dz[(size_t)(b0 + r) * hidden + n] = h;
+        tile[r][n] = h;
+    }
+    __syncthreads();
+    for (uint32_t idx = threadIdx.x; idx < TILE * hidden; idx += blockDim.x) {
+        const uint32_t n = idx / TILE, r = idx % TILE;
+        dzt[(size_t)n * batch + b0 + r] = tile[r][n];
     }
 }
 
@@ -610,7 +634,12 @@ extern "C" int pacuda_mlp_forward(pacuda_mlp *m, const float *x, uint32_t batch,
         mlp_infer<<<(batch + ITILE - 1) / ITILE, 256, 2 * ITILE * IH * sizeof(bf16), mlp_stream()>>>(ia);
         return hip_failed(hipGetLastError(), "mlp inference") ? PACMAN_RECURRENCE_DEVICE_ERROR : PACMAN_RECURRENCE_OK;
     }
-    mlp_forward<<<batch / TILE, 256, lds, mlp_stream()>>>(a);
+    // workgroups of 64 points when that gives two per compute unit, else of 16
+    if (batch / TILE >= 512) {
+        mlp_forward<4><<<batch / TILE, 256, lds, mlp_stream()>>>(a);
+    } else {
+        mlp_forward<1><<<batch / 16, 256, lds / 4, mlp_stream()>>>(a);
+    }
     if (hip_failed(hipGetLastError(), "mlp forward")) return PACMAN_RECURRENCE_DEVICE_ERROR;
     return PACMAN_RECURRENCE_OK;
 }
@@ -623,7 +652,7 @@ extern "C" int pacuda_mlp_backward(pacuda_mlp *m, const float *y, const float *t
     const size_t B = batch;
     auto htrow = [&](int l) { return n.ht + (l < 0 ? 0 : (n.in_pad + (size_t)l * H) * B); };  // H_l^T rows (l = -1: x^T)
     if (hip_failed(hipMemsetAsync(n.loss, 0, 4, s), "mlp")) return PACMAN_RECURRENCE_DEVICE_ERROR;
-    mlp_output_grad<<<batch, 256, 0, s>>>(y, target, batch, H, n.z + (size_t)(L - 1) * B * H, n.params + n.wout_off,
+    mlp_output_grad<<<batch / TILE, 256, 0, s>>>(y, target, batch, H, n.z + (size_t)(L - 1) * B * H, n.params + n.wout_off,
                                           n.delta, n.loss, n.dz[0], n.dzt);
     // output layer: dW_out[k] = sum_b delta[b] H_last[b][k]; db_out = sum_b delta[b]
     row_dot<bf16><<<H, 256, 0, s>>>(htrow(L - 1), n.delta, batch, n.grads + n.wout_off);

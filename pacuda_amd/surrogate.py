@@ -60,11 +60,13 @@ class MLP:
         if self.lib.pacuda_mlp_forward(self.handle, d_x.ptr, batch, d_y.ptr, 1 if train else 0) != 0:
             raise RuntimeError("pacuda_mlp_forward failed")
 
-    def backward(self, d_y, d_t, batch):
+    def backward(self, d_y, d_t, batch, want_loss=True):
+        """gradients of the mean squared error; returns the batch loss (a device synchronisation)
+        when want_loss, else None (the launches stay queued)"""
         loss = C.c_float()
-        if self.lib.pacuda_mlp_backward(self.handle, d_y.ptr, d_t.ptr, batch, C.byref(loss)) != 0:
+        if self.lib.pacuda_mlp_backward(self.handle, d_y.ptr, d_t.ptr, batch, C.byref(loss) if want_loss else None) != 0:
             raise RuntimeError("pacuda_mlp_backward failed")
-        return loss.value
+        return loss.value if want_loss else None
 
     def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
         self.step += 1

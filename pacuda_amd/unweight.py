@@ -30,7 +30,7 @@ import time
 import numpy as np
 
 from . import abi, plans, phase_space as ps
-from .runtime import Device, FusedPlan, evaluate
+from .runtime import Device, DeviceArray, FusedPlan, evaluate
 from .surrogate import MLP, he_init
 
 GEV2_TO_PB = 0.3893793721e9
@@ -209,34 +209,51 @@ class Unweighting:
 
     def train(self, x, logm2, logp=None, epochs=30, lr=1e-3, seed=0, init_seed=5, log=None):
         """fit the network to the normalised residual log(|M|^2 / P) (or log|M|^2 without the
-        prior) with Adam and a cosine learning-rate decay; returns the mean loss per epoch"""
+        prior) with Adam and a cosine learning-rate decay; returns the loss of each epoch's last
+        batch. The training set stays on the device: each epoch uploads only its permutation and
+        gathers the shuffled rows there, and the steps queue without host synchronisation."""
         target = logm2 - logp if (self.prior and logp is not None) else logm2
         self.mean, self.std = float(np.mean(target)), float(np.std(target))
         t = ((target - self.mean) / self.std).astype(np.float32)
         self.mlp.set_params(he_init(self.gen.width, self.mlp.hidden, self.mlp.layers, init_seed))
         n = len(t) // self.batch * self.batch
+        if n == 0:
+            raise ValueError(f"training needs at least one batch of {self.batch} points")
         rng = np.random.default_rng(seed)
-        d_x = self.dev.empty(self.batch * self.gen.width, np.float32)
-        d_t = self.dev.empty(self.batch, np.float32)
+        w = self.gen.width
+        lib = self.dev.lib
+        d_xall = self.dev.array(np.ascontiguousarray(x, dtype=np.float32))
+        d_tall = self.dev.array(t)
+        d_x = self.dev.empty(n * w, np.float32)
+        d_t = self.dev.empty(n, np.float32)
         d_y = self.dev.empty(self.batch, np.float32)
+        d_idx = self.dev.empty(n, np.uint32)
+        dummy = self.dev.empty(1)
         history = []
-        steps = epochs * (n // self.batch)
+        batches = n // self.batch
+        steps = epochs * batches
         step = 0
         for epoch in range(epochs):
-            order = rng.permutation(len(t))[:n]
-            losses = []
-            for b in range(n // self.batch):
-                sel = order[b * self.batch:(b + 1) * self.batch]
-                d_x.upload(x[sel])
-                d_t.upload(t[sel])
-                self.mlp.forward(d_x, self.batch, d_y, train=True)
-                losses.append(self.mlp.backward(d_y, d_t, self.batch))
+            order = rng.permutation(len(t))[:n].astype(np.uint32)
+            self.dev.synchronize()  # the previous epoch's steps are done with d_x, d_t
+            d_idx.upload(order)
+            for src, width, dst in ((d_xall, w, d_x), (d_tall, 1, d_t)):
+                st = lib.pacuda_gather(dummy.ptr, 1, 0, d_idx.ptr, n, dummy.ptr, src.ptr, width, dst.ptr)
+                if st != 0:
+                    raise RuntimeError(f"pacuda_gather: status {st}")
+            self.dev.synchronize()  # (the gather runs on the runner's stream, training on the network's)
+            for b in range(batches):
+                xb = DeviceArray.view(d_x, b * self.batch * w * 4, self.batch * w * 4)
+                tb = DeviceArray.view(d_t, b * self.batch * 4, self.batch * 4)
+                self.mlp.forward(xb, self.batch, d_y, train=True)
+                loss = self.mlp.backward(d_y, tb, self.batch, want_loss=b == batches - 1)
                 self.mlp.adam(lr * 0.5 * (1 + math.cos(math.pi * step / steps)))
                 step += 1
-            history.append(float(np.mean(losses)))
+            history.append(float(loss))
             if log:
                 log(f"epoch {epoch}: loss {history[-1]:.4f}")
-        for d in (d_x, d_t, d_y):
+        self.dev.synchronize()
+        for d in (d_xall, d_tall, d_x, d_t, d_y, d_idx, dummy):
             d.free()
         return history
 

@@ -80,10 +80,10 @@ def test_training_reduces_loss():
     net.destroy()
 
 
-@pytest.mark.parametrize("layers,batch", [(1, 64), (2, 192), (4, 4096 + 64)])
+@pytest.mark.parametrize("layers,batch", [(1, 64), (2, 192), (4, 4096 + 64), (2, 32768)])
 def test_inference_kernel(layers, batch):
     """the inference kernel (hidden 256, forward without train) against the oracle and against
-    the training forward, on whole and ragged 128-point tiles"""
+    the training forward (16- and 64-point workgroups), on whole and ragged 128-point tiles"""
     inputs, hidden = 14, 256
     print("mlp inference", layers, batch, flush=True)
     rng = np.random.default_rng(layers * 7 + batch)
