@@ -31,6 +31,7 @@ from pacuda_amd.unweight import Generator, Unweighting  # noqa: E402
 
 METRIC = "|M|^2 points/s (f64 bit-exact) + unweighted events/s, surrogate vs plain"
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+HBM_PEAK_TBS = 8.0        # MI355X HBM3E
 
 
 def log(msg):
@@ -324,6 +325,7 @@ def main():
         cpu = cpu_baseline(plan, ext_host, total64, sample)
         log(f"cpu {cpu['value']:.4g} points/s")
 
+    traffic = measured_traffic("fused_kernel<KF64>", n, len(plan.rows), args.waves_per_cu or 8)
     line = {
         "metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
@@ -335,8 +337,11 @@ def main():
         "roofline": {"bound": "mfma", "pipe": "fp64 VALU (MI355X FP64 vector peak = matrix peak)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": measured_traffic("fused_kernel<KF64>", n, len(plan.rows), args.waves_per_cu or 8),
+                     "traffic": traffic,
                      "traffic_unit": "bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     # the slot traffic beyond L2 at this run's kernel time, against the HBM peak
+                     "traffic_tb_per_s": traffic / (kernel_ms * 1e-3) / 1e12 if traffic else None,
+                     "traffic_frac_of_hbm_peak": traffic / (kernel_ms * 1e-3) / 1e12 / HBM_PEAK_TBS if traffic else None,
                      "kernel": "fused_kernel<KF64>", "kernel_ms": kernel_ms,
                      "algorithmic_bytes_per_point": 20 * 8 + 8},
         "cpu_baseline": cpu,
