@@ -561,10 +561,15 @@ void sort_contribution_runs(std::vector<Row> &rows) {
     }
 }
 
-// An executable row: kind, op, flags and the fields in word order.
+// An executable row: kind, op, flags and the fields in word order; z: the known-zero component
+// masks of a fused-path instruction (0 for the reference's own programs): z[0] the first loaded
+// operand (contribution parent 0, the in-place input, closure parent 0), z[1] contribution or
+// closure parent 1, z[2] a contribution's old destination value, z[3] the stored value. A masked
+// load reads +0 and a masked store is skipped, as the fused kernels do in the slot.
 struct Op {
     int kind, op, flags;
     uint32_t f[8];
+    uint32_t z[4];
 };
 
 bool program(const std::vector<Row> &rows, std::vector<Op> &ops, uint32_t *sources) {
@@ -635,10 +640,14 @@ void run_sources(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
             int c = source_values(code, mass, p, v);
             for (int k = 0; k < c; k++) {
                 W s = cx(fr * v[k].re - fi * v[k].im, fr * v[k].im + fi * v[k].re);
-                st_cur<R>(a, (dst + k) * a.ld + i, cx(narrow<R>(s.re), narrow<R>(s.im)));
+                if (!((o.z[3] >> k) & 1)) st_cur<R>(a, (dst + k) * a.ld + i, cx(narrow<R>(s.re), narrow<R>(s.im)));
             }
         }
     }
+}
+
+template <class R> Cx<R> ld_cur_z(const Arena &a, size_t at, uint32_t z, int k) {
+    return ((z >> k) & 1) ? cx(R(0.0), R(0.0)) : ld_cur<R>(a, at);
 }
 
 template <class K>
@@ -660,8 +669,8 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
             for (size_t i = i0; i < i1; i++) {
                 C l[6], r[4], v[6];
                 R q0[4], q1[4];
-                for (int k = 0; k < c0; k++) l[k] = ld_cur<R>(a, (p0 + k) * a.ld + i);
-                for (int k = 0; k < c1; k++) r[k] = ld_cur<R>(a, (p1 + k) * a.ld + i);
+                for (int k = 0; k < c0; k++) l[k] = ld_cur_z<R>(a, (p0 + k) * a.ld + i, o.z[0], k);
+                for (int k = 0; k < c1; k++) r[k] = ld_cur_z<R>(a, (p1 + k) * a.ld + i, o.z[1], k);
                 if (o.op == 6)
                     for (int k = 0; k < 4; k++) {
                         q0[k] = ld_form<R, Den>(a.mom, (f0 * 4 + k) * a.ld + i);
@@ -673,10 +682,11 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
                     C c = mul(v[k], scale);
                     if (!(o.flags & F_FIRST)) ac[k] = add(ac[k], c);
                     else if (o.flags & F_INIT) ac[k] = c;
-                    else ac[k] = add(ld_cur<R>(a, (dst + k) * a.ld + i), c);
+                    else ac[k] = add(ld_cur_z<R>(a, (dst + k) * a.ld + i, o.z[2], k), c);
                 }
                 if ((o.flags & F_LAST) && !(o.flags & F_HOLD))
-                    for (int k = 0; k < cd; k++) st_cur<R>(a, (dst + k) * a.ld + i, ac[k]);
+                    for (int k = 0; k < cd; k++)
+                        if (!((o.z[3] >> k) & 1)) st_cur<R>(a, (dst + k) * a.ld + i, ac[k]);
             }
         } else if (o.kind == K_CLEAR) {
             for (uint32_t k = 0; k < o.f[1]; k++)
@@ -696,11 +706,12 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
                 R reciprocal = ToReal<R, Den>::of(reciprocal_minkowski_norm(q[0], q[1], q[2], q[3]));
                 C c[4], v[4];
                 C *ac = &acc[6 * (i - i0)];
-                for (int k = 0; k < m; k++) c[k] = (o.flags & F_FROMACC) ? ac[k] : ld_cur<R>(a, (base + k) * a.ld + i);
+                for (int k = 0; k < m; k++)
+                    c[k] = (o.flags & F_FROMACC) ? ac[k] : ld_cur_z<R>(a, (base + k) * a.ld + i, o.z[0], k);
                 propagator_values(o.op, c, p, reciprocal, rf, v);
                 for (int k = 0; k < m; k++) {
                     if (o.flags & F_HOLD) ac[k] = v[k];
-                    else st_cur<R>(a, (base + k) * a.ld + i, v[k]);
+                    else if (!((o.z[3] >> k) & 1)) st_cur<R>(a, (base + k) * a.ld + i, v[k]);
                 }
             }
         } else if (o.kind == K_IDENTITY) {
@@ -709,9 +720,10 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
                 for (size_t i = i0; i < i1; i++) {
                     size_t at = (o.f[0] + k) * a.ld + i;
                     C *ac = &acc[6 * (i - i0)];
-                    C v = identity_value(fr, fi, (o.flags & F_FROMACC) ? ac[k] : ld_cur<R>(a, at));
+                    const int b = k < 16 ? (int)k : 31;  // (no mask bit beyond 16 components)
+                    C v = identity_value(fr, fi, (o.flags & F_FROMACC) ? ac[k] : ld_cur_z<R>(a, at, o.z[0], b));
                     if (o.flags & F_HOLD) ac[k] = v;
-                    else st_cur<R>(a, at, v);
+                    else if (!((o.z[3] >> b) & 1)) st_cur<R>(a, at, v);
                 }
         } else if (o.kind == K_CLOSURE) {
             uint32_t p0 = o.f[0], p1 = o.f[1], amp = o.f[2], x = o.f[3], first = o.f[4], cnt = o.f[5];
@@ -720,7 +732,9 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
                 size_t at = amp * a.ld + i;
                 Amp dr = ldr<Amp>(a.are, at), di = ldr<Amp>(a.aim, at);
                 for (uint32_t k = 0; k < cnt; k++) {
-                    C v = closure_increment(ld_cur<R>(a, (p0 + k) * a.ld + i), ld_cur<R>(a, (p1 + k) * a.ld + i),
+                    const int b = k < 8 ? (int)k : 31;
+                    C v = closure_increment(ld_cur_z<R>(a, (p0 + k) * a.ld + i, o.z[0], b),
+                                            ld_cur_z<R>(a, (p1 + k) * a.ld + i, o.z[1], b),
                                             ldr<R>(a.fre, first + k), ldr<R>(a.fim, first + k), rfr, rfi);
                     dr = dr + widen(v.re);
                     di = di + widen(v.im);
@@ -1236,8 +1250,9 @@ int refcpu_evaluate(uint32_t kernel, const uint64_t *words, uint32_t word_count,
 // The same plan run from an instruction stream of the library's fused path (pacuda_plan_program:
 // 8 u32 per instruction, w[0] = kind | op << 8 | flags << 16, the operands in w[1..7]); current
 // plane indices >= 2^24 (on-chip slots: LDS, then the register tier) are mapped after the slot's
-// `slot_planes` planes, and form fields keep their low 24 bits (the bits above name the register
-// slot a fused kernel keeps the form in). Lets the tests check, on the CPU, that the library's
+// `slot_planes` planes, form fields keep their low 24 bits (the bits above name the register
+// slot a fused kernel keeps the form in), and the known-zero masks in spare bits are honoured
+// as the kernels honour them in the slot (see Op). Lets the tests check, on the CPU, that the library's
 // program passes leave every value unchanged.
 int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t count, uint32_t sources,
                             uint32_t slot_planes, uint32_t lds_planes, uint32_t amplitude_planes, uint32_t form_count,
@@ -1262,18 +1277,29 @@ int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t cou
         switch (o.kind) {
         case K_CONTRIB:
             o.f[0] = (o.flags & F_INIT) ? 1 : 0;
-            o.f[1] = w[1];
+            o.f[1] = w[1] & 0xffff;
             o.f[2] = plane(w[2]); o.f[3] = plane(w[3]); o.f[4] = plane(w[4]);
-            o.f[5] = form(w[5]); o.f[6] = form(w[6]); o.f[7] = w[7];
+            o.f[5] = form(w[5]); o.f[6] = form(w[6]); o.f[7] = w[7] & 0xffff;
+            o.z[0] = (w[0] >> 22) & 63; o.z[1] = w[0] >> 28; o.z[2] = (w[1] >> 16) & 63; o.z[3] = (w[7] >> 16) & 63;
             break;
         case K_CLEAR: o.f[0] = plane(w[1]); o.f[1] = w[2]; break;
-        case K_PROP: o.f[0] = plane(w[1]); o.f[1] = form(w[2]); o.f[2] = w[3]; break;
-        case K_IDENTITY: o.f[0] = plane(w[1]); o.f[1] = w[2]; o.f[2] = w[3]; break;
+        case K_PROP:
+            o.f[0] = plane(w[1]); o.f[1] = form(w[2]); o.f[2] = w[3];
+            o.z[0] = w[4] & 15; o.z[3] = (w[4] >> 8) & 15;
+            break;
+        case K_IDENTITY:
+            o.f[0] = plane(w[1]); o.f[1] = w[2]; o.f[2] = w[3];
+            o.z[0] = w[4] & 0xffff; o.z[3] = w[4] >> 16;
+            break;
         case K_CLOSURE:
             o.f[0] = plane(w[1]); o.f[1] = plane(w[2]);
             o.f[2] = w[3]; o.f[3] = w[4]; o.f[4] = w[5]; o.f[5] = w[6];
+            o.z[0] = w[7] & 0xff; o.z[1] = (w[7] >> 8) & 0xff;
             break;
-        case K_SOURCE: o.f[0] = plane(w[1]); o.f[1] = w[2]; o.f[2] = w[3]; o.f[3] = w[4]; o.f[4] = w[5]; break;
+        case K_SOURCE:
+            o.f[0] = plane(w[1]); o.f[1] = w[2]; o.f[2] = w[3]; o.f[3] = w[4]; o.f[4] = w[5];
+            o.z[3] = w[6] & 15;
+            break;
         default: return PACMAN_RECURRENCE_INVALID;
         }
     }

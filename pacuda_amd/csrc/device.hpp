@@ -15,6 +15,9 @@
 #pragma clang fp contract(off)
 
 #define PCD __device__ __forceinline__
+// the row formulas and complex helpers also run on the host, over the zero lattice of the
+// fused path's structural-zero pass
+#define PCHD __host__ __device__ __forceinline__
 
 namespace pacuda {
 
@@ -76,13 +79,13 @@ PCD D operator/(D x, D y) {
 
 // ---- complex values ------------------------------------------------------------------------
 template <class R> struct Cx { R re, im; };
-template <class R> PCD Cx<R> cx(R re, R im) { Cx<R> c; c.re = re; c.im = im; return c; }
-template <class R> PCD Cx<R> add(Cx<R> a, Cx<R> b) { return cx(a.re + b.re, a.im + b.im); }
-template <class R> PCD Cx<R> sub(Cx<R> a, Cx<R> b) { return cx(a.re - b.re, a.im - b.im); }
-template <class R> PCD Cx<R> neg(Cx<R> a) { return cx(-a.re, -a.im); }
-template <class R> PCD Cx<R> mul(Cx<R> a, Cx<R> b) { return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
-template <class R> PCD Cx<R> mul_i(Cx<R> a) { return cx(-a.im, a.re); }
-template <class R> PCD Cx<R> mul_real(Cx<R> a, R s) { return cx(a.re * s, a.im * s); }
+template <class R> PCHD Cx<R> cx(R re, R im) { Cx<R> c; c.re = re; c.im = im; return c; }
+template <class R> PCHD Cx<R> add(Cx<R> a, Cx<R> b) { return cx(a.re + b.re, a.im + b.im); }
+template <class R> PCHD Cx<R> sub(Cx<R> a, Cx<R> b) { return cx(a.re - b.re, a.im - b.im); }
+template <class R> PCHD Cx<R> neg(Cx<R> a) { return cx(-a.re, -a.im); }
+template <class R> PCHD Cx<R> mul(Cx<R> a, Cx<R> b) { return cx(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+template <class R> PCHD Cx<R> mul_i(Cx<R> a) { return cx(-a.im, a.re); }
+template <class R> PCHD Cx<R> mul_real(Cx<R> a, R s) { return cx(a.re * s, a.im * s); }
 
 // ---- uniform tables ----------------------------------------------------------------------------
 // A pointer into the constant address space: a load at a wavefront-uniform address becomes a
@@ -195,15 +198,16 @@ template <> struct Cur<float> {
 // one lane's view of the v1 arena: point i (v1 programs carry no forwarding flags)
 struct PlaneArena {
     static constexpr bool forwarding = false;
+    static constexpr bool zero_masks = false;  // v1 programs carry no zero masks
     Arena a;
     size_t i;
-    template <class R> PCD Cx<R> cur(uint32_t p) const { return Cur<R>::ld(a, (size_t)p * a.ld + i); }
-    template <class R> PCD void put(uint32_t p, Cx<R> v) const { Cur<R>::st(a, (size_t)p * a.ld + i, v); }
-    template <class R, int N> PCD void curs(uint32_t p, Cx<R> *v) const {
+    template <class R> PCD Cx<R> cur(uint32_t p, uint32_t = 0) const { return Cur<R>::ld(a, (size_t)p * a.ld + i); }
+    template <class R> PCD void put(uint32_t p, Cx<R> v, uint32_t = 0) const { Cur<R>::st(a, (size_t)p * a.ld + i, v); }
+    template <class R, int N> PCD void curs(uint32_t p, Cx<R> *v, uint32_t = 0) const {
 #pragma unroll
         for (int k = 0; k < N; k++) v[k] = cur<R>(p + k);
     }
-    template <class R, int N> PCD void puts(uint32_t p, const Cx<R> *v) const {
+    template <class R, int N> PCD void puts(uint32_t p, const Cx<R> *v, uint32_t = 0) const {
 #pragma unroll
         for (int k = 0; k < N; k++) put<R>(p + k, v[k]);
     }
@@ -336,6 +340,7 @@ template <class R, bool REG = false, bool FREG = false> struct SlotArena {
     typedef typename Packed<R>::T V;
     typedef Buffer<sizeof(V)> B;
     static constexpr bool forwarding = true;
+    static constexpr bool zero_masks = true;  // a masked component is neither loaded nor stored in the slot
     static constexpr uint32_t PLANE = 64 * sizeof(V);  // bytes per current plane
     __amdgpu_buffer_rsrc_t slot;
     uint32_t amp_base, form_base;  // byte offsets of the amplitude and form regions in the slot
@@ -350,20 +355,28 @@ template <class R, bool REG = false, bool FREG = false> struct SlotArena {
         extern __shared__ __attribute__((aligned(16))) char pacuda_lds[];
         return reinterpret_cast<V *>(pacuda_lds + off);
     }
-    PCD V mem_ld(uint32_t p) const { return B::template load<V>(slot, idx * (uint32_t)sizeof(V), p * PLANE); }
-    PCD void mem_st(uint32_t p, V v) const { B::store(v, slot, idx * (uint32_t)sizeof(V), p * PLANE); }
+    // z = 1: a known zero, so the access is put out of the slot's range (the load returns +0 and
+    // the store is dropped, with no memory traffic)
+    static constexpr uint32_t OUT_OF_RANGE = 0x80000000u;
+    PCD V mem_ld(uint32_t p, uint32_t z = 0) const {
+        return B::template load<V>(slot, idx * (uint32_t)sizeof(V) + z * OUT_OF_RANGE, p * PLANE);
+    }
+    PCD void mem_st(uint32_t p, V v, uint32_t z = 0) const {
+        B::store(v, slot, idx * (uint32_t)sizeof(V) + z * OUT_OF_RANGE, p * PLANE);
+    }
 
-    template <class Q> PCD Cx<R> cur(uint32_t p) const {
+    template <class Q> PCD Cx<R> cur(uint32_t p, uint32_t z = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) return Packed<R>::get(*lds_at(lds + (p - LDS_PLANE) * PLANE));
-        return Packed<R>::get(mem_ld(p));
+        return Packed<R>::get(mem_ld(p, z));
     }
-    template <class Q> PCD void put(uint32_t p, Cx<R> v) const {
+    template <class Q> PCD void put(uint32_t p, Cx<R> v, uint32_t z = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) *lds_at(lds + (p - LDS_PLANE) * PLANE) = Packed<R>::set(v);
-        else mem_st(p, Packed<R>::set(v));
+        else mem_st(p, Packed<R>::set(v), z);
     }
-    template <class Q, int N> PCD void curs(uint32_t p, Cx<R> *v) const {
+    // zmask bit k: component k is a known zero (the LDS and register tiers hold it anyway)
+    template <class Q, int N> PCD void curs(uint32_t p, Cx<R> *v, uint32_t zmask = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) {
             if constexpr (REG && N == 4) {
@@ -377,10 +390,10 @@ template <class R, bool REG = false, bool FREG = false> struct SlotArena {
             for (int k = 0; k < N; k++) v[k] = Packed<R>::get(*lds_at(o + k * PLANE));
         } else {
 #pragma unroll
-            for (int k = 0; k < N; k++) v[k] = Packed<R>::get(mem_ld(p + k));
+            for (int k = 0; k < N; k++) v[k] = Packed<R>::get(mem_ld(p + k, (zmask >> k) & 1));
         }
     }
-    template <class Q, int N> PCD void puts(uint32_t p, const Cx<R> *v) const {
+    template <class Q, int N> PCD void puts(uint32_t p, const Cx<R> *v, uint32_t zmask = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) {
             if constexpr (REG && N == 4) {
@@ -394,7 +407,7 @@ template <class R, bool REG = false, bool FREG = false> struct SlotArena {
             for (int k = 0; k < N; k++) *lds_at(o + k * PLANE) = Packed<R>::set(v[k]);
         } else {
 #pragma unroll
-            for (int k = 0; k < N; k++) mem_st(p + k, Packed<R>::set(v[k]));
+            for (int k = 0; k < N; k++) mem_st(p + k, Packed<R>::set(v[k]), (zmask >> k) & 1);
         }
     }
     // form field f: a register slot when bits 24 and up are set (REG kernels), else a slot form
@@ -430,10 +443,10 @@ template <class R, bool REG = false, bool FREG = false> struct SlotArena {
 template <class R> PCD R reciprocal_minkowski_norm(R p0, R p1, R p2, R p3) {
     return R(1.0) / (p0 * p0 - p1 * p1 - p2 * p2 - p3 * p3);
 }
-template <class R> PCD Cx<R> minkowski_dot(const Cx<R> *l, const Cx<R> *r) {
+template <class R> PCHD Cx<R> minkowski_dot(const Cx<R> *l, const Cx<R> *r) {
     return sub(sub(sub(mul(l[0], r[0]), mul(l[1], r[1])), mul(l[2], r[2])), mul(l[3], r[3]));
 }
-template <class R> PCD Cx<R> minkowski_real_dot(const Cx<R> *l, const R *q) {
+template <class R> PCHD Cx<R> minkowski_real_dot(const Cx<R> *l, const R *q) {
     return sub(sub(sub(mul_real(l[0], q[0]), mul_real(l[1], q[1])), mul_real(l[2], q[2])), mul_real(l[3], q[3]));
 }
 
@@ -446,7 +459,7 @@ template <int OP> struct Shape {
 
 // ops 2-6 before the scale (convert/CONVENTIONS.md:71-79 in runner/recurrence_runner.cu:244-282's order)
 template <int OP, class R>
-PCD void contribution_values(const Cx<R> *l, const Cx<R> *r, const R *q0, const R *q1, Cx<R> *o) {
+PCHD void contribution_values(const Cx<R> *l, const Cx<R> *r, const R *q0, const R *q1, Cx<R> *o) {
     if constexpr (OP == 2) {
         o[0] = add(add(sub(neg(mul_i(mul(l[0], r[3]))), mul_i(mul(l[1], r[1]))), mul(l[1], r[2])), mul_i(mul(l[0], r[0])));
         o[1] = add(add(sub(neg(mul(l[0], r[2])), mul_i(mul(l[0], r[1]))), mul_i(mul(l[1], r[0]))), mul_i(mul(l[1], r[3])));
@@ -480,7 +493,7 @@ PCD void contribution_values(const Cx<R> *l, const Cx<R> *r, const R *q0, const 
 
 // ops 7-9 (runner/recurrence_runner.cu:284-314)
 template <int OP, class R>
-PCD void propagator_values(const Cx<R> *c, const R *p, R reciprocal, Cx<R> row_factor, Cx<R> *o) {
+PCHD void propagator_values(const Cx<R> *c, const R *p, R reciprocal, Cx<R> row_factor, Cx<R> *o) {
     if constexpr (OP == 9) {
         Cx<R> scale = mul_real(mul(row_factor, cx(R(0.0), R(-1.0))), reciprocal);
 #pragma unroll
@@ -504,9 +517,9 @@ PCD void propagator_values(const Cx<R> *c, const R *p, R reciprocal, Cx<R> row_f
     }
 }
 
-template <class R> PCD Cx<R> identity_value(R fr, R fi, Cx<R> v) { return cx(fr * v.re - fi * v.im, fr * v.im + fi * v.re); }
+template <class R> PCHD Cx<R> identity_value(R fr, R fi, Cx<R> v) { return cx(fr * v.re - fi * v.im, fr * v.im + fi * v.re); }
 
-template <class R> PCD Cx<R> closure_increment(Cx<R> l, Cx<R> r, R cr, R ci, R rfr, R rfi) {
+template <class R> PCHD Cx<R> closure_increment(Cx<R> l, Cx<R> r, R cr, R ci, R rfr, R rfi) {
     R scale_re = cr * rfr - ci * rfi;
     R scale_im = cr * rfi + ci * rfr;
     R product_re = l.re * r.re - l.im * r.im;

@@ -421,6 +421,122 @@ int row_accesses(const Instr &o, Access *a) {
     return -1;
 }
 
+// Structural zeros. A real value of the program is known to be zero for every input when it is
+// a product with a known zero or a sum of known zeros: a massless vector source's time component
+// (ext_gluon writes 0 there) and the imaginary part of its last component, a constant 0, and
+// what the row formulas make of those. The pass runs the row formulas themselves (the same
+// templates the kernels run) over this two-point lattice, in program order, and records per
+// operand which components are zero in both parts; the fused kernels then neither load nor store
+// such a component in the slot (a load gives +0). Replacing a zero of either sign by +0 leaves
+// every nonzero value of the program unchanged (x + 0 = x, x * 0 = 0 and no current is a
+// divisor), so the outputs keep their bits whenever the reference's are finite.
+struct ZeroVal {
+    bool zero;  // known to be zero (of either sign) for every input
+    ZeroVal() = default;
+    constexpr ZeroVal(double v) : zero(v == 0.0) {}
+    static ZeroVal known(bool z) {
+        ZeroVal r;
+        r.zero = z;
+        return r;
+    }
+};
+inline ZeroVal operator+(ZeroVal a, ZeroVal b) { return ZeroVal::known(a.zero && b.zero); }
+inline ZeroVal operator-(ZeroVal a, ZeroVal b) { return ZeroVal::known(a.zero && b.zero); }
+inline ZeroVal operator-(ZeroVal a) { return a; }
+inline ZeroVal operator*(ZeroVal a, ZeroVal b) { return ZeroVal::known(a.zero || b.zero); }
+
+// mask fields (the fused programs only): contribution: w[0] bits 22-27 parent 0, bits 28-31
+// parent 1, w[1] bits 16-21 the destination's old value, w[7] bits 16-21 the stored value
+// (the scale slot and factor keep bits 0-15); propagator and identity: w[4] = load mask | store
+// mask << 8 (propagator) or << 16 (identity); closure: w[7] = parent 0 | parent 1 << 8; source:
+// w[6] = store mask. Bit k: component k is zero.
+void zero_masks(std::vector<Instr> &ops, const pacuda_plan_desc *d) {
+    typedef Cx<ZeroVal> ZC;
+    const ZeroVal N = ZeroVal::known(false);
+    const ZC unknown = cx(N, N);
+    std::vector<ZC> st(d->current_planes, unknown);
+    auto zc = [](double re, double im) { return cx(ZeroVal(re), ZeroVal(im)); };
+    auto factor = [&](uint32_t k) { return zc(d->factor_re[k], d->factor_im[k]); };
+    auto mask = [](const ZC *v, uint32_t n) {
+        uint32_t m = 0;
+        for (uint32_t k = 0; k < n; k++)
+            if (v[k].re.zero && v[k].im.zero) m |= 1u << k;
+        return m;
+    };
+    ZC acc[6];
+    for (Instr &o : ops) {
+        const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff, flags = o.w[0] >> 16;
+        if (kind == K_SOURCE) {
+            const uint32_t family = o.w[5] & 7, c = family == 0 ? 1 : family == 1 ? 2 : 4;
+            ZC v[4] = {unknown, unknown, unknown, unknown};
+            if (family == 0) v[0] = zc(1.0, 0.0);
+            if (family >= 3 && d->scale_re[o.w[4]] == 0.0) {  // ext_gluon
+                v[0] = zc(0.0, 0.0);
+                v[3] = cx(N, ZeroVal(0.0));
+            }
+            const ZC f = factor(o.w[3]);
+            for (uint32_t k = 0; k < c; k++) st[o.w[1] + k] = identity_value(f.re, f.im, v[k]);
+            o.w[6] = mask(&st[o.w[1]], c);
+        } else if (kind == K_CLEAR) {
+            for (uint32_t k = 0; k < o.w[2]; k++) st[o.w[1] + k] = zc(0.0, 0.0);
+        } else if (kind == K_CONTRIB) {
+            int c0, c1, cd;
+            contribution_shape(op, &c0, &c1, &cd);
+            ZC l[6], r[4], v[6];
+            ZeroVal q0[4] = {N, N, N, N}, q1[4] = {N, N, N, N};
+            for (int k = 0; k < c0; k++) l[k] = st[o.w[2] + k];
+            for (int k = 0; k < c1; k++) r[k] = st[o.w[3] + k];
+            switch (op) {
+            case 2: contribution_values<2>(l, r, q0, q1, v); break;
+            case 3: contribution_values<3>(l, r, q0, q1, v); break;
+            case 4: contribution_values<4>(l, r, q0, q1, v); break;
+            case 5: contribution_values<5>(l, r, q0, q1, v); break;
+            default: contribution_values<6>(l, r, q0, q1, v); break;
+            }
+            const ZC scale = mul(zc(d->scale_re[o.w[1]], d->scale_im[o.w[1]]), factor(o.w[7]));
+            uint32_t old = 0, stored = 0;
+            if ((flags & F_FIRST) && !(flags & F_INIT)) old = mask(&st[o.w[4]], cd);
+            for (int k = 0; k < cd; k++) {
+                const ZC c = mul(v[k], scale);
+                if (!(flags & F_FIRST)) acc[k] = add(acc[k], c);
+                else if (flags & F_INIT) acc[k] = c;
+                else acc[k] = add(st[o.w[4] + k], c);
+            }
+            if (flags & F_LAST) {
+                for (int k = 0; k < cd; k++) st[o.w[4] + k] = acc[k];
+                stored = mask(acc, cd);
+            }
+            o.w[0] |= mask(l, c0) << 22 | mask(r, c1) << 28;
+            o.w[1] |= old << 16;
+            o.w[7] |= stored << 16;
+        } else if (kind == K_PROP) {
+            const uint32_t m = op == 9 ? 4 : 2;
+            ZC c[4], v[4];
+            ZeroVal p[4] = {N, N, N, N};
+            for (uint32_t k = 0; k < m; k++) c[k] = st[o.w[1] + k];
+            if (op == 9) propagator_values<9>(c, p, N, factor(o.w[3]), v);
+            else if (op == 7) propagator_values<7>(c, p, N, factor(o.w[3]), v);
+            else propagator_values<8>(c, p, N, factor(o.w[3]), v);
+            o.w[4] = mask(c, m) | mask(v, m) << 8;
+            for (uint32_t k = 0; k < m; k++) st[o.w[1] + k] = v[k];
+        } else if (kind == K_IDENTITY) {
+            const uint32_t n = o.w[2];
+            const ZC f = factor(o.w[3]);
+            uint32_t in = 0, out = 0;
+            for (uint32_t k = 0; k < n; k++) {
+                ZC &u = st[o.w[1] + k];
+                if (u.re.zero && u.im.zero) in |= (k < 16 ? 1u : 0u) << (k & 15);
+                u = identity_value(f.re, f.im, u);
+                if (u.re.zero && u.im.zero) out |= (k < 16 ? 1u : 0u) << (k & 15);
+            }
+            o.w[4] = n <= 16 ? in | out << 16 : 0;
+        } else if (kind == K_CLOSURE) {
+            const uint32_t n = o.w[6];
+            o.w[7] = n <= 8 ? mask(&st[o.w[1]], n) | mask(&st[o.w[2]], n) << 8 : 0;
+        }
+    }
+}
+
 // Forwarding: an in-place row (a propagator, or an identity of at most 6 components) whose
 // planes were last touched by the final row of the contribution run that produces them (or by
 // an in-place row on the same planes) moves right after that row; the producer keeps its result
@@ -692,6 +808,12 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
         fprintf(stderr, "pacman recurrence runner: the program reads a current plane before writing it\n");
         return PACMAN_RECURRENCE_INVALID;
     }
+    // (the mask fields share the scale slot's and factor's words)
+    if (d->scale_count > 0x10000 || d->factor_count > 0x10000) {
+        fprintf(stderr, "pacman recurrence runner: the fused path takes at most 65536 scales and factors\n");
+        return PACMAN_RECURRENCE_INVALID;
+    }
+    zero_masks(hp.ops, d);
     // the slot keeps only the live currents
     c.planes = d->current_planes;
     if (const uint32_t compacted = compact_planes(hp.ops, d->current_planes)) c.planes = compacted;

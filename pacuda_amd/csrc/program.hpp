@@ -64,7 +64,16 @@ HostProgram build_program(const uint64_t *words, uint32_t count, const uint8_t *
 void contribution_shape(int op, int *c0, int *c1, int *cd);
 
 // ---- the interpreter (device) ----------------------------------------------------------------
-// A is the arena (PlaneArena for the v1 path, SlotArena for the fused path).
+// A is the arena (PlaneArena for the v1 path, SlotArena for the fused path). The fused programs
+// carry known-zero component masks in spare bits (A::zero_masks): contribution: w[0] bits 22-27
+// parent 0, bits 28-31 parent 1, w[1] bits 16-21 the destination's old value, w[7] bits 16-21
+// the stored value (the scale slot and factor are bits 0-15 then); propagator: w[4] = load |
+// store << 8; identity: w[4] = load | store << 16; closure: w[7] = parent 0 | parent 1 << 8;
+// source: w[6] = store. Bit k: component k is zero for every input.
+template <class A> PCD uint32_t field16(uint32_t w) { return A::zero_masks ? (w & 0xffffu) : w; }
+template <class A> PCD uint32_t zbits(uint32_t w, int shift, uint32_t width) {
+    return A::zero_masks ? (w >> shift) & ((1u << width) - 1u) : 0u;
+}
 
 template <class K, int OP, class A>
 PCD void contribution(const A &a, const Instr &o, int flags, Cx<typename K::real> *acc) {
@@ -72,12 +81,12 @@ PCD void contribution(const A &a, const Instr &o, int flags, Cx<typename K::real
     typedef typename K::den Den;
     typedef Cx<R> C;
     constexpr int c0 = Shape<OP>::c0, c1 = Shape<OP>::c1, cd = Shape<OP>::cd;
-    const uint32_t dst = o.w[4];
-    C scale = mul(cx(ldc<R>(a.sre(), o.w[1]), ldc<R>(a.sim(), o.w[1])), cx(ldc<R>(a.fre(), o.w[7]), ldc<R>(a.fim(), o.w[7])));
+    const uint32_t dst = o.w[4], slot = field16<A>(o.w[1]), x = field16<A>(o.w[7]);
+    C scale = mul(cx(ldc<R>(a.sre(), slot), ldc<R>(a.sim(), slot)), cx(ldc<R>(a.fre(), x), ldc<R>(a.fim(), x)));
     C l[c0], r[c1], v[cd];
     R q0[4], q1[4];
-    a.template curs<R, c0>(o.w[2], l);
-    a.template curs<R, c1>(o.w[3], r);
+    a.template curs<R, c0>(o.w[2], l, zbits<A>(o.w[0], 22, 6));
+    a.template curs<R, c1>(o.w[3], r, zbits<A>(o.w[0], 28, 4));
     if constexpr (OP == 6) {
         double f0[4], f1[4];
         a.form(o.w[5], f0);
@@ -97,11 +106,11 @@ PCD void contribution(const A &a, const Instr &o, int flags, Cx<typename K::real
         for (int k = 0; k < cd; k++) acc[k] = mul(v[k], scale);
     } else {
         C old[cd];
-        a.template curs<R, cd>(dst, old);
+        a.template curs<R, cd>(dst, old, zbits<A>(o.w[1], 16, 6));
 #pragma unroll
         for (int k = 0; k < cd; k++) acc[k] = add(old[k], mul(v[k], scale));
     }
-    if ((flags & F_LAST) && !(flags & F_HOLD)) a.template puts<R, cd>(dst, acc);
+    if ((flags & F_LAST) && !(flags & F_HOLD)) a.template puts<R, cd>(dst, acc, zbits<A>(o.w[7], 16, 6));
 }
 
 template <class K, int OP, class A>
@@ -126,14 +135,14 @@ PCD void propagator(const A &a, const Instr &o, int flags, Cx<typename K::real> 
 #pragma unroll
         for (int k = 0; k < m; k++) c[k] = acc[k];
     } else {
-        a.template curs<R, m>(base, c);
+        a.template curs<R, m>(base, c, zbits<A>(o.w[4], 0, 4));
     }
     propagator_values<OP>(c, p, reciprocal, cx(ldc<R>(a.fre(), o.w[3]), ldc<R>(a.fim(), o.w[3])), v);
     if (flags & F_HOLD) {
 #pragma unroll
         for (int k = 0; k < m; k++) acc[k] = v[k];
     } else {
-        a.template puts<R, m>(base, v);
+        a.template puts<R, m>(base, v, zbits<A>(o.w[4], 8, 4));
     }
 }
 
@@ -176,10 +185,11 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
                 dr = d.re;
                 di = d.im;
             }
+            const uint32_t zl = zbits<A>(o.w[7], 0, 8), zr = zbits<A>(o.w[7], 8, 8);
             if (o.w[6] == 4) {  // four-plane currents: whole-object accesses
                 C l[4], r[4];
-                a.template curs<R, 4>(o.w[1], l);
-                a.template curs<R, 4>(o.w[2], r);
+                a.template curs<R, 4>(o.w[1], l, zl);
+                a.template curs<R, 4>(o.w[2], r, zr);
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     C v = closure_increment(l[k], r[k], ldc<R>(a.fre(), o.w[5] + k), ldc<R>(a.fim(), o.w[5] + k), rfr, rfi);
@@ -188,7 +198,8 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
                 }
             } else {
                 for (uint32_t k = 0; k < o.w[6]; k++) {
-                    C v = closure_increment(a.template cur<R>(o.w[1] + k), a.template cur<R>(o.w[2] + k),
+                    C v = closure_increment(a.template cur<R>(o.w[1] + k, (zl >> (k & 7)) & (k < 8)),
+                                            a.template cur<R>(o.w[2] + k, (zr >> (k & 7)) & (k < 8)),
                                             ldc<R>(a.fre(), o.w[5] + k), ldc<R>(a.fim(), o.w[5] + k), rfr, rfi);
                     dr = dr + widen(v.re);
                     di = di + widen(v.im);
@@ -197,14 +208,14 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
             a.template put_amp<Amp>(o.w[3], cx(dr, di));
         } else if (kind == K_IDENTITY) {
             R fr = ldc<R>(a.fre(), o.w[3]), fi = ldc<R>(a.fim(), o.w[3]);
-            const uint32_t count = o.w[2];
+            const uint32_t count = o.w[2], zin = zbits<A>(o.w[4], 0, 16), zout = zbits<A>(o.w[4], 16, 16);
             if (count == 4) {  // four-plane currents: whole-object accesses
                 C u[4], v[4];
                 if (flags & F_FROMACC) {
 #pragma unroll
                     for (int k = 0; k < 4; k++) v[k] = identity_value(fr, fi, acc[k]);
                 } else {
-                    a.template curs<R, 4>(o.w[1], u);
+                    a.template curs<R, 4>(o.w[1], u, zin);
 #pragma unroll
                     for (int k = 0; k < 4; k++) v[k] = identity_value(fr, fi, u[k]);
                 }
@@ -212,7 +223,7 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
 #pragma unroll
                     for (int k = 0; k < 4; k++) acc[k] = v[k];
                 } else {
-                    a.template puts<R, 4>(o.w[1], v);
+                    a.template puts<R, 4>(o.w[1], v, zout);
                 }
             } else if (flags & (F_FROMACC | F_HOLD)) {  // at most 6 components (the accumulators)
 #pragma unroll
@@ -220,15 +231,17 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
                     if ((uint32_t)k < count) {
                         C u;
                         if (flags & F_FROMACC) u = acc[k];
-                        else u = a.template cur<R>(o.w[1] + k);
+                        else u = a.template cur<R>(o.w[1] + k, (zin >> k) & 1);
                         const C v = identity_value(fr, fi, u);
                         if (flags & F_HOLD) acc[k] = v;
-                        else a.template put<R>(o.w[1] + k, v);
+                        else a.template put<R>(o.w[1] + k, v, (zout >> k) & 1);
                     }
                 }
             } else {
-                for (uint32_t k = 0; k < count; k++)
-                    a.template put<R>(o.w[1] + k, identity_value(fr, fi, a.template cur<R>(o.w[1] + k)));
+                for (uint32_t k = 0; k < count; k++) {
+                    const uint32_t zi = k < 16 ? (zin >> k) & 1 : 0, zo = k < 16 ? (zout >> k) & 1 : 0;
+                    a.template put<R>(o.w[1] + k, identity_value(fr, fi, a.template cur<R>(o.w[1] + k, zi)), zo);
+                }
             }
         } else if (kind == K_CLEAR) {
             if (o.w[2] == 4) {
@@ -260,12 +273,13 @@ __device__ inline void run_sources_point(const Instr *__restrict__ prog, uint32_
                 out[k] = cx(narrow<R>(s.re), narrow<R>(s.im));
             }
         }
+        const uint32_t z = zbits<A>(o.w[6], 0, 4);
         if (c == 4) {  // four-plane currents: a whole-object store
-            a.template puts<R, 4>(o.w[1], out);
+            a.template puts<R, 4>(o.w[1], out, z);
         } else {
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (k < c) a.template put<R>(o.w[1] + k, out[k]);
+                if (k < c) a.template put<R>(o.w[1] + k, out[k], (z >> k) & 1);
         }
     }
 }

@@ -71,6 +71,13 @@ def test_passes_do_their_work(lib):
     assert len(fwd) == len(plain)
     moved = np.count_nonzero((fwd[:, 0] >> 16) & F_FROMACC)
     assert moved == np.count_nonzero((fwd[:, 0] >> 16) & F_HOLD) > 0
+    # structural zeros: the gluon sources' time components and the tensor components built from
+    # them are masked (bits 22-31 of a contribution's first word), and the oracle run above,
+    # which reads +0 for every masked component and skips masked stores, kept every bit
+    contrib = fwd[(fwd[:, 0] & 0xff) == 2]
+    assert np.count_nonzero(contrib[:, 0] >> 22) > 0
+    sources = fwd[(fwd[:, 0] & 0xff) == 6]
+    assert np.all(sources[:, 6] & 1)  # component 0 of every massless vector source
     # the first closure of each amplitude plane starts from zero (no load of the zeroed plane)
     closures = fwd[(fwd[:, 0] & 0xff) == 5]
     zero = closures[((closures[:, 0] >> 16) & F_ZERO) != 0]
