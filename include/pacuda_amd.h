@@ -70,7 +70,10 @@ typedef struct {
  * 1: with forwarding through the accumulators, 2 + v: with LDS variant
  * v (10, 13, 20, 26, 40 planes; plane indices >= 2^24 are on-chip slots), 7 + v: LDS variant v
  * with a register tier of 16 planes (whole four-plane currents at slots from the variant's LDS
- * capacity on). Each instruction is 8 u32: w[0] =
+ * capacity on). A plane field with bit 31 or 30 set names slot plane (field & 0xffffff) with an
+ * on-chip copy at slot ((field & 0xffffff) + (field >> 24)) & 63: bit 31, a read loads the slot
+ * and keeps the copy; bit 30, a write also stores to the slot (reads use the copy). Each
+ * instruction is 8 u32: w[0] =
  * kind | op << 8 | flags << 16 (kinds 1 clear, 2 contribution, 3 propagator, 4 identity, 5
  * closure, 6 source; flags 1 init, 2 first, 4 last, 8 hold, 16 from the accumulators). out (may
  * be NULL) receives up to `capacity` instructions; *count their number, *sources the leading
@@ -104,7 +107,7 @@ typedef struct {
     uint32_t abi_version;   /* PACMAN_RECURRENCE_ABI */
     uint32_t struct_size;   /* sizeof(pacuda_launch) */
     uint32_t kernel;        /* PACMAN_RECURRENCE_KERNEL_* */
-    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 8 (f64), 16 (df64), 12 (mixed) */
+    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 12 (f64, mixed), 16 (df64) */
     uint32_t block;         /* threads per workgroup, a multiple of 64 up to 256, 0: 256 */
     uint32_t reserved;
     void *stream;           /* hipStream_t to queue on, or NULL: the library's stream */

@@ -566,11 +566,37 @@ void sort_contribution_runs(std::vector<Row> &rows) {
 // operand (contribution parent 0, the in-place input, closure parent 0), z[1] contribution or
 // closure parent 1, z[2] a contribution's old destination value, z[3] the stored value. A masked
 // load reads +0 and a masked store is skipped, as the fused kernels do in the slot.
+// dual[k] / chip[k]: plane field f[k] of a fused program names a slot plane with a copy on chip
+// (the library's DUAL_FILL: a read loads the slot and keeps the copy, a write goes on chip;
+// DUAL_THROUGH: reads on chip, writes on chip and to the slot); zero in the reference's programs
+enum { DUAL_FILL = 1u << 31, DUAL_THROUGH = 1u << 30 };
 struct Op {
     int kind, op, flags;
     uint32_t f[8];
     uint32_t z[4];
+    uint32_t dual[8], chip[8];
 };
+
+// a current component of plane field k (component c, known-zero bit b of mask z) read or written
+// as the op says where it lives
+template <class R> Cx<R> ld_cur_z(const Arena &a, size_t at, uint32_t z, int k);
+template <class R> Cx<R> ld_field(const Arena &a, const Op &o, int k, uint32_t c, uint32_t z, int b, size_t i) {
+    if (o.dual[k] & DUAL_FILL) {
+        const Cx<R> v = ld_cur_z<R>(a, (o.f[k] + c) * a.ld + i, z, b);
+        st_cur<R>(a, (o.chip[k] + c) * a.ld + i, v);
+        return v;
+    }
+    return ld_cur_z<R>(a, ((o.dual[k] ? o.chip[k] : o.f[k]) + c) * a.ld + i, z, b);
+}
+template <class R> void st_field(const Arena &a, const Op &o, int k, uint32_t c, uint32_t z, int b, Cx<R> v, size_t i) {
+    const bool masked = (z >> b) & 1;
+    if (o.dual[k]) {  // on chip every component is kept (as the kernels keep it)
+        st_cur<R>(a, (o.chip[k] + c) * a.ld + i, v);
+        if ((o.dual[k] & DUAL_THROUGH) && !masked) st_cur<R>(a, (o.f[k] + c) * a.ld + i, v);
+    } else if (!masked) {
+        st_cur<R>(a, (o.f[k] + c) * a.ld + i, v);
+    }
+}
 
 bool program(const std::vector<Row> &rows, std::vector<Op> &ops, uint32_t *sources) {
     for (size_t n = 0; n < rows.size(); n++) {
@@ -640,7 +666,7 @@ void run_sources(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
             int c = source_values(code, mass, p, v);
             for (int k = 0; k < c; k++) {
                 W s = cx(fr * v[k].re - fi * v[k].im, fr * v[k].im + fi * v[k].re);
-                if (!((o.z[3] >> k) & 1)) st_cur<R>(a, (dst + k) * a.ld + i, cx(narrow<R>(s.re), narrow<R>(s.im)));
+                st_field<R>(a, o, 0, k, o.z[3], k, cx(narrow<R>(s.re), narrow<R>(s.im)), i);
             }
         }
     }
@@ -669,8 +695,8 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
             for (size_t i = i0; i < i1; i++) {
                 C l[6], r[4], v[6];
                 R q0[4], q1[4];
-                for (int k = 0; k < c0; k++) l[k] = ld_cur_z<R>(a, (p0 + k) * a.ld + i, o.z[0], k);
-                for (int k = 0; k < c1; k++) r[k] = ld_cur_z<R>(a, (p1 + k) * a.ld + i, o.z[1], k);
+                for (int k = 0; k < c0; k++) l[k] = ld_field<R>(a, o, 2, k, o.z[0], k, i);
+                for (int k = 0; k < c1; k++) r[k] = ld_field<R>(a, o, 3, k, o.z[1], k, i);
                 if (o.op == 6)
                     for (int k = 0; k < 4; k++) {
                         q0[k] = ld_form<R, Den>(a.mom, (f0 * 4 + k) * a.ld + i);
@@ -682,15 +708,15 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
                     C c = mul(v[k], scale);
                     if (!(o.flags & F_FIRST)) ac[k] = add(ac[k], c);
                     else if (o.flags & F_INIT) ac[k] = c;
-                    else ac[k] = add(ld_cur_z<R>(a, (dst + k) * a.ld + i, o.z[2], k), c);
+                    else ac[k] = add(ld_field<R>(a, o, 4, k, o.z[2], k, i), c);
                 }
                 if ((o.flags & F_LAST) && !(o.flags & F_HOLD))
                     for (int k = 0; k < cd; k++)
-                        if (!((o.z[3] >> k) & 1)) st_cur<R>(a, (dst + k) * a.ld + i, ac[k]);
+                        st_field<R>(a, o, 4, k, o.z[3], k, ac[k], i);
             }
         } else if (o.kind == K_CLEAR) {
             for (uint32_t k = 0; k < o.f[1]; k++)
-                for (size_t i = i0; i < i1; i++) st_cur<R>(a, (o.f[0] + k) * a.ld + i, cx(R(0.0), R(0.0)));
+                for (size_t i = i0; i < i1; i++) st_field<R>(a, o, 0, k, 0, 0, cx(R(0.0), R(0.0)), i);
         } else if (o.kind == K_PROP) {
             int m = o.op == 9 ? 4 : 2;
             uint32_t base = o.f[0], form = o.f[1], x = o.f[2];
@@ -707,23 +733,22 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
                 C c[4], v[4];
                 C *ac = &acc[6 * (i - i0)];
                 for (int k = 0; k < m; k++)
-                    c[k] = (o.flags & F_FROMACC) ? ac[k] : ld_cur_z<R>(a, (base + k) * a.ld + i, o.z[0], k);
+                    c[k] = (o.flags & F_FROMACC) ? ac[k] : ld_field<R>(a, o, 0, k, o.z[0], k, i);
                 propagator_values(o.op, c, p, reciprocal, rf, v);
                 for (int k = 0; k < m; k++) {
                     if (o.flags & F_HOLD) ac[k] = v[k];
-                    else if (!((o.z[3] >> k) & 1)) st_cur<R>(a, (base + k) * a.ld + i, v[k]);
+                    else st_field<R>(a, o, 0, k, o.z[3], k, v[k], i);
                 }
             }
         } else if (o.kind == K_IDENTITY) {
             R fr = ldr<R>(a.fre, o.f[2]), fi = ldr<R>(a.fim, o.f[2]);
             for (uint32_t k = 0; k < o.f[1]; k++)
                 for (size_t i = i0; i < i1; i++) {
-                    size_t at = (o.f[0] + k) * a.ld + i;
                     C *ac = &acc[6 * (i - i0)];
                     const int b = k < 16 ? (int)k : 31;  // (no mask bit beyond 16 components)
-                    C v = identity_value(fr, fi, (o.flags & F_FROMACC) ? ac[k] : ld_cur_z<R>(a, at, o.z[0], b));
+                    C v = identity_value(fr, fi, (o.flags & F_FROMACC) ? ac[k] : ld_field<R>(a, o, 0, k, o.z[0], b, i));
                     if (o.flags & F_HOLD) ac[k] = v;
-                    else if (!((o.z[3] >> b) & 1)) st_cur<R>(a, at, v);
+                    else st_field<R>(a, o, 0, k, o.z[3], b, v, i);
                 }
         } else if (o.kind == K_CLOSURE) {
             uint32_t p0 = o.f[0], p1 = o.f[1], amp = o.f[2], x = o.f[3], first = o.f[4], cnt = o.f[5];
@@ -733,8 +758,8 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
                 Amp dr = ldr<Amp>(a.are, at), di = ldr<Amp>(a.aim, at);
                 for (uint32_t k = 0; k < cnt; k++) {
                     const int b = k < 8 ? (int)k : 31;
-                    C v = closure_increment(ld_cur_z<R>(a, (p0 + k) * a.ld + i, o.z[0], b),
-                                            ld_cur_z<R>(a, (p1 + k) * a.ld + i, o.z[1], b),
+                    C v = closure_increment(ld_field<R>(a, o, 0, k, o.z[0], b, i),
+                                            ld_field<R>(a, o, 1, k, o.z[1], b, i),
                                             ldr<R>(a.fre, first + k), ldr<R>(a.fim, first + k), rfr, rfi);
                     dr = dr + widen(v.re);
                     di = di + widen(v.im);
@@ -1250,7 +1275,9 @@ int refcpu_evaluate(uint32_t kernel, const uint64_t *words, uint32_t word_count,
 // The same plan run from an instruction stream of the library's fused path (pacuda_plan_program:
 // 8 u32 per instruction, w[0] = kind | op << 8 | flags << 16, the operands in w[1..7]); current
 // plane indices >= 2^24 (on-chip slots: LDS, then the register tier) are mapped after the slot's
-// `slot_planes` planes, form fields keep their low 24 bits (the bits above name the register
+// `slot_planes` planes, as are the on-chip copies of dual fields (bits 30-31 set: a slot plane in
+// the low 24 bits whose copy sits at on-chip slot (plane + bits 24-29) & 63, read and written as
+// the op's dual flags say), form fields keep their low 24 bits (the bits above name the register
 // slot a fused kernel keeps the form in), and the known-zero masks in spare bits are honoured
 // as the kernels honour them in the slot (see Op). Lets the tests check, on the CPU, that the library's
 // program passes leave every value unchanged.
@@ -1265,6 +1292,17 @@ int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t cou
     if (kernel > KERNEL_MIXED64 || block == 0 || !instr) return PACMAN_RECURRENCE_INVALID;
     const uint32_t LDS = 1u << 24;
     auto plane = [&](uint32_t p) { return p >= LDS ? slot_planes + (p - LDS) : p; };
+    // plane field k of op o from word w: a slot plane, an on-chip slot, or a slot plane with an
+    // on-chip copy at ((p + (w >> 24)) & 63)
+    auto field = [&](Op &o, int k, uint32_t w) {
+        if (w >= LDS && (w & (DUAL_FILL | DUAL_THROUGH))) {
+            o.f[k] = w & (LDS - 1);
+            o.dual[k] = w & (DUAL_FILL | DUAL_THROUGH);
+            o.chip[k] = slot_planes + (((w & (LDS - 1)) + (w >> 24)) & 63);
+        } else {
+            o.f[k] = plane(w);
+        }
+    };
     auto form = [](uint32_t f) { return f & 0xffffffu; };
     std::vector<Op> ops(count);
     for (uint32_t n = 0; n < count; n++) {
@@ -1278,26 +1316,26 @@ int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t cou
         case K_CONTRIB:
             o.f[0] = (o.flags & F_INIT) ? 1 : 0;
             o.f[1] = w[1] & 0xffff;
-            o.f[2] = plane(w[2]); o.f[3] = plane(w[3]); o.f[4] = plane(w[4]);
+            field(o, 2, w[2]); field(o, 3, w[3]); field(o, 4, w[4]);
             o.f[5] = form(w[5]); o.f[6] = form(w[6]); o.f[7] = w[7] & 0xffff;
             o.z[0] = (w[0] >> 22) & 63; o.z[1] = w[0] >> 28; o.z[2] = (w[1] >> 16) & 63; o.z[3] = (w[7] >> 16) & 63;
             break;
-        case K_CLEAR: o.f[0] = plane(w[1]); o.f[1] = w[2]; break;
+        case K_CLEAR: field(o, 0, w[1]); o.f[1] = w[2]; break;
         case K_PROP:
-            o.f[0] = plane(w[1]); o.f[1] = form(w[2]); o.f[2] = w[3];
+            field(o, 0, w[1]); o.f[1] = form(w[2]); o.f[2] = w[3];
             o.z[0] = w[4] & 15; o.z[3] = (w[4] >> 8) & 15;
             break;
         case K_IDENTITY:
-            o.f[0] = plane(w[1]); o.f[1] = w[2]; o.f[2] = w[3];
+            field(o, 0, w[1]); o.f[1] = w[2]; o.f[2] = w[3];
             o.z[0] = w[4] & 0xffff; o.z[3] = w[4] >> 16;
             break;
         case K_CLOSURE:
-            o.f[0] = plane(w[1]); o.f[1] = plane(w[2]);
+            field(o, 0, w[1]); field(o, 1, w[2]);
             o.f[2] = w[3]; o.f[3] = w[4]; o.f[4] = w[5]; o.f[5] = w[6];
             o.z[0] = w[7] & 0xff; o.z[1] = (w[7] >> 8) & 0xff;
             break;
         case K_SOURCE:
-            o.f[0] = plane(w[1]); o.f[1] = w[2]; o.f[2] = w[3]; o.f[3] = w[4]; o.f[4] = w[5];
+            field(o, 0, w[1]); o.f[1] = w[2]; o.f[2] = w[3]; o.f[3] = w[4]; o.f[4] = w[5];
             o.z[3] = w[6] & 15;
             break;
         default: return PACMAN_RECURRENCE_INVALID;

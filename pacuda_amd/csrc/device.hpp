@@ -251,6 +251,13 @@ template <> struct Packed<float> {
 // wavefront's LDS region for s below the launch's LDS capacity, else (s - capacity) in the lane's
 // register tier; only the fused path's passes write such indices.
 constexpr uint32_t LDS_PLANE = 1u << 24;
+// A slot plane p with an on-chip copy at slot s: flags | ((s - p) & 63) << 24 | p, so that the
+// field plus c names component c in both places (s + c < 64). DUAL_FILL: a read loads from the
+// slot and keeps a copy on chip, a write goes on chip; DUAL_THROUGH: a read reads on chip (unless
+// DUAL_FILL), a write goes on chip and to the slot.
+constexpr uint32_t DUAL_FILL = 1u << 31, DUAL_THROUGH = 1u << 30, DUAL = DUAL_FILL | DUAL_THROUGH;
+PCHD uint32_t dual_chip(uint32_t f) { return ((f & (LDS_PLANE - 1)) + (f >> 24)) & 63; }
+PCHD uint32_t dual_field(uint32_t flags, uint32_t p, uint32_t s) { return flags | ((s - p) & 63) << 24 | p; }
 // the register tier: REG_PLANES planes in units of one four-plane current
 constexpr uint32_t REG_PLANES = 16;
 // ... and FORM_REGS momentum forms: a form field of an instruction with bits 24 and up set
@@ -365,29 +372,76 @@ template <class R, bool REG = false, bool FREG = false> struct SlotArena {
         B::store(v, slot, idx * (uint32_t)sizeof(V) + z * OUT_OF_RANGE, p * PLANE);
     }
 
+    // single planes: the slot or LDS (the register tier holds whole four-plane currents only)
     template <class Q> PCD Cx<R> cur(uint32_t p, uint32_t z = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
-        if (p >= LDS_PLANE) return Packed<R>::get(*lds_at(lds + (p - LDS_PLANE) * PLANE));
+        if (p >= LDS_PLANE) {
+            if (p & DUAL) {
+                V *c = lds_at(lds + dual_chip(p) * PLANE);
+                if (!(p & DUAL_FILL)) return Packed<R>::get(*c);
+                const V x = mem_ld(p & (LDS_PLANE - 1), z);
+                *c = x;
+                return Packed<R>::get(x);
+            }
+            return Packed<R>::get(*lds_at(lds + (p - LDS_PLANE) * PLANE));
+        }
         return Packed<R>::get(mem_ld(p, z));
     }
     template <class Q> PCD void put(uint32_t p, Cx<R> v, uint32_t z = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
-        if (p >= LDS_PLANE) *lds_at(lds + (p - LDS_PLANE) * PLANE) = Packed<R>::set(v);
-        else mem_st(p, Packed<R>::set(v), z);
+        if (p >= LDS_PLANE) {
+            if (p & DUAL) {
+                *lds_at(lds + dual_chip(p) * PLANE) = Packed<R>::set(v);
+                if (p & DUAL_THROUGH) mem_st(p & (LDS_PLANE - 1), Packed<R>::set(v), z);
+                return;
+            }
+            *lds_at(lds + (p - LDS_PLANE) * PLANE) = Packed<R>::set(v);
+        } else {
+            mem_st(p, Packed<R>::set(v), z);
+        }
+    }
+    // whole objects on chip at slot s: the register tier from lds_cap up, else LDS
+    template <int N> PCD void chip_get(uint32_t s, Cx<R> *v) const {
+        if constexpr (REG && N == 4) {
+            if (s >= lds_cap) {
+                regs->get((s - lds_cap) / 4, v);
+                return;
+            }
+        }
+        const uint32_t o = lds + s * PLANE;
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = Packed<R>::get(*lds_at(o + k * PLANE));
+    }
+    template <int N> PCD void chip_set(uint32_t s, const Cx<R> *v) const {
+        if constexpr (REG && N == 4) {
+            if (s >= lds_cap) {
+                regs->set((s - lds_cap) / 4, v);
+                return;
+            }
+        }
+        const uint32_t o = lds + s * PLANE;
+#pragma unroll
+        for (int k = 0; k < N; k++) *lds_at(o + k * PLANE) = Packed<R>::set(v[k]);
     }
     // zmask bit k: component k is a known zero (the LDS and register tiers hold it anyway)
     template <class Q, int N> PCD void curs(uint32_t p, Cx<R> *v, uint32_t zmask = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) {
-            if constexpr (REG && N == 4) {
-                if (p - LDS_PLANE >= lds_cap) {
-                    regs->get((p - LDS_PLANE - lds_cap) / 4, v);
-                    return;
-                }
-            }
-            const uint32_t o = lds + (p - LDS_PLANE) * PLANE;
+            if (p & DUAL) {
+                if (p & DUAL_FILL) {  // (copies go to LDS: the register tier takes objects on writes)
+                    const uint32_t o = lds + dual_chip(p) * PLANE;
 #pragma unroll
-            for (int k = 0; k < N; k++) v[k] = Packed<R>::get(*lds_at(o + k * PLANE));
+                    for (int k = 0; k < N; k++) {
+                        const V x = mem_ld((p & (LDS_PLANE - 1)) + k, (zmask >> k) & 1);
+                        *lds_at(o + k * PLANE) = x;
+                        v[k] = Packed<R>::get(x);
+                    }
+                } else {
+                    chip_get<N>(dual_chip(p), v);
+                }
+                return;
+            }
+            chip_get<N>(p - LDS_PLANE, v);
         } else {
 #pragma unroll
             for (int k = 0; k < N; k++) v[k] = Packed<R>::get(mem_ld(p + k, (zmask >> k) & 1));
@@ -396,15 +450,16 @@ template <class R, bool REG = false, bool FREG = false> struct SlotArena {
     template <class Q, int N> PCD void puts(uint32_t p, const Cx<R> *v, uint32_t zmask = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) {
-            if constexpr (REG && N == 4) {
-                if (p - LDS_PLANE >= lds_cap) {
-                    regs->set((p - LDS_PLANE - lds_cap) / 4, v);
-                    return;
-                }
-            }
-            const uint32_t o = lds + (p - LDS_PLANE) * PLANE;
+            if (p & DUAL) {
+                chip_set<N>(dual_chip(p), v);
+                if (p & DUAL_THROUGH) {
 #pragma unroll
-            for (int k = 0; k < N; k++) *lds_at(o + k * PLANE) = Packed<R>::set(v[k]);
+                    for (int k = 0; k < N; k++)
+                        mem_st((p & (LDS_PLANE - 1)) + k, Packed<R>::set(v[k]), (zmask >> k) & 1);
+                }
+                return;
+            }
+            chip_set<N>(p - LDS_PLANE, v);
         } else {
 #pragma unroll
             for (int k = 0; k < N; k++) mem_st(p + k, Packed<R>::set(v[k]), (zmask >> k) & 1);

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <mutex>
 #include <array>
+#include <unordered_map>
 #include <vector>
 
 #include "pacman_recurrence.h"
@@ -85,15 +86,16 @@ __global__ void reset_counter(uint32_t *counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
 }
 
-// Wavefronts per SIMD a kernel is built for: the register-tier kernels take two in the f64 and
-// df64 policies (up to 256 VGPRs) and three in the mixed ones (168 VGPRs), and also hold
-// FORM_REGS forms; the other kernels take four (128 VGPRs).
-template <class K, bool REG> constexpr int waves_per_simd() {
-    return REG ? (__is_same(typename K::real, float) ? 3 : 2) : 4;
+// The kernels' register tiers: TIER 2 keeps REG_PLANES current planes and FORM_REGS forms in
+// registers, built for two wavefronts per SIMD in the f64 and df64 policies (up to 256 VGPRs)
+// and three in the mixed ones (168 VGPRs); TIER 1 (f64 and df64 only) keeps the forms, for three
+// wavefronts; TIER 0 keeps neither, for four (128 VGPRs).
+template <class K, int TIER> constexpr int waves_per_simd() {
+    return TIER == 2 ? (__is_same(typename K::real, float) ? 3 : 2) : TIER == 1 ? 3 : 4;
 }
 
-template <class K, bool REG>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_simd<K, REG>())))
+template <class K, int TIER>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_simd<K, TIER>())))
 fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start, uint32_t item_count,
              uint32_t *counter, double *scratch, size_t slot_doubles, uint32_t lds_per_wave, uint32_t lds_variant,
              uint32_t lds_cap) {
@@ -124,7 +126,7 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
         const uint32_t C = pl.current_planes, A = pl.amplitude_planes, F = pl.form_count;
         typedef typename Packed<R>::T V;
         // forms in registers where the kernel has room for them (up to three wavefronts per SIMD)
-        constexpr bool FREG = REG && waves_per_simd<K, REG>() <= 3;
+        constexpr bool REG = TIER == 2, FREG = TIER >= 1 && waves_per_simd<K, TIER>() <= 3;
         SlotArena<R, REG, FREG> a;
         RegFile<R> regs;
         FormRegs fregs;
@@ -421,6 +423,159 @@ int row_accesses(const Instr &o, Access *a) {
     return -1;
 }
 
+// Row scheduling. The rows form blocks: each leading source, each contribution run (FIRST to
+// LAST on one destination: its partial sums live in the accumulators, so it stays whole), each
+// other row. A block depends on the earlier blocks that write a plane it reads or writes, or read
+// a plane it writes, and closures into one amplitude keep their order; every order that honours
+// these dependences runs the same operations on the same operands, and adds the same terms to
+// each destination in the same order, so every value keeps its bits. A list scheduler builds one
+// such order (sources first, in place): among the ready blocks it takes the one with the highest
+// score, ties to the earliest. A plane counts as recent while fewer than `window` plane touches
+// have passed since its last one (a stand-in for the on-chip tiers); the score adds
+// `w_recent` per recent plane a block reads and `w_last` per plane it reads for the last time.
+// Returns an empty program when a row is not understood.
+std::vector<Instr> schedule_rows(const std::vector<Instr> &ops, uint32_t sources, uint32_t planes, uint32_t window,
+                                 int w_recent, int w_last) {
+    struct Block {
+        uint32_t first, count;
+        std::vector<uint32_t> reads, writes;  // plane indices
+        int amplitude = -1;
+    };
+    std::vector<Block> blocks;
+    for (size_t n = 0; n < ops.size();) {
+        const int kind = ops[n].w[0] & 0xff;
+        Block b;
+        b.first = (uint32_t)n;
+        size_t m = n;
+        if (kind == K_CONTRIB) {
+            if (!((ops[n].w[0] >> 16) & F_FIRST)) return {};
+            while (m < ops.size() && (ops[m].w[0] & 0xff) == K_CONTRIB && ops[m].w[4] == ops[n].w[4] &&
+                   !((ops[m].w[0] >> 16) & F_LAST))
+                m++;
+            if (m == ops.size() || (ops[m].w[0] & 0xff) != K_CONTRIB || ops[m].w[4] != ops[n].w[4]) return {};
+        }
+        b.count = (uint32_t)(m - n + 1);
+        for (size_t r = n; r <= m; r++) {
+            const Instr &o = ops[r];
+            const int k = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff;
+            auto add = [&](std::vector<uint32_t> &v, uint32_t base, uint32_t len) {
+                for (uint32_t j = 0; j < len; j++) v.push_back(base + j);
+            };
+            if (k == K_SOURCE || k == K_CLEAR) {
+                Access a[3];
+                row_accesses(o, a);
+                add(b.writes, o.w[a[0].word], a[0].len);
+            } else if (k == K_CONTRIB) {
+                int c0, c1, cd;
+                contribution_shape(op, &c0, &c1, &cd);
+                add(b.reads, o.w[2], c0);
+                add(b.reads, o.w[3], c1);
+                add(b.reads, o.w[4], cd);  // (read when the run adds to the old value)
+                add(b.writes, o.w[4], cd);
+            } else if (k == K_PROP || k == K_IDENTITY) {
+                const uint32_t len = k == K_PROP ? (op == 9 ? 4u : 2u) : o.w[2];
+                add(b.reads, o.w[1], len);
+                add(b.writes, o.w[1], len);
+            } else if (k == K_CLOSURE) {
+                add(b.reads, o.w[1], o.w[6]);
+                add(b.reads, o.w[2], o.w[6]);
+                b.amplitude = (int)o.w[3];
+            } else {
+                return {};
+            }
+        }
+        for (auto *v : {&b.reads, &b.writes}) {
+            std::sort(v->begin(), v->end());
+            v->erase(std::unique(v->begin(), v->end()), v->end());
+            if (!v->empty() && v->back() >= planes) return {};
+        }
+        blocks.push_back(std::move(b));
+        n = m + 1;
+    }
+    const uint32_t B = (uint32_t)blocks.size();
+    std::vector<std::vector<uint32_t>> succ(B);
+    std::vector<uint32_t> indeg(B, 0);
+    {
+        std::vector<int> writer(planes, -1);
+        std::vector<std::vector<uint32_t>> readers(planes);
+        std::unordered_map<int, uint32_t> last_amplitude;
+        auto edge = [&](uint32_t from, uint32_t to) {
+            if (from == to) return;
+            succ[from].push_back(to);
+            indeg[to]++;
+        };
+        for (uint32_t b = 0; b < B; b++) {
+            const Block &x = blocks[b];
+            for (uint32_t p : x.reads)
+                if (writer[p] >= 0) edge((uint32_t)writer[p], b);
+            for (uint32_t p : x.writes) {
+                if (writer[p] >= 0) edge((uint32_t)writer[p], b);
+                for (uint32_t r : readers[p]) edge(r, b);
+            }
+            if (x.amplitude >= 0) {
+                auto it = last_amplitude.find(x.amplitude);
+                if (it != last_amplitude.end()) edge(it->second, b);
+                last_amplitude[x.amplitude] = b;
+            }
+            for (uint32_t p : x.reads) readers[p].push_back(b);
+            for (uint32_t p : x.writes) {
+                writer[p] = (int)b;
+                readers[p].clear();
+            }
+        }
+    }
+    std::vector<uint32_t> pending_reads(planes, 0);  // unscheduled blocks reading each plane
+    for (const Block &x : blocks)
+        for (uint32_t p : x.reads) pending_reads[p]++;
+    std::vector<uint64_t> touched(planes, 0);  // clock of the last touch + 1 (0: never)
+    uint64_t clock = 0;
+    std::vector<uint32_t> ready;
+    std::vector<Instr> out;
+    out.reserve(ops.size());
+    auto emit = [&](uint32_t b) {
+        const Block &x = blocks[b];
+        for (uint32_t r = 0; r < x.count; r++) out.push_back(ops[x.first + r]);
+        for (uint32_t p : x.reads) pending_reads[p]--;
+        for (uint32_t p : x.reads) touched[p] = ++clock;
+        for (uint32_t p : x.writes) touched[p] = ++clock;
+        for (uint32_t s : succ[b])
+            if (--indeg[s] == 0) ready.push_back(s);
+    };
+    // the leading sources stay in front, in order (the kernels run them as a phase of their own)
+    uint32_t lead = 0;
+    while (lead < B && blocks[lead].first < sources) lead++;
+    for (uint32_t b = lead; b < B; b++)
+        if (indeg[b] == 0) ready.push_back(b);
+    for (uint32_t b = 0; b < lead; b++) {
+        if (indeg[b] != 0) return {};
+        emit(b);
+    }
+    ready.erase(std::remove_if(ready.begin(), ready.end(), [&](uint32_t r) { return r < lead; }), ready.end());
+    while (!ready.empty()) {
+        size_t best = 0;
+        int64_t best_score = INT64_MIN;
+        for (size_t k = 0; k < ready.size(); k++) {
+            const Block &x = blocks[ready[k]];
+            int64_t s = 0;
+            for (uint32_t p : x.reads) {
+                if (touched[p] && clock - (touched[p] - 1) < window) s += w_recent;
+                if (pending_reads[p] == 1) s += w_last;
+            }
+            s = s * (int64_t)B * 2 - ready[k];  // ties: the earliest block
+            if (s > best_score) {
+                best_score = s;
+                best = k;
+            }
+        }
+        const uint32_t b = ready[best];
+        ready[best] = ready.back();
+        ready.pop_back();
+        emit(b);
+    }
+    if (out.size() != ops.size()) return {};
+    return out;
+}
+
 // Structural zeros. A real value of the program is known to be zero for every input when it is
 // a product with a known zero or a sum of known zeros: a massless vector source's time component
 // (ext_gluon writes 0 there) and the imaginary part of its last component, a constant 0, and
@@ -687,6 +842,265 @@ uint32_t assign_lds(std::vector<Instr> &ops, uint32_t capacity, uint32_t base = 
     return used;
 }
 
+// On-chip allocation with fills and write-through. The program is straight-line code, so the
+// on-chip slots are allocated as registers are in a basic block: the current objects (as in
+// assign_lds) move on chip for parts of their lives. An access to an object in the slot takes
+// on-chip room when there is some — free slots, or slots whose holders are next read later than
+// this object is (the furthest-next-read rule) — and the object stays there until it is evicted
+// or its value dies. A read that brings it on chip loads from the slot and keeps the copy
+// (DUAL_FILL); writes while on chip go on chip only, and when an object is evicted before a later
+// read, its last write also stores to the slot (DUAL_THROUGH, set afterwards: nothing reads the
+// slot copy in between). An access that finds no room goes to the slot. LDS slots are
+// [0, lds_cap) (any object, contiguous), the register tier [lds_cap, lds_cap + reg) (four-plane
+// objects at multiples of four). A replay of the rewritten program checks that every read finds
+// the value last written (in the slot only its unmasked components). Returns false with the
+// program unchanged when an access does not name a whole object or the replay fails.
+bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, uint32_t *lds_used,
+                      uint32_t *reg_used) {
+    struct Acc {
+        uint32_t row;
+        int word;
+        uint32_t len;
+        bool rd, wr, create;
+        uint32_t rmask, wmask;
+        int obj;
+        uint32_t field;
+    };
+    std::vector<Acc> acc;
+    for (size_t n = 0; n < ops.size(); n++) {
+        const Instr &o = ops[n];
+        const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff, flags = o.w[0] >> 16;
+        const uint32_t r = (uint32_t)n;
+        if (kind == K_CONTRIB) {
+            int c0, c1, cd;
+            contribution_shape(op, &c0, &c1, &cd);
+            acc.push_back({r, 2, (uint32_t)c0, true, false, false, (o.w[0] >> 22) & 0x3f, 0, -1, 0});
+            acc.push_back({r, 3, (uint32_t)c1, true, false, false, o.w[0] >> 28, 0, -1, 0});
+            const bool rd = (flags & F_FIRST) && !(flags & F_INIT), wr = (flags & F_LAST) && !(flags & F_HOLD),
+                       cr = (flags & F_FIRST) && (flags & F_INIT);
+            if (rd || wr || cr)
+                acc.push_back({r, 4, (uint32_t)cd, rd, wr, cr, (o.w[1] >> 16) & 0x3f, (o.w[7] >> 16) & 0x3f, -1, 0});
+        } else if (kind == K_PROP) {
+            acc.push_back({r, 1, op == 9 ? 4u : 2u, !(flags & F_FROMACC), !(flags & F_HOLD), false, o.w[4] & 0xff,
+                           (o.w[4] >> 8) & 0xff, -1, 0});
+        } else if (kind == K_IDENTITY) {
+            acc.push_back({r, 1, o.w[2], !(flags & F_FROMACC), !(flags & F_HOLD), false, o.w[4] & 0xffff, o.w[4] >> 16,
+                           -1, 0});
+        } else if (kind == K_CLOSURE) {
+            acc.push_back({r, 1, o.w[6], true, false, false, o.w[7] & 0xff, 0, -1, 0});
+            acc.push_back({r, 2, o.w[6], true, false, false, (o.w[7] >> 8) & 0xff, 0, -1, 0});
+        } else if (kind == K_CLEAR) {
+            acc.push_back({r, 1, o.w[2], false, true, true, 0, 0, -1, 0});
+        } else if (kind == K_SOURCE) {
+            Access a[3];
+            row_accesses(o, a);
+            acc.push_back({r, 1, a[0].len, false, true, true, 0, o.w[6] & 15, -1, 0});
+        } else {
+            return false;
+        }
+    }
+    // objects: a creating access starts one; every other access names a whole live object
+    struct Obj { uint32_t base, len; };
+    std::vector<Obj> objs;
+    std::vector<int> live;
+    uint32_t slot_top = 0;
+    for (Acc &x : acc) {
+        if (x.len == 0) continue;
+        const uint32_t at = ops[x.row].w[x.word];
+        if (at >= LDS_PLANE || at + x.len > LDS_PLANE || x.len > 63) return false;
+        if (live.size() < at + x.len) live.resize(at + x.len, -1);
+        slot_top = std::max(slot_top, at + x.len);
+        if (x.create) {
+            objs.push_back({at, x.len});
+            for (uint32_t j = 0; j < x.len; j++) live[at + j] = (int)objs.size() - 1;
+        }
+        const int o = live[at];
+        if (o < 0 || objs[o].base != at || objs[o].len != x.len) return false;
+        x.obj = o;
+        x.field = at;
+    }
+    // the next read of the value each access leaves (none when the object is written over or dies first)
+    const uint32_t NONE = UINT32_MAX;
+    std::vector<uint32_t> next_read(acc.size(), NONE), ahead(objs.size(), NONE);
+    for (size_t i = acc.size(); i-- > 0;) {
+        const Acc &x = acc[i];
+        if (x.obj < 0 || !(x.rd || x.wr)) continue;
+        next_read[i] = ahead[x.obj];
+        if (x.rd) ahead[x.obj] = (uint32_t)i;
+        else ahead[x.obj] = NONE;  // a write over the value
+    }
+    const uint32_t cap = lds_cap + reg;
+    std::vector<int> occ(cap, -1), pos(objs.size(), -1);
+    std::vector<uint32_t> next_of(objs.size(), NONE), last_write(objs.size(), 0);
+    std::vector<char> dirty(objs.size(), 0);
+    uint32_t used_lds = 0, used_reg = 0;
+    auto leave = [&](int o) {
+        for (uint32_t j = 0; j < objs[o].len; j++) occ[pos[o] + j] = -1;
+        pos[o] = -1;
+    };
+    // (the objects a row names are not displaced by its own accesses: a row may read its operands
+    // one component at a time, interleaved)
+    // and the room of a value that dies in a row is free from the next row on
+    std::vector<uint32_t> named_in(objs.size(), UINT32_MAX);
+    std::vector<int> dying;
+    for (size_t i = 0; i < acc.size(); i++) {
+        Acc &x = acc[i];
+        if (i == 0 || acc[i - 1].row != x.row) {
+            for (int o : dying) leave(o);
+            dying.clear();
+            for (size_t j = i; j < acc.size() && acc[j].row == x.row; j++)
+                if (acc[j].obj >= 0) named_in[acc[j].obj] = x.row;
+        }
+        if (x.obj < 0 || !(x.rd || x.wr)) continue;
+        const int o = x.obj;
+        const uint32_t t = next_read[i], n = x.len, p = objs[o].base;
+        dying.erase(std::remove(dying.begin(), dying.end(), o), dying.end());
+        if (pos[o] >= 0) {
+            x.field = LDS_PLANE + (uint32_t)pos[o];
+            if (x.wr) {
+                dirty[o] = 1;
+                last_write[o] = (uint32_t)i;
+            }
+        } else if (t != NONE) {
+            // the room with the latest next read among the holders it displaces (free room first)
+            int best = -1;
+            uint64_t best_key = 0;
+            auto consider = [&](uint32_t s) {
+                uint32_t soonest = NONE;
+                int dirt = 0;
+                for (uint32_t j = 0; j < n; j++) {
+                    const int v = occ[s + j];
+                    if (v < 0) continue;
+                    if (next_of[v] <= t || named_in[v] == x.row) return;
+                    soonest = std::min(soonest, next_of[v]);
+                    dirt += dirty[v];
+                }
+                const uint64_t key = ((uint64_t)soonest << 8) | (uint64_t)(255 - std::min(dirt, 255));
+                if (best < 0 || key > best_key) {
+                    best = (int)s;
+                    best_key = key;
+                }
+            };
+            if (reg && n == 4 && !x.rd)  // (a read's copy goes to LDS)
+                for (uint32_t s = lds_cap; s + 4 <= cap; s += 4) consider(s);
+            for (uint32_t s = 0; s + n <= lds_cap; s++) consider(s);
+            if (best >= 0) {
+                for (uint32_t j = 0; j < n; j++) {
+                    const int v = occ[best + j];
+                    if (v < 0) continue;
+                    if (dirty[v]) {  // its last write also stores to the slot
+                        Acc &w = acc[last_write[v]];
+                        w.field = (w.field & DUAL) ? (w.field | DUAL_THROUGH)
+                                                   : dual_field(DUAL_THROUGH, objs[v].base, w.field - LDS_PLANE);
+                        dirty[v] = 0;
+                    }
+                    leave(v);
+                }
+                pos[o] = best;
+                for (uint32_t j = 0; j < n; j++) occ[best + j] = o;
+                x.field = x.rd ? dual_field(DUAL_FILL, p, (uint32_t)best) : LDS_PLANE + (uint32_t)best;
+                dirty[o] = x.wr;
+                if (x.wr) last_write[o] = (uint32_t)i;
+                if ((uint32_t)best >= lds_cap) used_reg = std::max(used_reg, (uint32_t)best + n - lds_cap);
+                else used_lds = std::max(used_lds, (uint32_t)best + n);
+            }
+        }
+        next_of[o] = t;
+        if (t == NONE && pos[o] >= 0) {  // the value dies here
+            dirty[o] = 0;
+            dying.push_back(o);
+        }
+    }
+    if (!used_lds && !used_reg) return false;
+    // replay: each location holds (object, version) of its last write
+    std::vector<std::pair<int, uint32_t>> slot(slot_top, {-1, 0}), chip(cap, {-1, 0});
+    std::vector<uint32_t> version(objs.size(), 0);
+    // (within a row, no on-chip copy a read keeps may overlap another access's on-chip planes: the
+    // kernels may interleave a row's operand reads component by component)
+    std::vector<std::array<uint32_t, 3>> row_chip;  // (first, end, is a copy) of the row's accesses
+    for (size_t i = 0; i < acc.size(); i++) {
+        const Acc &x = acc[i];
+        if (i == 0 || acc[i - 1].row != x.row) row_chip.clear();
+        if (x.obj < 0 || !(x.rd || x.wr)) continue;
+        if (x.rd && x.field >= LDS_PLANE) {
+            const bool copy = (x.field & DUAL_FILL) != 0;
+            const uint32_t s0 = (x.field & DUAL) ? dual_chip(x.field) : x.field - LDS_PLANE;
+            for (const auto &c : row_chip)
+                if ((copy || c[2]) && s0 < c[1] && c[0] < s0 + x.len) return false;
+            row_chip.push_back({s0, s0 + x.len, copy ? 1u : 0u});
+        }
+        const std::pair<int, uint32_t> now{x.obj, version[x.obj]};
+        const uint32_t f = x.field;
+        const bool dual = f >= LDS_PLANE && (f & DUAL), onchip = f >= LDS_PLANE && !(f & DUAL);
+        const uint32_t q = dual ? (f & (LDS_PLANE - 1)) : f, s = dual ? dual_chip(f) : f - LDS_PLANE;
+        if ((dual || onchip) && (s + x.len > cap || (s >= lds_cap && (x.len != 4 || (s - lds_cap) % 4))))
+            return false;
+        if (x.rd && dual && (f & DUAL_FILL) && s + x.len > lds_cap) return false;
+        if (x.rd) {
+            if (onchip || (dual && !(f & DUAL_FILL))) {
+                for (uint32_t j = 0; j < x.len; j++)
+                    if (chip[s + j] != now) return false;
+            } else {
+                for (uint32_t j = 0; j < x.len; j++)
+                    if (!((x.rmask >> j) & 1) && slot[q + j] != now) return false;
+                if (dual)
+                    for (uint32_t j = 0; j < x.len; j++) chip[s + j] = now;
+            }
+        }
+        if (x.wr) {
+            const std::pair<int, uint32_t> next{x.obj, ++version[x.obj]};
+            if (onchip || dual)
+                for (uint32_t j = 0; j < x.len; j++) chip[s + j] = next;
+            if (!onchip && (!dual || (f & DUAL_THROUGH)))
+                for (uint32_t j = 0; j < x.len; j++)
+                    if (!((x.wmask >> j) & 1)) slot[q + j] = next;
+        }
+    }
+    for (const Acc &x : acc)
+        if (x.obj >= 0) ops[x.row].w[x.word] = x.field;
+    *lds_used = used_lds;
+    *reg_used = used_reg;
+    return true;
+}
+
+// The slot's share of a fused program's work: the current components it loads from and stores
+// to the slot per point (on-chip planes and known-zero components take none).
+uint64_t slot_accesses(const std::vector<Instr> &ops) {
+    uint64_t n = 0;
+    auto count = [&](uint32_t f, uint32_t len, uint32_t zero, bool load = true) {
+        if (f >= LDS_PLANE && !(f & (load ? DUAL_FILL : DUAL_THROUGH))) return;
+        for (uint32_t j = 0; j < len; j++) n += !((zero >> j) & 1);
+    };
+    for (const Instr &o : ops) {
+        const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff, flags = o.w[0] >> 16;
+        if (kind == K_CONTRIB) {
+            int c0, c1, cd;
+            contribution_shape(op, &c0, &c1, &cd);
+            count(o.w[2], c0, (o.w[0] >> 22) & 0x3f);
+            count(o.w[3], c1, (o.w[0] >> 28) & 0xf);
+            if ((flags & F_FIRST) && !(flags & F_INIT)) count(o.w[4], cd, (o.w[1] >> 16) & 0x3f);
+            if ((flags & F_LAST) && !(flags & F_HOLD)) count(o.w[4], cd, (o.w[7] >> 16) & 0x3f, false);
+        } else if (kind == K_PROP) {
+            const uint32_t m = op == 9 ? 4 : 2;
+            if (!(flags & F_FROMACC)) count(o.w[1], m, o.w[4] & 0xff);
+            if (!(flags & F_HOLD)) count(o.w[1], m, (o.w[4] >> 8) & 0xff, false);
+        } else if (kind == K_IDENTITY) {
+            if (!(flags & F_FROMACC)) count(o.w[1], o.w[2], o.w[4] & 0xffff);
+            if (!(flags & F_HOLD)) count(o.w[1], o.w[2], o.w[4] >> 16, false);
+        } else if (kind == K_CLOSURE) {
+            count(o.w[1], o.w[6], o.w[7] & 0xff);
+            count(o.w[2], o.w[6], (o.w[7] >> 8) & 0xff);
+        } else if (kind == K_CLEAR) {
+            count(o.w[1], o.w[2], 0, false);
+        } else if (kind == K_SOURCE) {
+            Access a[3];
+            row_accesses(o, a);
+            count(o.w[1], a[0].len, o.w[6], false);
+        }
+    }
+    return n;
+}
+
 struct FusedState {
     std::mutex lock;
     double *scratch = nullptr;
@@ -817,6 +1231,7 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
         return PACMAN_RECURRENCE_INVALID;
     }
     zero_masks(hp.ops, d);
+    const std::vector<Instr> built = hp.ops;
     // the slot keeps only the live currents
     c.planes = d->current_planes;
     if (const uint32_t compacted = compact_planes(hp.ops, d->current_planes)) c.planes = compacted;
@@ -825,14 +1240,58 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
     forward_in_place(hp.ops);
     c.zero_fill = mark_first_closures(hp.ops, d->amplitude_planes);
     c.fwd = hp.ops;
-    for (int v = 0; v < LDS_VARIANTS; v++) {
-        c.lds[v] = c.fwd;
-        c.lds_used[v] = assign_lds(c.lds[v], LDS_CAPS[v]);
-        c.reg[v] = c.lds[v];
-        c.reg_used[v] = assign_lds(c.reg[v], REG_PLANES, LDS_CAPS[v], 4);
+    // the on-chip programs: the passes over the rows as built and over each candidate order, the
+    // allocator's placement (the static one where it gives up), keeping per variant and tier the
+    // program that leaves the fewest slot accesses
+    std::vector<std::vector<Instr>> orders{built};
+    static const int weights[][2] = {{1, 0}, {1, 1}, {0, 1}, {2, 1}, {1, 2}, {4, 1}};
+    for (uint32_t window : {24u, 72u})
+        for (const auto &w : weights) {
+            std::vector<Instr> o = schedule_rows(built, hp.sources, d->current_planes, window, w[0], w[1]);
+            if (!o.empty()) orders.push_back(std::move(o));
+        }
+    uint64_t best[2][LDS_VARIANTS];
+    bool have[2][LDS_VARIANTS] = {};
+    uint32_t reg_lds_used[LDS_VARIANTS] = {};
+    for (const std::vector<Instr> &order : orders) {
+        std::vector<Instr> ops = order;
+        uint32_t planes = d->current_planes;
+        if (const uint32_t compacted = compact_planes(ops, d->current_planes)) planes = compacted;
+        forward_in_place(ops);
+        mark_first_closures(ops, d->amplitude_planes);
+        for (int v = 0; v < LDS_VARIANTS; v++)
+            for (int tier = 0; tier < 2; tier++) {  // LDS only; LDS and the register tier
+                const uint32_t reg = tier ? REG_PLANES : 0;
+                std::vector<Instr> prog = ops;
+                uint32_t lds_used = 0, reg_used = 0;
+                if (!allocate_on_chip(prog, LDS_CAPS[v], reg, &lds_used, &reg_used)) {
+                    if (have[tier][v]) continue;
+                    prog = ops;
+                    lds_used = assign_lds(prog, LDS_CAPS[v]);
+                    if (reg) reg_used = assign_lds(prog, reg, LDS_CAPS[v], 4);
+                }
+                const uint64_t cost = slot_accesses(prog);
+                if (have[tier][v] && cost >= best[tier][v]) continue;
+                have[tier][v] = true;
+                best[tier][v] = cost;
+                if (tier) {
+                    c.reg[v] = std::move(prog);
+                    c.reg_used[v] = reg_used;
+                    reg_lds_used[v] = lds_used;
+                } else {
+                    c.lds[v] = std::move(prog);
+                    c.lds_used[v] = lds_used;
+                }
+                c.planes = std::max(c.planes, planes);
+            }
     }
+    // (the LDS region of a wavefront holds what either program of a variant places there)
+    for (int v = 0; v < LDS_VARIANTS; v++) c.lds_used[v] = std::max(c.lds_used[v], reg_lds_used[v]);
     choose_form_regs(c.fwd, d->form_count, c.reg_forms);
-    for (int v = 0; v < LDS_VARIANTS; v++) mark_form_regs(c.reg[v], c.reg_forms);
+    for (int v = 0; v < LDS_VARIANTS; v++) {
+        mark_form_regs(c.lds[v], c.reg_forms);
+        mark_form_regs(c.reg[v], c.reg_forms);
+    }
     return PACMAN_RECURRENCE_OK;
 }
 
@@ -1010,7 +1469,7 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
     sc.block = l->block ? l->block : 256;
     // default occupancy per kernel (measured on g g > g g g and g g > g g g g)
     const uint32_t per_cu = l->waves_per_cu                                  ? l->waves_per_cu
-                            : l->kernel == PACMAN_RECURRENCE_KERNEL_F64     ? 8
+                            : l->kernel == PACMAN_RECURRENCE_KERNEL_F64     ? 12
                             : l->kernel == PACMAN_RECURRENCE_KERNEL_DF64    ? 16
                                                                             : 12;
     sc.per_cu = per_cu;
@@ -1021,31 +1480,44 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
     return PACMAN_RECURRENCE_OK;
 }
 
-template <class K, bool REG> bool allow_lds() {
-    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(fused_kernel<K, REG>),
+template <class K, int TIER> bool allow_lds() {
+    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(fused_kernel<K, TIER>),
                                                            hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES),
                                        "fused kernel LDS");
     return ok;
 }
 
-// whether a launch of `kernel` at `per_cu` wavefronts per CU runs the register-tier kernel: when
-// its wavefronts fit the CU's registers
-bool use_regs(uint32_t kernel, uint32_t per_cu) {
+// The register tier of a launch of `kernel` at `per_cu` wavefronts per CU: the largest whose
+// wavefronts fit the CU's registers
+int tier_of(uint32_t kernel, uint32_t per_cu) {
     const bool mixed = kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64;
-    return per_cu <= 4u * (uint32_t)(mixed ? waves_per_simd<KMixed, true>() : waves_per_simd<KF64, true>());
+    if (mixed) return per_cu <= 4u * (uint32_t)waves_per_simd<KMixed, 2>() ? 2 : 0;
+    return per_cu <= 4u * (uint32_t)waves_per_simd<KF64, 2>() ? 2 : per_cu <= 4u * (uint32_t)waves_per_simd<KF64, 1>() ? 1 : 0;
 }
 
-template <class K> bool allow_lds(bool reg) { return reg ? allow_lds<K, true>() : allow_lds<K, false>(); }
+template <class K> constexpr bool has_tier1() { return !__is_same(typename K::real, float); }
+
+template <class K> bool allow_lds(int tier) {
+    if constexpr (has_tier1<K>())
+        if (tier == 1) return allow_lds<K, 1>();
+    return tier == 2 ? allow_lds<K, 2>() : allow_lds<K, 0>();
+}
 
 template <class K>
-void launch(const Schedule &sc, hipStream_t s, bool reg, size_t bytes, const ItemDev *items, const uint32_t *starts,
+void launch(const Schedule &sc, hipStream_t s, int tier, size_t bytes, const ItemDev *items, const uint32_t *starts,
             uint32_t count, uint32_t *counter, double *scratch, uint32_t per_wave, uint32_t variant) {
-    if (reg)
-        fused_kernel<K, true><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
-                                                                 variant, LDS_CAPS[variant]);
+    if constexpr (has_tier1<K>())
+        if (tier == 1) {
+            fused_kernel<K, 1><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot,
+                                                                per_wave, variant, LDS_CAPS[variant]);
+            return;
+        }
+    if (tier == 2)
+        fused_kernel<K, 2><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
+                                                            variant, LDS_CAPS[variant]);
     else
-        fused_kernel<K, false><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot,
-                                                                  per_wave, variant, LDS_CAPS[variant]);
+        fused_kernel<K, 0><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
+                                                            variant, LDS_CAPS[variant]);
 }
 
 // the LDS attribute of the kernel a launch of `sc` needs, set once (before any graph capture)
@@ -1054,11 +1526,11 @@ bool lds_ready(uint32_t kernel, const Schedule &sc) {
     size_t bytes;
     lds_shape(kernel, sc, &per_wave, &bytes, &variant);
     if (!bytes) return true;
-    const bool reg = use_regs(kernel, sc.per_cu);
-    if (kernel == PACMAN_RECURRENCE_KERNEL_F64) return allow_lds<KF64>(reg);
-    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED) return allow_lds<KMixed>(reg);
-    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) return allow_lds<KMixed64>(reg);
-    return allow_lds<KDF64>(reg);
+    const int tier = tier_of(kernel, sc.per_cu);
+    if (kernel == PACMAN_RECURRENCE_KERNEL_F64) return allow_lds<KF64>(tier);
+    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED) return allow_lds<KMixed>(tier);
+    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) return allow_lds<KMixed64>(tier);
+    return allow_lds<KDF64>(tier);
 }
 
 bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
@@ -1067,23 +1539,23 @@ bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const Item
     size_t bytes;
     lds_shape(kernel, sc, &per_wave, &bytes, &variant);
     reset_counter<<<1, 64, 0, s>>>(counter);
-    const bool reg = use_regs(kernel, sc.per_cu);
+    const int tier = tier_of(kernel, sc.per_cu);
     switch (kernel) {
     case PACMAN_RECURRENCE_KERNEL_F64:
-        if (bytes && !allow_lds<KF64>(reg)) return false;
-        launch<KF64>(sc, s, reg, bytes, items, starts, count, counter, scratch, per_wave, variant);
+        if (bytes && !allow_lds<KF64>(tier)) return false;
+        launch<KF64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
         break;
     case PACMAN_RECURRENCE_KERNEL_MIXED:
-        if (bytes && !allow_lds<KMixed>(reg)) return false;
-        launch<KMixed>(sc, s, reg, bytes, items, starts, count, counter, scratch, per_wave, variant);
+        if (bytes && !allow_lds<KMixed>(tier)) return false;
+        launch<KMixed>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
         break;
     case PACMAN_RECURRENCE_KERNEL_MIXED64:
-        if (bytes && !allow_lds<KMixed64>(reg)) return false;
-        launch<KMixed64>(sc, s, reg, bytes, items, starts, count, counter, scratch, per_wave, variant);
+        if (bytes && !allow_lds<KMixed64>(tier)) return false;
+        launch<KMixed64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
         break;
     default:
-        if (bytes && !allow_lds<KDF64>(reg)) return false;
-        launch<KDF64>(sc, s, reg, bytes, items, starts, count, counter, scratch, per_wave, variant);
+        if (bytes && !allow_lds<KDF64>(tier)) return false;
+        launch<KDF64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
         break;
     }
     return true;
