@@ -325,7 +325,7 @@ def main():
         cpu = cpu_baseline(plan, ext_host, total64, sample)
         log(f"cpu {cpu['value']:.4g} points/s")
 
-    traffic = measured_traffic("fused_kernel<KF64>", n, len(plan.rows), args.waves_per_cu or 8)
+    traffic = measured_traffic("fused_kernel<KF64>", n, len(plan.rows), args.waves_per_cu or 12)  # (the f64 default)
     line = {
         "metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,

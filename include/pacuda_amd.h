@@ -107,7 +107,7 @@ typedef struct {
     uint32_t abi_version;   /* PACMAN_RECURRENCE_ABI */
     uint32_t struct_size;   /* sizeof(pacuda_launch) */
     uint32_t kernel;        /* PACMAN_RECURRENCE_KERNEL_* */
-    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 12 (f64, mixed), 16 (df64) */
+    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 12 (f64), 16 (df64, mixed) */
     uint32_t block;         /* threads per workgroup, a multiple of 64 up to 256, 0: 256 */
     uint32_t reserved;
     void *stream;           /* hipStream_t to queue on, or NULL: the library's stream */

@@ -260,9 +260,10 @@ PCHD uint32_t dual_chip(uint32_t f) { return ((f & (LDS_PLANE - 1)) + (f >> 24))
 PCHD uint32_t dual_field(uint32_t flags, uint32_t p, uint32_t s) { return flags | ((s - p) & 63) << 24 | p; }
 // the register tier: REG_PLANES planes in units of one four-plane current
 constexpr uint32_t REG_PLANES = 16;
-// ... and FORM_REGS momentum forms: a form field of an instruction with bits 24 and up set
-// names the register slot ((field >> 24) - 1) holding form (field & 0xffffff)
-constexpr uint32_t FORM_REGS = 4;
+// ... and up to FORM_REGS momentum forms: a form field of an instruction with bits 24 and up set
+// names the register slot ((field >> 24) - 1) holding form (field & 0xffffff), when the kernel
+// keeps that many forms (else the form is read from the slot)
+constexpr uint32_t FORM_REGS = 6;
 constexpr uint32_t FORM_SLOT_SHIFT = 24;
 
 // Raw buffer accesses: a wavefront-uniform resource (base, size), a per-lane byte offset
@@ -313,26 +314,27 @@ template <class R> struct RegFile {
 static_assert(REG_PLANES == 16, "RegFile holds four units");
 
 // The register-resident forms of one lane (four raw doubles each), selected like RegFile.
-struct FormRegs {
-    double f0[4], f1[4], f2[4], f3[4];
+// N register-resident forms of one lane, selected by a uniform slot (unrolled compares: a
+// runtime index into the array would put it in scratch memory)
+template <int N> struct FormRegs {
+    double f[N][4];
     PCD void get(uint32_t s, double *raw) const {
-        switch (s) {
-        case 0: for (int k = 0; k < 4; k++) raw[k] = f0[k]; break;
-        case 1: for (int k = 0; k < 4; k++) raw[k] = f1[k]; break;
-        case 2: for (int k = 0; k < 4; k++) raw[k] = f2[k]; break;
-        default: for (int k = 0; k < 4; k++) raw[k] = f3[k]; break;
-        }
+#pragma unroll
+        for (int q = 0; q < N; q++)
+            if (s == (uint32_t)q) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) raw[k] = f[q][k];
+            }
     }
     PCD void set(uint32_t s, const double *raw) {
-        switch (s) {
-        case 0: for (int k = 0; k < 4; k++) f0[k] = raw[k]; break;
-        case 1: for (int k = 0; k < 4; k++) f1[k] = raw[k]; break;
-        case 2: for (int k = 0; k < 4; k++) f2[k] = raw[k]; break;
-        default: for (int k = 0; k < 4; k++) f3[k] = raw[k]; break;
-        }
+#pragma unroll
+        for (int q = 0; q < N; q++)
+            if (s == (uint32_t)q) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) f[q][k] = raw[k];
+            }
     }
 };
-static_assert(FORM_REGS == 4, "FormRegs holds four forms");
 
 // The fused path's arena: one wavefront's scratch slot for 64 points, each plane 64 lanes of one
 // vector: currents as Packed<R> (PLANE bytes per plane), then amplitudes as (re, im) raw pairs
@@ -341,9 +343,10 @@ static_assert(FORM_REGS == 4, "FormRegs holds four forms");
 // (lane offset, uniform plane offset). Every operand of a row is one whole object of the LDS
 // pass, so one residency test serves all its components.
 // REG: the lane has a register tier; an on-chip slot at or above lds_cap is in it (only whole
-// four-plane currents go there, so only the four-plane accessors look). FREG: the lane also holds
-// the register-resident forms (else a marked form field is read from the slot's copy).
-template <class R, bool REG = false, bool FREG = false> struct SlotArena {
+// four-plane currents go there, so only the four-plane accessors look). NF: the lane also holds
+// the first NF register-resident forms (a form field marked with a later slot, or any marked
+// field when NF is 0, is read from the slot's copy).
+template <class R, bool REG = false, int NF = 0> struct SlotArena {
     typedef typename Packed<R>::T V;
     typedef Buffer<sizeof(V)> B;
     static constexpr bool forwarding = true;
@@ -356,7 +359,7 @@ template <class R, bool REG = false, bool FREG = false> struct SlotArena {
     uint32_t lds;    // byte address of the lane's value in LDS plane 0 of the wavefront's region
     uint32_t lds_cap;     // on-chip slots below this are LDS planes
     RegFile<R> *regs;     // the register tier (REG only)
-    FormRegs *fregs;      // the register-resident forms (FREG only)
+    FormRegs<NF ? NF : 1> *fregs;  // the register-resident forms (NF > 0 only)
 
     static PCD V *lds_at(uint32_t off) {
         extern __shared__ __attribute__((aligned(16))) char pacuda_lds[];
@@ -467,8 +470,8 @@ template <class R, bool REG = false, bool FREG = false> struct SlotArena {
     }
     // form field f: a register slot when bits 24 and up are set (REG kernels), else a slot form
     PCD void form(uint32_t f, double *raw) const {
-        if constexpr (FREG) {
-            if (f >> FORM_SLOT_SHIFT) {
+        if constexpr (NF > 0) {
+            if ((f >> FORM_SLOT_SHIFT) && (f >> FORM_SLOT_SHIFT) <= (uint32_t)NF) {
                 fregs->get((f >> FORM_SLOT_SHIFT) - 1, raw);
                 return;
             }

@@ -86,12 +86,17 @@ __global__ void reset_counter(uint32_t *counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
 }
 
-// The kernels' register tiers: TIER 2 keeps REG_PLANES current planes and FORM_REGS forms in
+// The kernels' register tiers: TIER 2 keeps REG_PLANES current planes and four forms in
 // registers, built for two wavefronts per SIMD in the f64 and df64 policies (up to 256 VGPRs)
-// and three in the mixed ones (168 VGPRs); TIER 1 (f64 and df64 only) keeps the forms, for three
-// wavefronts; TIER 0 keeps neither, for four (128 VGPRs).
+// and three in the mixed ones (168 VGPRs); TIER 1 keeps only forms: FORM_REGS for three
+// wavefronts (f64, df64), four for four wavefronts (mixed); TIER 0 keeps neither, for four
+// (128 VGPRs).
+template <class K> constexpr bool f32_currents() { return __is_same(typename K::real, float); }
 template <class K, int TIER> constexpr int waves_per_simd() {
-    return TIER == 2 ? (__is_same(typename K::real, float) ? 3 : 2) : TIER == 1 ? 3 : 4;
+    return TIER == 2 ? (f32_currents<K>() ? 3 : 2) : TIER == 1 ? (f32_currents<K>() ? 4 : 3) : 4;
+}
+template <class K, int TIER> constexpr int register_forms() {
+    return TIER == 2 ? 4 : TIER == 1 ? (f32_currents<K>() ? 4 : (int)FORM_REGS) : 0;
 }
 
 template <class K, int TIER>
@@ -126,10 +131,11 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
         const uint32_t C = pl.current_planes, A = pl.amplitude_planes, F = pl.form_count;
         typedef typename Packed<R>::T V;
         // forms in registers where the kernel has room for them (up to three wavefronts per SIMD)
-        constexpr bool REG = TIER == 2, FREG = TIER >= 1 && waves_per_simd<K, TIER>() <= 3;
-        SlotArena<R, REG, FREG> a;
+        constexpr bool REG = TIER == 2;
+        constexpr int NF = register_forms<K, TIER>();
+        SlotArena<R, REG, NF> a;
         RegFile<R> regs;
-        FormRegs fregs;
+        FormRegs<NF ? NF : 1> fregs;
         a.regs = &regs;
         a.fregs = &fregs;
         a.lds_cap = lds_cap;
@@ -187,9 +193,9 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
         // the program's constants: df64 pairs for every kernel but f64 (as the v1 path converts them)
         const double *cons = DF ? pl.cdf : pl.c64;
         a.sre_ = cons; a.sim_ = cons + S; a.fre_ = cons + 2 * S; a.fim_ = cons + 2 * S + X;
-        if constexpr (FREG) {  // the register-resident forms, in the kernel's form layout
+        if constexpr (NF > 0) {  // the register-resident forms, in the kernel's form layout
 #pragma unroll
-            for (uint32_t q = 0; q < FORM_REGS; q++) {
+            for (uint32_t q = 0; q < (uint32_t)NF; q++) {
                 if (pl.reg_forms[q] < F) {
                     double raw[4];
                     a.form(pl.reg_forms[q], raw);
@@ -1471,7 +1477,7 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
     const uint32_t per_cu = l->waves_per_cu                                  ? l->waves_per_cu
                             : l->kernel == PACMAN_RECURRENCE_KERNEL_F64     ? 12
                             : l->kernel == PACMAN_RECURRENCE_KERNEL_DF64    ? 16
-                                                                            : 12;
+                                                                            : 16;
     sc.per_cu = per_cu;
     const uint32_t waves_per_block = sc.block / LANES;
     uint64_t waves = std::min<uint64_t>((uint64_t)compute_units() * per_cu, std::max<uint64_t>(sc.units, 1));
@@ -1489,30 +1495,27 @@ template <class K, int TIER> bool allow_lds() {
 
 // The register tier of a launch of `kernel` at `per_cu` wavefronts per CU: the largest whose
 // wavefronts fit the CU's registers
+template <class K> int tier_of(uint32_t per_cu) {
+    return per_cu <= 4u * (uint32_t)waves_per_simd<K, 2>()   ? 2
+           : per_cu <= 4u * (uint32_t)waves_per_simd<K, 1>() ? 1
+                                                             : 0;
+}
 int tier_of(uint32_t kernel, uint32_t per_cu) {
     const bool mixed = kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64;
-    if (mixed) return per_cu <= 4u * (uint32_t)waves_per_simd<KMixed, 2>() ? 2 : 0;
-    return per_cu <= 4u * (uint32_t)waves_per_simd<KF64, 2>() ? 2 : per_cu <= 4u * (uint32_t)waves_per_simd<KF64, 1>() ? 1 : 0;
+    return mixed ? tier_of<KMixed>(per_cu) : tier_of<KF64>(per_cu);
 }
 
-template <class K> constexpr bool has_tier1() { return !__is_same(typename K::real, float); }
-
 template <class K> bool allow_lds(int tier) {
-    if constexpr (has_tier1<K>())
-        if (tier == 1) return allow_lds<K, 1>();
-    return tier == 2 ? allow_lds<K, 2>() : allow_lds<K, 0>();
+    return tier == 2 ? allow_lds<K, 2>() : tier == 1 ? allow_lds<K, 1>() : allow_lds<K, 0>();
 }
 
 template <class K>
 void launch(const Schedule &sc, hipStream_t s, int tier, size_t bytes, const ItemDev *items, const uint32_t *starts,
             uint32_t count, uint32_t *counter, double *scratch, uint32_t per_wave, uint32_t variant) {
-    if constexpr (has_tier1<K>())
-        if (tier == 1) {
-            fused_kernel<K, 1><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot,
-                                                                per_wave, variant, LDS_CAPS[variant]);
-            return;
-        }
-    if (tier == 2)
+    if (tier == 1)
+        fused_kernel<K, 1><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
+                                                            variant, LDS_CAPS[variant]);
+    else if (tier == 2)
         fused_kernel<K, 2><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
                                                             variant, LDS_CAPS[variant]);
     else

@@ -18,6 +18,7 @@ import refcpu
 
 LDS_PLANE = 1 << 24
 REG_PLANES = 16
+FORM_REGS = 6
 F_HOLD, F_FROMACC, F_ZERO = 8, 16, 32
 
 
@@ -86,7 +87,7 @@ def test_passes_do_their_work(lib):
         prog, _, _, lds = plan_program(lib, plan, variant)
         assert 0 < lds <= cap
         assert np.any(prog[:, 1:5] >= LDS_PLANE)
-        # the register-tier variant: the same LDS slots, plus whole four-plane currents after them
+        # the register-tier variant: LDS slots, plus whole four-plane currents after them
         reg, _, _, onchip = plan_program(lib, plan, variant + 5)
         assert cap < onchip <= cap + REG_PLANES
         tier = (reg[:, 1:5] >= LDS_PLANE + cap) & (reg[:, 1:5] < LDS_PLANE + cap + REG_PLANES)
@@ -95,4 +96,4 @@ def test_passes_do_their_work(lib):
         kind, op = reg[:, 0] & 0xff, (reg[:, 0] >> 8) & 0xff
         fields = np.concatenate([reg[(kind == 2) & (op == 6)][:, 5:7].ravel(), reg[kind == 3][:, 2]])
         slots = fields >> 24
-        assert set(np.unique(slots)) == {0, 1, 2, 3, 4}
+        assert set(np.unique(slots)) == set(range(FORM_REGS + 1))
