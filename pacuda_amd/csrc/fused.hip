@@ -429,36 +429,39 @@ int row_accesses(const Instr &o, Access *a) {
     return -1;
 }
 
-// Row scheduling. The rows form blocks: each leading source, each contribution run (FIRST to
-// LAST on one destination: its partial sums live in the accumulators, so it stays whole), each
-// other row. A block depends on the earlier blocks that write a plane it reads or writes, or read
-// a plane it writes, and closures into one amplitude keep their order; every order that honours
-// these dependences runs the same operations on the same operands, and adds the same terms to
-// each destination in the same order, so every value keeps its bits. A list scheduler builds one
-// such order (sources first, in place): among the ready blocks it takes the one with the highest
-// score, ties to the earliest. A plane counts as recent while fewer than `window` plane touches
-// have passed since its last one (a stand-in for the on-chip tiers); the score adds
-// `w_recent` per recent plane a block reads and `w_last` per plane it reads for the last time.
-// Returns an empty program when a row is not understood.
-std::vector<Instr> schedule_rows(const std::vector<Instr> &ops, uint32_t sources, uint32_t planes, uint32_t window,
-                                 int w_recent, int w_last) {
+// Row blocks and their dependences. The rows form blocks: each leading source, each contribution
+// run (FIRST to LAST on one destination: its partial sums live in the accumulators, so it stays
+// whole), each other row. A block depends on the earlier blocks that write a plane it reads or
+// writes, or read a plane it writes, and closures into one amplitude keep their order; every
+// order that honours these dependences runs the same operations on the same operands, and adds
+// the same terms to each destination in the same order, so every value keeps its bits. The
+// leading sources stay in front, in order (the kernels run them as a phase of their own).
+struct BlockGraph {
     struct Block {
         uint32_t first, count;
         std::vector<uint32_t> reads, writes;  // plane indices
         int amplitude = -1;
     };
     std::vector<Block> blocks;
+    std::vector<std::vector<uint32_t>> succ, pred;
+    uint32_t lead = 0;  // the leading source blocks
+    bool ok = false;
+};
+
+BlockGraph block_graph(const std::vector<Instr> &ops, uint32_t sources, uint32_t planes) {
+    BlockGraph g;
+    auto &blocks = g.blocks;
     for (size_t n = 0; n < ops.size();) {
         const int kind = ops[n].w[0] & 0xff;
-        Block b;
+        BlockGraph::Block b;
         b.first = (uint32_t)n;
         size_t m = n;
         if (kind == K_CONTRIB) {
-            if (!((ops[n].w[0] >> 16) & F_FIRST)) return {};
+            if (!((ops[n].w[0] >> 16) & F_FIRST)) return g;
             while (m < ops.size() && (ops[m].w[0] & 0xff) == K_CONTRIB && ops[m].w[4] == ops[n].w[4] &&
                    !((ops[m].w[0] >> 16) & F_LAST))
                 m++;
-            if (m == ops.size() || (ops[m].w[0] & 0xff) != K_CONTRIB || ops[m].w[4] != ops[n].w[4]) return {};
+            if (m == ops.size() || (ops[m].w[0] & 0xff) != K_CONTRIB || ops[m].w[4] != ops[n].w[4]) return g;
         }
         b.count = (uint32_t)(m - n + 1);
         for (size_t r = n; r <= m; r++) {
@@ -487,51 +490,71 @@ std::vector<Instr> schedule_rows(const std::vector<Instr> &ops, uint32_t sources
                 add(b.reads, o.w[2], o.w[6]);
                 b.amplitude = (int)o.w[3];
             } else {
-                return {};
+                return g;
             }
         }
         for (auto *v : {&b.reads, &b.writes}) {
             std::sort(v->begin(), v->end());
             v->erase(std::unique(v->begin(), v->end()), v->end());
-            if (!v->empty() && v->back() >= planes) return {};
+            if (!v->empty() && v->back() >= planes) return g;
         }
         blocks.push_back(std::move(b));
         n = m + 1;
     }
     const uint32_t B = (uint32_t)blocks.size();
-    std::vector<std::vector<uint32_t>> succ(B);
-    std::vector<uint32_t> indeg(B, 0);
-    {
-        std::vector<int> writer(planes, -1);
-        std::vector<std::vector<uint32_t>> readers(planes);
-        std::unordered_map<int, uint32_t> last_amplitude;
-        auto edge = [&](uint32_t from, uint32_t to) {
-            if (from == to) return;
-            succ[from].push_back(to);
-            indeg[to]++;
-        };
-        for (uint32_t b = 0; b < B; b++) {
-            const Block &x = blocks[b];
-            for (uint32_t p : x.reads)
-                if (writer[p] >= 0) edge((uint32_t)writer[p], b);
-            for (uint32_t p : x.writes) {
-                if (writer[p] >= 0) edge((uint32_t)writer[p], b);
-                for (uint32_t r : readers[p]) edge(r, b);
-            }
-            if (x.amplitude >= 0) {
-                auto it = last_amplitude.find(x.amplitude);
-                if (it != last_amplitude.end()) edge(it->second, b);
-                last_amplitude[x.amplitude] = b;
-            }
-            for (uint32_t p : x.reads) readers[p].push_back(b);
-            for (uint32_t p : x.writes) {
-                writer[p] = (int)b;
-                readers[p].clear();
-            }
+    g.succ.assign(B, {});
+    g.pred.assign(B, {});
+    std::vector<int> writer(planes, -1);
+    std::vector<std::vector<uint32_t>> readers(planes);
+    std::unordered_map<int, uint32_t> last_amplitude;
+    auto edge = [&](uint32_t from, uint32_t to) {
+        if (from == to) return;
+        g.succ[from].push_back(to);
+        g.pred[to].push_back(from);
+    };
+    for (uint32_t b = 0; b < B; b++) {
+        const BlockGraph::Block &x = blocks[b];
+        for (uint32_t p : x.reads)
+            if (writer[p] >= 0) edge((uint32_t)writer[p], b);
+        for (uint32_t p : x.writes) {
+            if (writer[p] >= 0) edge((uint32_t)writer[p], b);
+            for (uint32_t r : readers[p]) edge(r, b);
+        }
+        if (x.amplitude >= 0) {
+            auto it = last_amplitude.find(x.amplitude);
+            if (it != last_amplitude.end()) edge(it->second, b);
+            last_amplitude[x.amplitude] = b;
+        }
+        for (uint32_t p : x.reads) readers[p].push_back(b);
+        for (uint32_t p : x.writes) {
+            writer[p] = (int)b;
+            readers[p].clear();
         }
     }
+    while (g.lead < B && blocks[g.lead].first < sources) {
+        if (!g.pred[g.lead].empty()) return g;
+        g.lead++;
+    }
+    g.ok = true;
+    return g;
+}
+
+// Row scheduling: a list scheduler builds one dependence-honouring order: among the ready blocks
+// it takes the one with the highest score, ties to the earliest. A plane counts as recent while
+// fewer than `window` plane touches have passed since its last one (a stand-in for the on-chip
+// tiers); the score adds `w_recent` per recent plane a block reads and `w_last` per plane it
+// reads for the last time. Returns an empty program when a row is not understood.
+std::vector<Instr> schedule_rows(const std::vector<Instr> &ops, uint32_t sources, uint32_t planes, uint32_t window,
+                                 int w_recent, int w_last) {
+    const BlockGraph g = block_graph(ops, sources, planes);
+    if (!g.ok) return {};
+    const auto &blocks = g.blocks;
+    const uint32_t B = (uint32_t)blocks.size();
+    std::vector<uint32_t> indeg(B);
+    for (uint32_t b = 0; b < B; b++) indeg[b] = (uint32_t)g.pred[b].size();
+    const auto &succ = g.succ;
     std::vector<uint32_t> pending_reads(planes, 0);  // unscheduled blocks reading each plane
-    for (const Block &x : blocks)
+    for (const auto &x : blocks)
         for (uint32_t p : x.reads) pending_reads[p]++;
     std::vector<uint64_t> touched(planes, 0);  // clock of the last touch + 1 (0: never)
     uint64_t clock = 0;
@@ -539,7 +562,7 @@ std::vector<Instr> schedule_rows(const std::vector<Instr> &ops, uint32_t sources
     std::vector<Instr> out;
     out.reserve(ops.size());
     auto emit = [&](uint32_t b) {
-        const Block &x = blocks[b];
+        const auto &x = blocks[b];
         for (uint32_t r = 0; r < x.count; r++) out.push_back(ops[x.first + r]);
         for (uint32_t p : x.reads) pending_reads[p]--;
         for (uint32_t p : x.reads) touched[p] = ++clock;
@@ -547,21 +570,16 @@ std::vector<Instr> schedule_rows(const std::vector<Instr> &ops, uint32_t sources
         for (uint32_t s : succ[b])
             if (--indeg[s] == 0) ready.push_back(s);
     };
-    // the leading sources stay in front, in order (the kernels run them as a phase of their own)
-    uint32_t lead = 0;
-    while (lead < B && blocks[lead].first < sources) lead++;
+    const uint32_t lead = g.lead;
     for (uint32_t b = lead; b < B; b++)
         if (indeg[b] == 0) ready.push_back(b);
-    for (uint32_t b = 0; b < lead; b++) {
-        if (indeg[b] != 0) return {};
-        emit(b);
-    }
+    for (uint32_t b = 0; b < lead; b++) emit(b);
     ready.erase(std::remove_if(ready.begin(), ready.end(), [&](uint32_t r) { return r < lead; }), ready.end());
     while (!ready.empty()) {
         size_t best = 0;
         int64_t best_score = INT64_MIN;
         for (size_t k = 0; k < ready.size(); k++) {
-            const Block &x = blocks[ready[k]];
+            const auto &x = blocks[ready[k]];
             int64_t s = 0;
             for (uint32_t p : x.reads) {
                 if (touched[p] && clock - (touched[p] - 1) < window) s += w_recent;
