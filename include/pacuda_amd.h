@@ -70,7 +70,9 @@ typedef struct {
  * 1: with forwarding through the accumulators, 2 + v: with LDS variant
  * v (10, 13, 20, 26, 40 planes; plane indices >= 2^24 are on-chip slots), 7 + v: LDS variant v
  * with a register tier of 16 planes (whole four-plane currents at slots from the variant's LDS
- * capacity on). A plane field with bit 31 or 30 set names slot plane (field & 0xffffff) with an
+ * capacity on), 12 + v and 17 + v: the same two for the f64 kernel, whose programs store no
+ * source current in the slot but run its source row again (kind 6 after the leading sources)
+ * where a read would have loaded it. A plane field with bit 31 or 30 set names slot plane (field & 0xffffff) with an
  * on-chip copy at slot ((field & 0xffffff) + (field >> 24)) & 63: bit 31, a read loads the slot
  * and keeps the copy; bit 30, a write also stores to the slot (reads use the copy). Each
  * instruction is 8 u32: w[0] =

@@ -767,6 +767,8 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
                 str(a.are, at, dr);
                 str(a.aim, at, di);
             }
+        } else if (o.kind == K_SOURCE) {  // a source row made again (the library's f64 programs)
+            run_sources<R>(&o, 1, a, i0, i1);
         }
     }
 }
@@ -1335,7 +1337,7 @@ int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t cou
             o.z[0] = w[7] & 0xff; o.z[1] = (w[7] >> 8) & 0xff;
             break;
         case K_SOURCE:
-            field(o, 0, w[1]); o.f[1] = w[2]; o.f[2] = w[3]; o.f[3] = w[4]; o.f[4] = w[5];
+            field(o, 0, w[1]); o.f[1] = form(w[2]); o.f[2] = w[3]; o.f[3] = w[4]; o.f[4] = w[5];
             o.z[3] = w[6] & 15;
             break;
         default: return PACMAN_RECURRENCE_INVALID;

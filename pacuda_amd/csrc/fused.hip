@@ -40,6 +40,8 @@ constexpr int LDS_VARIANTS = 5;
 struct PlanDev {
     const Instr *prog_lds[LDS_VARIANTS];  // with forwarding and LDS-resident currents (LDS_CAPS[v] planes)
     const Instr *prog_reg[LDS_VARIANTS];  // the same with a register tier of REG_PLANES planes after them
+    const Instr *prog_rlds[LDS_VARIANTS], *prog_rreg[LDS_VARIANTS];  // the f64 kernel's (sources made again)
+    uint32_t count_rlds[LDS_VARIANTS], count_rreg[LDS_VARIANTS];     // their instruction counts
     uint32_t reg_forms[FORM_REGS];        // the form each register slot of prog_reg holds (UINT32_MAX: none)
     uint32_t zero_fill;                   // 1: some amplitude plane has no F_ZERO closure, so all start zeroed
     const Instr *prog_fwd;                // with forwarding only (launches that give the wavefronts no LDS)
@@ -165,21 +167,50 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
         const uint32_t S = pl.scale_count, X = pl.factor_count;
         a.sre_ = pl.c64; a.sim_ = pl.c64 + S; a.fre_ = pl.c64 + 2 * S; a.fim_ = pl.c64 + 2 * S + X;
         // (selects, not an index: a runtime index into the local copy of the plan would put it in scratch)
+        constexpr bool REMAKE = __is_same(R, double) && __is_same(typename K::den, double);
         const Instr *prog;
-        if constexpr (REG)
+        uint32_t count = pl.count;
+        if constexpr (REMAKE && REG) {
+            prog = !lds_per_wave       ? pl.prog_fwd
+                   : lds_variant == 0 ? pl.prog_rreg[0]
+                   : lds_variant == 1 ? pl.prog_rreg[1]
+                   : lds_variant == 2 ? pl.prog_rreg[2]
+                   : lds_variant == 3 ? pl.prog_rreg[3]
+                                      : pl.prog_rreg[4];
+            count = !lds_per_wave       ? pl.count
+                    : lds_variant == 0 ? pl.count_rreg[0]
+                    : lds_variant == 1 ? pl.count_rreg[1]
+                    : lds_variant == 2 ? pl.count_rreg[2]
+                    : lds_variant == 3 ? pl.count_rreg[3]
+                                       : pl.count_rreg[4];
+        } else if constexpr (REMAKE) {
+            prog = !lds_per_wave       ? pl.prog_fwd
+                   : lds_variant == 0 ? pl.prog_rlds[0]
+                   : lds_variant == 1 ? pl.prog_rlds[1]
+                   : lds_variant == 2 ? pl.prog_rlds[2]
+                   : lds_variant == 3 ? pl.prog_rlds[3]
+                                      : pl.prog_rlds[4];
+            count = !lds_per_wave       ? pl.count
+                    : lds_variant == 0 ? pl.count_rlds[0]
+                    : lds_variant == 1 ? pl.count_rlds[1]
+                    : lds_variant == 2 ? pl.count_rlds[2]
+                    : lds_variant == 3 ? pl.count_rlds[3]
+                                       : pl.count_rlds[4];
+        } else if constexpr (REG) {
             prog = !lds_per_wave       ? pl.prog_fwd
                    : lds_variant == 0 ? pl.prog_reg[0]
                    : lds_variant == 1 ? pl.prog_reg[1]
                    : lds_variant == 2 ? pl.prog_reg[2]
                    : lds_variant == 3 ? pl.prog_reg[3]
                                       : pl.prog_reg[4];
-        else
+        } else {
             prog = !lds_per_wave       ? pl.prog_fwd
                    : lds_variant == 0 ? pl.prog_lds[0]
                    : lds_variant == 1 ? pl.prog_lds[1]
                    : lds_variant == 2 ? pl.prog_lds[2]
                    : lds_variant == 3 ? pl.prog_lds[3]
                                       : pl.prog_lds[4];
+        }
         run_sources_point<R>(prog, pl.sources, a);
         if constexpr (DF_FORMS) {
             for (uint32_t f = 0; f < F; f++) {
@@ -204,7 +235,7 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
             }
         }
         if constexpr (K::forwarding)
-            interpret<K>(prog + pl.sources, pl.count - pl.sources, a);
+            interpret<K>(prog + pl.sources, count - pl.sources, a);
         else
             interpret<K>(pl.prog_plain + pl.sources, pl.count_plain - pl.sources, a);
         // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
@@ -876,11 +907,15 @@ uint32_t assign_lds(std::vector<Instr> &ops, uint32_t capacity, uint32_t base = 
 // read, its last write also stores to the slot (DUAL_THROUGH, set afterwards: nothing reads the
 // slot copy in between). An access that finds no room goes to the slot. LDS slots are
 // [0, lds_cap) (any object, contiguous), the register tier [lds_cap, lds_cap + reg) (four-plane
-// objects at multiples of four). A replay of the rewritten program checks that every read finds
-// the value last written (in the slot only its unmasked components). Returns false with the
-// program unchanged when an access does not name a whole object or the replay fails.
+// objects at multiples of four). With `recompute`, a current a leading source row makes (and no
+// row writes again) is never stored to the slot: a read that finds it off chip is preceded by a
+// copy of its source row that makes it again, into on-chip room taken by the same rule but
+// displacing whatever it must (the same operations on the same f64 inputs: the same bits). A
+// replay of the rewritten program checks that every read finds the value last written (in the
+// slot only its unmasked components). Returns false with the program unchanged when an access
+// does not name a whole object or the replay fails.
 bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, uint32_t *lds_used,
-                      uint32_t *reg_used) {
+                      uint32_t *reg_used, bool recompute = false) {
     struct Acc {
         uint32_t row;
         int word;
@@ -943,6 +978,21 @@ bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, u
         x.obj = o;
         x.field = at;
     }
+    // the currents made again instead of stored: made by a source row, never written again
+    std::vector<char> again(objs.size(), 0);
+    std::vector<uint32_t> maker(objs.size(), 0);  // the access that makes each object
+    if (recompute) {
+        for (size_t i = 0; i < acc.size(); i++) {
+            const Acc &x = acc[i];
+            if (x.obj < 0) continue;
+            if (x.create) {
+                again[x.obj] = (ops[x.row].w[0] & 0xff) == K_SOURCE;
+                maker[x.obj] = (uint32_t)i;
+            } else if (x.wr) {
+                again[x.obj] = 0;
+            }
+        }
+    }
     // the next read of the value each access leaves (none when the object is written over or dies first)
     const uint32_t NONE = UINT32_MAX;
     std::vector<uint32_t> next_read(acc.size(), NONE), ahead(objs.size(), NONE);
@@ -967,6 +1017,8 @@ bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, u
     // and the room of a value that dies in a row is free from the next row on
     std::vector<uint32_t> named_in(objs.size(), UINT32_MAX);
     std::vector<int> dying;
+    struct Remake { uint32_t before; int obj; uint32_t slot; };  // a source row made again before access `before`
+    std::vector<Remake> remakes;
     for (size_t i = 0; i < acc.size(); i++) {
         Acc &x = acc[i];
         if (i == 0 || acc[i - 1].row != x.row) {
@@ -979,13 +1031,14 @@ bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, u
         const int o = x.obj;
         const uint32_t t = next_read[i], n = x.len, p = objs[o].base;
         dying.erase(std::remove(dying.begin(), dying.end(), o), dying.end());
+        const bool remake = again[o] && x.rd && pos[o] < 0;  // (made again into on-chip room)
         if (pos[o] >= 0) {
             x.field = LDS_PLANE + (uint32_t)pos[o];
-            if (x.wr) {
+            if (x.wr && !again[o]) {
                 dirty[o] = 1;
                 last_write[o] = (uint32_t)i;
             }
-        } else if (t != NONE) {
+        } else if (t != NONE || remake) {
             // the room with the latest next read among the holders it displaces (free room first)
             int best = -1;
             uint64_t best_key = 0;
@@ -1005,9 +1058,10 @@ bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, u
                     best_key = key;
                 }
             };
-            if (reg && n == 4 && !x.rd)  // (a read's copy goes to LDS)
+            if (reg && n == 4 && (!x.rd || remake))  // (a read's copy goes to LDS)
                 for (uint32_t s = lds_cap; s + 4 <= cap; s += 4) consider(s);
             for (uint32_t s = 0; s + n <= lds_cap; s++) consider(s);
+
             if (best >= 0) {
                 for (uint32_t j = 0; j < n; j++) {
                     const int v = occ[best + j];
@@ -1022,8 +1076,13 @@ bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, u
                 }
                 pos[o] = best;
                 for (uint32_t j = 0; j < n; j++) occ[best + j] = o;
-                x.field = x.rd ? dual_field(DUAL_FILL, p, (uint32_t)best) : LDS_PLANE + (uint32_t)best;
-                dirty[o] = x.wr;
+                if (remake) {  // (else the read loads the slot's copy)
+                    remakes.push_back({(uint32_t)i, o, (uint32_t)best});
+                    x.field = LDS_PLANE + (uint32_t)best;
+                } else {
+                    x.field = x.rd ? dual_field(DUAL_FILL, p, (uint32_t)best) : LDS_PLANE + (uint32_t)best;
+                }
+                dirty[o] = x.wr && !again[o];
                 if (x.wr) last_write[o] = (uint32_t)i;
                 if ((uint32_t)best >= lds_cap) used_reg = std::max(used_reg, (uint32_t)best + n - lds_cap);
                 else used_lds = std::max(used_lds, (uint32_t)best + n);
@@ -1042,9 +1101,14 @@ bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, u
     // (within a row, no on-chip copy a read keeps may overlap another access's on-chip planes: the
     // kernels may interleave a row's operand reads component by component)
     std::vector<std::array<uint32_t, 3>> row_chip;  // (first, end, is a copy) of the row's accesses
+    size_t next_remake = 0;
     for (size_t i = 0; i < acc.size(); i++) {
         const Acc &x = acc[i];
         if (i == 0 || acc[i - 1].row != x.row) row_chip.clear();
+        for (; next_remake < remakes.size() && remakes[next_remake].before == i; next_remake++) {
+            const Remake &m = remakes[next_remake];  // the value itself again (no new version)
+            for (uint32_t j = 0; j < objs[m.obj].len; j++) chip[m.slot + j] = {m.obj, version[m.obj]};
+        }
         if (x.obj < 0 || !(x.rd || x.wr)) continue;
         if (x.rd && x.field >= LDS_PLANE) {
             const bool copy = (x.field & DUAL_FILL) != 0;
@@ -1082,6 +1146,21 @@ bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, u
     }
     for (const Acc &x : acc)
         if (x.obj >= 0) ops[x.row].w[x.word] = x.field;
+    if (!remakes.empty()) {  // the source rows made again, each before the row that reads it
+        std::vector<Instr> out;
+        out.reserve(ops.size() + remakes.size());
+        size_t k = 0;
+        for (uint32_t r = 0; r < ops.size(); r++) {
+            for (; k < remakes.size() && acc[remakes[k].before].row == r; k++) {
+                Instr m = ops[acc[maker[remakes[k].obj]].row];
+                m.w[1] = LDS_PLANE + remakes[k].slot;
+                m.w[6] = 0;
+                out.push_back(m);
+            }
+            out.push_back(ops[r]);
+        }
+        ops.swap(out);
+    }
     *lds_used = used_lds;
     *reg_used = used_reg;
     return true;
@@ -1156,6 +1235,9 @@ struct Compiled {
     uint32_t sources = 0, planes = 0;
     std::vector<Instr> plain, fwd, lds[LDS_VARIANTS], reg[LDS_VARIANTS];
     uint32_t lds_used[LDS_VARIANTS] = {}, reg_used[LDS_VARIANTS] = {};
+    // the same for the f64 kernel, whose programs make sources again instead of storing them
+    std::vector<Instr> rlds[LDS_VARIANTS], rreg[LDS_VARIANTS];
+    uint32_t rreg_used[LDS_VARIANTS] = {};
     uint32_t reg_forms[FORM_REGS];
     bool zero_fill = true;
 };
@@ -1177,8 +1259,8 @@ bool mark_first_closures(std::vector<Instr> &ops, uint32_t amplitude_planes) {
     return first < amplitude_planes;
 }
 
-// The register-resident forms: the FORM_REGS forms the program's rows (sources aside, which
-// take their forms from the slot in f64) read most often go to register slots 0.. in that order;
+// The register-resident forms: the FORM_REGS forms the program's rows (the leading sources aside,
+// which run before the registers are filled) read most often go to register slots 0.. in that order;
 // their form fields in the rows are rewritten to f | (slot + 1) << FORM_SLOT_SHIFT. The same
 // choice serves every program of a plan (they differ only in where currents live).
 void choose_form_regs(const std::vector<Instr> &ops, uint32_t form_count, uint32_t *reg_forms) {
@@ -1199,7 +1281,7 @@ void choose_form_regs(const std::vector<Instr> &ops, uint32_t form_count, uint32
         reg_forms[s] = s < form_count && uses[order[s]] ? order[s] : UINT32_MAX;
 }
 
-void mark_form_regs(std::vector<Instr> &ops, const uint32_t *reg_forms) {
+void mark_form_regs(std::vector<Instr> &ops, const uint32_t *reg_forms, uint32_t sources) {
     auto mark = [&](uint32_t &w) {
         for (uint32_t s = 0; s < FORM_REGS; s++)
             if (reg_forms[s] == w) {
@@ -1207,12 +1289,13 @@ void mark_form_regs(std::vector<Instr> &ops, const uint32_t *reg_forms) {
                 return;
             }
     };
-    for (Instr &o : ops) {
+    for (size_t n = 0; n < ops.size(); n++) {
+        Instr &o = ops[n];
         const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff;
         if (kind == K_CONTRIB && op == 6) {
             mark(o.w[5]);
             mark(o.w[6]);
-        } else if (kind == K_PROP) {
+        } else if (kind == K_PROP || (kind == K_SOURCE && n >= sources)) {  // (a source row made again)
             mark(o.w[2]);
         }
     }
@@ -1274,48 +1357,43 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
             std::vector<Instr> o = schedule_rows(built, hp.sources, d->current_planes, window, w[0], w[1]);
             if (!o.empty()) orders.push_back(std::move(o));
         }
-    uint64_t best[2][LDS_VARIANTS];
-    bool have[2][LDS_VARIANTS] = {};
-    uint32_t reg_lds_used[LDS_VARIANTS] = {};
+    uint64_t best[2][2][LDS_VARIANTS];  // [sources made again][tier][variant]
+    bool have[2][2][LDS_VARIANTS] = {};
+    uint32_t lds_of[2][2][LDS_VARIANTS] = {};
     for (const std::vector<Instr> &order : orders) {
         std::vector<Instr> ops = order;
         uint32_t planes = d->current_planes;
         if (const uint32_t compacted = compact_planes(ops, d->current_planes)) planes = compacted;
         forward_in_place(ops);
         mark_first_closures(ops, d->amplitude_planes);
-        for (int v = 0; v < LDS_VARIANTS; v++)
-            for (int tier = 0; tier < 2; tier++) {  // LDS only; LDS and the register tier
-                const uint32_t reg = tier ? REG_PLANES : 0;
-                std::vector<Instr> prog = ops;
-                uint32_t lds_used = 0, reg_used = 0;
-                if (!allocate_on_chip(prog, LDS_CAPS[v], reg, &lds_used, &reg_used)) {
-                    if (have[tier][v]) continue;
-                    prog = ops;
-                    lds_used = assign_lds(prog, LDS_CAPS[v]);
-                    if (reg) reg_used = assign_lds(prog, reg, LDS_CAPS[v], 4);
+        for (int again = 0; again < 2; again++)
+            for (int v = 0; v < LDS_VARIANTS; v++)
+                for (int tier = 0; tier < 2; tier++) {  // LDS only; LDS and the register tier
+                    const uint32_t reg = tier ? REG_PLANES : 0;
+                    std::vector<Instr> prog = ops;
+                    uint32_t lds_used = 0, reg_used = 0;
+                    if (!allocate_on_chip(prog, LDS_CAPS[v], reg, &lds_used, &reg_used, again)) {
+                        if (have[again][tier][v]) continue;
+                        prog = ops;
+                        lds_used = assign_lds(prog, LDS_CAPS[v]);
+                        if (reg) reg_used = assign_lds(prog, reg, LDS_CAPS[v], 4);
+                    }
+                    const uint64_t cost = slot_accesses(prog);
+                    if (have[again][tier][v] && cost >= best[again][tier][v]) continue;
+                    have[again][tier][v] = true;
+                    best[again][tier][v] = cost;
+                    lds_of[again][tier][v] = lds_used;
+                    (again ? (tier ? c.rreg : c.rlds) : (tier ? c.reg : c.lds))[v] = std::move(prog);
+                    if (tier) (again ? c.rreg_used : c.reg_used)[v] = reg_used;
+                    c.planes = std::max(c.planes, planes);
                 }
-                const uint64_t cost = slot_accesses(prog);
-                if (have[tier][v] && cost >= best[tier][v]) continue;
-                have[tier][v] = true;
-                best[tier][v] = cost;
-                if (tier) {
-                    c.reg[v] = std::move(prog);
-                    c.reg_used[v] = reg_used;
-                    reg_lds_used[v] = lds_used;
-                } else {
-                    c.lds[v] = std::move(prog);
-                    c.lds_used[v] = lds_used;
-                }
-                c.planes = std::max(c.planes, planes);
-            }
     }
-    // (the LDS region of a wavefront holds what either program of a variant places there)
-    for (int v = 0; v < LDS_VARIANTS; v++) c.lds_used[v] = std::max(c.lds_used[v], reg_lds_used[v]);
+    // (the LDS region of a wavefront holds what any program of a variant places there)
+    for (int v = 0; v < LDS_VARIANTS; v++)
+        c.lds_used[v] = std::max(std::max(lds_of[0][0][v], lds_of[0][1][v]), std::max(lds_of[1][0][v], lds_of[1][1][v]));
     choose_form_regs(c.fwd, d->form_count, c.reg_forms);
-    for (int v = 0; v < LDS_VARIANTS; v++) {
-        mark_form_regs(c.lds[v], c.reg_forms);
-        mark_form_regs(c.reg[v], c.reg_forms);
-    }
+    for (int v = 0; v < LDS_VARIANTS; v++)
+        for (auto *prog : {&c.lds[v], &c.reg[v], &c.rlds[v], &c.rreg[v]}) mark_form_regs(*prog, c.reg_forms, c.sources);
     return PACMAN_RECURRENCE_OK;
 }
 
@@ -1323,18 +1401,24 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
 
 extern "C" int pacuda_plan_program(const pacuda_plan_desc *d, uint32_t variant, uint32_t *out, uint32_t capacity,
                                    uint32_t *count, uint32_t *sources, uint32_t *planes, uint32_t *lds_planes) {
-    if (!count || variant >= 2 + 2 * LDS_VARIANTS) return PACMAN_RECURRENCE_INVALID;
+    if (!count || variant >= 2 + 4 * LDS_VARIANTS) return PACMAN_RECURRENCE_INVALID;
     Compiled c;
     const int st = compile_plan(d, c);
     if (st) return st;
-    const uint32_t v = variant >= 2 + LDS_VARIANTS ? variant - 2 - LDS_VARIANTS : variant - 2;
-    const bool reg = variant >= 2 + LDS_VARIANTS;
-    const std::vector<Instr> &ops = variant == 0 ? c.plain : variant == 1 ? c.fwd : reg ? c.reg[v] : c.lds[v];
+    // 2 + v: LDS variant v, + LDS_VARIANTS: with the register tier, + 2 LDS_VARIANTS: the f64
+    // kernel's programs (sources made again)
+    const uint32_t set = variant < 2 ? 0 : (variant - 2) / LDS_VARIANTS, v = variant < 2 ? 0 : (variant - 2) % LDS_VARIANTS;
+    const bool reg = set == 1 || set == 3, again = set >= 2;
+    const std::vector<Instr> &ops = variant == 0 ? c.plain
+                                    : variant == 1 ? c.fwd
+                                    : again ? (reg ? c.rreg[v] : c.rlds[v])
+                                            : (reg ? c.reg[v] : c.lds[v]);
     *count = (uint32_t)ops.size();
     if (sources) *sources = c.sources;
     if (planes) *planes = c.planes;
     // on-chip slots the program names: LDS slots, then (register variants) the register tier
-    if (lds_planes) *lds_planes = variant < 2 ? 0 : reg && c.reg_used[v] ? LDS_CAPS[v] + c.reg_used[v] : c.lds_used[v];
+    const uint32_t reg_used = again ? c.rreg_used[v] : c.reg_used[v];
+    if (lds_planes) *lds_planes = variant < 2 ? 0 : reg && reg_used ? LDS_CAPS[v] + reg_used : c.lds_used[v];
     if (out) {
         if (capacity < ops.size()) return PACMAN_RECURRENCE_INVALID;
         memcpy(out, ops.data(), ops.size() * sizeof(Instr));
@@ -1367,6 +1451,10 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     for (int v = 0; v < LDS_VARIANTS; v++) {
         h.prog_lds[v] = upload(p, cp.lds[v].data(), cp.lds[v].size(), &ok);
         h.prog_reg[v] = upload(p, cp.reg[v].data(), cp.reg[v].size(), &ok);
+        h.prog_rlds[v] = upload(p, cp.rlds[v].data(), cp.rlds[v].size(), &ok);
+        h.prog_rreg[v] = upload(p, cp.rreg[v].data(), cp.rreg[v].size(), &ok);
+        h.count_rlds[v] = (uint32_t)cp.rlds[v].size();
+        h.count_rreg[v] = (uint32_t)cp.rreg[v].size();
         h.lds_planes[v] = cp.lds_used[v];
     }
     h.count = (uint32_t)cp.fwd.size();

@@ -243,6 +243,8 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
                     a.template put<R>(o.w[1] + k, identity_value(fr, fi, a.template cur<R>(o.w[1] + k, zi)), zo);
                 }
             }
+        } else if (kind == K_SOURCE) {  // a source row made again (programs of the f64 kernel only)
+            if constexpr (__is_same(R, double) && __is_same(typename K::den, double)) source_row<R>(o, a);
         } else if (kind == K_CLEAR) {
             if (o.w[2] == 4) {
                 const C z[4] = {cx(R(0.0), R(0.0)), cx(R(0.0), R(0.0)), cx(R(0.0), R(0.0)), cx(R(0.0), R(0.0))};
@@ -254,34 +256,36 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
     }
 }
 
-// The leading source instructions, in f64 from f64 forms, scales and factors, then narrowed to
-// the kernel's current layout (runner/recurrence_runner.cu:766-785).
-template <class R, class A>
-__device__ inline void run_sources_point(const Instr *__restrict__ prog, uint32_t count, const A &a) {
-    for (uint32_t n = 0; n < count; n++) {
-        const Instr o = cnst(prog)[n];
-        double p[4];
-        a.form(o.w[2], p);
-        double fr = ldc<double>(a.fre(), o.w[3]), fi = ldc<double>(a.fim(), o.w[3]), mass = ldc<double>(a.sre(), o.w[4]);
-        W v[4];
-        const int c = source_values(o.w[5], mass, p, v);
-        Cx<R> out[4];
+// One source instruction, in f64 from an f64 form, scale and factor, then narrowed to the
+// kernel's current layout (runner/recurrence_runner.cu:766-785).
+template <class R, class A> __device__ inline void source_row(const Instr &o, const A &a) {
+    double p[4];
+    a.form(o.w[2], p);
+    double fr = ldc<double>(a.fre(), o.w[3]), fi = ldc<double>(a.fim(), o.w[3]), mass = ldc<double>(a.sre(), o.w[4]);
+    W v[4];
+    const int c = source_values(o.w[5], mass, p, v);
+    Cx<R> out[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {  // c <= 4: a constant bound keeps v in registers
-            if (k < c) {
-                W s = cx(fr * v[k].re - fi * v[k].im, fr * v[k].im + fi * v[k].re);
-                out[k] = cx(narrow<R>(s.re), narrow<R>(s.im));
-            }
-        }
-        const uint32_t z = zbits<A>(o.w[6], 0, 4);
-        if (c == 4) {  // four-plane currents: a whole-object store
-            a.template puts<R, 4>(o.w[1], out, z);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (k < c) a.template put<R>(o.w[1] + k, out[k], (z >> k) & 1);
+    for (int k = 0; k < 4; k++) {  // c <= 4: a constant bound keeps v in registers
+        if (k < c) {
+            W s = cx(fr * v[k].re - fi * v[k].im, fr * v[k].im + fi * v[k].re);
+            out[k] = cx(narrow<R>(s.re), narrow<R>(s.im));
         }
     }
+    const uint32_t z = zbits<A>(o.w[6], 0, 4);
+    if (c == 4) {  // four-plane currents: a whole-object store
+        a.template puts<R, 4>(o.w[1], out, z);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k < c) a.template put<R>(o.w[1] + k, out[k], (z >> k) & 1);
+    }
+}
+
+// the leading source instructions
+template <class R, class A>
+__device__ inline void run_sources_point(const Instr *__restrict__ prog, uint32_t count, const A &a) {
+    for (uint32_t n = 0; n < count; n++) source_row<R>(cnst(prog)[n], a);
 }
 
 }  // namespace pacuda

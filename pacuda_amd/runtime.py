@@ -104,6 +104,12 @@ def plan_program(lib, plan, variant):
     a = plan.arrays()
     d = plan_desc(plan, a)
     n, src, planes, lds = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+    # (one compilation when a generous buffer holds the program, else a count query first)
+    guess = np.zeros((4 * len(plan.rows) + 64, 8), dtype=np.uint32)
+    st = lib.pacuda_plan_program(C.byref(d), variant, guess.ctypes.data_as(u32p), len(guess), C.byref(n), C.byref(src),
+                                 C.byref(planes), C.byref(lds))
+    if st == 0:
+        return guess[:n.value].copy(), src.value, planes.value, lds.value
     st = lib.pacuda_plan_program(C.byref(d), variant, None, 0, C.byref(n), None, None, None)
     if st != 0:
         raise RuntimeError(f"pacuda_plan_program: status {st}")

@@ -197,7 +197,7 @@ def test_plan_compilation_rejects_bad_descriptions(lib):
     assert compile_with(words=sym)[0] == abi.INVALID
     # variant out of range, missing count, too small a buffer
     d = plan_desc(plan, a)
-    assert lib.pacuda_plan_program(C.byref(d), 12, None, 0, C.byref(n), None, None, None) == abi.INVALID
+    assert lib.pacuda_plan_program(C.byref(d), 22, None, 0, C.byref(n), None, None, None) == abi.INVALID
     assert lib.pacuda_plan_program(C.byref(d), 1, None, 0, None, None, None, None) == abi.INVALID
     small = np.zeros((1, 8), dtype=np.uint32)
     assert lib.pacuda_plan_program(C.byref(d), 1, small.ctypes.data_as(abi.u32p), 1, C.byref(n), None, None,

@@ -5,7 +5,8 @@ through the accumulators, and with each LDS-residency variant, alone and with th
 (the ones the kernels run) — and the oracle runs each instruction stream
 (refcpu_evaluate_program, on-chip slots mapped to extra planes) beside its own plan run from the
 words. Every value must be equal bit for bit, in
-the f64, df64 and mixed policies, for 4–7 gluon plans, quark-line plans and every p p > j j j
+the f64, df64 and mixed policies (the f64 kernel's own programs, which run source rows again
+instead of loading sources, in f64), for 4–7 gluon plans, quark-line plans and every p p > j j j
 subprocess plan; and the passes must have done their work (fewer planes, forwarded rows, LDS
 slots)."""
 import numpy as np
@@ -41,7 +42,8 @@ def _check(lib, name, plan, points, seed, kernels):
     arrays = plan.arrays()
     for kernel in kernels:
         want_res, want_tot = refcpu.evaluate(arrays, counts, ext, points, kernel=kernel)
-        for variant in range(12):
+        # (12-21: the f64 kernel's programs, which run source rows again)
+        for variant in range(22) if kernel == abi.KERNEL_F64 else range(12):
             instr, sources, planes, lds = plan_program(lib, plan, variant)
             if variant == 0:
                 assert not np.any((instr[:, 0] >> 16) & (F_HOLD | F_FROMACC)), name
@@ -163,3 +165,19 @@ def test_on_chip_allocation(lib):
     assert np.all((((fill & 0xffffff) + (fill >> 24)) & 63) < 20)
     assert _slot_accesses(fwd) == 2210
     assert _slot_accesses(reg) < 1000
+
+
+def test_sources_made_again(lib):
+    """the f64 kernel's programs run a source row again where a read would have filled an on-chip
+    copy from the slot, and leave fewer slot accesses than the programs the other kernels run"""
+    plan = plans.gluon_plan(5)
+    for variant in (3, 4, 8, 9):
+        prog, sources, _, _ = plan_program(lib, plan, variant)
+        again, _, _, _ = plan_program(lib, plan, variant + 10)
+        kind = again[:, 0] & 0xff
+        assert np.all(kind[:sources] == 6) and np.count_nonzero(kind[sources:] == 6) > 0
+        assert not np.any((prog[sources:, 0] & 0xff) == 6)
+        # each one writes an on-chip slot (plain on-chip field) and stores nothing to the slot
+        made = again[sources:][kind[sources:] == 6]
+        assert np.all((made[:, 1] >= LDS_PLANE) & (made[:, 1] < LDS_PLANE + 64))
+        assert _slot_accesses(again) < _slot_accesses(prog)
