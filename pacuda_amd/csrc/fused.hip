@@ -907,10 +907,11 @@ uint32_t assign_lds(std::vector<Instr> &ops, uint32_t capacity, uint32_t base = 
 // read, its last write also stores to the slot (DUAL_THROUGH, set afterwards: nothing reads the
 // slot copy in between). An access that finds no room goes to the slot. LDS slots are
 // [0, lds_cap) (any object, contiguous), the register tier [lds_cap, lds_cap + reg) (four-plane
-// objects at multiples of four). With `recompute`, a current a leading source row makes (and no
-// row writes again) is never stored to the slot: a read that finds it off chip is preceded by a
-// copy of its source row that makes it again, into on-chip room taken by the same rule but
-// displacing whatever it must (the same operations on the same f64 inputs: the same bits). A
+// objects at multiples of four). With `recompute`, a read that would fill an on-chip copy of a
+// current a leading vector-source row makes (and no row writes again) is preceded instead by a
+// copy of that source row, which makes the value again into the same room (the same operations
+// on the same f64 inputs: the same bits), so the slot load disappears; reads that find no room
+// still load the slot's copy, and evicting such a current never needs a store. A
 // replay of the rewritten program checks that every read finds the value last written (in the
 // slot only its unmasked components). Returns false with the program unchanged when an access
 // does not name a whole object or the replay fails.
@@ -985,8 +986,8 @@ bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, u
         for (size_t i = 0; i < acc.size(); i++) {
             const Acc &x = acc[i];
             if (x.obj < 0) continue;
-            if (x.create) {
-                again[x.obj] = (ops[x.row].w[0] & 0xff) == K_SOURCE;
+            if (x.create) {  // (vector sources: a spinor's two components are cheaper to load)
+                again[x.obj] = (ops[x.row].w[0] & 0xff) == K_SOURCE && (ops[x.row].w[5] & 7) >= 3;
                 maker[x.obj] = (uint32_t)i;
             } else if (x.wr) {
                 again[x.obj] = 0;
@@ -1378,7 +1379,9 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
                         lds_used = assign_lds(prog, LDS_CAPS[v]);
                         if (reg) reg_used = assign_lds(prog, reg, LDS_CAPS[v], 4);
                     }
-                    const uint64_t cost = slot_accesses(prog);
+                    // (a source row made again is charged as two slot accesses: its f64 arithmetic)
+                    uint64_t cost = slot_accesses(prog);
+                    for (size_t r = c.sources; r < prog.size(); r++) cost += 2 * ((prog[r].w[0] & 0xff) == K_SOURCE);
                     if (have[again][tier][v] && cost >= best[again][tier][v]) continue;
                     have[again][tier][v] = true;
                     best[again][tier][v] = cost;
@@ -1388,6 +1391,14 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
                     c.planes = std::max(c.planes, planes);
                 }
     }
+    // the f64 kernel runs the other kernels' program where making sources again does not pay
+    for (int v = 0; v < LDS_VARIANTS; v++)
+        for (int tier = 0; tier < 2; tier++)
+            if (best[1][tier][v] >= best[0][tier][v]) {
+                (tier ? c.rreg : c.rlds)[v] = (tier ? c.reg : c.lds)[v];
+                if (tier) c.rreg_used[v] = c.reg_used[v];
+                lds_of[1][tier][v] = lds_of[0][tier][v];
+            }
     // (the LDS region of a wavefront holds what any program of a variant places there)
     for (int v = 0; v < LDS_VARIANTS; v++)
         c.lds_used[v] = std::max(std::max(lds_of[0][0][v], lds_of[0][1][v]), std::max(lds_of[1][0][v], lds_of[1][1][v]));
