@@ -147,6 +147,8 @@ PCD void propagator(const A &a, const Instr &o, int flags, Cx<typename K::real> 
 }
 
 // One lane runs `count` instructions for its point (runner/recurrence_runner.cu:787-838).
+template <class R, class A> __device__ inline void source_row(const Instr &o, const A &a);
+
 template <class K, class A>
 __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count, const A &a) {
     typedef typename K::real R;
