@@ -42,7 +42,7 @@ struct PlanDev {
     const Instr *prog_reg[LDS_VARIANTS];  // the same with a register tier of REG_PLANES planes after them
     const Instr *prog_rlds[LDS_VARIANTS], *prog_rreg[LDS_VARIANTS];  // the f64 kernel's (sources made again)
     uint32_t count_rlds[LDS_VARIANTS], count_rreg[LDS_VARIANTS];     // their instruction counts
-    uint32_t reg_forms[FORM_REGS];        // the form each register slot of prog_reg holds (UINT32_MAX: none)
+    uint32_t reg_forms[FORM_REGS];        // the form each register slot holds (UINT32_MAX: none), every program
     uint32_t zero_fill;                   // 1: some amplitude plane has no F_ZERO closure, so all start zeroed
     const Instr *prog_fwd;                // with forwarding only (launches that give the wavefronts no LDS)
     const Instr *prog_plain;              // neither (kernels without forwarding)
