@@ -72,6 +72,8 @@ constexpr int LANES = 64;
 // whose LDS fits the wavefronts it asks per CU (f64, 256-thread blocks: 40 at 4 wavefronts, 20 at
 // 8, 13 at 12, 10 at 16; 64-thread blocks: 26 at 6)
 constexpr uint32_t LDS_CAPS[LDS_VARIANTS] = {10, 13, 20, 26, 40};
+// (a dual field names its on-chip slot in six bits: LDS and register-tier slots below 64)
+static_assert(LDS_CAPS[LDS_VARIANTS - 1] + REG_PLANES <= 64, "on-chip slots of dual fields");
 constexpr uint32_t LDS_BYTES = 160 * 1024;  // LDS per CU (gfx950), the most one workgroup may take
 
 // an amplitude's raw value as a double: the df64 kernels' pairs are summed in f64
