@@ -92,15 +92,14 @@ __global__ void reset_counter(uint32_t *counter) {
 
 // The kernels' register tiers: TIER 2 keeps REG_PLANES current planes and four forms in
 // registers, built for two wavefronts per SIMD in the f64 and df64 policies (up to 256 VGPRs)
-// and three in the mixed ones (168 VGPRs); TIER 1 keeps only forms: FORM_REGS for three
-// wavefronts (f64, df64), four for four wavefronts (mixed); TIER 0 keeps neither, for four
-// (128 VGPRs).
+// and three in the mixed ones (168 VGPRs); TIER 1 keeps only the FORM_REGS forms, for three
+// wavefronts (f64, df64) or four (mixed); TIER 0 keeps neither, for four (128 VGPRs).
 template <class K> constexpr bool f32_currents() { return __is_same(typename K::real, float); }
 template <class K, int TIER> constexpr int waves_per_simd() {
     return TIER == 2 ? (f32_currents<K>() ? 3 : 2) : TIER == 1 ? (f32_currents<K>() ? 4 : 3) : 4;
 }
 template <class K, int TIER> constexpr int register_forms() {
-    return TIER == 2 ? 4 : TIER == 1 ? (f32_currents<K>() ? 4 : (int)FORM_REGS) : 0;
+    return TIER == 2 ? 4 : TIER == 1 ? (int)FORM_REGS : 0;
 }
 
 template <class K, int TIER>
