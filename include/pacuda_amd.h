@@ -120,7 +120,9 @@ typedef struct {
  * Evaluates every item in one persistent launch: each wavefront takes 64-point units of the
  * items off a device work queue and runs forms, sources, program and reduction for them with its
  * arena in an on-device scratch slot. Results equal the v1 path bit for bit for the same kernel.
- * Asynchronous on the stream unless kernel_ms is set. 0 OK, 2 invalid, 3 device error.
+ * Asynchronous on the stream unless kernel_ms is set. Calls may go to different streams: the
+ * item table, work counter and slots are shared by all calls, so each call first waits on the
+ * host until the previous call's launch has ended. 0 OK, 2 invalid, 3 device error.
  */
 int pacuda_evaluate(const pacuda_item *items, uint32_t count, const pacuda_launch *launch);
 

@@ -1156,6 +1156,14 @@ int refcpu_v1_symjit_kernel(const char *sha256_hex) { (void)sha256_hex; return -
 void *refcpu_v1_host_alloc(size_t bytes) { return malloc(bytes ? bytes : 1); }
 void refcpu_v1_host_free(void *p) { free(p); }
 
+// Test hook: the momentum forms of `points` points from external momenta ([(slot*4+k)*points+i])
+// into mom ([(f*4+k)*points+i]), as FORMS_ON_DEVICE computes them.
+int refcpu_fill_forms(const double *external, uint32_t points, const uint32_t *form_terms, uint32_t form_count,
+                      const uint32_t *slots, const double *coefficients, double *mom) {
+    fill_forms(external, points, form_terms, form_count, slots, coefficients, mom, points, 0, points, 0);
+    return PACMAN_RECURRENCE_OK;
+}
+
 int refcpu_test_sources(const uint32_t *codes, const double *masses, const double *momenta, uint32_t count,
                         double *out_re, double *out_im) {
     for (uint32_t i = 0; i < count; i++) {

@@ -40,6 +40,8 @@ def load():
             f64p, C.c_uint32, f64p, f64p, C.c_uint32, C.c_int]
         lib.refcpu_test_sources.restype = C.c_int
         lib.refcpu_test_sources.argtypes = [u32p, f64p, f64p, C.c_uint32, f64p, f64p]
+        lib.refcpu_fill_forms.restype = C.c_int
+        lib.refcpu_fill_forms.argtypes = [f64p, C.c_uint32, u32p, C.c_uint32, u32p, f64p, f64p]
         lib.refcpu_reset.restype = None
         for name in ("run", "reduce", "reduce_resolved", "reduce_sample", "configure"):
             getattr(lib, "refcpu_v1_" + name).restype = C.c_int
@@ -94,6 +96,18 @@ def evaluate_program(instr, sources, slot_planes, lds_planes, plan_arrays, count
     if st != 0:
         raise RuntimeError(f"refcpu_evaluate_program: status {st}")
     return res, tot
+
+
+def fill_forms(plan_arrays, external, points):
+    """the plan's momentum forms of `points` points: [form][4][points]"""
+    lib = load()
+    a = plan_arrays
+    count = len(a["form_terms"]) // 2
+    ext = np.ascontiguousarray(external, dtype=np.float64)
+    mom = np.zeros((count, 4, points))
+    assert lib.refcpu_fill_forms(_p(ext, f64p), points, _p(a["form_terms"], u32p), count, _p(a["term_slots"], u32p),
+                                 _p(a["term_coefficients"], f64p), _p(mom, f64p)) == 0
+    return mom
 
 
 def test_sources(codes, masses, momenta):

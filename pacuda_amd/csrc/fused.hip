@@ -1215,6 +1215,10 @@ struct FusedState {
     uint32_t *unit_start = nullptr;
     size_t item_capacity = 0;
     hipEvent_t start = nullptr, stop = nullptr;
+    // recorded after each launch on the stream it went to: the next call, on whatever stream,
+    // waits for it before it rewrites the item table, the counter or the slots
+    hipEvent_t last = nullptr;
+    bool last_pending = false;
 };
 FusedState fs;
 
@@ -1693,6 +1697,12 @@ extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const p
     const size_t need = (size_t)sc.waves * sc.slot;
     hipStream_t s = l->stream ? (hipStream_t)l->stream : runner_stream();
     if (!s) return PACMAN_RECURRENCE_DEVICE_ERROR;
+    // the previous launch (on any stream) reads the item table and uses the counter and slots
+    // until it ends
+    if (fs.last_pending) {
+        if (hip_failed(hipEventSynchronize(fs.last), "previous launch")) return PACMAN_RECURRENCE_DEVICE_ERROR;
+        fs.last_pending = false;
+    }
     if (need > fs.scratch_doubles) {
         if (fs.scratch) {
             (void)hipDeviceSynchronize();
@@ -1716,18 +1726,20 @@ extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const p
         fs.item_capacity = count;
     }
     if (!fs.counter && hip_failed(hipMalloc((void **)&fs.counter, 64), "counter")) return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (!fs.start && (hip_failed(hipEventCreate(&fs.start), "event") || hip_failed(hipEventCreate(&fs.stop), "event")))
+    if (!fs.start && (hip_failed(hipEventCreate(&fs.start), "event") || hip_failed(hipEventCreate(&fs.stop), "event") ||
+                      hip_failed(hipEventCreateWithFlags(&fs.last, hipEventDisableTiming), "event")))
         return PACMAN_RECURRENCE_DEVICE_ERROR;
-    // the item table: no earlier launch on the stream may still read it, and the host copies go
-    // out of scope on return, so wait for the stream and copy synchronously
-    if (hip_failed(hipStreamSynchronize(s), "items") ||
-        hip_failed(hipMemcpy(fs.items, sc.items.data(), count * sizeof(ItemDev), hipMemcpyHostToDevice), "items") ||
+    // the item table: the previous launch has ended (above), and the host copies go out of scope
+    // on return, so copy synchronously; the caller's earlier work on `s` (which may write the
+    // momenta this launch reads) stays ordered before the launch by the stream itself
+    if (hip_failed(hipMemcpy(fs.items, sc.items.data(), count * sizeof(ItemDev), hipMemcpyHostToDevice), "items") ||
         hip_failed(hipMemcpy(fs.unit_start, sc.starts.data(), (count + 1) * sizeof(uint32_t), hipMemcpyHostToDevice), "items"))
         return PACMAN_RECURRENCE_DEVICE_ERROR;
     if (l->kernel_ms) (void)hipEventRecord(fs.start, s);
     if (!launch_fused(l->kernel, sc, s, fs.items, fs.unit_start, count, fs.counter, fs.scratch) ||
-        hip_failed(hipGetLastError(), "fused launch"))
+        hip_failed(hipGetLastError(), "fused launch") || hip_failed(hipEventRecord(fs.last, s), "event"))
         return PACMAN_RECURRENCE_DEVICE_ERROR;
+    fs.last_pending = true;
     if (l->kernel_ms) {
         if (hip_failed(hipEventRecord(fs.stop, s), "event") || hip_failed(hipEventSynchronize(fs.stop), "kernel"))
             return PACMAN_RECURRENCE_DEVICE_ERROR;

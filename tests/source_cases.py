@@ -6,7 +6,12 @@ legend_cases: the case generator of tests/check_sources.py:75-90 restated (Pytho
 patch_cases: the generator of rusticol/patches/0009-...patch:64-117 restated (xorshift64 from
   0x9e3779b97f4a7c15, 40 momenta per mass in both energy signs, the beam axis first).
 legend(): the closed forms of convert/CONVENTIONS.md:52-61 (HELAS-style spinors and polarisation
-  vectors) written out independently, in Python complex arithmetic, for a 1e-13 check."""
+  vectors) in Python complex arithmetic, for a 1e-13 check. It follows the branch structure of the
+  reference's own Python legend (tests/check_sources.py:10-72), so it pins the oracle to that
+  legend, not to an independent derivation; the physical identities in tests/test_oracle.py
+  (transversality, normalisation, spinor completeness) are the independent pins.
+CPU only: tests/golden/make_source_cases.py writes these cases and values into a data file, which
+the GPU tests load."""
 import math
 import random
 

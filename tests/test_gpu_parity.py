@@ -5,7 +5,8 @@
   internal host-vs-device KATs against the HIP library for every kernel, and its printed
   amplitudes equal the oracle's emulation of the same v1 calls byte for byte;
 - the source wavefunctions (pacman_recurrence_v1_test_sources) equal the oracle's bit for bit on
-  the case generators of tests/check_sources.py and rusticol/patches/0009 (restated here);
+  the cases of tests/check_sources.py's and rusticol/patches/0009's generators, committed as data
+  with their closed-form values (tests/golden/source_cases.npz);
 - generated n-gluon plans through the v1 path and the fused path equal the oracle's whole plan
   run bit for bit in every kernel (f64, df64, mixed, mixed64), on ragged point counts."""
 import ctypes as C
@@ -20,7 +21,6 @@ from pacuda_amd import abi, plans, phase_space as ps
 import harness
 from harness import V1, FusedPlan, DeviceArray, fused_evaluate, bits_equal
 import refcpu
-import source_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -51,11 +51,12 @@ def test_capabilities(lib):
 
 
 def test_sources_bitwise(lib):
-    cases = source_cases.legend_cases(seed=3, per_family=300) + source_cases.patch_cases()
-    codes = np.array([c[0] for c in cases], dtype=np.uint32)
-    masses = np.array([c[1] for c in cases])
-    mom = np.array([c[2] for c in cases]).reshape(-1)
-    n = len(cases)
+    """the source wavefunctions on the GPU equal the oracle's bit for bit and the closed forms of
+    convert/CONVENTIONS.md:52-61 within 1e-13, on the committed case vectors
+    (tests/golden/source_cases.npz)"""
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "source_cases.npz"))
+    codes, masses, mom = g["codes"], g["masses"], g["momenta"].reshape(-1)
+    n = len(codes)
     print("sources", n, flush=True)
     re, im = np.zeros(4 * n), np.zeros(4 * n)
     st = lib.pacman_recurrence_v1_test_sources(abi.ptr(codes, abi.u32p), abi.ptr(masses), abi.ptr(mom), n,
@@ -64,7 +65,12 @@ def test_sources_bitwise(lib):
     _, ore, oim = refcpu.test_sources(codes, masses, mom)
     bad = np.nonzero((re.view(np.uint64) != ore.reshape(-1).view(np.uint64)) |
                      (im.view(np.uint64) != oim.reshape(-1).view(np.uint64)))[0]
-    assert len(bad) == 0, f"{len(bad)} components differ, first case {bad[0] // 4}: {cases[bad[0] // 4]}"
+    assert len(bad) == 0, f"{len(bad)} components differ, first case {bad[0] // 4}: code {codes[bad[0] // 4]}"
+    got = (re + 1j * im).reshape(n, 4)
+    want = g["legend_re"] + 1j * g["legend_im"]
+    scale = np.maximum(np.max(np.abs(got), axis=1), 1e-300)
+    err = np.max(np.abs(got - want), axis=1) / scale
+    assert np.max(err) < 1e-13, f"case {int(np.argmax(err))}: {np.max(err)}"
 
 
 def _points(n, count, seed):

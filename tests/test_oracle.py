@@ -5,10 +5,11 @@
   emulation for every kernel (its host-side reductions and sampling are checked bit for bit
   inside it), and tests/runner_variants.sh's criteria hold: f64 sorted == unsorted byte for byte,
   df64 within 1e-9 and mixed/mixed64 within 1e-4 of the largest f64 value;
-- the source wavefunctions equal the closed forms of convert/CONVENTIONS.md:52-61 (restated
-  independently in tests/source_cases.py) within 1e-13 relative, on the reference's two case
-  generators, and satisfy the textbook identities (transversality, normalisation);
-- the momentum forms equal their brute-force sums;
+- the source wavefunctions equal the closed forms of convert/CONVENTIONS.md:52-61 (restated in
+  tests/source_cases.py after the reference's own Python legend) within 1e-13 relative, on the
+  reference's two case generators, and satisfy the textbook identities (transversality,
+  normalisation, spinor completeness), which do not depend on that legend;
+- the momentum forms equal their brute-force sums bit for bit;
 - whole generated plans reproduce Parke-Taylor (tests/test_physics.py).
 A mistake planted in the oracle's arithmetic (a dropped term, a sign, a swapped operand) fails
 at least one of these."""
@@ -20,7 +21,12 @@ import pytest
 
 import harness
 import refcpu
-import source_cases
+
+
+def _sc():
+    """the restated case generators and legend (CPU only; GPU runs load tests/golden data)"""
+    import source_cases
+    return source_cases
 
 KERNEL_ARGS = [("f64",), ("f64", "nosort"), ("df64",), (), ("mixed",), ("mixed64",)]
 
@@ -58,14 +64,14 @@ def test_runner_variants_criteria(rectest_outputs):
 
 @pytest.mark.parametrize("which", ["legend", "patch"])
 def test_sources_match_legend(which):
-    cases = source_cases.legend_cases() if which == "legend" else source_cases.patch_cases()
+    cases = _sc().legend_cases() if which == "legend" else _sc().patch_cases()
     codes = [c[0] for c in cases]
     masses = [c[1] for c in cases]
     mom = np.array([c[2] for c in cases]).reshape(-1)
     got, _, _ = refcpu.test_sources(codes, masses, mom)
     worst = 0.0
     for i, (code, m, k, name, h, ch) in enumerate(cases):
-        want = [complex(x) for x in source_cases.legend(name, k, h, ch, m)] + [0j] * 4
+        want = [complex(x) for x in _sc().legend(name, k, h, ch, m)] + [0j] * 4
         scale = max(1e-300, max(abs(x) for x in got[i]))
         err = max(abs(got[i][j] - want[j]) for j in range(4)) / scale
         worst = max(worst, err)
@@ -86,7 +92,7 @@ def test_polarisation_identities():
         e = math.sqrt(float(p3 @ p3))
         sgn = rng.choice([-1.0, 1.0])
         for h in (-1, 1):
-            codes.append(source_cases.code(3, 0, h, 0))
+            codes.append(_sc().code(3, 0, h, 0))
             masses.append(0.0)
             moms.append([sgn * e, *(sgn * p3)])
     got, _, _ = refcpu.test_sources(codes, masses, np.array(moms).reshape(-1))
@@ -109,7 +115,7 @@ def test_massless_spinor_completeness():
         p = [e, *p3]
         us = []
         for h in (-1, 1):
-            got, _, _ = refcpu.test_sources([source_cases.code(2, 0, h, 0)], [0.0], np.array(p))
+            got, _, _ = refcpu.test_sources([_sc().code(2, 0, h, 0)], [0.0], np.array(p))
             us.append(got[0])
         m = sum(np.outer(u, np.conj(u)) for u in us)
         # pslash gamma^0 in the chiral basis: [[p.sigma, 0], [0, p.sigmabar]] up to the block order
@@ -125,14 +131,48 @@ def test_massless_spinor_completeness():
 
 
 def test_forms_brute_force():
-    """the momentum forms of a plan equal sum(coefficient * leg momentum) (CONVENTIONS.md:44)"""
-    from pacuda_amd import plans, phase_space as ps
+    """the oracle's momentum forms (the FORMS_ON_DEVICE computation, runner/recurrence_runner.cu:
+    745-764) equal the brute-force sum 0.0 + c0 p(s0) + c1 p(s1) + ... over the external momenta
+    (convert/CONVENTIONS.md:44), bit for bit, for a gluon plan, a quark-line plan and a crossed
+    p p > j j j subprocess; and the forms of a gluon plan are the momenta flowing through its
+    propagators (sums of cyclically consecutive legs), so the full sum is zero"""
+    from pacuda_amd import plans, processes, phase_space as ps
+    subs = processes.subprocesses()
+    for plan in (plans.gluon_plan(5), plans.quark_line_plan(5), subs[len(subs) // 2].plan):
+        points = 37
+        mom, _ = ps.generate(plan.legs, points, seed=9)
+        ext = ps.external_planes(mom, plan.crossing)
+        got = refcpu.fill_forms(plan.arrays(), ext, points)
+        assert got.shape == (len(plan.forms), 4, points)
+        for f, form in enumerate(plan.forms):
+            want = 0.0
+            for leg, c in form:
+                want = want + c * ext[leg * 4:(leg + 1) * 4]
+            assert np.array_equal(got[f].view(np.uint64), np.asarray(want).view(np.uint64)), (plan.name, f, form)
     plan = plans.gluon_plan(5)
     mom, _ = ps.generate(5, 8, seed=9)
     ext = ps.external_planes(mom, plan.crossing)
-    for f in plan.forms:
-        want = sum(c * ext[leg * 4:(leg + 1) * 4] for leg, c in f)
-        assert want.shape == (4, 8)
+    got = refcpu.fill_forms(plan.arrays(), ext, 8)
+    legs = [ext[4 * l:4 * l + 4] for l in range(5)]
+    for f, form in enumerate(plan.forms):
+        members = sorted(leg for leg, _ in form)
+        assert all(c == 1.0 for _, c in form), form
+        # a cyclically consecutive run of legs
+        assert any(members == sorted((a + k) % 5 for k in range(len(members))) for a in range(5)), members
+        assert np.allclose(got[f], sum(legs[l] for l in members), rtol=0, atol=1e-9)
     # conservation: the form of legs 1..n-1 is minus the crossed momentum of leg 0
-    full = sum(ext[leg * 4:(leg + 1) * 4] for leg in range(1, 5))
-    assert np.allclose(full, -ext[0:4], atol=1e-9)
+    full = sum(legs[leg] for leg in range(1, 5))
+    assert np.allclose(full, -legs[0], atol=1e-9)
+
+
+def test_source_fixture_is_current():
+    """tests/golden/source_cases.npz (what the GPU test loads) is what its script writes now"""
+    import importlib.util
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    spec = importlib.util.spec_from_file_location("make_source_cases", os.path.join(here, "make_source_cases.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    want = m.build()
+    got = np.load(os.path.join(here, "source_cases.npz"))
+    for k, v in want.items():
+        assert np.array_equal(got[k], v), k
