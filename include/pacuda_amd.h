@@ -127,6 +127,20 @@ typedef struct {
 int pacuda_evaluate(const pacuda_item *items, uint32_t count, const pacuda_launch *launch);
 
 /*
+ * pacuda_evaluate without allocation. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The same launch and results as pacuda_evaluate, with the work counter, the
+ * item table and the wavefronts' slots in one fixed 256 MiB device array of the library (each GPU
+ * holds its own copy until the process ends) instead of buffers the library allocates. Limits:
+ * at most 4096 items; as many resident wavefronts as the slots fit (at most the launch's, at least
+ * one workgroup: a plan whose slots for one workgroup exceed the array is refused, status 2);
+ * kernel_ms must be NULL; stream NULL is the null stream. The call waits for `stream` before it
+ * rewrites the item table, so consecutive calls on one stream are ordered; calls that overlap on
+ * one GPU from different streams or threads share the array and race.
+ */
+int pacuda_evaluate_static(const pacuda_item *items, uint32_t count, const pacuda_launch *launch);
+
+/*
  * A sweep: the same batch of items evaluated again and again (a Monte Carlo loop that rewrites
  * the momenta in place between iterations). pacuda_sweep_create uploads the item table once,
  * gives the sweep its own counter and scratch, and captures the launch into a HIP graph;

@@ -1750,6 +1750,93 @@ extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const p
     return PACMAN_RECURRENCE_OK;
 }
 
+// ---- the evaluation without allocation ---------------------------------------------------------
+//
+// pacuda_evaluate_static runs the same launch as pacuda_evaluate with every buffer it needs
+// beyond the caller's inputs and outputs (the work counter, the item table, the units' first
+// indices and the wavefronts' slots) in one file-scope device array, so it never allocates.
+// Layout of the array: [counter, 256 B][items, STATIC_ITEMS x ItemDev][unit starts,
+// STATIC_ITEMS + 1 u32][slots, the rest]. Each GPU holds its own copy; its address comes from
+// hipGetSymbolAddress once per device and is the only name host code uses for it.
+namespace {
+
+constexpr size_t STATIC_BYTES = 256ull << 20;
+constexpr uint32_t STATIC_ITEMS = 4096;
+constexpr size_t STATIC_ITEMS_AT = 256;
+constexpr size_t STATIC_STARTS_AT = STATIC_ITEMS_AT + ((STATIC_ITEMS * sizeof(ItemDev) + 255) & ~(size_t)255);
+constexpr size_t STATIC_SLOTS_AT = STATIC_STARTS_AT + (((STATIC_ITEMS + 1) * sizeof(uint32_t) + 255) & ~(size_t)255);
+static_assert(STATIC_SLOTS_AT < STATIC_BYTES / 64, "the static scratch is mostly slots");
+
+static __device__ __align__(256) double scratch[STATIC_BYTES / sizeof(double)];
+
+constexpr int STATIC_DEVICES = 64;
+std::mutex static_lock;
+char *static_base[STATIC_DEVICES];  // per device: the array's device address (nullptr: not yet)
+
+// the device address of the current device's array, looked up once per device
+char *static_area(int *device) {
+    if (hip_failed(hipGetDevice(device), "device") || *device < 0 || *device >= STATIC_DEVICES) return nullptr;
+    if (!static_base[*device]) {
+        void *p = nullptr;
+        if (hip_failed(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "static scratch address")) return nullptr;
+        static_base[*device] = (char *)p;
+    }
+    return static_base[*device];
+}
+
+}  // namespace
+
+extern "C" int pacuda_evaluate_static(const pacuda_item *items, uint32_t count, const pacuda_launch *l) {
+    Schedule sc;
+    int st = schedule(items, count, l, sc);
+    if (st) return st;
+    if (l->kernel_ms) {
+        fprintf(stderr, "pacuda: pacuda_evaluate_static does not time its launch (kernel_ms must be NULL)\n");
+        return PACMAN_RECURRENCE_INVALID;
+    }
+    if (count > STATIC_ITEMS) {
+        fprintf(stderr, "pacuda: pacuda_evaluate_static takes at most %u items (%u given)\n", STATIC_ITEMS, count);
+        return PACMAN_RECURRENCE_INVALID;
+    }
+    if (count == 0 || sc.units == 0) return PACMAN_RECURRENCE_OK;
+    // the resident wavefronts whose slots fit: at most the launch's, at least one workgroup
+    const size_t slot_bytes = (size_t)sc.slot * sizeof(double);
+    const uint64_t fit = (STATIC_BYTES - STATIC_SLOTS_AT) / std::max<size_t>(slot_bytes, 1);
+    const uint32_t waves_per_block = sc.block / LANES;
+    if (fit < waves_per_block) {
+        fprintf(stderr,
+                "pacuda: pacuda_evaluate_static: %u wavefronts of %.1f MB slots exceed the static scratch of %zu MB\n",
+                waves_per_block, slot_bytes / 1e6, STATIC_BYTES >> 20);
+        return PACMAN_RECURRENCE_INVALID;
+    }
+    if (sc.waves > fit) {
+        sc.grid = (uint32_t)(fit / waves_per_block);
+        sc.waves = (uint64_t)sc.grid * waves_per_block;
+    }
+    std::lock_guard<std::mutex> guard(static_lock);
+    int device = 0;
+    char *base = static_area(&device);
+    if (!base) return PACMAN_RECURRENCE_DEVICE_ERROR;
+    uint32_t *counter = (uint32_t *)base;
+    ItemDev *table = (ItemDev *)(base + STATIC_ITEMS_AT);
+    uint32_t *starts = (uint32_t *)(base + STATIC_STARTS_AT);
+    double *slots = (double *)(base + STATIC_SLOTS_AT);
+    // (the null stream when the caller gives none: the library's own stream is created on first
+    // use, which this entry point must not do)
+    hipStream_t s = (hipStream_t)l->stream;
+    // the previous call's launch may still read the table: wait for the stream, then copy
+    // synchronously (the host copies go out of scope on return)
+    if (hip_failed(hipStreamSynchronize(s), "static items") ||
+        hip_failed(hipMemcpy(table, sc.items.data(), count * sizeof(ItemDev), hipMemcpyHostToDevice), "static items") ||
+        hip_failed(hipMemcpy(starts, sc.starts.data(), (count + 1) * sizeof(uint32_t), hipMemcpyHostToDevice),
+                   "static items"))
+        return PACMAN_RECURRENCE_DEVICE_ERROR;
+    if (!launch_fused(l->kernel, sc, s, table, starts, count, counter, slots) ||
+        hip_failed(hipGetLastError(), "static fused launch"))
+        return PACMAN_RECURRENCE_DEVICE_ERROR;
+    return PACMAN_RECURRENCE_OK;
+}
+
 // A sweep: one batch of items with its own item table, counter and scratch, captured once into a
 // HIP graph (the counter reset kernel and the fused kernel) and replayed by pacuda_sweep_run.
 struct pacuda_sweep {
