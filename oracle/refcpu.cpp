@@ -44,6 +44,8 @@
 #include <vector>
 
 #include "pacman_recurrence.h"
+// the runner's generated symjit kernels (the functions its op-13 rows call)
+#include "../pacuda_amd/csrc/plane_kernels.hpp"
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -570,11 +572,15 @@ void sort_contribution_runs(std::vector<Row> &rows) {
 // (the library's DUAL_FILL: a read loads the slot and keeps the copy, a write goes on chip;
 // DUAL_THROUGH: reads on chip, writes on chip and to the slot); zero in the reference's programs
 enum { DUAL_FILL = 1u << 31, DUAL_THROUGH = 1u << 30 };
+// bind: a symjit row's binding (kernel, inputs, output pairs, direct, input codes, output
+// components), copied from the op-14 row before it
+constexpr int BIND_WORDS = 4 + PACUDA_MAX_IN + PACUDA_MAX_OUT / 2;
 struct Op {
     int kind, op, flags;
     uint32_t f[8];
     uint32_t z[4];
     uint32_t dual[8], chip[8];
+    uint32_t bind[BIND_WORDS];
 };
 
 // a current component of plane field k (component c, known-zero bit b of mask z) read or written
@@ -599,15 +605,32 @@ template <class R> void st_field(const Arena &a, const Op &o, int k, uint32_t c,
 }
 
 bool program(const std::vector<Row> &rows, std::vector<Op> &ops, uint32_t *sources) {
+    const Row *binding = nullptr;  // the last binding (op 14) before a symjit row (op 13)
     for (size_t n = 0; n < rows.size(); n++) {
         const Row &r = rows[n];
-        if (r.op == 13 || r.op == 14) {
-            fprintf(stderr, "refcpu: a symjit row, but the oracle has no symjit kernels\n");
-            return false;
+        if (r.op == 14) {
+            binding = &r;
+            continue;
         }
         Op o;
         memset(&o, 0, sizeof o);
         o.op = r.op;
+        if (r.op == 13) {
+            // a binding of a kernel the runner carries, within its input and output limits
+            // (runner/recurrence_runner.cu:1142-1151)
+            if (!binding || binding->f[0] >= PACUDA_KERNEL_COUNT || binding->f[1] > PACUDA_MAX_IN ||
+                2 * binding->f[2] > PACUDA_MAX_OUT) {
+                fprintf(stderr, "refcpu: a symjit row without a valid binding\n");
+                return false;
+            }
+            o.kind = K_SYMJIT;
+            o.flags = r.f[0] ? F_INIT : 0;
+            memcpy(o.f, r.f, sizeof o.f);
+            for (int k = 0; k < 4; k++) o.bind[k] = binding->f[k];
+            for (size_t k = 0; k < binding->data.size(); k++) o.bind[4 + k] = binding->data[k];
+            ops.push_back(o);
+            continue;
+        }
         if (r.op >= 2 && r.op <= 6) {
             bool first = n == 0 || rows[n - 1].op < 2 || rows[n - 1].op > 6 || rows[n - 1].f[4] != r.f[4];
             bool last = n + 1 == rows.size() || rows[n + 1].op < 2 || rows[n + 1].op > 6 || rows[n + 1].f[4] != r.f[4];
@@ -668,6 +691,65 @@ void run_sources(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
                 W s = cx(fr * v[k].re - fi * v[k].im, fr * v[k].im + fi * v[k].re);
                 st_field<R>(a, o, 0, k, o.z[3], k, cx(narrow<R>(s.re), narrow<R>(s.im)), i);
             }
+        }
+    }
+}
+
+inline double f64_of(double x) { return x; }
+inline double f64_of(float x) { return (double)x; }
+inline double f64_of(D x) { return (double)x.hi + (double)x.lo; }
+
+// A symjit row (runner/recurrence_runner.cu:631-684) per point: the binding's inputs as f64
+// (kind 0 a parent's component, 1 a form component, 2 a destination component, 3 the real part of
+// a scale, else the row factor; bit 3 which parent or form, bit 4 the imaginary part, bits 5 and
+// up the component or slot), the kernel, then each output pair times the factor (re fr - im fi,
+// re fi + im fr) added to the destination component unless the row initialises it, or stored
+// unscaled when the binding is direct and the factor is exactly (1, +0); narrowed to R.
+template <class K> void symjit_row(const Op &o, const Arena &a, size_t i0, size_t i1) {
+    typedef typename K::real R;
+    typedef typename K::den Den;
+    const uint32_t kernel = o.bind[0], n_in = o.bind[1], pairs = o.bind[2], direct = o.bind[3];
+    const uint32_t parent[2] = {o.f[1], o.f[2]}, dst = o.f[3], form[2] = {o.f[4], o.f[5]}, x = o.f[6];
+    const double fr = f64_of(ldr<R>(a.fre, x)), fi = f64_of(ldr<R>(a.fim, x));
+    uint64_t frb, fib;
+    memcpy(&frb, &fr, 8);
+    memcpy(&fib, &fi, 8);
+    const bool as_is = direct && frb == 0x3ff0000000000000ull && fib == 0;
+    for (size_t i = i0; i < i1; i++) {
+        double in[PACUDA_MAX_IN], out[PACUDA_MAX_OUT];
+        for (uint32_t k = 0; k < n_in; k++) {
+            const uint32_t c = o.bind[4 + k], kind = c & 7, which = (c >> 3) & 1, im = (c >> 4) & 1, n = c >> 5;
+            double v;
+            if (kind == 0 || kind == 2) {
+                const Cx<R> z = ld_cur<R>(a, ((kind == 0 ? parent[which] : dst) + n) * a.ld + i);
+                v = f64_of(im ? z.im : z.re);
+            } else if (kind == 1) {
+                v = f64_of(ldr<Den>(a.mom, ((size_t)form[which] * 4 + n) * a.ld + i));
+            } else if (kind == 3) {
+                v = f64_of(ldr<R>(a.sre, n));
+            } else {
+                v = im ? fi : fr;
+            }
+            in[k] = v;
+        }
+        pacuda_kernel(kernel, false, in, out);
+        for (uint32_t j = 0; j < pairs; j++) {
+            const double re = out[2 * j], imv = out[2 * j + 1];
+            const size_t at = (dst + o.bind[4 + n_in + j]) * a.ld + i;
+            double sr, si;
+            if (as_is) {
+                sr = re;
+                si = imv;
+            } else {
+                sr = re * fr - imv * fi;
+                si = re * fi + imv * fr;
+                if (!(o.flags & F_INIT)) {
+                    const Cx<R> d = ld_cur<R>(a, at);
+                    sr = f64_of(d.re) + sr;
+                    si = f64_of(d.im) + si;
+                }
+            }
+            st_cur<R>(a, at, cx(narrow<R>(sr), narrow<R>(si)));
         }
     }
 }
@@ -769,6 +851,8 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
             }
         } else if (o.kind == K_SOURCE) {  // a source row made again (the library's f64 programs)
             run_sources<R>(&o, 1, a, i0, i1);
+        } else if (o.kind == K_SYMJIT) {
+            symjit_row<K>(o, a, i0, i1);
         }
     }
 }
@@ -1151,10 +1235,25 @@ int refcpu_v1_reduce_sample(const pacman_recurrence_reduce_sample *r) {
     release();
     return s;
 }
-uint32_t refcpu_v1_capabilities(void) { return PACMAN_RECURRENCE_CAP_SOURCES | PACMAN_RECURRENCE_CAP_FORMS; }
-int refcpu_v1_symjit_kernel(const char *sha256_hex) { (void)sha256_hex; return -1; }
+uint32_t refcpu_v1_capabilities(void) {
+    return PACMAN_RECURRENCE_CAP_SOURCES | PACMAN_RECURRENCE_CAP_FORMS |
+           (PACUDA_KERNEL_COUNT > 0 ? PACMAN_RECURRENCE_CAP_SYMJIT : 0u);
+}
+int refcpu_v1_symjit_kernel(const char *sha256_hex) {
+    if (!sha256_hex) return -1;
+    for (int k = 0; k < PACUDA_KERNEL_COUNT; k++)
+        if (!strcasecmp(pacuda_kernel_sha256[k], sha256_hex)) return k;
+    return -1;
+}
 void *refcpu_v1_host_alloc(size_t bytes) { return malloc(bytes ? bytes : 1); }
 void refcpu_v1_host_free(void *p) { free(p); }
+
+// Test hook: generated kernel k on `in` (PACUDA_MAX_IN doubles) into `out` (PACUDA_MAX_OUT).
+int refcpu_plane_kernel(uint32_t k, const double *in, double *out) {
+    if (k >= PACUDA_KERNEL_COUNT) return PACMAN_RECURRENCE_INVALID;
+    pacuda_kernel(k, false, in, out);
+    return PACMAN_RECURRENCE_OK;
+}
 
 // Test hook: the momentum forms of `points` points from external momenta ([(slot*4+k)*points+i])
 // into mom ([(f*4+k)*points+i]), as FORMS_ON_DEVICE computes them.

@@ -40,6 +40,8 @@ def load():
             f64p, C.c_uint32, f64p, f64p, C.c_uint32, C.c_int]
         lib.refcpu_test_sources.restype = C.c_int
         lib.refcpu_test_sources.argtypes = [u32p, f64p, f64p, C.c_uint32, f64p, f64p]
+        lib.refcpu_plane_kernel.restype = C.c_int
+        lib.refcpu_plane_kernel.argtypes = [C.c_uint32, f64p, f64p]
         lib.refcpu_fill_forms.restype = C.c_int
         lib.refcpu_fill_forms.argtypes = [f64p, C.c_uint32, u32p, C.c_uint32, u32p, f64p, f64p]
         lib.refcpu_reset.restype = None
@@ -96,6 +98,16 @@ def evaluate_program(instr, sources, slot_planes, lds_planes, plan_arrays, count
     if st != 0:
         raise RuntimeError(f"refcpu_evaluate_program: status {st}")
     return res, tot
+
+
+def plane_kernel(k, inputs, outputs):
+    """generated symjit kernel k on `inputs` (f64): its first `outputs` results"""
+    lib = load()
+    a = np.zeros(64)
+    a[:len(inputs)] = inputs
+    out = np.zeros(64)
+    assert lib.refcpu_plane_kernel(k, _p(a, f64p), _p(out, f64p)) == 0
+    return out[:outputs]
 
 
 def fill_forms(plan_arrays, external, points):

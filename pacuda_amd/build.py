@@ -12,7 +12,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_DIR = os.path.join(HERE, "lib")
 LIBRARY = os.path.join(LIB_DIR, "librecurrence.so")
-SOURCES = ["runner.hip", "fused.hip", "surrogate.hip", "phase.hip", "unweight.hip"]
+SOURCES = ["runner.hip", "plan_compiler.hip", "fused.hip", "surrogate.hip", "phase.hip", "unweight.hip"]
+KERNEL_DIR = os.path.join(HERE, "kernels")
+KERNEL_HEADER = os.path.join(HERE, "csrc", "plane_kernels.hpp")
+GEN_TOOL = os.path.join(LIB_DIR, "pacuda-gen")
 ARCH = os.environ.get("PACUDA_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -61,6 +64,26 @@ def build_check_tool(verbose=False):
     os.replace(CHECK_TOOL + ".tmp", CHECK_TOOL)
 
 
+def kernel_files():
+    """the plane-kernel files the runner carries, in kernel-index order (by file name)"""
+    return sorted(os.path.join(KERNEL_DIR, f) for f in os.listdir(KERNEL_DIR) if f.endswith(".pk"))
+
+
+def build_kernels(verbose=False):
+    """pacuda-gen (host C++), then the generated kernels' header from pacuda_amd/kernels/*.pk"""
+    os.makedirs(LIB_DIR, exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", os.path.join(HERE, "csrc", "pacuda_gen.cpp"), "-o", GEN_TOOL + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(GEN_TOOL + ".tmp", GEN_TOOL)
+    cmd = [GEN_TOOL, "hip", *kernel_files(), "-o", KERNEL_HEADER + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(KERNEL_HEADER + ".tmp", KERNEL_HEADER)
+
+
 def build_oracle(verbose=False):
     targets = ["all"]
     if os.path.exists("/root/reference/runner/rectest.c"):
@@ -72,6 +95,7 @@ def build_oracle(verbose=False):
 
 
 def build(verbose=False):
+    build_kernels(verbose)
     build_oracle(verbose)
     build_check_tool(verbose)
     return build_library(verbose)

@@ -183,7 +183,9 @@ struct Arena {
     float *fcre, *fcim;  // mixed kernels: the currents as f32 planes
     const double *mom;
     const double *sre, *sim, *fre, *fim;
+    const uint32_t *bind;  // the program's symjit bindings (none: nullptr)
     size_t ld;
+    uint32_t points;  // the call's points
 };
 
 template <class R> struct Cur {
@@ -226,6 +228,9 @@ struct PlaneArena {
     PCD const double *sim() const { return a.sim; }
     PCD const double *fre() const { return a.fre; }
     PCD const double *fim() const { return a.fim; }
+    PCD const uint32_t *bind() const { return a.bind; }
+    PCD size_t point() const { return i; }
+    PCD size_t points() const { return a.points; }
 };
 
 // A complex current component as one vector: (re, im) doubles, (re.hi, re.lo, im.hi, im.lo) for
@@ -496,6 +501,11 @@ template <class R, bool REG = false, int NF = 0> struct SlotArena {
     PCD const double *sim() const { return sim_; }
     PCD const double *fre() const { return fre_; }
     PCD const double *fim() const { return fim_; }
+    const uint32_t *bind_;  // the plan's symjit bindings
+    uint32_t point_, points_;  // the lane's point in its item, the item's points
+    PCD const uint32_t *bind() const { return bind_; }
+    PCD uint32_t point() const { return point_; }
+    PCD uint32_t points() const { return points_; }
 };
 
 template <class R> PCD R reciprocal_minkowski_norm(R p0, R p1, R p2, R p3) {

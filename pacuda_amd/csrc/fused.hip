@@ -28,6 +28,7 @@
 
 #include "pacman_recurrence.h"
 #include "pacuda_amd.h"
+#include "plan_compiler.hpp"
 #include "program.hpp"
 #include "runtime.hpp"
 
@@ -56,6 +57,7 @@ struct PlanDev {
     const double *coefficients;
     const uint32_t *offsets, *planes;
     const double *weights;
+    const uint32_t *bindings;             // the program's symjit bindings
     double numerator, denominator;
     uint32_t factor, pad;
 };
@@ -146,6 +148,9 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
         a.amp_base = C * 1024u;
         a.form_base = a.amp_base + A * 1024u;
         a.idx = lane;
+        a.bind_ = pl.bindings;
+        a.point_ = i;
+        a.points_ = it.points;
         a.lds = (threadIdx.x / LANES) * lds_per_wave * (uint32_t)(sizeof(V) / 2) + lane * (uint32_t)sizeof(V);
         // the momentum forms in f64 (runner/recurrence_runner.cu:745-764), four per lane
         for (uint32_t f = 0; f < F; f++) {
@@ -1246,6 +1251,7 @@ struct Compiled {
     uint32_t rreg_used[LDS_VARIANTS] = {};
     uint32_t reg_forms[FORM_REGS];
     bool zero_fill = true;
+    std::vector<uint32_t> bindings;
 };
 
 // F_ZERO on the first closure of every amplitude plane (closures are the only rows that touch
@@ -1325,7 +1331,8 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
             return PACMAN_RECURRENCE_INVALID;
     for (uint32_t t = 0; t < d->term_count; t++)
         if (d->term_slots[t] >= d->external_count) return PACMAN_RECURRENCE_INVALID;
-    HostProgram hp = build_program(d->words, d->word_count, d->width, !d->no_sort);
+    CompiledProgram hp = compile_program(d->words, d->word_count, d->width, !d->no_sort,
+                                         {PACUDA_KERNEL_COUNT, PACUDA_MAX_IN, PACUDA_MAX_OUT});
     if (hp.error) {
         fprintf(stderr, "pacman recurrence runner: %s\n", hp.error);
         return PACMAN_RECURRENCE_INVALID;
@@ -1333,6 +1340,24 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
     if (!program_fits(hp.need, d->current_planes, d->amplitude_planes, d->form_count, d->scale_count, d->factor_count)) {
         fprintf(stderr, "pacman recurrence runner: the program indexes past the plan's planes or tables\n");
         return PACMAN_RECURRENCE_INVALID;
+    }
+    c.bindings = hp.bindings;
+    if (hp.symjit) {
+        // Symjit rows read and write single components their bindings name, which the passes
+        // below (whole-object compaction, forwarding, on-chip placement, zero masks) do not model:
+        // such plans run their program as built, every current plane in the slot.
+        c.planes = d->current_planes;
+        c.sources = hp.sources;
+        c.plain = c.fwd = hp.ops;
+        // (every variant is the program as built, which places nothing on chip: the plan joins
+        // launches of any LDS variant without taking LDS)
+        for (int v = 0; v < LDS_VARIANTS; v++) {
+            c.lds[v] = c.reg[v] = c.rlds[v] = c.rreg[v] = hp.ops;
+            c.lds_used[v] = 1;
+        }
+        for (uint32_t &f : c.reg_forms) f = UINT32_MAX;
+        c.zero_fill = true;
+        return PACMAN_RECURRENCE_OK;
     }
     if (!reads_after_writes(hp.ops, d->current_planes)) {
         fprintf(stderr, "pacman recurrence runner: the program reads a current plane before writing it\n");
@@ -1494,6 +1519,7 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     h.offsets = upload(p, d->offsets, (size_t)d->outputs + 1, &ok);
     h.planes = upload(p, d->planes, count, &ok);
     h.weights = upload(p, d->weights, count, &ok);
+    h.bindings = upload(p, cp.bindings.data(), cp.bindings.size(), &ok);
     h.numerator = d->numerator;
     h.denominator = d->denominator;
     h.factor = d->numerator != d->denominator;

@@ -1,12 +1,7 @@
 // SPDX-License-Identifier: 0BSD
 //
-// Plan words -> device instructions (host side), and the interpreter loop (device side) that
-// one lane runs for its point.
-//
-// Host: decode the 64-bit words with the plan's field widths (runner/recurrence_runner.cu:
-// 914-975), stably sort each run of contribution rows by destination when that cannot change a
-// value (:977-1007), then emit one 32-byte instruction per row with F_FIRST/F_LAST marking the
-// rows of one destination that a lane accumulates in registers (:1009-1054).
+// The instruction format the host's plan compiler (plan_compiler.hpp) emits, and the
+// interpreter loop (device side) that one lane runs for its point.
 //
 // Device: the instruction stream is the same for every lane of a wavefront, so each
 // instruction is fetched with scalar loads into SGPRs and dispatched by a uniform branch; the
@@ -17,10 +12,15 @@
 #include <vector>
 
 #include "device.hpp"
+#include "plane_kernels.hpp"
 
 namespace pacuda {
 
 enum { K_CLEAR = 1, K_CONTRIB, K_PROP, K_IDENTITY, K_CLOSURE, K_SOURCE, K_SYMJIT };
+// a symjit binding's input code: bits 0-2 the kind below, bit 3 which parent or form, bit 4 the
+// imaginary part, bits 5 and up the component (or the scale slot) (runner/recurrence_runner.cu:
+// 582, 631-684)
+enum { SB_PARENT, SB_MOMENTUM, SB_DESTINATION, SB_SCALE, SB_FACTOR };
 // F_HOLD: the row's result stays in the accumulators for the next row instead of being stored;
 // F_FROMACC: an in-place row (propagator, identity) takes its input from the accumulators
 // instead of loading it; F_ZERO: a closure that is the first to touch its amplitude starts from
@@ -33,26 +33,15 @@ enum { F_INIT = 1, F_FIRST = 2, F_LAST = 4, F_HOLD = 8, F_FROMACC = 16, F_ZERO =
 // count; propagator: w[1] base, w[2] form, w[3] factor; identity: w[1] base, w[2] count, w[3]
 // factor; closure: w[1] parent 0, w[2] parent 1, w[3] amplitude, w[4] factor, w[5] first
 // component factor, w[6] count; source: w[1] destination, w[2] form, w[3] factor, w[4] mass
-// slot, w[5] template.
+// slot, w[5] template; symjit: w[1] parent 0, w[2] parent 1, w[3] destination, w[4] form 0 |
+// form 1 << 16, w[5] factor, w[6] the binding's offset in the binding table (kernel, inputs,
+// output pairs, direct, then the input codes and the output components).
 struct alignas(32) Instr {
     uint32_t w[8];
 };
 
-struct HostRow {
-    int op;
-    uint32_t f[8];
-    std::vector<uint32_t> data;
-};
-
-// a program ready for upload, or an error message; `need` holds one more than the largest index
-// the program touches per table (the call's tables must be at least that long)
+// the tables a program indexes (the extents a compiled program records per table)
 enum { N_CURRENT, N_AMPLITUDE, N_FORM, N_SCALE, N_FACTOR, N_COUNT };
-struct HostProgram {
-    std::vector<Instr> ops;
-    uint32_t sources = 0;
-    uint64_t need[N_COUNT] = {0, 0, 0, 0, 0};
-    const char *error = nullptr;
-};
 // true if a call with these table sizes keeps every index of the program in bounds
 inline bool program_fits(const uint64_t *need, uint64_t currents, uint64_t amplitudes, uint64_t forms, uint64_t scales,
                          uint64_t factors) {
@@ -60,8 +49,6 @@ inline bool program_fits(const uint64_t *need, uint64_t currents, uint64_t ampli
            need[N_SCALE] <= scales && need[N_FACTOR] <= factors;
 }
 
-HostProgram build_program(const uint64_t *words, uint32_t count, const uint8_t *width, bool sort);
-void contribution_shape(int op, int *c0, int *c1, int *cd);
 
 // ---- the interpreter (device) ----------------------------------------------------------------
 // A is the arena (PlaneArena for the v1 path, SlotArena for the fused path). The fused programs
@@ -143,6 +130,66 @@ PCD void propagator(const A &a, const Instr &o, int flags, Cx<typename K::real> 
         for (int k = 0; k < m; k++) acc[k] = v[k];
     } else {
         a.template puts<R, m>(base, v, zbits<A>(o.w[4], 8, 4));
+    }
+}
+
+// a current, form or constant value as the f64 a symjit kernel takes (df64: hi + lo in f64)
+PCD double as_f64(double x) { return x; }
+PCD double as_f64(float x) { return (double)x; }
+PCD double as_f64(D x) { return (double)x.hi + (double)x.lo; }
+
+// A symjit row (op 13), as RustiCol's executors run it (runner/recurrence_runner.cu:631-684): the
+// kernel's inputs gathered in f64 by the binding's codes, the generated kernel, then each output
+// pair (re, im) times the row's exact factor (re fr - im fi, re fi + im fr, each operation rounded
+// once) added to the destination component unless the row initialises it -- or stored as it is
+// when the binding is direct and the factor is exactly (1, +0) -- and narrowed to the kernel's
+// current type.
+template <class K, class A> __device__ __noinline__ void symjit_row(const A &a, const Instr &o, int flags) {
+    typedef typename K::real R;
+    typedef typename K::den Den;
+    const uint32_t *b = a.bind() + o.w[6];
+    const uint32_t kernel = cnst(b)[0], inputs = cnst(b)[1], pairs = cnst(b)[2], direct = cnst(b)[3];
+    const uint32_t dst = o.w[3];
+    const double fr = as_f64(ldc<R>(a.fre(), o.w[5])), fi = as_f64(ldc<R>(a.fim(), o.w[5]));
+    double in[PACUDA_MAX_IN], out[PACUDA_MAX_OUT];
+    for (uint32_t k = 0; k < inputs; k++) {
+        const uint32_t code = cnst(b)[4 + k], which = (code >> 3) & 1, imag = (code >> 4) & 1, index = code >> 5;
+        const uint32_t kind = code & 7;
+        double v;
+        if (kind == SB_PARENT || kind == SB_DESTINATION) {
+            const Cx<R> x = a.template cur<R>((kind == SB_PARENT ? (which ? o.w[2] : o.w[1]) : dst) + index);
+            v = as_f64(imag ? x.im : x.re);
+        } else if (kind == SB_MOMENTUM) {
+            const uint32_t q = (which ? o.w[4] >> 16 : o.w[4] & 0xffff) * 4 + index;
+            double raw[4];
+            a.form(q / 4, raw);
+            v = as_f64(from_raw<Den>(raw[q % 4]));
+        } else if (kind == SB_SCALE) {
+            v = as_f64(ldc<R>(a.sre(), index));
+        } else {
+            v = imag ? fi : fr;
+        }
+        in[k] = v;
+    }
+    const bool simd = PACUDA_SIMD_LANES > 0 &&
+                      a.point() < a.points() / (PACUDA_SIMD_LANES > 0 ? PACUDA_SIMD_LANES : 1) * PACUDA_SIMD_LANES;
+    pacuda_kernel(kernel, simd, in, out);
+    const bool as_is = direct && __builtin_bit_cast(uint64_t, fr) == __builtin_bit_cast(uint64_t, 1.0) &&
+                       __builtin_bit_cast(uint64_t, fi) == 0;
+    for (uint32_t j = 0; j < pairs; j++) {
+        const double re = out[2 * j], im = out[2 * j + 1];
+        const uint32_t plane = dst + cnst(b)[4 + inputs + j];
+        double sr = re, si = im;
+        if (!as_is) {
+            sr = re * fr - im * fi;
+            si = re * fi + im * fr;
+            if (!(flags & F_INIT)) {
+                const Cx<R> d = a.template cur<R>(plane);
+                sr = as_f64(d.re) + sr;
+                si = as_f64(d.im) + si;
+            }
+        }
+        a.template put<R>(plane, cx(narrow<R>(sr), narrow<R>(si)));
     }
 }
 
@@ -245,6 +292,8 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
                     a.template put<R>(o.w[1] + k, identity_value(fr, fi, a.template cur<R>(o.w[1] + k, zi)), zo);
                 }
             }
+        } else if (kind == K_SYMJIT) {
+            symjit_row<K>(a, o, flags);
         } else if (kind == K_SOURCE) {  // a source row made again (programs of the f64 kernel only)
             if constexpr (__is_same(R, double) && __is_same(typename K::den, double)) source_row<R>(o, a);
         } else if (kind == K_CLEAR) {

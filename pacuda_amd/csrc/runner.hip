@@ -5,27 +5,31 @@
 //
 // One lane per phase-space point, 64 points per wavefront, the plan's instructions in SGPRs
 // (scalar loads, uniform dispatch), the arena plane-major (structure of arrays) so each value
-// access is one coalesced 512-byte wavefront load. The host side keeps the reference's call
-// semantics (runner/recurrence_runner.cu:1290-1865): the arena and the compiled programs persist
-// between calls, only the named plane runs move, df64/mixed layouts are converted in place,
-// KEEP holds the lock until the same thread's reduction. All device work of a call is queued on
-// one stream (default priority) and synchronised once.
+// access is one coalesced 512-byte wavefront load. The arena and the compiled programs persist
+// between calls; only the named plane runs move; df64/mixed layouts are converted in place; all
+// device work of a call is queued on one stream (default priority) and synchronised once.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <strings.h>
-#include <time.h>
 
 #include <algorithm>
-#include <map>
+#include <atomic>
+#include <chrono>
+#include <iterator>
+#include <memory>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "pacman_recurrence.h"
 #include "pacuda_amd.h"
+#include "plan_compiler.hpp"
 #include "program.hpp"
 #include "runtime.hpp"
 
@@ -194,169 +198,22 @@ __global__ void test_sources_kernel(const uint32_t *codes, const double *masses,
     for (int k = 0; k < 4; k++) { out_re[4 * i + k] = v[k].re; out_im[4 * i + k] = v[k].im; }
 }
 
-// ---- host: programs --------------------------------------------------------------------------
+
+// ---- host ------------------------------------------------------------------------------------
+//
+// The v1 call contract (runner/pacman_recurrence.h, runner/recurrence_runner.cu:1290-1865):
+// status codes, which calls are refused, what a call moves between host and device, the kept
+// amplitudes and the lock rules. The structure here:
+//   Runner       the process's one device state: arenas, compiled programs, kept amplitudes,
+//                configuration, statistics; created on first use and never destroyed (the HIP
+//                runtime may be gone when static destructors run)
+//   RunPlan      one run call: every check on the host first, then the device work queued on
+//                the library's stream in the order the data needs, then one synchronisation
+//   Reduction    the three reductions of a kept run, over one upload of their tables
+//   RunnerLock   one lock for all entry points; a kept run leaves it with its thread until that
+//                thread's reduction
 
 namespace pacuda {
-
-void contribution_shape(int op, int *c0, int *c1, int *cd) {
-    *c0 = op == 4 ? 6 : (op == 5 || op == 6) ? 4 : 2;
-    *c1 = 4;
-    *cd = (op == 4 || op == 6) ? 4 : op == 5 ? 6 : 2;
-}
-
-static std::vector<HostRow> decode_words(const uint64_t *words, uint32_t count, const uint8_t *width) {
-    std::vector<HostRow> rows;
-    rows.reserve(count);
-    for (uint32_t n = 0; n < count; n++) {
-        uint64_t w = words[n];
-        HostRow r;
-        memset(r.f, 0, sizeof r.f);
-        if ((w & 15) == 14) {  // symjit binding: kernel 16, inputs 8, output pairs 8, direct 1, data words
-            r.op = 14;
-            r.f[0] = (uint32_t)(w >> 4) & 0xffff;
-            r.f[1] = (uint32_t)(w >> 20) & 0xff;
-            r.f[2] = (uint32_t)(w >> 28) & 0xff;
-            r.f[3] = (uint32_t)(w >> 36) & 1;
-            uint32_t codes = r.f[1] + r.f[2];
-            for (uint32_t k = 0; k < codes; k++) {
-                uint64_t d = n + 1 + k / 2 < count ? words[n + 1 + k / 2] : 0;
-                r.data.push_back((uint32_t)(k % 2 ? d >> 32 : d));
-            }
-            n += (codes + 1) / 2;
-            rows.push_back(std::move(r));
-            continue;
-        }
-        r.op = (int)(w & 15);
-        w >>= 4;
-        const int P = width[W_P], A = width[W_A], F = width[W_F], X = width[W_X], S = width[W_S], K = width[W_K];
-        int fields[8] = {0}, nf = 0;
-        auto set = [&](std::initializer_list<int> l) { for (int b : l) fields[nf++] = b; };
-        switch (r.op) {
-        case 1: set({P, P}); break;
-        case 2: case 3: case 4: case 5: case 6: set({1, S, P, P, P, F, F, X}); break;
-        case 7: case 8: case 9: set({P, F, X}); break;
-        case 10: set({P, K, X}); break;
-        case 11: set({P, P, A, X, X, K}); break;
-        case 12: set({P, F, X, S, 10}); break;
-        case 13: set({1, P, P, P, F, F, X}); break;
-        default: break;
-        }
-        for (int k = 0; k < nf; k++) {
-            r.f[k] = (uint32_t)(w & ((1ull << fields[k]) - 1));
-            w >>= fields[k];
-        }
-        rows.push_back(std::move(r));
-    }
-    return rows;
-}
-
-// each maximal run of contribution rows, stably by destination, when no row of the run reads a
-// component the run writes and the destinations do not overlap (each destination keeps its order)
-static void sort_runs(std::vector<HostRow> &rows) {
-    size_t n = 0;
-    while (n < rows.size()) {
-        if (rows[n].op < 2 || rows[n].op > 6) { n++; continue; }
-        size_t end = n;
-        while (end < rows.size() && rows[end].op >= 2 && rows[end].op <= 6) end++;
-        std::unordered_map<uint32_t, uint32_t> owner;  // written component -> its destination base
-        bool safe = true;
-        for (size_t k = n; k < end && safe; k++) {
-            int c0, c1, cd;
-            contribution_shape(rows[k].op, &c0, &c1, &cd);
-            for (int q = 0; q < cd; q++) {
-                auto it = owner.emplace(rows[k].f[4] + q, rows[k].f[4]);
-                if (it.first->second != rows[k].f[4]) safe = false;
-            }
-        }
-        for (size_t k = n; k < end && safe; k++) {
-            int c0, c1, cd;
-            contribution_shape(rows[k].op, &c0, &c1, &cd);
-            for (int q = 0; q < c0 && safe; q++) safe = !owner.count(rows[k].f[2] + q);
-            for (int q = 0; q < c1 && safe; q++) safe = !owner.count(rows[k].f[3] + q);
-        }
-        if (safe)
-            std::stable_sort(rows.begin() + n, rows.begin() + end,
-                             [](const HostRow &x, const HostRow &y) { return x.f[4] < y.f[4]; });
-        n = end;
-    }
-}
-
-HostProgram build_program(const uint64_t *words, uint32_t count, const uint8_t *width, bool sort) {
-    HostProgram p;
-    std::vector<HostRow> rows = decode_words(words, count, width);
-    if (sort) sort_runs(rows);
-    for (size_t n = 0; n < rows.size(); n++) {
-        const HostRow &r = rows[n];
-        if (r.op == 13 || r.op == 14) {
-            p.error = "a symjit row, but this runner has no symjit kernels";
-            return p;
-        }
-        if (r.op < 1 || r.op > 12) {
-            p.error = "a word with an unknown op";
-            return p;
-        }
-        Instr o;
-        memset(&o, 0, sizeof o);
-        int kind, flags = 0;
-        if (r.op >= 2 && r.op <= 6) {
-            bool first = n == 0 || rows[n - 1].op < 2 || rows[n - 1].op > 6 || rows[n - 1].f[4] != r.f[4];
-            bool last = n + 1 == rows.size() || rows[n + 1].op < 2 || rows[n + 1].op > 6 || rows[n + 1].f[4] != r.f[4];
-            kind = K_CONTRIB;
-            flags = (r.f[0] ? F_INIT : 0) | (first ? F_FIRST : 0) | (last ? F_LAST : 0);
-            for (int k = 1; k < 8; k++) o.w[k] = r.f[k];
-        } else {
-            kind = r.op == 1 ? K_CLEAR : r.op <= 9 ? K_PROP : r.op == 10 ? K_IDENTITY : r.op == 11 ? K_CLOSURE : K_SOURCE;
-            for (int k = 0; k < 7; k++) o.w[k + 1] = r.f[k];
-        }
-        o.w[0] = (uint32_t)kind | (uint32_t)r.op << 8 | (uint32_t)flags << 16;
-        p.ops.push_back(o);
-    }
-    // the extent of every table the program indexes
-    auto use = [&](int table, uint64_t first, uint64_t count) {
-        if (count) p.need[table] = std::max<uint64_t>(p.need[table], first + count);
-    };
-    for (const Instr &o : p.ops) {
-        const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff;
-        if (kind == K_CONTRIB) {
-            int c0, c1, cd;
-            contribution_shape(op, &c0, &c1, &cd);
-            use(N_CURRENT, o.w[2], c0);
-            use(N_CURRENT, o.w[3], c1);
-            use(N_CURRENT, o.w[4], cd);
-            use(N_SCALE, o.w[1], 1);
-            use(N_FACTOR, o.w[7], 1);
-            if (op == 6) { use(N_FORM, o.w[5], 1); use(N_FORM, o.w[6], 1); }
-        } else if (kind == K_CLEAR) {
-            use(N_CURRENT, o.w[1], o.w[2]);
-        } else if (kind == K_PROP) {
-            use(N_CURRENT, o.w[1], op == 9 ? 4 : 2);
-            use(N_FORM, o.w[2], 1);
-            use(N_FACTOR, o.w[3], 1);
-        } else if (kind == K_IDENTITY) {
-            use(N_CURRENT, o.w[1], o.w[2]);
-            use(N_FACTOR, o.w[3], 1);
-        } else if (kind == K_CLOSURE) {
-            use(N_CURRENT, o.w[1], o.w[6]);
-            use(N_CURRENT, o.w[2], o.w[6]);
-            use(N_AMPLITUDE, o.w[3], 1);
-            use(N_FACTOR, o.w[4], 1);
-            use(N_FACTOR, o.w[5], o.w[6]);
-        } else if (kind == K_SOURCE) {
-            uint32_t family = o.w[5] & 7;
-            use(N_CURRENT, o.w[1], family == 0 ? 1 : family == 1 ? 2 : 4);
-            use(N_FORM, o.w[2], 1);
-            use(N_FACTOR, o.w[3], 1);
-            use(N_SCALE, o.w[4], 1);
-        }
-    }
-    while (p.sources < p.ops.size() && (p.ops[p.sources].w[0] & 0xff) == K_SOURCE) p.sources++;
-    for (size_t n = p.sources; n < p.ops.size(); n++)
-        if ((p.ops[n].w[0] & 0xff) == K_SOURCE) {
-            p.error = "a source row follows other rows";
-            return p;
-        }
-    return p;
-}
 
 bool hip_failed(hipError_t e, const char *step) {
     if (e == hipSuccess) return false;
@@ -365,603 +222,754 @@ bool hip_failed(hipError_t e, const char *step) {
 }
 
 hipStream_t runner_stream() {
-    static hipStream_t s = nullptr;
-    if (!s && hip_failed(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream")) s = nullptr;
-    return s;
+    static hipStream_t stream = [] {
+        hipStream_t s = nullptr;
+        return hip_failed(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream") ? nullptr : s;
+    }();
+    return stream;
 }
 
 }  // namespace pacuda
 
-// ---- host: the v1 runner state ----------------------------------------------------------------
-
 namespace {
 
-struct Program {
-    Instr *ops = nullptr;
+constexpr int OK = PACMAN_RECURRENCE_OK, INVALID = PACMAN_RECURRENCE_INVALID, DEVICE_ERROR = PACMAN_RECURRENCE_DEVICE_ERROR;
+constexpr KernelLimits GENERATED_KERNELS = {PACUDA_KERNEL_COUNT, PACUDA_MAX_IN, PACUDA_MAX_OUT};
+
+// A device array that grows on demand; growing drops the contents (so does the reference's arena:
+// the caller uploads what a program reads).
+template <class T> class DeviceArray {
+  public:
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray &) = delete;
+    DeviceArray &operator=(const DeviceArray &) = delete;
+    ~DeviceArray() { release(); }
+    bool fit(size_t n, const char *what) {
+        n = std::max<size_t>(n, 1);
+        if (n <= size_) return true;
+        release();
+        if (hip_failed(hipMalloc((void **)&p_, n * sizeof(T)), what)) {
+            p_ = nullptr;
+            return false;
+        }
+        size_ = n;
+        return true;
+    }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        size_ = 0;
+    }
+    T *data() const { return p_; }
+
+  private:
+    T *p_ = nullptr;
+    size_t size_ = 0;
+};
+
+// pacman_recurrence_options as the runner uses them
+struct Config {
+    uint32_t kernel = PACMAN_RECURRENCE_KERNEL_DF64;
+    uint32_t block = 128;
+    bool sort = true;
+    // kernels other than f64 keep the uploaded values, the constants and (df64, mixed) the forms
+    // as df64 pairs; the mixed kernels keep the currents as separate f32 planes
+    bool df64_layout() const { return kernel != PACMAN_RECURRENCE_KERNEL_F64; }
+    bool df64_forms() const { return kernel == PACMAN_RECURRENCE_KERNEL_DF64 || kernel == PACMAN_RECURRENCE_KERNEL_MIXED; }
+    bool f32_currents() const {
+        return kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64;
+    }
+    bool operator==(const Config &o) const { return kernel == o.kernel && block == o.block && sort == o.sort; }
+};
+
+struct Stats {
+    bool enabled = false;
+    uint64_t calls = 0, points = 0, reductions = 0, copies = 0, bytes = 0;
+    double seconds = 0;
+    void copied(uint64_t count, uint64_t b) {
+        copies += count;
+        bytes += b;
+    }
+    void print() const {
+        if (!enabled || calls == 0) return;
+        fprintf(stderr,
+                "pacman recurrence runner: %llu calls, %llu points, %llu reductions, %llu copies (%.1f MB), "
+                "%.3f s (%.2f us/point)\n",
+                (unsigned long long)calls, (unsigned long long)points, (unsigned long long)reductions,
+                (unsigned long long)copies, bytes / 1e6, seconds, points ? seconds * 1e6 / points : 0.0);
+    }
+};
+
+// wall time of a call, taken only when statistics are on
+class CallTimer {
+  public:
+    explicit CallTimer(bool on) : on_(on) {
+        if (on_) t0_ = std::chrono::steady_clock::now();
+    }
+    double seconds() const {
+        return on_ ? std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count() : 0.0;
+    }
+
+  private:
+    bool on_;
+    std::chrono::steady_clock::time_point t0_;
+};
+
+// A compiled program on the device.
+struct DeviceProgram {
+    DeviceArray<Instr> ops;
+    DeviceArray<uint32_t> bindings;
     uint32_t count = 0, sources = 0;
     uint64_t need[N_COUNT] = {0, 0, 0, 0, 0};
 };
 
-enum { KERNEL_DF64 = 0, KERNEL_F64 = 1, KERNEL_MIXED = 2, KERNEL_MIXED64 = 3 };
-struct Options {
-    int kernel = KERNEL_DF64;
-    bool df64 = true, df64_forms = true;
-    int block = 128;
-    bool sort = true;
+// Programs by words key (0: not kept -- the next uncached call replaces it).
+class ProgramCache {
+  public:
+    const DeviceProgram *get(const pacman_recurrence_call &c, bool sort, hipStream_t s) {
+        if (c.words_key) {
+            auto it = keyed_.find(c.words_key);
+            if (it != keyed_.end()) return it->second.get();
+        }
+        CompiledProgram cp = compile_program(c.words, c.word_count, c.width, sort, GENERATED_KERNELS);
+        if (cp.error) {
+            fprintf(stderr, "pacman recurrence runner: %s\n", cp.error);
+            return nullptr;
+        }
+        std::unique_ptr<DeviceProgram> p(new DeviceProgram);
+        p->count = (uint32_t)cp.ops.size();
+        p->sources = cp.sources;
+        std::copy(cp.need, cp.need + N_COUNT, p->need);
+        // synchronous copies: the host vectors end with this function
+        if (!p->ops.fit(cp.ops.size(), "program") || !p->bindings.fit(cp.bindings.size(), "bindings") ||
+            (!cp.ops.empty() &&
+             hip_failed(hipMemcpy(p->ops.data(), cp.ops.data(), cp.ops.size() * sizeof(Instr), hipMemcpyHostToDevice),
+                        "program")) ||
+            (!cp.bindings.empty() &&
+             hip_failed(hipMemcpy(p->bindings.data(), cp.bindings.data(), cp.bindings.size() * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice),
+                        "bindings")))
+            return nullptr;
+        if (c.words_key) return (keyed_[c.words_key] = std::move(p)).get();
+        // the replaced program may still be in use by queued work
+        if (unkeyed_ && hip_failed(hipStreamSynchronize(s), "program")) return nullptr;
+        unkeyed_ = std::move(p);
+        return unkeyed_.get();
+    }
+
+  private:
+    std::unordered_map<uint64_t, std::unique_ptr<DeviceProgram>> keyed_;
+    std::unique_ptr<DeviceProgram> unkeyed_;
 };
 
-template <class T> struct Buf {
-    T *p = nullptr;
-    size_t capacity = 0;
-    // at least n elements; a grown buffer loses its contents
-    bool reserve(size_t n, const char *what) {
-        if (n <= capacity) return true;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        capacity = 0;
-        if (hip_failed(hipMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(T)), what)) return false;
-        capacity = n;
+// The amplitudes a KEEP run left on the device for the next reduction.
+struct Kept {
+    bool valid = false;
+    bool df64 = false;  // stored as df64 pairs
+    size_t stride = 0;
+    uint32_t points = 0, planes = 0;
+};
+
+struct Runner {
+    Config config;
+    bool started = false;  // a run was attempted: the configuration is fixed
+    Stats stats;
+    Kept kept;
+    ProgramCache programs;
+    // the arena: currents (f64 or df64 pairs; f32 planes beside them for the mixed kernels),
+    // amplitudes, momentum forms, the constants (scales re, im, factors re, im)
+    DeviceArray<double> cre, cim, are, aim, forms, constants;
+    DeviceArray<float> fcre, fcim;
+    // FORMS_ON_DEVICE inputs
+    DeviceArray<double> external, coefficients;
+    DeviceArray<uint32_t> form_terms, term_slots;
+    // reduction inputs and outputs
+    DeviceArray<uint32_t> rplanes, roffsets, rhelicity, rchosen;
+    DeviceArray<double> rweights, routput, rrandom, rscale, rtotal;
+
+    void account(size_t points, double seconds) {
+        stats.calls++;
+        stats.points += points;
+        stats.seconds += seconds;
+    }
+};
+
+Runner &runner() {
+    static Runner *r = [] {
+        Runner *n = new Runner;  // never destroyed: see above
+        atexit([] { runner().stats.print(); });
+        return n;
+    }();
+    return *r;
+}
+
+unsigned grid_stride_blocks(size_t count) { return (unsigned)std::min<size_t>(4096, std::max<size_t>(1, (count + 255) / 256)); }
+
+// A plane run: planes [first, first + count) of a table of `planes` planes.
+struct PlaneRun {
+    uint32_t first, count;
+};
+bool runs_in_range(const uint32_t *runs, uint32_t n, uint32_t planes) {
+    for (uint32_t r = 0; r < n; r++)
+        if (runs[2 * r] > planes || runs[2 * r + 1] > planes - runs[2 * r]) return false;
+    return true;
+}
+
+// One run call.
+class RunPlan {
+  public:
+    RunPlan(Runner &r, const pacman_recurrence_call &c) : r_(r), c_(c), s_(runner_stream()) {}
+
+    int execute() {
+        const CallTimer timer(r_.stats.enabled);
+        int status = check();
+        if (status == OK) status = reserve();
+        if (status == OK) status = stage_inputs();
+        if (status == OK) status = prepare();
+        if (status == OK) status = compute();
+        if (status == OK) status = deliver();
+        if (status != OK) return status;
+        if (keep_) r_.kept = Kept{true, r_.config.df64_layout(), c_.stride, c_.points, c_.amplitude_planes};
+        r_.account(c_.points, timer.seconds());
+        return OK;
+    }
+
+  private:
+    Runner &r_;
+    const pacman_recurrence_call &c_;
+    hipStream_t s_;
+    bool keep_ = false, forms_on_device_ = false;
+    const DeviceProgram *program_ = nullptr;
+    Arena arena_{};
+
+    size_t plane_doubles(uint32_t planes) const { return (size_t)planes * c_.stride; }
+    double *scale_re() const { return r_.constants.data(); }
+    double *scale_im() const { return scale_re() + c_.scale_count; }
+    double *factor_re() const { return scale_im() + c_.scale_count; }
+    double *factor_im() const { return factor_re() + c_.factor_count; }
+    size_t constant_count() const { return 2 * ((size_t)c_.scale_count + c_.factor_count); }
+
+    // everything the host can check before the device is touched
+    int check() {
+        if (c_.points == 0 || c_.points > c_.stride || !c_.words || (c_.zero_amplitude_runs && !c_.zero_amplitude))
+            return INVALID;
+        r_.kept.valid = false;  // a run ends any kept one, whatever its outcome
+        keep_ = c_.flags & PACMAN_RECURRENCE_KEEP_AMPLITUDES;
+        forms_on_device_ = c_.flags & PACMAN_RECURRENCE_FORMS_ON_DEVICE;
+        if (forms_on_device_) {
+            if (!c_.external_momenta || !c_.form_terms || (c_.term_count && (!c_.term_slots || !c_.term_coefficients)))
+                return INVALID;
+            if (!runs_in_range(c_.form_terms, c_.momentum_planes / 4, c_.term_count)) return INVALID;
+            for (uint32_t t = 0; t < c_.term_count; t++)
+                if (c_.term_slots[t] >= c_.external_count) return INVALID;
+        }
+        if (!runs_in_range(c_.zero_amplitude, c_.zero_amplitude_runs, c_.amplitude_planes)) return INVALID;
+        // a run list past its table cannot be copied: a device error, as the copy's failure
+        if (!runs_in_range(c_.upload_current, c_.upload_current_runs, c_.current_planes) ||
+            !runs_in_range(c_.upload_amplitude, c_.upload_amplitude_runs, c_.amplitude_planes) ||
+            !runs_in_range(c_.download_current, c_.download_current_runs, c_.current_planes) ||
+            !runs_in_range(c_.download_amplitude, c_.download_amplitude_runs, c_.amplitude_planes))
+            return DEVICE_ERROR;
+        return s_ ? OK : DEVICE_ERROR;
+    }
+
+    int reserve() {
+        const size_t currents = plane_doubles(c_.current_planes), amplitudes = plane_doubles(c_.amplitude_planes);
+        bool ok = r_.cre.fit(currents, "current arena") && r_.cim.fit(currents, "current arena") &&
+                  r_.are.fit(amplitudes, "amplitudes") && r_.aim.fit(amplitudes, "amplitudes") &&
+                  r_.forms.fit(plane_doubles(c_.momentum_planes), "momenta") &&
+                  r_.constants.fit(constant_count(), "factors");
+        if (ok && r_.config.f32_currents())
+            ok = r_.fcre.fit(currents, "current arena (f32)") && r_.fcim.fit(currents, "current arena (f32)");
+        if (ok && forms_on_device_)
+            ok = r_.external.fit((size_t)c_.external_count * 4 * c_.points, "external momenta") &&
+                 r_.form_terms.fit(2 * (size_t)(c_.momentum_planes / 4), "form terms") &&
+                 r_.coefficients.fit(c_.term_count, "form terms") && r_.term_slots.fit(c_.term_count, "form terms");
+        return ok ? OK : DEVICE_ERROR;
+    }
+
+    bool upload(void *dst, const void *src, size_t bytes, const char *what) {
+        return !bytes || !hip_failed(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s_), what);
+    }
+    // the first `points` values of each plane of a run, between the caller's planes and the arena
+    bool copy_run(double *device, double *host, PlaneRun run, bool to_device) {
+        if (run.count == 0) return true;
+        const size_t pitch = c_.stride * sizeof(double), at = (size_t)run.first * c_.stride;
+        const hipError_t e = to_device ? hipMemcpy2DAsync(device + at, pitch, host + at, pitch, c_.points * sizeof(double),
+                                                          run.count, hipMemcpyHostToDevice, s_)
+                                       : hipMemcpy2DAsync(host + at, pitch, device + at, pitch, c_.points * sizeof(double),
+                                                          run.count, hipMemcpyDeviceToHost, s_);
+        if (hip_failed(e, "copy")) return false;
+        r_.stats.copied(1, (uint64_t)run.count * c_.points * sizeof(double));
         return true;
     }
+    bool copy_runs(double *dre, double *dim, double *hre, double *him, const uint32_t *runs, uint32_t n, bool to_device) {
+        for (uint32_t k = 0; k < n; k++) {
+            const PlaneRun run{runs[2 * k], runs[2 * k + 1]};
+            if (!copy_run(dre, hre, run, to_device) || !copy_run(dim, him, run, to_device)) return false;
+        }
+        return true;
+    }
+
+    int stage_inputs() {
+        // momentum forms: from the external momenta on the device, or the caller's planes
+        const uint32_t form_count = c_.momentum_planes / 4;
+        if (forms_on_device_) {
+            const size_t external = (size_t)c_.external_count * 4 * c_.points;
+            if (!upload(r_.external.data(), c_.external_momenta, external * sizeof(double), "external momenta") ||
+                !upload(r_.form_terms.data(), c_.form_terms, 2 * (size_t)form_count * sizeof(uint32_t), "form terms") ||
+                !upload(r_.term_slots.data(), c_.term_slots, c_.term_count * sizeof(uint32_t), "form terms") ||
+                !upload(r_.coefficients.data(), c_.term_coefficients, c_.term_count * sizeof(double), "form terms"))
+                return DEVICE_ERROR;
+            r_.stats.copied(1, external * sizeof(double));
+            if (form_count) {
+                fill_forms_kernel<<<(c_.points + 255) / 256, 256, 0, s_>>>(r_.external.data(), c_.points,
+                                                                           r_.form_terms.data(), form_count,
+                                                                           r_.term_slots.data(), r_.coefficients.data(),
+                                                                           r_.forms.data(), c_.stride);
+                if (hip_failed(hipGetLastError(), "forms")) return DEVICE_ERROR;
+            }
+        } else if (c_.momentum_planes && !copy_run(r_.forms.data(), const_cast<double *>(c_.momenta), {0, c_.momentum_planes}, true)) {
+            return DEVICE_ERROR;
+        }
+        // scales and factors, then the named current and amplitude runs, then the zeroed runs
+        if (!upload(scale_re(), c_.scale_re, c_.scale_count * sizeof(double), "scales") ||
+            !upload(scale_im(), c_.scale_im, c_.scale_count * sizeof(double), "scales") ||
+            !upload(factor_re(), c_.factor_re, c_.factor_count * sizeof(double), "factors") ||
+            !upload(factor_im(), c_.factor_im, c_.factor_count * sizeof(double), "factors") ||
+            !copy_runs(r_.cre.data(), r_.cim.data(), c_.current_re, c_.current_im, c_.upload_current,
+                       c_.upload_current_runs, true) ||
+            !copy_runs(r_.are.data(), r_.aim.data(), c_.amplitude_re, c_.amplitude_im, c_.upload_amplitude,
+                       c_.upload_amplitude_runs, true))
+            return DEVICE_ERROR;
+        for (uint32_t k = 0; k < c_.zero_amplitude_runs; k++) {
+            const PlaneRun run{c_.zero_amplitude[2 * k], c_.zero_amplitude[2 * k + 1]};
+            if (run.count == 0) continue;
+            const size_t pitch = c_.stride * sizeof(double), at = (size_t)run.first * c_.stride;
+            for (double *plane : {r_.are.data(), r_.aim.data()})
+                if (hip_failed(hipMemset2DAsync(plane + at, pitch, 0, c_.points * sizeof(double), run.count, s_), "zero"))
+                    return DEVICE_ERROR;
+        }
+        return OK;
+    }
+
+    int prepare() {
+        program_ = r_.programs.get(c_, r_.config.sort, s_);
+        if (!program_) return DEVICE_ERROR;
+        // a program indexing past this call's tables would read or write outside the arena
+        if (!program_fits(program_->need, c_.current_planes, c_.amplitude_planes, c_.momentum_planes / 4, c_.scale_count,
+                          c_.factor_count)) {
+            fprintf(stderr, "pacman recurrence runner: the program indexes past the call's planes or tables\n");
+            return DEVICE_ERROR;
+        }
+        arena_.cre = r_.cre.data();
+        arena_.cim = r_.cim.data();
+        arena_.are = r_.are.data();
+        arena_.aim = r_.aim.data();
+        arena_.fcre = r_.fcre.data();
+        arena_.fcim = r_.fcim.data();
+        arena_.mom = r_.forms.data();
+        arena_.sre = scale_re();
+        arena_.sim = scale_im();
+        arena_.fre = factor_re();
+        arena_.fim = factor_im();
+        arena_.bind = program_->bindings.data();
+        arena_.ld = c_.stride;
+        arena_.points = c_.points;
+        return OK;
+    }
+
+    // in-place layout changes of plane runs: f64 <-> df64 pairs, or f64 <-> the f32 planes
+    bool to_df64(double *p, PlaneRun run, bool forward) {
+        const size_t n = (size_t)run.count * c_.points;
+        if (!n) return true;
+        double *at = p + (size_t)run.first * c_.stride;
+        if (forward) to_df64_planes_kernel<<<grid_stride_blocks(n), 256, 0, s_>>>(at, run.count, c_.stride, c_.points);
+        else to_f64_planes_kernel<<<grid_stride_blocks(n), 256, 0, s_>>>(at, run.count, c_.stride, c_.points);
+        return !hip_failed(hipGetLastError(), "conversion");
+    }
+    bool runs_to_df64(double *re, double *im, const uint32_t *runs, uint32_t n, bool forward) {
+        for (uint32_t k = 0; k < n; k++) {
+            const PlaneRun run{runs[2 * k], runs[2 * k + 1]};
+            if (!to_df64(re, run, forward) || !to_df64(im, run, forward)) return false;
+        }
+        return true;
+    }
+    bool runs_to_f32(const uint32_t *runs, uint32_t n, bool forward) {
+        for (uint32_t k = 0; k < n; k++) {
+            const size_t planes = runs[2 * k + 1], at = (size_t)runs[2 * k] * c_.stride, count = planes * c_.points;
+            if (!count) continue;
+            const std::pair<double *, float *> parts[2] = {{r_.cre.data() + at, r_.fcre.data() + at},
+                                                           {r_.cim.data() + at, r_.fcim.data() + at}};
+            for (const auto &part : parts) {
+                if (forward)
+                    f64_to_f32_planes_kernel<<<grid_stride_blocks(count), 256, 0, s_>>>(part.first, part.second, planes,
+                                                                                      c_.stride, c_.points);
+                else
+                    f32_to_f64_planes_kernel<<<grid_stride_blocks(count), 256, 0, s_>>>(part.second, part.first, planes,
+                                                                                      c_.stride, c_.points);
+            }
+            if (hip_failed(hipGetLastError(), "conversion")) return false;
+        }
+        return true;
+    }
+
+    template <class K> bool launch_program() {
+        const Instr *ops = program_->ops.data() + program_->sources;
+        const uint32_t count = program_->count - program_->sources;
+        const unsigned block = r_.config.block, grid = (c_.points + block - 1) / block;
+        program_kernel<K><<<grid, block, 0, s_>>>(ops, count, arena_, c_.points);
+        return !hip_failed(hipGetLastError(), "launch");
+    }
+    template <class R> bool launch_sources() {
+        if (!program_->sources) return true;
+        sources_kernel<R><<<(c_.points + 255) / 256, 256, 0, s_>>>(program_->ops.data(), program_->sources, arena_, c_.points);
+        return !hip_failed(hipGetLastError(), "sources");
+    }
+
+    int compute() {
+        const Config &cfg = r_.config;
+        // the sources in f64, written in the kernel's current layout
+        const bool sources = cfg.kernel == PACMAN_RECURRENCE_KERNEL_F64 ? launch_sources<double>()
+                             : cfg.f32_currents()                      ? launch_sources<float>()
+                                                                        : launch_sources<D>();
+        if (!sources) return DEVICE_ERROR;
+        // the uploaded values into the kernel's layout
+        if (cfg.f32_currents() && !runs_to_f32(c_.upload_current, c_.upload_current_runs, true)) return DEVICE_ERROR;
+        if (cfg.df64_layout()) {
+            const size_t constants = constant_count();
+            bool ok = (cfg.f32_currents() ||
+                       runs_to_df64(r_.cre.data(), r_.cim.data(), c_.upload_current, c_.upload_current_runs, true)) &&
+                      runs_to_df64(r_.are.data(), r_.aim.data(), c_.upload_amplitude, c_.upload_amplitude_runs, true) &&
+                      (!cfg.df64_forms() || to_df64(r_.forms.data(), {0, c_.momentum_planes}, true));
+            if (ok && constants) {
+                to_df64_kernel<<<grid_stride_blocks(constants), 256, 0, s_>>>(r_.constants.data(), constants);
+                ok = !hip_failed(hipGetLastError(), "conversion");
+            }
+            if (!ok) return DEVICE_ERROR;
+        }
+        bool launched;
+        switch (cfg.kernel) {
+        case PACMAN_RECURRENCE_KERNEL_F64: launched = launch_program<KF64>(); break;
+        case PACMAN_RECURRENCE_KERNEL_MIXED: launched = launch_program<KMixed>(); break;
+        case PACMAN_RECURRENCE_KERNEL_MIXED64: launched = launch_program<KMixed64>(); break;
+        default: launched = launch_program<KDF64>(); break;
+        }
+        return launched ? OK : DEVICE_ERROR;
+    }
+
+    int deliver() {
+        const Config &cfg = r_.config;
+        // kept amplitudes stay on the device, untouched
+        const uint32_t amplitude_runs = keep_ ? 0 : c_.download_amplitude_runs;
+        const uint32_t *amp = c_.download_amplitude, *cur = c_.download_current;
+        const uint32_t current_runs = c_.download_current_runs;
+        // back to f64 for the copies, then the copies, then (df64 layouts) the arena as it was
+        if (cfg.df64_layout()) {
+            const bool ok = runs_to_df64(r_.are.data(), r_.aim.data(), amp, amplitude_runs, false) &&
+                            (cfg.f32_currents() ? runs_to_f32(cur, current_runs, false)
+                                                : runs_to_df64(r_.cre.data(), r_.cim.data(), cur, current_runs, false));
+            if (!ok) return DEVICE_ERROR;
+        }
+        if (!copy_runs(r_.are.data(), r_.aim.data(), c_.amplitude_re, c_.amplitude_im, amp, amplitude_runs, false) ||
+            !copy_runs(r_.cre.data(), r_.cim.data(), c_.current_re, c_.current_im, cur, current_runs, false))
+            return DEVICE_ERROR;
+        if (cfg.df64_layout()) {
+            const bool ok = runs_to_df64(r_.are.data(), r_.aim.data(), amp, amplitude_runs, true) &&
+                            (cfg.f32_currents() || runs_to_df64(r_.cre.data(), r_.cim.data(), cur, current_runs, true));
+            if (!ok) return DEVICE_ERROR;
+        }
+        return hip_failed(hipStreamSynchronize(s_), "kernel") ? DEVICE_ERROR : OK;
+    }
 };
 
-struct State {
-    Buf<double> cre, cim, are, aim, mom, small, external, coefficients, rweights, routput, rrandom, rscale, rtotal;
-    Buf<float> fcre, fcim;
-    Buf<uint32_t> form_terms, slots, rplanes, roffsets, rhelicity, rchosen;
-    bool kept = false, kept_df64 = false;
-    size_t kept_stride = 0, kept_points = 0, kept_planes = 0;
-    std::unordered_map<uint64_t, Program> programs;
-    Program scratch;
-    Options options;
-    bool runs_started = false;
-    uint64_t calls = 0, points = 0, copies = 0, bytes = 0, reductions = 0;
-    double seconds = 0;
-    bool stats = false;
-};
-State st;
+// The reductions of a kept run: |a|^2 sums over the kept amplitude planes, per point
+// (runner/pacman_recurrence.h:53-118).
+class Reduction {
+  public:
+    explicit Reduction(Runner &r) : r_(r), s_(runner_stream()) {}
 
-double now() {
-    struct timespec t;
-    clock_gettime(CLOCK_MONOTONIC, &t);
-    return t.tv_sec + t.tv_nsec * 1e-9;
-}
-
-void print_stats() {
-    if (!st.stats || st.calls == 0) return;
-    fprintf(stderr,
-            "pacman recurrence runner: %llu calls, %llu points, %llu reductions, %llu copies (%.1f MB), %.3f s (%.2f us/point)\n",
-            (unsigned long long)st.calls, (unsigned long long)st.points, (unsigned long long)st.reductions,
-            (unsigned long long)st.copies, st.bytes / 1e6, st.seconds, st.points ? st.seconds * 1e6 / st.points : 0.0);
-}
-
-void initialize() {
-    static bool done = false;
-    if (!done) {
-        done = true;
-        atexit(print_stats);
-    }
-}
-
-unsigned stride_grid(size_t count) { return (unsigned)std::min<size_t>(4096, std::max<size_t>(1, (count + 255) / 256)); }
-
-// the first `points` values of `n` planes of `stride` doubles between host and device
-bool copy_planes(void *dst, const void *src, size_t n, size_t stride, size_t points, hipMemcpyKind kind) {
-    size_t pitch = stride * sizeof(double);
-    if (hip_failed(hipMemcpy2DAsync(dst, pitch, src, pitch, points * sizeof(double), n, kind, runner_stream()), "copy"))
-        return false;
-    st.copies++;
-    st.bytes += n * points * sizeof(double);
-    return true;
-}
-
-bool copy_runs(double *dre, double *dim, double *hre, double *him, const uint32_t *runs, uint32_t count, uint32_t planes,
-               size_t stride, size_t points, bool to_device) {
-    for (uint32_t r = 0; r < count; r++) {
-        uint32_t first = runs[2 * r], n = runs[2 * r + 1];
-        if (first > planes || n > planes - first) return false;
-        if (n == 0) continue;
-        size_t off = (size_t)first * stride;
-        bool ok = to_device ? copy_planes(dre + off, hre + off, n, stride, points, hipMemcpyHostToDevice) &&
-                                  copy_planes(dim + off, him + off, n, stride, points, hipMemcpyHostToDevice)
-                            : copy_planes(hre + off, dre + off, n, stride, points, hipMemcpyDeviceToHost) &&
-                                  copy_planes(him + off, dim + off, n, stride, points, hipMemcpyDeviceToHost);
-        if (!ok) return false;
-    }
-    return true;
-}
-
-bool convert_planes(bool to_df, double *p, size_t planes, size_t stride, size_t points) {
-    size_t count = planes * points;
-    if (count == 0) return true;
-    if (to_df) to_df64_planes_kernel<<<stride_grid(count), 256, 0, runner_stream()>>>(p, planes, stride, points);
-    else to_f64_planes_kernel<<<stride_grid(count), 256, 0, runner_stream()>>>(p, planes, stride, points);
-    return !hip_failed(hipGetLastError(), "conversion");
-}
-
-bool convert_runs(bool to_df, double *re, double *im, const uint32_t *runs, uint32_t count, size_t stride, size_t points) {
-    for (uint32_t r = 0; r < count; r++) {
-        size_t off = (size_t)runs[2 * r] * stride, n = runs[2 * r + 1];
-        if (!convert_planes(to_df, re + off, n, stride, points) || !convert_planes(to_df, im + off, n, stride, points))
-            return false;
-    }
-    return true;
-}
-
-bool convert_current_runs(bool to_f32, const uint32_t *runs, uint32_t count, size_t stride, size_t points) {
-    for (uint32_t r = 0; r < count; r++) {
-        size_t off = (size_t)runs[2 * r] * stride, n = runs[2 * r + 1], total = n * points;
-        if (total == 0) continue;
-        for (int part = 0; part < 2; part++) {
-            double *d = (part ? st.cim.p : st.cre.p) + off;
-            float *f = (part ? st.fcim.p : st.fcre.p) + off;
-            if (to_f32) f64_to_f32_planes_kernel<<<stride_grid(total), 256, 0, runner_stream()>>>(d, f, n, stride, points);
-            else f32_to_f64_planes_kernel<<<stride_grid(total), 256, 0, runner_stream()>>>(f, d, n, stride, points);
+    int plain(const pacman_recurrence_reduce *q) {
+        if (!q || q->abi_version != PACMAN_RECURRENCE_ABI || q->struct_size != sizeof(*q) ||
+            (q->count && (!q->planes || !q->weights)) || !q->output)
+            return INVALID;
+        if (!matches_kept(q->points) || !planes_kept(q->planes, q->count)) return INVALID;
+        const CallTimer timer(r_.stats.enabled);
+        const size_t bytes = (size_t)q->points * sizeof(double);
+        if (!s_ || !upload_tables(q->planes, q->weights, q->count, nullptr, 0) || !r_.routput.fit(q->points, "reduction") ||
+            !upload(r_.routput.data(), q->output, bytes, "reduction"))
+            return DEVICE_ERROR;
+        if (q->points) {
+            const unsigned grid = (q->points + 255) / 256;
+            if (r_.kept.df64)
+                reduce_kernel<true><<<grid, 256, 0, s_>>>(r_.are.data(), r_.aim.data(), r_.kept.stride, r_.rplanes.data(),
+                                                          r_.rweights.data(), q->count, r_.routput.data(), q->points);
+            else
+                reduce_kernel<false><<<grid, 256, 0, s_>>>(r_.are.data(), r_.aim.data(), r_.kept.stride, r_.rplanes.data(),
+                                                           r_.rweights.data(), q->count, r_.routput.data(), q->points);
+            if (hip_failed(hipGetLastError(), "reduction") || !download(q->output, r_.routput.data(), bytes, "reduction"))
+                return DEVICE_ERROR;
         }
-        if (hip_failed(hipGetLastError(), "conversion")) return false;
+        return finish(2, 2 * bytes, timer);
     }
-    return true;
-}
 
-const Program *prepare(const pacman_recurrence_call *call) {
-    if (call->words_key) {
-        auto it = st.programs.find(call->words_key);
-        if (it != st.programs.end()) return &it->second;
-    }
-    HostProgram hp = build_program(call->words, call->word_count, call->width, st.options.sort);
-    if (hp.error) {
-        fprintf(stderr, "pacman recurrence runner: %s\n", hp.error);
-        return nullptr;
-    }
-    Program p;
-    p.count = (uint32_t)hp.ops.size();
-    p.sources = hp.sources;
-    memcpy(p.need, hp.need, sizeof p.need);
-    // a synchronous copy: the host instructions go out of scope on return
-    if (hip_failed(hipMalloc((void **)&p.ops, std::max<size_t>(1, hp.ops.size()) * sizeof(Instr)), "program") ||
-        (!hp.ops.empty() &&
-         hip_failed(hipMemcpy(p.ops, hp.ops.data(), hp.ops.size() * sizeof(Instr), hipMemcpyHostToDevice), "program")))
-        return nullptr;
-    if (call->words_key) return &(st.programs[call->words_key] = p);
-    if (st.scratch.ops) {
-        (void)hipStreamSynchronize(runner_stream());
-        (void)hipFree(st.scratch.ops);
-    }
-    st.scratch = p;
-    return &st.scratch;
-}
-
-bool launch_program(const Program &p, const Arena &a, uint32_t points) {
-    const unsigned block = (unsigned)st.options.block, grid = (points + block - 1) / block;
-    const Instr *ops = p.ops + p.sources;
-    const uint32_t count = p.count - p.sources;
-    hipStream_t s = runner_stream();
-    switch (st.options.kernel) {
-    case KERNEL_F64: program_kernel<KF64><<<grid, block, 0, s>>>(ops, count, a, points); break;
-    case KERNEL_MIXED: program_kernel<KMixed><<<grid, block, 0, s>>>(ops, count, a, points); break;
-    case KERNEL_MIXED64: program_kernel<KMixed64><<<grid, block, 0, s>>>(ops, count, a, points); break;
-    default: program_kernel<KDF64><<<grid, block, 0, s>>>(ops, count, a, points); break;
-    }
-    return !hip_failed(hipGetLastError(), "launch");
-}
-
-bool launch_sources(const Program &p, const Arena &a, uint32_t points) {
-    if (p.sources == 0) return true;
-    const unsigned block = 256, grid = (points + block - 1) / block;
-    hipStream_t s = runner_stream();
-    switch (st.options.kernel) {
-    case KERNEL_F64: sources_kernel<double><<<grid, block, 0, s>>>(p.ops, p.sources, a, points); break;
-    case KERNEL_MIXED:
-    case KERNEL_MIXED64: sources_kernel<float><<<grid, block, 0, s>>>(p.ops, p.sources, a, points); break;
-    default: sources_kernel<D><<<grid, block, 0, s>>>(p.ops, p.sources, a, points); break;
-    }
-    return !hip_failed(hipGetLastError(), "sources");
-}
-
-int run_locked(const pacman_recurrence_call *in) {
-    st.runs_started = true;
-    initialize();
-    const size_t v1_size = offsetof(pacman_recurrence_call, zero_amplitude);
-    const size_t v2_size = offsetof(pacman_recurrence_call, external_momenta);
-    if (!in || in->abi_version != PACMAN_RECURRENCE_ABI ||
-        (in->struct_size != sizeof(pacman_recurrence_call) && in->struct_size != v1_size && in->struct_size != v2_size))
-        return PACMAN_RECURRENCE_INVALID;
-    pacman_recurrence_call full;
-    memset(&full, 0, sizeof full);
-    memcpy(&full, in, in->struct_size);
-    const pacman_recurrence_call *call = &full;
-    if (call->points == 0 || call->points > call->stride || !call->words || (call->zero_amplitude_runs && !call->zero_amplitude))
-        return PACMAN_RECURRENCE_INVALID;
-    st.kept = false;
-    const bool keep = call->flags & PACMAN_RECURRENCE_KEEP_AMPLITUDES;
-    const double start = st.stats ? now() : 0;
-    const size_t stride = call->stride, points = call->points;
-    const size_t current_doubles = (size_t)call->current_planes * stride;
-    const size_t amplitude_doubles = (size_t)call->amplitude_planes * stride;
-    hipStream_t s = runner_stream();
-    if (!s) return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (!st.cre.reserve(current_doubles, "current arena") || !st.cim.reserve(current_doubles, "current arena") ||
-        !st.are.reserve(amplitude_doubles, "amplitudes") || !st.aim.reserve(amplitude_doubles, "amplitudes"))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    const Options opt = st.options;
-    const bool mixed = opt.kernel == KERNEL_MIXED || opt.kernel == KERNEL_MIXED64;
-    if (mixed && (!st.fcre.reserve(current_doubles, "current arena (f32)") ||
-                  !st.fcim.reserve(current_doubles, "current arena (f32)")))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    const size_t momentum_doubles = (size_t)call->momentum_planes * stride;
-    const size_t small = 2 * ((size_t)call->scale_count + call->factor_count);
-    if (!st.mom.reserve(std::max<size_t>(1, momentum_doubles), "momenta") ||
-        !st.small.reserve(std::max<size_t>(1, small), "factors"))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    double *dsre = st.small.p, *dsim = dsre + call->scale_count;
-    double *dfre = dsim + call->scale_count, *dfim = dfre + call->factor_count;
-    const bool forms_on_device = call->flags & PACMAN_RECURRENCE_FORMS_ON_DEVICE;
-    if (forms_on_device) {
-        const uint32_t form_count = call->momentum_planes / 4;
-        const size_t external_doubles = (size_t)call->external_count * 4 * points;
-        if (!call->external_momenta || !call->form_terms || (call->term_count && (!call->term_slots || !call->term_coefficients)))
-            return PACMAN_RECURRENCE_INVALID;
-        for (uint32_t f = 0; f < form_count; f++)
-            if (call->form_terms[2 * f] > call->term_count || call->form_terms[2 * f + 1] > call->term_count - call->form_terms[2 * f])
-                return PACMAN_RECURRENCE_INVALID;
-        for (uint32_t t = 0; t < call->term_count; t++)
-            if (call->term_slots[t] >= call->external_count) return PACMAN_RECURRENCE_INVALID;
-        if (!st.external.reserve(std::max<size_t>(1, external_doubles), "external momenta") ||
-            !st.form_terms.reserve(std::max<size_t>(1, 2 * (size_t)form_count), "form terms") ||
-            !st.coefficients.reserve(std::max<size_t>(1, call->term_count), "form terms") ||
-            !st.slots.reserve(std::max<size_t>(1, call->term_count), "form terms"))
-            return PACMAN_RECURRENCE_DEVICE_ERROR;
-        if ((external_doubles && hip_failed(hipMemcpyAsync(st.external.p, call->external_momenta, external_doubles * 8,
-                                                           hipMemcpyHostToDevice, s), "external momenta")) ||
-            (form_count && hip_failed(hipMemcpyAsync(st.form_terms.p, call->form_terms, 2 * (size_t)form_count * 4,
-                                                     hipMemcpyHostToDevice, s), "form terms")) ||
-            (call->term_count &&
-             (hip_failed(hipMemcpyAsync(st.slots.p, call->term_slots, call->term_count * 4, hipMemcpyHostToDevice, s), "form terms") ||
-              hip_failed(hipMemcpyAsync(st.coefficients.p, call->term_coefficients, call->term_count * 8,
-                                        hipMemcpyHostToDevice, s), "form terms"))))
-            return PACMAN_RECURRENCE_DEVICE_ERROR;
-        st.copies += 1;
-        st.bytes += external_doubles * 8;
-        if (form_count) {
-            fill_forms_kernel<<<(unsigned)((points + 255) / 256), 256, 0, s>>>(st.external.p, (uint32_t)points, st.form_terms.p,
-                                                                               form_count, st.slots.p, st.coefficients.p,
-                                                                               st.mom.p, stride);
-            if (hip_failed(hipGetLastError(), "forms")) return PACMAN_RECURRENCE_DEVICE_ERROR;
-        }
-    }
-    if ((momentum_doubles && !forms_on_device &&
-         !copy_planes(st.mom.p, call->momenta, call->momentum_planes, stride, points, hipMemcpyHostToDevice)) ||
-        (call->scale_count &&
-         (hip_failed(hipMemcpyAsync(dsre, call->scale_re, call->scale_count * 8, hipMemcpyHostToDevice, s), "scales") ||
-          hip_failed(hipMemcpyAsync(dsim, call->scale_im, call->scale_count * 8, hipMemcpyHostToDevice, s), "scales"))) ||
-        (call->factor_count &&
-         (hip_failed(hipMemcpyAsync(dfre, call->factor_re, call->factor_count * 8, hipMemcpyHostToDevice, s), "factors") ||
-          hip_failed(hipMemcpyAsync(dfim, call->factor_im, call->factor_count * 8, hipMemcpyHostToDevice, s), "factors"))))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (!copy_runs(st.cre.p, st.cim.p, call->current_re, call->current_im, call->upload_current, call->upload_current_runs,
-                   call->current_planes, stride, points, true) ||
-        !copy_runs(st.are.p, st.aim.p, call->amplitude_re, call->amplitude_im, call->upload_amplitude,
-                   call->upload_amplitude_runs, call->amplitude_planes, stride, points, true))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    for (uint32_t r = 0; r < call->zero_amplitude_runs; r++) {
-        uint32_t first = call->zero_amplitude[2 * r], n = call->zero_amplitude[2 * r + 1];
-        if (first > call->amplitude_planes || n > call->amplitude_planes - first) return PACMAN_RECURRENCE_INVALID;
-        if (n == 0) continue;
-        size_t pitch = stride * 8, width = points * 8;
-        if (hip_failed(hipMemset2DAsync(st.are.p + (size_t)first * stride, pitch, 0, width, n, s), "zero") ||
-            hip_failed(hipMemset2DAsync(st.aim.p + (size_t)first * stride, pitch, 0, width, n, s), "zero"))
-            return PACMAN_RECURRENCE_DEVICE_ERROR;
-    }
-    Arena a;
-    a.cre = st.cre.p; a.cim = st.cim.p; a.are = st.are.p; a.aim = st.aim.p;
-    a.fcre = st.fcre.p; a.fcim = st.fcim.p; a.mom = st.mom.p;
-    a.sre = dsre; a.sim = dsim; a.fre = dfre; a.fim = dfim; a.ld = stride;
-    const Program *prepared = prepare(call);
-    if (!prepared) return PACMAN_RECURRENCE_DEVICE_ERROR;
-    // a program indexing past this call's tables would read or write outside the arena
-    if (!program_fits(prepared->need, call->current_planes, call->amplitude_planes, call->momentum_planes / 4,
-                      call->scale_count, call->factor_count)) {
-        fprintf(stderr, "pacman recurrence runner: the program indexes past the call's planes or tables\n");
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    }
-    if (!launch_sources(*prepared, a, (uint32_t)points)) return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (mixed && !convert_current_runs(true, call->upload_current, call->upload_current_runs, stride, points))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (opt.df64) {
-        bool ok = (mixed || convert_runs(true, st.cre.p, st.cim.p, call->upload_current, call->upload_current_runs, stride, points)) &&
-                  convert_runs(true, st.are.p, st.aim.p, call->upload_amplitude, call->upload_amplitude_runs, stride, points) &&
-                  (!opt.df64_forms || convert_planes(true, st.mom.p, call->momentum_planes, stride, points));
-        if (ok && small) {
-            to_df64_kernel<<<stride_grid(small), 256, 0, s>>>(st.small.p, small);
-            ok = !hip_failed(hipGetLastError(), "conversion");
-        }
-        if (!ok) return PACMAN_RECURRENCE_DEVICE_ERROR;
-    }
-    if (!launch_program(*prepared, a, (uint32_t)points)) return PACMAN_RECURRENCE_DEVICE_ERROR;
-    const uint32_t amplitude_runs = keep ? 0 : call->download_amplitude_runs;
-    if (opt.df64 &&
-        (!convert_runs(false, st.are.p, st.aim.p, call->download_amplitude, amplitude_runs, stride, points) ||
-         (mixed ? !convert_current_runs(false, call->download_current, call->download_current_runs, stride, points)
-                : !convert_runs(false, st.cre.p, st.cim.p, call->download_current, call->download_current_runs, stride, points))))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (!copy_runs(st.are.p, st.aim.p, call->amplitude_re, call->amplitude_im, call->download_amplitude, amplitude_runs,
-                   call->amplitude_planes, stride, points, false) ||
-        !copy_runs(st.cre.p, st.cim.p, call->current_re, call->current_im, call->download_current,
-                   call->download_current_runs, call->current_planes, stride, points, false))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (opt.df64 &&
-        (!convert_runs(true, st.are.p, st.aim.p, call->download_amplitude, amplitude_runs, stride, points) ||
-         (!mixed && !convert_runs(true, st.cre.p, st.cim.p, call->download_current, call->download_current_runs, stride, points))))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (hip_failed(hipStreamSynchronize(s), "kernel")) return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (keep) {
-        st.kept = true;
-        st.kept_df64 = opt.df64;
-        st.kept_stride = stride;
-        st.kept_points = points;
-        st.kept_planes = call->amplitude_planes;
-    }
-    st.calls++;
-    st.points += points;
-    if (st.stats) st.seconds += now() - start;
-    return PACMAN_RECURRENCE_OK;
-}
-
-// validation shared by reduce_resolved and reduce_sample
-int check_outputs(const uint32_t *offsets, uint32_t outputs, const uint32_t *planes, const double *weights, uint32_t points) {
-    if (!st.kept || points != st.kept_points) return PACMAN_RECURRENCE_INVALID;
-    uint32_t count = offsets[outputs];
-    if (offsets[0] != 0 || (count && (!planes || !weights))) return PACMAN_RECURRENCE_INVALID;
-    for (uint32_t o = 0; o < outputs; o++)
-        if (offsets[o] > offsets[o + 1]) return PACMAN_RECURRENCE_INVALID;
-    for (uint32_t k = 0; k < count; k++)
-        if (planes[k] >= st.kept_planes) return PACMAN_RECURRENCE_INVALID;
-    return PACMAN_RECURRENCE_OK;
-}
-
-bool upload_reduction(const uint32_t *planes, const double *weights, uint32_t count, const uint32_t *offsets, uint32_t outputs) {
-    hipStream_t s = runner_stream();
-    if (!st.rplanes.reserve(std::max<size_t>(count, 1), "reduction") || !st.rweights.reserve(std::max<size_t>(count, 1), "reduction") ||
-        (offsets && !st.roffsets.reserve(outputs + 1, "reduction")))
-        return false;
-    return !((count && (hip_failed(hipMemcpyAsync(st.rplanes.p, planes, count * 4, hipMemcpyHostToDevice, s), "reduction") ||
-                        hip_failed(hipMemcpyAsync(st.rweights.p, weights, count * 8, hipMemcpyHostToDevice, s), "reduction"))) ||
-             (offsets && hip_failed(hipMemcpyAsync(st.roffsets.p, offsets, (outputs + 1) * 4, hipMemcpyHostToDevice, s), "reduction")));
-}
-
-int reduce_resolved_locked(const pacman_recurrence_reduce_resolved *r) {
-    initialize();
-    if (!r || r->abi_version != PACMAN_RECURRENCE_ABI || r->struct_size != sizeof(pacman_recurrence_reduce_resolved) ||
-        !r->offsets || !r->output || r->outputs == 0 || r->outputs > 65535)
-        return PACMAN_RECURRENCE_INVALID;
-    int c = check_outputs(r->offsets, r->outputs, r->planes, r->weights, r->points);
-    if (c) return c;
-    if (r->points == 0) return PACMAN_RECURRENCE_OK;
-    const size_t span = (size_t)(r->outputs - 1) * r->output_stride + (size_t)(r->points - 1) * r->point_stride + 1;
-    const bool zero = r->flags & PACMAN_RECURRENCE_OUTPUT_ZERO;
-    const double start = st.stats ? now() : 0;
-    hipStream_t s = runner_stream();
-    const uint32_t count = r->offsets[r->outputs];
-    // OUTPUT_ZERO clears the whole span on the device (the values between the outputs come back
-    // as zeros too); otherwise the span goes up and comes back
-    if (!upload_reduction(r->planes, r->weights, count, r->offsets, r->outputs) || !st.routput.reserve(span, "reduction") ||
-        (zero ? hip_failed(hipMemsetAsync(st.routput.p, 0, span * 8, s), "reduction")
-              : hip_failed(hipMemcpyAsync(st.routput.p, r->output, span * 8, hipMemcpyHostToDevice, s), "reduction")))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    const unsigned grid = (r->points + 255) / 256;
-    if (st.kept_df64)
-        reduce_resolved_kernel<true><<<grid, 256, 0, s>>>(st.are.p, st.aim.p, st.kept_stride, st.rplanes.p, st.rweights.p,
-                                                          st.roffsets.p, r->outputs, st.routput.p, r->points,
-                                                          r->output_stride, r->point_stride);
-    else
-        reduce_resolved_kernel<false><<<grid, 256, 0, s>>>(st.are.p, st.aim.p, st.kept_stride, st.rplanes.p, st.rweights.p,
-                                                           st.roffsets.p, r->outputs, st.routput.p, r->points,
-                                                           r->output_stride, r->point_stride);
-    if (hip_failed(hipGetLastError(), "reduction") ||
-        hip_failed(hipMemcpyAsync(r->output, st.routput.p, span * 8, hipMemcpyDeviceToHost, s), "reduction") ||
-        hip_failed(hipStreamSynchronize(s), "reduction"))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    st.copies += zero ? 1 : 2;
-    st.bytes += (zero ? 1 : 2) * span * 8;
-    st.reductions++;
-    if (st.stats) st.seconds += now() - start;
-    return PACMAN_RECURRENCE_OK;
-}
-
-int reduce_sample_locked(const pacman_recurrence_reduce_sample *r) {
-    initialize();
-    if (!r || r->abi_version != PACMAN_RECURRENCE_ABI || r->struct_size != sizeof(pacman_recurrence_reduce_sample) ||
-        !r->offsets || !r->helicity || !r->random || !r->total || !r->chosen || r->outputs == 0)
-        return PACMAN_RECURRENCE_INVALID;
-    int c = check_outputs(r->offsets, r->outputs, r->planes, r->weights, r->points);
-    if (c) return c;
-    if (r->points == 0) return PACMAN_RECURRENCE_OK;
-    const double start = st.stats ? now() : 0;
-    hipStream_t s = runner_stream();
-    const size_t n = r->points;
-    const uint32_t count = r->offsets[r->outputs];
-    if (!upload_reduction(r->planes, r->weights, count, r->offsets, r->outputs) ||
-        !st.rhelicity.reserve(r->outputs, "sampling") || !st.rrandom.reserve(n, "sampling") ||
-        !st.rscale.reserve(n, "sampling") || !st.rtotal.reserve(n, "sampling") || !st.rchosen.reserve(n, "sampling"))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (hip_failed(hipMemcpyAsync(st.rhelicity.p, r->helicity, r->outputs * 4, hipMemcpyHostToDevice, s), "sampling") ||
-        hip_failed(hipMemcpyAsync(st.rrandom.p, r->random, n * 8, hipMemcpyHostToDevice, s), "sampling") ||
-        (r->scale && hip_failed(hipMemcpyAsync(st.rscale.p, r->scale, n * 8, hipMemcpyHostToDevice, s), "sampling")))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    const bool factor = r->numerator != r->denominator;
-    const unsigned grid = (unsigned)((n + 255) / 256);
-    const double *scale = r->scale ? st.rscale.p : nullptr;
-    if (st.kept_df64)
-        reduce_sample_kernel<true><<<grid, 256, 0, s>>>(st.are.p, st.aim.p, st.kept_stride, st.rplanes.p, st.rweights.p,
-                                                        st.roffsets.p, st.rhelicity.p, r->outputs, r->points, st.rrandom.p,
-                                                        scale, factor, r->numerator, r->denominator, st.rtotal.p, st.rchosen.p);
-    else
-        reduce_sample_kernel<false><<<grid, 256, 0, s>>>(st.are.p, st.aim.p, st.kept_stride, st.rplanes.p, st.rweights.p,
-                                                         st.roffsets.p, st.rhelicity.p, r->outputs, r->points, st.rrandom.p,
-                                                         scale, factor, r->numerator, r->denominator, st.rtotal.p, st.rchosen.p);
-    if (hip_failed(hipGetLastError(), "sampling") ||
-        hip_failed(hipMemcpyAsync(r->total, st.rtotal.p, n * 8, hipMemcpyDeviceToHost, s), "sampling") ||
-        hip_failed(hipMemcpyAsync(r->chosen, st.rchosen.p, n * 4, hipMemcpyDeviceToHost, s), "sampling") ||
-        hip_failed(hipStreamSynchronize(s), "sampling"))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    st.copies += r->scale ? 4 : 3;
-    st.bytes += n * (8 * (r->scale ? 3 : 2) + 4);
-    st.reductions++;
-    if (st.stats) st.seconds += now() - start;
-    return PACMAN_RECURRENCE_OK;
-}
-
-int reduce_locked(const pacman_recurrence_reduce *r) {
-    initialize();
-    if (!r || r->abi_version != PACMAN_RECURRENCE_ABI || r->struct_size != sizeof(pacman_recurrence_reduce) ||
-        (r->count && (!r->planes || !r->weights)) || !r->output)
-        return PACMAN_RECURRENCE_INVALID;
-    if (!st.kept || r->points != st.kept_points) return PACMAN_RECURRENCE_INVALID;
-    for (uint32_t k = 0; k < r->count; k++)
-        if (r->planes[k] >= st.kept_planes) return PACMAN_RECURRENCE_INVALID;
-    const double start = st.stats ? now() : 0;
-    hipStream_t s = runner_stream();
-    const size_t bytes = (size_t)r->points * 8;
-    if (!upload_reduction(r->planes, r->weights, r->count, nullptr, 0) ||
-        !st.routput.reserve(std::max<uint32_t>(r->points, 1), "reduction") ||
-        (bytes && hip_failed(hipMemcpyAsync(st.routput.p, r->output, bytes, hipMemcpyHostToDevice, s), "reduction")))
-        return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (r->points) {
-        const unsigned grid = (r->points + 255) / 256;
-        if (st.kept_df64)
-            reduce_kernel<true><<<grid, 256, 0, s>>>(st.are.p, st.aim.p, st.kept_stride, st.rplanes.p, st.rweights.p,
-                                                     r->count, st.routput.p, r->points);
+    int resolved(const pacman_recurrence_reduce_resolved *q) {
+        if (!q || q->abi_version != PACMAN_RECURRENCE_ABI || q->struct_size != sizeof(*q) || !q->offsets || !q->output ||
+            q->outputs == 0 || q->outputs > 65535)
+            return INVALID;
+        const int valid = check_outputs(q->offsets, q->outputs, q->planes, q->weights, q->points);
+        if (valid != OK || q->points == 0) return valid;
+        const CallTimer timer(r_.stats.enabled);
+        // the output span: every (output, point) value and whatever lies between them
+        const size_t span = (size_t)(q->outputs - 1) * q->output_stride + (size_t)(q->points - 1) * q->point_stride + 1;
+        const bool zero = q->flags & PACMAN_RECURRENCE_OUTPUT_ZERO;
+        // OUTPUT_ZERO clears the span on the device (the values between the outputs come back as
+        // zeros too); otherwise the caller's span goes up first
+        if (!s_ || !upload_tables(q->planes, q->weights, q->offsets[q->outputs], q->offsets, q->outputs) ||
+            !r_.routput.fit(span, "reduction") ||
+            (zero ? hip_failed(hipMemsetAsync(r_.routput.data(), 0, span * sizeof(double), s_), "reduction")
+                  : !upload(r_.routput.data(), q->output, span * sizeof(double), "reduction")))
+            return DEVICE_ERROR;
+        const unsigned grid = (q->points + 255) / 256;
+        if (r_.kept.df64)
+            reduce_resolved_kernel<true><<<grid, 256, 0, s_>>>(r_.are.data(), r_.aim.data(), r_.kept.stride, r_.rplanes.data(),
+                                                               r_.rweights.data(), r_.roffsets.data(), q->outputs,
+                                                               r_.routput.data(), q->points, q->output_stride, q->point_stride);
         else
-            reduce_kernel<false><<<grid, 256, 0, s>>>(st.are.p, st.aim.p, st.kept_stride, st.rplanes.p, st.rweights.p,
-                                                      r->count, st.routput.p, r->points);
+            reduce_resolved_kernel<false><<<grid, 256, 0, s_>>>(r_.are.data(), r_.aim.data(), r_.kept.stride, r_.rplanes.data(),
+                                                                r_.rweights.data(), r_.roffsets.data(), q->outputs,
+                                                                r_.routput.data(), q->points, q->output_stride, q->point_stride);
         if (hip_failed(hipGetLastError(), "reduction") ||
-            hip_failed(hipMemcpyAsync(r->output, st.routput.p, bytes, hipMemcpyDeviceToHost, s), "reduction"))
-            return PACMAN_RECURRENCE_DEVICE_ERROR;
+            !download(q->output, r_.routput.data(), span * sizeof(double), "reduction"))
+            return DEVICE_ERROR;
+        return finish(zero ? 1 : 2, (zero ? 1 : 2) * span * sizeof(double), timer);
     }
-    if (hip_failed(hipStreamSynchronize(s), "reduction")) return PACMAN_RECURRENCE_DEVICE_ERROR;
-    st.copies += 2;
-    st.bytes += 2 * bytes;
-    st.reductions++;
-    if (st.stats) st.seconds += now() - start;
-    return PACMAN_RECURRENCE_OK;
+
+    int sample(const pacman_recurrence_reduce_sample *q) {
+        if (!q || q->abi_version != PACMAN_RECURRENCE_ABI || q->struct_size != sizeof(*q) || !q->offsets || !q->helicity ||
+            !q->random || !q->total || !q->chosen || q->outputs == 0)
+            return INVALID;
+        const int valid = check_outputs(q->offsets, q->outputs, q->planes, q->weights, q->points);
+        if (valid != OK || q->points == 0) return valid;
+        const CallTimer timer(r_.stats.enabled);
+        const size_t n = q->points;
+        if (!s_ || !upload_tables(q->planes, q->weights, q->offsets[q->outputs], q->offsets, q->outputs) ||
+            !r_.rhelicity.fit(q->outputs, "sampling") || !r_.rrandom.fit(n, "sampling") || !r_.rscale.fit(n, "sampling") ||
+            !r_.rtotal.fit(n, "sampling") || !r_.rchosen.fit(n, "sampling") ||
+            !upload(r_.rhelicity.data(), q->helicity, q->outputs * sizeof(uint32_t), "sampling") ||
+            !upload(r_.rrandom.data(), q->random, n * sizeof(double), "sampling") ||
+            (q->scale && !upload(r_.rscale.data(), q->scale, n * sizeof(double), "sampling")))
+            return DEVICE_ERROR;
+        const bool factor = q->numerator != q->denominator;
+        const double *scale = q->scale ? r_.rscale.data() : nullptr;
+        const unsigned grid = (unsigned)((n + 255) / 256);
+        if (r_.kept.df64)
+            reduce_sample_kernel<true><<<grid, 256, 0, s_>>>(r_.are.data(), r_.aim.data(), r_.kept.stride, r_.rplanes.data(),
+                                                             r_.rweights.data(), r_.roffsets.data(), r_.rhelicity.data(),
+                                                             q->outputs, q->points, r_.rrandom.data(), scale, factor,
+                                                             q->numerator, q->denominator, r_.rtotal.data(), r_.rchosen.data());
+        else
+            reduce_sample_kernel<false><<<grid, 256, 0, s_>>>(r_.are.data(), r_.aim.data(), r_.kept.stride, r_.rplanes.data(),
+                                                              r_.rweights.data(), r_.roffsets.data(), r_.rhelicity.data(),
+                                                              q->outputs, q->points, r_.rrandom.data(), scale, factor,
+                                                              q->numerator, q->denominator, r_.rtotal.data(), r_.rchosen.data());
+        if (hip_failed(hipGetLastError(), "sampling") || !download(q->total, r_.rtotal.data(), n * sizeof(double), "sampling") ||
+            !download(q->chosen, r_.rchosen.data(), n * sizeof(uint32_t), "sampling"))
+            return DEVICE_ERROR;
+        return finish(q->scale ? 4 : 3, n * (sizeof(double) * (q->scale ? 3 : 2) + sizeof(uint32_t)), timer);
+    }
+
+  private:
+    Runner &r_;
+    hipStream_t s_;
+
+    bool matches_kept(uint32_t points) const { return r_.kept.valid && points == r_.kept.points; }
+    bool planes_kept(const uint32_t *planes, uint32_t count) const {
+        for (uint32_t k = 0; k < count; k++)
+            if (planes[k] >= r_.kept.planes) return false;
+        return true;
+    }
+    // the offsets table of reduce_resolved / reduce_sample: from 0, non-decreasing, over kept planes
+    int check_outputs(const uint32_t *offsets, uint32_t outputs, const uint32_t *planes, const double *weights,
+                      uint32_t points) const {
+        if (!matches_kept(points)) return INVALID;
+        const uint32_t count = offsets[outputs];
+        if (offsets[0] != 0 || (count && (!planes || !weights))) return INVALID;
+        for (uint32_t o = 0; o < outputs; o++)
+            if (offsets[o] > offsets[o + 1]) return INVALID;
+        return planes_kept(planes, count) ? OK : INVALID;
+    }
+    bool upload(void *dst, const void *src, size_t bytes, const char *what) {
+        return !bytes || !hip_failed(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s_), what);
+    }
+    bool download(void *dst, const void *src, size_t bytes, const char *what) {
+        return !bytes || !hip_failed(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s_), what);
+    }
+    bool upload_tables(const uint32_t *planes, const double *weights, uint32_t count, const uint32_t *offsets,
+                       uint32_t outputs) {
+        return r_.rplanes.fit(count, "reduction") && r_.rweights.fit(count, "reduction") &&
+               (!offsets || r_.roffsets.fit((size_t)outputs + 1, "reduction")) &&
+               upload(r_.rplanes.data(), planes, count * sizeof(uint32_t), "reduction") &&
+               upload(r_.rweights.data(), weights, count * sizeof(double), "reduction") &&
+               (!offsets || upload(r_.roffsets.data(), offsets, ((size_t)outputs + 1) * sizeof(uint32_t), "reduction"));
+    }
+    int finish(uint64_t copies, uint64_t bytes, const CallTimer &timer) {
+        if (hip_failed(hipStreamSynchronize(s_), "reduction")) return DEVICE_ERROR;
+        r_.stats.copied(copies, bytes);
+        r_.stats.reductions++;
+        r_.stats.seconds += timer.seconds();
+        return OK;
+    }
+};
+
+// One lock serialises the entry points. A run that keeps its amplitudes returns with the lock
+// still held by its thread; that thread's next reduction (or failed call) lets it go. The owner
+// is a thread id, so a thread that already holds the lock passes straight through.
+class RunnerLock {
+  public:
+    void enter() {
+        if (owner_.load(std::memory_order_acquire) == std::this_thread::get_id()) return;
+        mutex_.lock();
+        owner_.store(std::this_thread::get_id(), std::memory_order_release);
+    }
+    void leave() {
+        if (owner_.load(std::memory_order_acquire) != std::this_thread::get_id()) return;
+        owner_.store(std::thread::id(), std::memory_order_release);
+        mutex_.unlock();
+    }
+
+  private:
+    std::mutex mutex_;
+    std::atomic<std::thread::id> owner_{};
+};
+
+RunnerLock &runner_lock() {
+    static RunnerLock *l = new RunnerLock;  // never destroyed, like the runner
+    return *l;
 }
 
-// One lock serialises the entry points; a KEEP run keeps it until the same thread's reduction
-// (runner/recurrence_runner.cu:1725-1748).
-std::mutex runner_lock;
-thread_local bool holding_runner_lock = false;
-void acquire_runner_lock() {
-    if (!holding_runner_lock) {
-        runner_lock.lock();
-        holding_runner_lock = true;
-    }
-}
-void release_runner_lock() {
-    if (holding_runner_lock) {
-        holding_runner_lock = false;
-        runner_lock.unlock();
-    }
+// the call as the current layout: an older, shorter struct reads as zeros past its end
+int normalise_call(const pacman_recurrence_call *in, pacman_recurrence_call *out) {
+    static constexpr uint32_t SIZES[] = {sizeof(pacman_recurrence_call), offsetof(pacman_recurrence_call, external_momenta),
+                                         offsetof(pacman_recurrence_call, zero_amplitude)};
+    if (!in || in->abi_version != PACMAN_RECURRENCE_ABI ||
+        std::find(std::begin(SIZES), std::end(SIZES), in->struct_size) == std::end(SIZES))
+        return INVALID;
+    memset(out, 0, sizeof *out);
+    memcpy(out, in, in->struct_size);
+    return OK;
 }
 
 }  // namespace
 
 extern "C" int pacman_recurrence_v1_run(const pacman_recurrence_call *in) {
-    acquire_runner_lock();
-    int status = run_locked(in);
-    if (!(status == PACMAN_RECURRENCE_OK && st.kept)) release_runner_lock();
+    RunnerLock &lock = runner_lock();
+    lock.enter();
+    Runner &r = runner();
+    r.started = true;  // even a refused run fixes the configuration
+    pacman_recurrence_call call;
+    int status = normalise_call(in, &call);
+    if (status == OK) status = RunPlan(r, call).execute();
+    if (!(status == OK && r.kept.valid)) lock.leave();
     return status;
 }
 
+extern "C" int pacman_recurrence_v1_reduce(const pacman_recurrence_reduce *q) {
+    runner_lock().enter();
+    const int status = Reduction(runner()).plain(q);
+    runner_lock().leave();
+    return status;
+}
+
+extern "C" int pacman_recurrence_v1_reduce_resolved(const pacman_recurrence_reduce_resolved *q) {
+    runner_lock().enter();
+    const int status = Reduction(runner()).resolved(q);
+    runner_lock().leave();
+    return status;
+}
+
+extern "C" int pacman_recurrence_v1_reduce_sample(const pacman_recurrence_reduce_sample *q) {
+    runner_lock().enter();
+    const int status = Reduction(runner()).sample(q);
+    runner_lock().leave();
+    return status;
+}
+
+// The kernel, block and sort are set once: the same configuration again is accepted (several
+// callers of one process share the runner), another one only before the first run (the kernel
+// fixes the arena's layout). Statistics, once asked for, stay on.
 extern "C" int pacman_recurrence_v1_configure(const pacman_recurrence_options *o) {
-    if (!o || o->abi_version != PACMAN_RECURRENCE_ABI || o->struct_size < sizeof(pacman_recurrence_options))
-        return PACMAN_RECURRENCE_INVALID;
-    if (o->kernel > PACMAN_RECURRENCE_KERNEL_MIXED64 || (o->block && (o->block < 32 || o->block > 1024)))
-        return PACMAN_RECURRENCE_INVALID;
-    acquire_runner_lock();
-    int status = PACMAN_RECURRENCE_OK;
-    Options n;
-    n.kernel = (int)o->kernel;
-    n.df64 = n.kernel != KERNEL_F64;
-    n.df64_forms = n.kernel == KERNEL_DF64 || n.kernel == KERNEL_MIXED;
-    n.block = o->block ? (int)o->block : 128;
-    n.sort = !o->no_sort;
-    const bool same = n.kernel == st.options.kernel && n.block == st.options.block && n.sort == st.options.sort;
-    if (st.runs_started && !same) {
+    if (!o || o->abi_version != PACMAN_RECURRENCE_ABI || o->struct_size < sizeof(pacman_recurrence_options) ||
+        o->kernel > PACMAN_RECURRENCE_KERNEL_MIXED64 || (o->block && (o->block < 32 || o->block > 1024)))
+        return INVALID;
+    Config wanted;
+    wanted.kernel = o->kernel;
+    wanted.block = o->block ? o->block : 128;
+    wanted.sort = !o->no_sort;
+    runner_lock().enter();
+    Runner &r = runner();
+    int status = OK;
+    if (r.started && !(wanted == r.config)) {
         fprintf(stderr, "pacman recurrence runner: another configuration after the first run\n");
-        status = PACMAN_RECURRENCE_INVALID;
+        status = INVALID;
     } else {
-        st.options = n;
-        st.stats = st.stats || o->stats != 0;
+        r.config = wanted;
+        r.stats.enabled = r.stats.enabled || o->stats;
     }
-    release_runner_lock();
+    runner_lock().leave();
     return status;
 }
 
 extern "C" uint32_t pacman_recurrence_v1_capabilities(void) {
-    return PACMAN_RECURRENCE_CAP_SOURCES | PACMAN_RECURRENCE_CAP_FORMS | PACUDA_CAP_FUSED;
+    return PACMAN_RECURRENCE_CAP_SOURCES | PACMAN_RECURRENCE_CAP_FORMS |
+           (PACUDA_KERNEL_COUNT > 0 ? PACMAN_RECURRENCE_CAP_SYMJIT : 0u) | PACUDA_CAP_FUSED;
 }
 
-// This build carries no generated symjit kernels (op 13/14 rows are refused), so no SHA-256
-// names one.
+// The index of the generated kernel whose plane file has this SHA-256 (hex, either case), or -1.
 extern "C" int pacman_recurrence_v1_symjit_kernel(const char *sha256_hex) {
-    (void)sha256_hex;
+    if (!sha256_hex) return -1;
+    for (int k = 0; k < PACUDA_KERNEL_COUNT; k++)
+        if (strcasecmp(pacuda_kernel_sha256[k], sha256_hex) == 0) return k;
     return -1;
 }
 
+// Test entry: the source wavefunctions of `count` (template, mass, momentum) items on the device.
 extern "C" int pacman_recurrence_v1_test_sources(const uint32_t *codes, const double *masses, const double *momenta,
                                                  uint32_t count, double *out_re, double *out_im) {
-    acquire_runner_lock();
-    uint32_t *dc = nullptr;
-    double *dm = nullptr, *dp = nullptr, *dre = nullptr, *dim = nullptr;
-    hipStream_t s = runner_stream();
-    size_t n = std::max<uint32_t>(count, 1);
-    bool ok = s && !hip_failed(hipMalloc((void **)&dc, n * 4), "test") && !hip_failed(hipMalloc((void **)&dm, n * 8), "test") &&
-              !hip_failed(hipMalloc((void **)&dp, 4 * n * 8), "test") && !hip_failed(hipMalloc((void **)&dre, 4 * n * 8), "test") &&
-              !hip_failed(hipMalloc((void **)&dim, 4 * n * 8), "test");
-    if (ok && count) {
-        ok = !hip_failed(hipMemcpyAsync(dc, codes, count * 4, hipMemcpyHostToDevice, s), "test") &&
-             !hip_failed(hipMemcpyAsync(dm, masses, count * 8, hipMemcpyHostToDevice, s), "test") &&
-             !hip_failed(hipMemcpyAsync(dp, momenta, 4 * (size_t)count * 8, hipMemcpyHostToDevice, s), "test");
-        if (ok) {
-            test_sources_kernel<<<(count + 127) / 128, 128, 0, s>>>(dc, dm, dp, count, dre, dim);
-            ok = !hip_failed(hipGetLastError(), "test") &&
-                 !hip_failed(hipMemcpyAsync(out_re, dre, 4 * (size_t)count * 8, hipMemcpyDeviceToHost, s), "test") &&
-                 !hip_failed(hipMemcpyAsync(out_im, dim, 4 * (size_t)count * 8, hipMemcpyDeviceToHost, s), "test") &&
-                 !hip_failed(hipStreamSynchronize(s), "test");
+    runner_lock().enter();
+    int status = OK;
+    {
+        hipStream_t s = runner_stream();
+        DeviceArray<uint32_t> dc;
+        DeviceArray<double> dm, dp, dre, dim;
+        const size_t n = count;
+        bool ok = s && dc.fit(n, "test") && dm.fit(n, "test") && dp.fit(4 * n, "test") && dre.fit(4 * n, "test") &&
+                  dim.fit(4 * n, "test");
+        if (ok && n) {
+            ok = !hip_failed(hipMemcpyAsync(dc.data(), codes, n * 4, hipMemcpyHostToDevice, s), "test") &&
+                 !hip_failed(hipMemcpyAsync(dm.data(), masses, n * 8, hipMemcpyHostToDevice, s), "test") &&
+                 !hip_failed(hipMemcpyAsync(dp.data(), momenta, 4 * n * 8, hipMemcpyHostToDevice, s), "test");
+            if (ok) {
+                test_sources_kernel<<<(count + 127) / 128, 128, 0, s>>>(dc.data(), dm.data(), dp.data(), count, dre.data(),
+                                                                        dim.data());
+                ok = !hip_failed(hipGetLastError(), "test") &&
+                     !hip_failed(hipMemcpyAsync(out_re, dre.data(), 4 * n * 8, hipMemcpyDeviceToHost, s), "test") &&
+                     !hip_failed(hipMemcpyAsync(out_im, dim.data(), 4 * n * 8, hipMemcpyDeviceToHost, s), "test");
+            }
         }
+        // (the buffers are freed at the end of the block, after the stream has finished with them)
+        if (s && hip_failed(hipStreamSynchronize(s), "test")) ok = false;
+        status = ok ? OK : DEVICE_ERROR;
     }
-    (void)hipFree(dc); (void)hipFree(dm); (void)hipFree(dp); (void)hipFree(dre); (void)hipFree(dim);
-    release_runner_lock();
-    return ok ? PACMAN_RECURRENCE_OK : PACMAN_RECURRENCE_DEVICE_ERROR;
+    runner_lock().leave();
+    return status;
 }
 
 extern "C" void *pacman_recurrence_v1_host_alloc(size_t bytes) {
     void *p = nullptr;
-    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
-    return p;
+    return hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) == hipSuccess ? p : nullptr;
 }
 
 extern "C" void pacman_recurrence_v1_host_free(void *p) {
     if (p) (void)hipHostFree(p);
-}
-
-extern "C" int pacman_recurrence_v1_reduce_sample(const pacman_recurrence_reduce_sample *r) {
-    acquire_runner_lock();
-    int status = reduce_sample_locked(r);
-    release_runner_lock();
-    return status;
-}
-
-extern "C" int pacman_recurrence_v1_reduce_resolved(const pacman_recurrence_reduce_resolved *r) {
-    acquire_runner_lock();
-    int status = reduce_resolved_locked(r);
-    release_runner_lock();
-    return status;
-}
-
-extern "C" int pacman_recurrence_v1_reduce(const pacman_recurrence_reduce *r) {
-    acquire_runner_lock();
-    int status = reduce_locked(r);
-    release_runner_lock();
-    return status;
 }

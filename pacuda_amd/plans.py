@@ -170,6 +170,15 @@ def encode(rows, width):
         elif kind == "source":
             _, dst, form, x, mass, code = r
             op, fields = 12, [(dst, P), (form, F), (x, X), (mass, S), (code, 10)]
+        elif kind == "symjit":
+            _, init, p0, p1, dst, f0, f1, x = r
+            op, fields = 13, [(init, 1), (p0, P), (p1, P), (dst, P), (f0, F), (f1, F), (x, X)]
+        elif kind == "binding":
+            # kernel, input codes, output components, direct: the word, then the codes two per
+            # data word, low half first
+            _, kernel, codes, components, direct = r
+            op, fields = 14, [(kernel, 16), (len(codes), 8), (len(components), 8), (direct, 1)]
+            data = list(codes) + list(components)
         else:
             raise ValueError(f"row {n}: {kind}")
         word, at = op, 4
@@ -181,7 +190,30 @@ def encode(rows, width):
         if at > 64:
             raise ValueError(f"row {n}: {at} bits")
         words.append(word)
+        if kind == "binding":
+            for k in range(0, len(data), 2):
+                lo = data[k]
+                hi = data[k + 1] if k + 1 < len(data) else 0
+                words.append(int(lo) | int(hi) << 32)
     return words
+
+
+# a symjit binding's input codes (runner/recurrence_runner.cu:580-582): kind, which operand,
+# imaginary part, component
+SB_PARENT, SB_MOMENTUM, SB_DESTINATION, SB_SCALE, SB_FACTOR = range(5)
+
+
+def binding_code(kind, which=0, imag=0, index=0):
+    return kind | which << 3 | imag << 4 | index << 5
+
+
+# the generated kernels' indices (the sorted file names of pacuda_amd/kernels/*.pk)
+KERNEL_BINDING_PROBE, KERNEL_FERMION_PAIR_SIGMA, KERNEL_FERMION_PAIR_SIGMABAR = range(3)
+
+
+def weyl_pair_codes():
+    """the inputs of the fermion-pair kernels: re, im of parent 0's two components, then parent 1's"""
+    return [binding_code(SB_PARENT, w, im, c) for w in (0, 1) for c in (0, 1) for im in (0, 1)]
 
 
 def decode_fields(word, width):

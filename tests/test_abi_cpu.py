@@ -55,9 +55,18 @@ def test_struct_layouts_match_the_reference_sizes():
 
 
 def test_capabilities_without_device(lib):
+    import hashlib
+    from pacuda_amd import build
     caps = lib.pacman_recurrence_v1_capabilities()
-    assert caps == abi.CAP_SOURCES | abi.CAP_FORMS | abi.CAP_FUSED
+    assert caps == abi.CAP_SOURCES | abi.CAP_FORMS | abi.CAP_SYMJIT | abi.CAP_FUSED
     assert lib.pacman_recurrence_v1_symjit_kernel(None) == -1
+    assert lib.pacman_recurrence_v1_symjit_kernel(b"0" * 64) == -1
+    # the generated kernels are named by the SHA-256 of their plane-kernel files, either case
+    # (runner/recurrence_runner.cu:1792-1797)
+    for k, path in enumerate(build.kernel_files()):
+        digest = hashlib.sha256(open(path, "rb").read()).hexdigest()
+        assert lib.pacman_recurrence_v1_symjit_kernel(digest.encode()) == k
+        assert lib.pacman_recurrence_v1_symjit_kernel(digest.upper().encode()) == k
 
 
 def _invalid_calls():
