@@ -157,7 +157,9 @@ def passes_cuts(mom, pt_min=20.0, eta_max=2.5, dr_min=0.4):
 
 
 def generate(n, points, seed, batch=1 << 20, ecm=ECM):
-    """`points` RAMBO points passing the cuts, and the number drawn to get them."""
+    """`points` RAMBO points passing the cuts, and the number drawn to get them. (The stream is
+    counter based, so the batch size changes only how many are drawn at a time, never which.)"""
+    batch = min(batch, max(4096, 2 * points))
     kept, drawn, offset = [], 0, 0
     have = 0
     while have < points:

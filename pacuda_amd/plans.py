@@ -397,6 +397,114 @@ def quark_line_plan(n, incoming=(0, 1), weight=1.0):
     return p
 
 
+# ---- two quark lines (four quarks and at most one gluon) ------------------------------------
+
+# the symjit row factor of the fermion-pair current: the vertex's i times the coupling, so that
+# the current replaces a gluon's polarisation in the closure of its line
+FERMION_PAIR = 1j * QUARK_GLUON
+
+
+def _fermion_pair(rows, chirality, left, right, dst, factor, init):
+    """K = factor * (l^T sigma^mu r) (chirality +1, the line of vertex op 2) or l^T sigmabar^mu r
+    (chirality -1, op 3): a symjit row (op 13) after its binding (op 14)"""
+    kernel = KERNEL_FERMION_PAIR_SIGMA if chirality == 1 else KERNEL_FERMION_PAIR_SIGMABAR
+    rows.append(("binding", kernel, weyl_pair_codes(), [0, 1, 2, 3], 0))
+    rows.append(("symjit", init, left, right, dst, 0, 0, factor))
+
+
+def four_quark_plan(incoming=(0, 3), gluon=True, weight=1.0):
+    """The colour ordering (qbar_M, g, q_S, qbar_S, q_M) of two massless quark lines of different
+    flavour (M, the main line, and S) and one gluon, every helicity configuration; without the
+    gluon (gluon=False) the four-quark ordering (qbar_M, q_S, qbar_S, q_M).
+
+    Legs (all outgoing; `incoming` legs get crossed momenta): 0 qbar_M, 1 the gluon, 2 q_S,
+    3 qbar_S, 4 q_M (without the gluon: 0, 1 q_S, 2 qbar_S, 3 q_M). The main line's current grows
+    from q_M as in quark_line_plan and absorbs the vector currents of the elements to its left:
+    the gluon's, the S pair's and that of the gluon and the S pair together. The S pair's vector
+    current is its fermion-pair current (a symjit row: the lines' two Weyl currents, the kernel
+    l^T sigma^mu r or its sigmabar twin, factor i/sqrt 2), propagated by op 9; with the gluon it
+    also takes the S line with the gluon absorbed (its quark propagated) and the three-gluon
+    vertex of the gluon and the S pair (op 6). Leading colour: the gluon sits on the colour line
+    from q_S to qbar_M.
+
+    Every helicity configuration is MHV or anti-MHV; |A|^2 = s_ab^2 s(qbar_M, q_S) s(qbar_S, q_M)
+    / prod s_{i,i+1} (a, b the two negative-helicity fermions for a positive gluon, the two
+    positive ones for a negative gluon; with no gluon s_ab^2 / s(q_S, qbar_S)^2), and the
+    amplitude vanishes when the gluon's polarisation is replaced by its momentum
+    (tests/test_physics.py). Output o: the main line's chirality (o < 4: -1), the S line's, the
+    gluon's helicity (bits from the most significant)."""
+    n = 5 if gluon else 4
+    p = Plan(n)
+    p.name = "qqbarQQbar+g" if gluon else "qqbarQQbar"
+    p.crossing = [(leg, -1.0 if leg in incoming else 1.0) for leg in range(n)]
+    mass0 = p.scale(0.0)
+    g3 = p.scale(complex(0.0, -SQRT_HALF))
+    gq = p.scale(QUARK_GLUON)
+    one = p.factor(1.0)
+    pair = p.factor(FERMION_PAIR)
+    rows = p.rows
+    qbar_m, q_s, qbar_s, q_m = (0, 2, 3, 4) if gluon else (0, 1, 2, 3)
+    J = {}
+    if gluon:
+        f = p.form([1])
+        for h in (-1, 1):
+            base = p.current(4)
+            rows.append(("source", base, f, one, mass0, source_code(FAMILY_VECTOR, 0, h, 0)))
+            J[h] = base
+
+    def helicity(leg, chirality):
+        return -chirality if leg in incoming else chirality
+
+    src, code_hel = {}, {}
+    for ch in (-1, 1):
+        # a quark of chirality ch and its line's antiquark of chirality -ch
+        for leg, orientation, chirality in ((q_m, 0, ch), (qbar_m, 1, -ch), (q_s, 0, ch), (qbar_s, 1, -ch)):
+            base = p.current(2)
+            h = helicity(leg, chirality)
+            rows.append(("source", base, p.form([leg]), one, mass0, source_code(FAMILY_WEYL, orientation, h, chirality)))
+            src[(leg, ch)] = base
+            code_hel[(leg, ch)] = h
+    components = p.factor_run([1.0, 1.0])
+    for ch_m in (-1, 1):
+        vertex_m, propagator_m = (2, 8) if ch_m == 1 else (3, 7)
+        for ch_s in (-1, 1):
+            vertex_s, propagator_s = (2, 8) if ch_s == 1 else (3, 7)
+            for h in ((-1, 1) if gluon else (None,)):
+                # the S pair's vector current, propagated
+                k_s = p.current(4)
+                _fermion_pair(rows, ch_s, src[(q_s, ch_s)], src[(qbar_s, ch_s)], k_s, pair, 1)
+                rows.append(("prop", 9, k_s, p.form([q_s, qbar_s]), one))
+                full = p.current(2)
+                if gluon:
+                    # the S line with the gluon absorbed, then the vector current of (g, S pair):
+                    # its fermion pair and the three-gluon vertex, propagated
+                    psi_s = p.current(2)
+                    rows.append(("contrib", vertex_s, 1, gq, src[(q_s, ch_s)], J[h], psi_s, 0, 0, one))
+                    rows.append(("prop", propagator_s, psi_s, p.form([1, q_s]), one))
+                    k_gs = p.current(4)
+                    _fermion_pair(rows, ch_s, psi_s, src[(qbar_s, ch_s)], k_gs, pair, 1)
+                    rows.append(("contrib", 6, 0, g3, J[h], k_s, k_gs, p.form([1]), p.form([q_s, qbar_s]), one))
+                    rows.append(("prop", 9, k_gs, p.form([1, q_s, qbar_s]), one))
+                    # the main line: the S pair next to q_M, then the gluon; or both at once
+                    psi_m = p.current(2)
+                    rows.append(("contrib", vertex_m, 1, gq, src[(q_m, ch_m)], k_s, psi_m, 0, 0, one))
+                    rows.append(("prop", propagator_m, psi_m, p.form([q_s, qbar_s, q_m]), one))
+                    rows.append(("contrib", vertex_m, 1, gq, psi_m, J[h], full, 0, 0, one))
+                    rows.append(("contrib", vertex_m, 0, gq, src[(q_m, ch_m)], k_gs, full, 0, 0, one))
+                else:
+                    rows.append(("contrib", vertex_m, 1, gq, src[(q_m, ch_m)], k_s, full, 0, 0, one))
+                amp = p.amplitude()
+                rows.append(("closure", src[(qbar_m, ch_m)], full, amp, one, components, 2))
+                p.outputs.append([(amp, weight)])
+                hel = [0] * n
+                hel[qbar_m], hel[q_m] = code_hel[(qbar_m, ch_m)], code_hel[(q_m, ch_m)]
+                hel[q_s], hel[qbar_s] = code_hel[(q_s, ch_s)], code_hel[(qbar_s, ch_s)]
+                if gluon:
+                    hel[1] = h
+                p.helicities.append(tuple(hel))
+    return p
+
+
 # ---- flop count (per point) ----------------------------------------------------------------
 # A real add, subtract or multiply counts 1, an fma 2, a divide or square root 1 (also counted
 # apart); negation and multiplication by i count 0 (the operations of runner/recurrence_runner.cu
@@ -406,9 +514,28 @@ CONTRIB_OUT = {2: 2, 3: 2, 4: 4, 5: 6, 6: 4}
 SOURCE_FLOPS = {FAMILY_SCALAR: 6, FAMILY_WEYL: 20, FAMILY_DIRAC: 30, FAMILY_VECTOR: 22}
 
 
+def kernel_flops():
+    """flops per call of each generated symjit kernel, counted from its plane-kernel file (the
+    rules above: copy, neg and constants free)"""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kernels")
+    cost = {"add": 1, "sub": 1, "mul": 1, "div": 1, "sqrt": 1, "fma": 2}
+    out = []
+    for name in sorted(f for f in os.listdir(here) if f.endswith(".pk")):
+        n = 0
+        for line in open(os.path.join(here, name)):
+            w = line.split("#")[0].split()
+            if len(w) >= 3 and w[1] == "=":
+                n += cost.get(w[2], 0)
+        out.append(n)
+    return out
+
+
 def plan_flops(plan):
     """(flops, divides) per point of one plan run: forms, sources, program and reduction."""
     flops, divs = 0, 0
+    kflops = kernel_flops()
+    binding = None
     for f in plan.forms:
         flops += 4 * (2 * len(f) - 1)
     for r in plan.rows:
@@ -431,6 +558,12 @@ def plan_flops(plan):
             comps = {0: 1, 1: 2, 2: 4, 3: 4}[fam]
             flops += SOURCE_FLOPS[fam] + 6 * comps
             divs += 3
+        elif k == "binding":
+            binding = r
+        elif k == "symjit":
+            # the kernel, then each output pair times the factor (6) and, accumulating, added (2)
+            pairs = len(binding[3])
+            flops += kflops[binding[1]] + 6 * pairs + (0 if r[1] else 2 * pairs)
     for out in plan.outputs:
         flops += 5 * len(out) + 1
     return flops, divs

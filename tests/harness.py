@@ -33,7 +33,12 @@ class V1:
         o = abi.Options(abi.ABI, C.sizeof(abi.Options), kernel, block, no_sort, 0)
         return self.configure(C.byref(o))
 
-    def run_plan(self, plan, ext, points, stride=None, keep=True):
+    def run_plan_keep_only(self, plan, ext, points, stride=None):
+        """the v1 run of run_plan with KEEP and no reduction: returns with the amplitudes kept
+        (and the runner's lock held by this thread)"""
+        self.run_plan(plan, ext, points, stride, keep=True, reduce=False)
+
+    def run_plan(self, plan, ext, points, stride=None, keep=True, reduce=True):
         """v1 run of a generated plan with the forms on the device, then reduce_resolved from
         zero: [points][outputs] (each output before the group factor)."""
         a = plan.arrays()
@@ -82,6 +87,8 @@ class V1:
             raise RuntimeError(f"run: status {st}")
         if not keep:
             return are.reshape(n_amp, stride)[:, :points], aim.reshape(n_amp, stride)[:, :points]
+        if not reduce:
+            return None
         outputs = len(a["offsets"]) - 1
         out = np.full(points * outputs, np.nan)
         r = abi.ReduceResolved(abi.ABI, C.sizeof(abi.ReduceResolved), ptr(a["planes"], u32p), ptr(a["weights"]),

@@ -178,10 +178,12 @@ def test_quark_line_mhv(n):
 
 def test_pp_jjj_subprocesses_closed_forms():
     """every subprocess of pacuda_amd/processes.py, with its crossing: each non-vanishing
-    helicity amplitude equals Parke-Taylor (gluons) or the quark-line MHV form"""
+    helicity amplitude equals Parke-Taylor (gluons), the quark-line MHV form or the two-line
+    form (four quarks and a gluon)"""
     from pacuda_amd import processes
     subs = processes.subprocesses()
-    assert len(subs) == 29
+    assert len(subs) == 29 + 152
+    assert len({sp.name for sp in subs}) == len(subs)
     mom, _ = ps.generate(5, 8, seed=300)
     n = 5
     for sp in subs:
@@ -191,6 +193,10 @@ def test_pp_jjj_subprocesses_closed_forms():
         q = ext.reshape(n, 4, 8).transpose(2, 0, 1)
         s = np.array([[2 * _mdot(q[:, i], q[:, j]) for j in range(n)] for i in range(n)])
         den = np.abs(np.prod([s[k, (k + 1) % n] for k in range(n)], axis=0))
+        if plan.name.startswith("qqbarQQbar"):
+            want = _fermion_pair_closed_form(plan, ext, 8)
+            assert np.allclose(res, want, rtol=1e-9, atol=0), sp.name
+            continue
         quark = plan.name.startswith("qqbar")
         for o, hel in enumerate(plan.helicities):
             out = [(-h if plan.crossing[l][1] < 0 else h) for l, h in enumerate(hel)]
@@ -234,3 +240,81 @@ def test_surrogate_prior_is_exact_below_six_gluons(n):
                               len(mom6), kernel=1, resolved=False)
     r6 = np.log(tot6) - ps.log_prior(mom6, plan6.crossing)
     assert 0.05 < np.std(r6) < 0.5
+
+
+def _fermion_pair_closed_form(plan, ext, points):
+    """|A|^2 per output of plans.four_quark_plan from its closed form (outgoing helicities)"""
+    n = plan.legs
+    q = ext.reshape(n, 4, points).transpose(2, 0, 1)
+    s = np.array([[2 * _mdot(q[:, i], q[:, j]) for j in range(n)] for i in range(n)])
+    qbar_m, q_s, qbar_s, q_m = (0, 2, 3, 4) if n == 5 else (0, 1, 2, 3)
+    fermions = (qbar_m, q_s, qbar_s, q_m)
+    want = []
+    for hel in plan.helicities:
+        out = [(-h if plan.crossing[l][1] < 0 else h) for l, h in enumerate(hel)]
+        # each line keeps its helicity: its quark and antiquark are of opposite helicity
+        assert out[q_m] == -out[qbar_m] and out[q_s] == -out[qbar_s]
+        if n == 4:
+            a, b = [l for l in fermions if out[l] < 0]
+            want.append(s[a, b] ** 2 / s[q_s, qbar_s] ** 2)
+        else:
+            sign = -1 if out[1] > 0 else 1  # MHV: the negative fermions; anti-MHV: the positive ones
+            a, b = [l for l in fermions if sign * out[l] > 0]
+            den = np.abs(np.prod([s[k, (k + 1) % n] for k in range(n)], axis=0))
+            want.append(s[a, b] ** 2 * np.abs(s[qbar_m, q_s] * s[qbar_s, q_m]) / den)
+    return np.array(want).T
+
+
+@pytest.mark.parametrize("gluon", [False, True])
+def test_four_quark_closed_forms(gluon):
+    """two quark lines of different flavour (and a gluon): every helicity amplitude equals the
+    closed form of plans.four_quark_plan's docstring -- with no gluon s_ab^2 / s_{S pair}^2 (one
+    gluon exchanged), with one the MHV form s_ab^2 s(qbar_M, q_S) s(qbar_S, q_M) / prod s_{i,i+1}
+    -- with two legs crossed as for q qbar -> Q Qbar (g)"""
+    n = 5 if gluon else 4
+    incoming = (0, 3) if gluon else (0, 2)
+    plan = plans.four_quark_plan(incoming=incoming, gluon=gluon)
+    mom, _ = ps.generate(n, 16, seed=400 + n)
+    # plan legs <- physical legs: the beams into the crossed slots
+    order = [0, 4, 2, 1, 3] if gluon else [0, 2, 1, 3]
+    mom = mom[:, order, :]
+    ext, res, tot = _run(plan, mom)
+    want = _fermion_pair_closed_form(plan, ext, mom.shape[0])
+    assert res.shape == want.shape == (16, 8 if gluon else 4)
+    assert np.allclose(res, want, rtol=1e-10, atol=0), np.max(np.abs(res / want - 1))
+
+
+def test_four_quark_gauge_invariance():
+    """the colour-ordered amplitude vanishes when the gluon's polarisation is replaced by its
+    momentum (the current k made by a symjit row copying the gluon's form), for every helicity
+    configuration of the fermions: the relative signs of the gluon on the main line, on the S
+    line and on the exchanged gluon (three-gluon vertex) are fixed by this"""
+    plan = plans.four_quark_plan(incoming=(0, 3))
+    mom, _ = ps.generate(5, 8, seed=410)
+    mom = mom[:, [0, 4, 2, 1, 3], :]
+    ext = ps.external_planes(mom, plan.crossing)
+    gauge = plans.four_quark_plan(incoming=(0, 3))
+    rows = [r for r in gauge.rows if not (r[0] == "source" and r[5] & 7 == plans.FAMILY_VECTOR)]
+    gluon_bases = [r[1] for r in gauge.rows if r[0] == "source" and r[5] & 7 == plans.FAMILY_VECTOR]
+    n_src = sum(1 for r in rows if r[0] == "source")
+    codes = []
+    for m in range(4):  # (k_m, +0) pairs: the form's component and the real factor's imaginary part
+        codes += [plans.binding_code(plans.SB_MOMENTUM, 0, 0, m), plans.binding_code(plans.SB_FACTOR, 0, 1, 0)]
+    k_rows = [("binding", plans.KERNEL_BINDING_PROBE, codes, [0, 1, 2, 3], 1)]
+    for base in gluon_bases:
+        k_rows.append(("symjit", 1, 0, 0, base, gauge.form([1]), gauge.form([1]), gauge.factor(1.0)))
+    gauge.rows = rows[:n_src] + k_rows + rows[n_src:]
+
+    def amplitudes(p):
+        lib = refcpu.load()
+        lib.refcpu_reset()
+        v1 = harness.V1(lib, prefix="refcpu_v1_")
+        assert v1.configure_kernel(1) == 0
+        are, aim = v1.run_plan(p, ext, ext.shape[1], keep=False)
+        lib.refcpu_reset()
+        return are + 1j * aim
+
+    a, ak = amplitudes(plan), amplitudes(gauge)
+    energy = np.abs(ext[4])  # the gluon's energy (leg 1)
+    assert np.max(np.abs(ak) / (np.max(np.abs(a), axis=0) * energy)) < 1e-12
+    assert np.min(np.max(np.abs(a), axis=0)) > 0
