@@ -104,7 +104,7 @@ template <class K, int TIER> constexpr int register_forms() {
     return TIER == 2 ? 4 : TIER == 1 ? (int)FORM_REGS : 0;
 }
 
-template <class K, int TIER>
+template <class K, int TIER, bool SYMJIT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_simd<K, TIER>())))
 fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start, uint32_t item_count,
              uint32_t *counter, double *scratch, size_t slot_doubles, uint32_t lds_per_wave, uint32_t lds_variant,
@@ -241,9 +241,9 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
             }
         }
         if constexpr (K::forwarding)
-            interpret<K>(prog + pl.sources, count - pl.sources, a);
+            interpret<K, SYMJIT>(prog + pl.sources, count - pl.sources, a);
         else
-            interpret<K>(pl.prog_plain + pl.sources, pl.count_plain - pl.sources, a);
+            interpret<K, SYMJIT>(pl.prog_plain + pl.sources, pl.count_plain - pl.sources, a);
         // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
         double t = 0.0;
         for (uint32_t o = 0; o < pl.outputs; o++) {
@@ -268,6 +268,7 @@ struct pacuda_plan {
     PlanDev *device = nullptr;
     std::vector<void *> allocations;
     uint64_t slot_doubles = 0;
+    bool symjit = false;  // the program holds symjit rows
 };
 
 namespace {
@@ -1252,6 +1253,7 @@ struct Compiled {
     uint32_t reg_forms[FORM_REGS];
     bool zero_fill = true;
     std::vector<uint32_t> bindings;
+    bool symjit = false;
 };
 
 // F_ZERO on the first closure of every amplitude plane (closures are the only rows that touch
@@ -1342,6 +1344,7 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
         return PACMAN_RECURRENCE_INVALID;
     }
     c.bindings = hp.bindings;
+    c.symjit = hp.symjit;
     if (hp.symjit) {
         // Symjit rows read and write single components their bindings name, which the passes
         // below (whole-object compaction, forwarding, on-chip placement, zero masks) do not model:
@@ -1520,6 +1523,7 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     h.planes = upload(p, d->planes, count, &ok);
     h.weights = upload(p, d->weights, count, &ok);
     h.bindings = upload(p, cp.bindings.data(), cp.bindings.size(), &ok);
+    p->symjit = cp.symjit;
     h.numerator = d->numerator;
     h.denominator = d->denominator;
     h.factor = d->numerator != d->denominator;
@@ -1559,6 +1563,7 @@ struct Schedule {
     uint32_t grid = 0, block = 0, per_cu = 0;
     uint32_t lds_planes[LDS_VARIANTS] = {};  // per variant, the most any item's plan uses
     bool lds_all[LDS_VARIANTS] = {true, true, true, true, true};  // every plan has the variant
+    bool symjit = false;  // some item's plan holds symjit rows
 };
 
 // The LDS variant of a launch, its LDS elements per wavefront and bytes per workgroup. A variant
@@ -1612,6 +1617,7 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
         sc.starts[j] = (uint32_t)sc.units;
         sc.units += (it.points + LANES - 1) / LANES;
         sc.slot = std::max<uint64_t>(sc.slot, it.plan->slot_doubles);
+        sc.symjit = sc.symjit || it.plan->symjit;
         for (int v = 0; v < LDS_VARIANTS; v++) {
             sc.lds_planes[v] = std::max(sc.lds_planes[v], it.plan->host.lds_planes[v]);
             sc.lds_all[v] = sc.lds_all[v] && it.plan->host.lds_planes[v] > 0;
@@ -1634,8 +1640,8 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
     return PACMAN_RECURRENCE_OK;
 }
 
-template <class K, int TIER> bool allow_lds() {
-    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(fused_kernel<K, TIER>),
+template <class K, int TIER, bool SJ> bool allow_lds() {
+    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(fused_kernel<K, TIER, SJ>),
                                                            hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES),
                                        "fused kernel LDS");
     return ok;
@@ -1653,22 +1659,30 @@ int tier_of(uint32_t kernel, uint32_t per_cu) {
     return mixed ? tier_of<KMixed>(per_cu) : tier_of<KF64>(per_cu);
 }
 
-template <class K> bool allow_lds(int tier) {
-    return tier == 2 ? allow_lds<K, 2>() : tier == 1 ? allow_lds<K, 1>() : allow_lds<K, 0>();
+template <class K> bool allow_lds(int tier, bool sj) {
+    if (sj) return tier == 2 ? allow_lds<K, 2, true>() : tier == 1 ? allow_lds<K, 1, true>() : allow_lds<K, 0, true>();
+    return tier == 2 ? allow_lds<K, 2, false>() : tier == 1 ? allow_lds<K, 1, false>() : allow_lds<K, 0, false>();
+}
+
+template <class K, int TIER>
+void launch_tier(const Schedule &sc, hipStream_t s, size_t bytes, const ItemDev *items, const uint32_t *starts,
+                 uint32_t count, uint32_t *counter, double *scratch, uint32_t per_wave, uint32_t variant) {
+    if (sc.symjit)
+        fused_kernel<K, TIER, true><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot,
+                                                                     per_wave, variant, LDS_CAPS[variant]);
+    else
+        fused_kernel<K, TIER, false><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot,
+                                                                      per_wave, variant, LDS_CAPS[variant]);
 }
 
 template <class K>
-void launch(const Schedule &sc, hipStream_t s, int tier, size_t bytes, const ItemDev *items, const uint32_t *starts,
+bool launch(const Schedule &sc, hipStream_t s, int tier, size_t bytes, const ItemDev *items, const uint32_t *starts,
             uint32_t count, uint32_t *counter, double *scratch, uint32_t per_wave, uint32_t variant) {
-    if (tier == 1)
-        fused_kernel<K, 1><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
-                                                            variant, LDS_CAPS[variant]);
-    else if (tier == 2)
-        fused_kernel<K, 2><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
-                                                            variant, LDS_CAPS[variant]);
-    else
-        fused_kernel<K, 0><<<sc.grid, sc.block, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, per_wave,
-                                                            variant, LDS_CAPS[variant]);
+    if (bytes && !allow_lds<K>(tier, sc.symjit)) return false;
+    if (tier == 1) launch_tier<K, 1>(sc, s, bytes, items, starts, count, counter, scratch, per_wave, variant);
+    else if (tier == 2) launch_tier<K, 2>(sc, s, bytes, items, starts, count, counter, scratch, per_wave, variant);
+    else launch_tier<K, 0>(sc, s, bytes, items, starts, count, counter, scratch, per_wave, variant);
+    return true;
 }
 
 // the LDS attribute of the kernel a launch of `sc` needs, set once (before any graph capture)
@@ -1678,10 +1692,10 @@ bool lds_ready(uint32_t kernel, const Schedule &sc) {
     lds_shape(kernel, sc, &per_wave, &bytes, &variant);
     if (!bytes) return true;
     const int tier = tier_of(kernel, sc.per_cu);
-    if (kernel == PACMAN_RECURRENCE_KERNEL_F64) return allow_lds<KF64>(tier);
-    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED) return allow_lds<KMixed>(tier);
-    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) return allow_lds<KMixed64>(tier);
-    return allow_lds<KDF64>(tier);
+    if (kernel == PACMAN_RECURRENCE_KERNEL_F64) return allow_lds<KF64>(tier, sc.symjit);
+    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED) return allow_lds<KMixed>(tier, sc.symjit);
+    if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) return allow_lds<KMixed64>(tier, sc.symjit);
+    return allow_lds<KDF64>(tier, sc.symjit);
 }
 
 bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
@@ -1693,23 +1707,14 @@ bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const Item
     const int tier = tier_of(kernel, sc.per_cu);
     switch (kernel) {
     case PACMAN_RECURRENCE_KERNEL_F64:
-        if (bytes && !allow_lds<KF64>(tier)) return false;
-        launch<KF64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
-        break;
+        return launch<KF64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
     case PACMAN_RECURRENCE_KERNEL_MIXED:
-        if (bytes && !allow_lds<KMixed>(tier)) return false;
-        launch<KMixed>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
-        break;
+        return launch<KMixed>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
     case PACMAN_RECURRENCE_KERNEL_MIXED64:
-        if (bytes && !allow_lds<KMixed64>(tier)) return false;
-        launch<KMixed64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
-        break;
+        return launch<KMixed64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
     default:
-        if (bytes && !allow_lds<KDF64>(tier)) return false;
-        launch<KDF64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
-        break;
+        return launch<KDF64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
     }
-    return true;
 }
 
 }  // namespace

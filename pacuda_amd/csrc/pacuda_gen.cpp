@@ -189,7 +189,7 @@ std::string emit(const std::vector<Kernel> &kernels) {
          "#include <stdint.h>\n\n"
          "#ifndef PACUDA_KFN\n"
          "#if defined(__HIPCC__)\n"
-         "#define PACUDA_KFN __host__ __device__ inline\n"
+         "#define PACUDA_KFN __host__ __device__ __attribute__((always_inline)) inline\n"
          "#else\n"
          "#define PACUDA_KFN inline\n"
          "#endif\n"

@@ -4,7 +4,7 @@
 
 #ifndef PACUDA_KFN
 #if defined(__HIPCC__)
-#define PACUDA_KFN __host__ __device__ inline
+#define PACUDA_KFN __host__ __device__ __attribute__((always_inline)) inline
 #else
 #define PACUDA_KFN inline
 #endif

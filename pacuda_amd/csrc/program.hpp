@@ -144,7 +144,10 @@ PCD double as_f64(D x) { return (double)x.hi + (double)x.lo; }
 // once) added to the destination component unless the row initialises it -- or stored as it is
 // when the binding is direct and the factor is exactly (1, +0) -- and narrowed to the kernel's
 // current type.
-template <class K, class A> __device__ __noinline__ void symjit_row(const A &a, const Instr &o, int flags) {
+// (inlined, with the gather and scatter loops unrolled to the kernels' limits, so that the in/out
+// arrays have constant indices and stay in registers)
+template <class K, class A> __device__ __attribute__((always_inline)) inline void symjit_row(const A &a, const Instr &o,
+                                                                                           int flags) {
     typedef typename K::real R;
     typedef typename K::den Den;
     const uint32_t *b = a.bind() + o.w[6];
@@ -152,7 +155,12 @@ template <class K, class A> __device__ __noinline__ void symjit_row(const A &a, 
     const uint32_t dst = o.w[3];
     const double fr = as_f64(ldc<R>(a.fre(), o.w[5])), fi = as_f64(ldc<R>(a.fim(), o.w[5]));
     double in[PACUDA_MAX_IN], out[PACUDA_MAX_OUT];
-    for (uint32_t k = 0; k < inputs; k++) {
+#pragma unroll
+    for (uint32_t k = 0; k < PACUDA_MAX_IN; k++) {
+        if (k >= inputs) {
+            in[k] = 0.0;
+            continue;
+        }
         const uint32_t code = cnst(b)[4 + k], which = (code >> 3) & 1, imag = (code >> 4) & 1, index = code >> 5;
         const uint32_t kind = code & 7;
         double v;
@@ -176,7 +184,9 @@ template <class K, class A> __device__ __noinline__ void symjit_row(const A &a, 
     pacuda_kernel(kernel, simd, in, out);
     const bool as_is = direct && __builtin_bit_cast(uint64_t, fr) == __builtin_bit_cast(uint64_t, 1.0) &&
                        __builtin_bit_cast(uint64_t, fi) == 0;
-    for (uint32_t j = 0; j < pairs; j++) {
+#pragma unroll
+    for (uint32_t j = 0; j < PACUDA_MAX_OUT / 2; j++) {
+        if (j >= pairs) break;
         const double re = out[2 * j], im = out[2 * j + 1];
         const uint32_t plane = dst + cnst(b)[4 + inputs + j];
         double sr = re, si = im;
@@ -196,7 +206,10 @@ template <class K, class A> __device__ __noinline__ void symjit_row(const A &a, 
 // One lane runs `count` instructions for its point (runner/recurrence_runner.cu:787-838).
 template <class R, class A> __device__ inline void source_row(const Instr &o, const A &a);
 
-template <class K, class A>
+// SYMJIT: whether the program may hold symjit rows. The symjit row is an out-of-line call with
+// stack arrays, so kernels instantiated with it allocate registers worse everywhere; programs
+// without symjit rows run an instance without the branch.
+template <class K, bool SYMJIT = false, class A>
 __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count, const A &a) {
     typedef typename K::real R;
     typedef typename K::amp Amp;
@@ -293,7 +306,7 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
                 }
             }
         } else if (kind == K_SYMJIT) {
-            symjit_row<K>(a, o, flags);
+            if constexpr (SYMJIT) symjit_row<K>(a, o, flags);
         } else if (kind == K_SOURCE) {  // a source row made again (programs of the f64 kernel only)
             if constexpr (__is_same(R, double) && __is_same(typename K::den, double)) source_row<R>(o, a);
         } else if (kind == K_CLEAR) {

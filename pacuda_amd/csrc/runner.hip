@@ -68,12 +68,12 @@ __global__ void __launch_bounds__(256) sources_kernel(const Instr *__restrict__ 
     run_sources_point<R>(prog, count, PlaneArena{a, i});
 }
 
-template <class K>
+template <class K, bool SYMJIT>
 __global__ void __launch_bounds__(1024) program_kernel(const Instr *__restrict__ prog, uint32_t count, Arena a,
                                                        uint32_t points) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= points) return;
-    interpret<K>(prog, count, PlaneArena{a, i});
+    interpret<K, SYMJIT>(prog, count, PlaneArena{a, i});
 }
 
 // in-place f64 <-> df64 (runner/recurrence_runner.cu:840-880)
@@ -321,6 +321,7 @@ struct DeviceProgram {
     DeviceArray<uint32_t> bindings;
     uint32_t count = 0, sources = 0;
     uint64_t need[N_COUNT] = {0, 0, 0, 0, 0};
+    bool symjit = false;  // the program holds symjit rows
 };
 
 // Programs by words key (0: not kept -- the next uncached call replaces it).
@@ -339,6 +340,7 @@ class ProgramCache {
         std::unique_ptr<DeviceProgram> p(new DeviceProgram);
         p->count = (uint32_t)cp.ops.size();
         p->sources = cp.sources;
+        p->symjit = cp.symjit;
         std::copy(cp.need, cp.need + N_COUNT, p->need);
         // synchronous copies: the host vectors end with this function
         if (!p->ops.fit(cp.ops.size(), "program") || !p->bindings.fit(cp.bindings.size(), "bindings") ||
@@ -618,7 +620,8 @@ class RunPlan {
         const Instr *ops = program_->ops.data() + program_->sources;
         const uint32_t count = program_->count - program_->sources;
         const unsigned block = r_.config.block, grid = (c_.points + block - 1) / block;
-        program_kernel<K><<<grid, block, 0, s_>>>(ops, count, arena_, c_.points);
+        if (program_->symjit) program_kernel<K, true><<<grid, block, 0, s_>>>(ops, count, arena_, c_.points);
+        else program_kernel<K, false><<<grid, block, 0, s_>>>(ops, count, arena_, c_.points);
         return !hip_failed(hipGetLastError(), "launch");
     }
     template <class R> bool launch_sources() {
