@@ -56,6 +56,27 @@ def points_on_device(dev, plan, n, seed):
     return out, int(idx[n - 1]) + 1
 
 
+def points_from_record_stream(dev, lib, plan, n, seed):
+    """n cut-passing points of the reference's own RAMBO stream (convert/src/record.rs:17-52, from
+    state phase_space.record_state(seed)), generated on the host by the library and uploaded in the
+    runner's layout"""
+    import ctypes as C
+    mom = np.zeros((n, plan.legs, 4))
+    drawn = C.c_uint64()
+    st = lib.pacuda_rambo_record(ps.record_state(seed), plan.legs, n, ps.ECM, 20.0, 2.5, 0.4,
+                                 mom.ctypes.data_as(abi.f64p), C.byref(drawn))
+    if st != 0:
+        raise RuntimeError(f"pacuda_rambo_record: status {st}")
+    return dev.array(ps.external_planes(mom, plan.crossing)), int(drawn.value)
+
+
+def host_threads():
+    """the CPUs this process may run on, within OMP_NUM_THREADS when that is set"""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(omp))) if omp.isdigit() and int(omp) > 0 else max(1, n)
+
+
 def timed_steps(lib, jobs, kernel, steps, waves_per_cu=0, block=0):
     """wall seconds of `steps` launches (synchronised) and the mean kernel ms (HIP events)"""
     ms = []
@@ -66,7 +87,7 @@ def timed_steps(lib, jobs, kernel, steps, waves_per_cu=0, block=0):
     return time.perf_counter() - t, float(np.mean(ms))
 
 
-def cpu_baseline(plan, ext_host, gpu_total, sample, runs=3):
+def cpu_baseline(plan, ext_host, gpu_total, sample, runs=5):
     """the oracle's OpenMP plan interpreter on the host cores: median of `runs` timed runs over
     `sample` points (all threads) and over 2^14 points (one thread), after a warm-up"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -74,7 +95,7 @@ def cpu_baseline(plan, ext_host, gpu_total, sample, runs=3):
     counts = dict(current_planes=plan.current_planes, amplitude_planes=plan.amplitude_planes,
                   form_count=len(plan.forms), external_count=plan.legs)
     arrays = plan.arrays()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     refcpu.evaluate(arrays, counts, ext_host[:, :4096].copy(), 4096, kernel=1, threads=threads, resolved=False)
     times = []
     for _ in range(runs):
@@ -161,7 +182,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--points", type=int, default=1 << 22)
     ap.add_argument("--waves-per-cu", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 21)
+    ap.add_argument("--stream", choices=("counter", "record"), default="counter",
+                    help="headline inputs: the GPU's counter-based RAMBO stream, or the reference's own "
+                         "xorshift stream (convert/src/record.rs) generated on the host")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 22)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-modes", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
@@ -187,8 +211,13 @@ def main():
     plan = plans.gluon_plan(5)
     n = args.points
     t0 = time.perf_counter()
-    d_ext, drawn = points_on_device(dev, plan, n, seed=2)
-    log(f"{n} points generated on the GPU in {time.perf_counter() - t0:.2f} s ({drawn} drawn)")
+    if args.stream == "record":
+        d_ext, drawn = points_from_record_stream(dev, lib, plan, n, seed=2)
+        log(f"{n} points of the record.rs stream generated on the host in {time.perf_counter() - t0:.2f} s "
+            f"({drawn} drawn)")
+    else:
+        d_ext, drawn = points_on_device(dev, plan, n, seed=2)
+        log(f"{n} points generated on the GPU in {time.perf_counter() - t0:.2f} s ({drawn} drawn)")
     fp = FusedPlan(lib, plan)
     d_tot = dev.empty(n)
     flops, divs = plans.plan_flops(plan)
@@ -330,7 +359,8 @@ def main():
     line = {
         "metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (RAMBO on the GPU, seed 2)",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": ("synthetic (RAMBO, the record.rs xorshift stream on the host, seed 2)" if args.stream == "record"
+                 else "synthetic (RAMBO on the GPU, counter-based stream, seed 2)"),
         "config": {"workload": f"g g > g g g LC plan run (one colour ordering, {len(plan.outputs)} helicities summed), "
                                f"f64, {n} RAMBO points, sqrt(s)=1 TeV, pT>20 GeV, |eta|<2.5, dR>0.4",
                    "points": n, "plan_rows": len(plan.rows), "current_planes": plan.current_planes,

@@ -201,6 +201,16 @@ int pacuda_rambo(uint64_t seed, uint64_t first, uint32_t points, uint32_t legs, 
                  const uint32_t *crossing_src, const double *crossing_sign, double *external, uint32_t stride,
                  float *features, uint8_t *pass, double pt_min, double eta_max, double dr_min, double *log_prior,
                  const uint32_t *index);
+
+/*
+ * The reference's RAMBO stream (convert/src/record.rs:17-52), on the host: xorshift64 from
+ * `state` (nonzero; record.rs takes seed | 1), points of `legs` massless partons at sqrt(s) = ecm
+ * as [point][leg][E, px, py, pz] (beams first), drawn until `points` pass the cuts (pT > pt_min,
+ * |eta| < eta_max, Delta R > dr_min between outgoing partons; pt_min <= 0: no cuts); `drawn`
+ * (may be NULL) gets the number drawn. 0 OK, 2 invalid.
+ */
+int pacuda_rambo_record(uint64_t state, uint32_t legs, uint32_t points, double ecm, double pt_min, double eta_max,
+                        double dr_min, double *momenta, uint64_t *drawn);
 /* out[r * count + j] = external[r * stride + index[j]] for rows r < rows; features likewise
  * ([j][width] <- [index[j]][width]) when features is not NULL (device pointers) */
 int pacuda_gather(const double *external, uint32_t stride, uint32_t rows, const uint32_t *index, uint32_t count,

@@ -133,6 +133,8 @@ EXTENSION_SYMBOLS = [
     ("pacuda_rambo", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, u32p, f64p, C.c_void_p,
                                C.c_uint32, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_void_p,
                                C.c_void_p]),
+    ("pacuda_rambo_record", C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, f64p, C.POINTER(C.c_uint64)]),
     ("pacuda_gather", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                 C.c_uint32, C.c_void_p]),
     ("pacuda_surrogate_weight", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double, C.c_double,

@@ -15,6 +15,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "pacman_recurrence.h"
 #include "pacuda_amd.h"
 #include "runtime.hpp"
@@ -209,4 +211,87 @@ extern "C" int pacuda_gather(const double *external, uint32_t stride, uint32_t r
     gather_kernel<<<(count + 255) / 256, 256, 0, runner_stream()>>>(external, stride, rows, index, count, out);
     if (features) gather_features<<<(count + 255) / 256, 256, 0, runner_stream()>>>(features, width, index, count, features_out);
     return hip_failed(hipGetLastError(), "gather") ? PACMAN_RECURRENCE_DEVICE_ERROR : PACMAN_RECURRENCE_OK;
+}
+
+// ---- the reference's own stream, on the host -------------------------------------------------
+//
+// convert/src/record.rs:17-52: one xorshift64 (13, 7, 17) stream, uniform u = ((s >> 11) + 0.5) /
+// 2^53; per outgoing parton c = 2u - 1, s = sqrt(1 - c c), f = 2 pi u, e = -ln(u u) (the two
+// uniforms in that order), then the boost and scale to sqrt(s) = ecm, beams (ecm/2, 0, 0, +-ecm/2).
+// Points that fail the cuts (pT, |eta|, Delta R of the outgoing partons, when pt_min > 0) are
+// drawn and dropped; `drawn` counts every point drawn up to the last one kept. The generation is
+// sequential, so it runs on the host; the points are then the same bits everywhere.
+namespace {
+
+bool passes(const double *p, uint32_t legs, double pt_min, double eta_max, double dr_min) {
+    double eta[MAX_LEGS], phi[MAX_LEGS];
+    for (uint32_t l = 2; l < legs; l++) {
+        const double *q = p + 4 * l;
+        const double pt = hypot(q[1], q[2]), mag = sqrt(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        eta[l] = 0.5 * log((mag + q[3]) / (mag - q[3]));
+        phi[l] = atan2(q[2], q[1]);
+        if (!(pt > pt_min) || !(fabs(eta[l]) < eta_max)) return false;
+    }
+    for (uint32_t a = 2; a < legs; a++)
+        for (uint32_t b = a + 1; b < legs; b++) {
+            double dphi = fabs(phi[a] - phi[b]);
+            if (dphi > M_PI) dphi = 2 * M_PI - dphi;
+            const double deta = eta[a] - eta[b];
+            if (!(deta * deta + dphi * dphi > dr_min * dr_min)) return false;
+        }
+    return true;
+}
+
+}  // namespace
+
+extern "C" int pacuda_rambo_record(uint64_t state, uint32_t legs, uint32_t points, double ecm, double pt_min,
+                                   double eta_max, double dr_min, double *momenta, uint64_t *drawn) {
+    if (legs < 3 || legs > MAX_LEGS || state == 0 || (points && !momenta)) return PACMAN_RECURRENCE_INVALID;
+    auto uniform = [&state]() {
+        state ^= state << 13;
+        state ^= state >> 7;
+        state ^= state << 17;
+        return ((double)(state >> 11) + 0.5) / 9007199254740992.0;
+    };
+    uint64_t count = 0;
+    std::vector<double> point(4 * legs);
+    for (uint32_t kept = 0; kept < points;) {
+        double q[MAX_LEGS][4];
+        const uint32_t m = legs - 2;
+        for (uint32_t k = 0; k < m; k++) {
+            const double c = 2.0 * uniform() - 1.0;
+            const double s = sqrt(1.0 - c * c);
+            const double f = 2.0 * M_PI * uniform();
+            const double u1 = uniform();
+            const double u2 = uniform();
+            const double e = -log(u1 * u2);
+            q[k][0] = e;
+            q[k][1] = e * s * cos(f);
+            q[k][2] = e * s * sin(f);
+            q[k][3] = e * c;
+        }
+        double total[4];
+        for (int k = 0; k < 4; k++) {
+            total[k] = 0.0;
+            for (uint32_t j = 0; j < m; j++) total[k] = total[k] + q[j][k];
+        }
+        const double mass = sqrt(total[0] * total[0] - total[1] * total[1] - total[2] * total[2] - total[3] * total[3]);
+        const double b[3] = {-total[1] / mass, -total[2] / mass, -total[3] / mass};
+        const double x = ecm / mass, g = total[0] / mass, a = 1.0 / (1.0 + g);
+        double *p = point.data();
+        p[0] = ecm / 2; p[1] = 0.0; p[2] = 0.0; p[3] = ecm / 2;
+        p[4] = ecm / 2; p[5] = 0.0; p[6] = 0.0; p[7] = -ecm / 2;
+        for (uint32_t j = 0; j < m; j++) {
+            const double bq = b[0] * q[j][1] + b[1] * q[j][2] + b[2] * q[j][3];
+            double *o = p + 4 * (j + 2);
+            o[0] = x * (g * q[j][0] + bq);
+            for (int k = 0; k < 3; k++) o[k + 1] = x * (q[j][k + 1] + b[k] * (q[j][0] + a * bq));
+        }
+        count++;
+        if (pt_min > 0 && !passes(p, legs, pt_min, eta_max, dr_min)) continue;
+        std::copy(point.begin(), point.end(), momenta + (size_t)kept * 4 * legs);
+        kept++;
+    }
+    if (drawn) *drawn = count;
+    return PACMAN_RECURRENCE_OK;
 }

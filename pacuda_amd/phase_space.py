@@ -95,9 +95,46 @@ def rambo(n, points, seed, offset=0, ecm=ECM):
     return _rambo_from(u, n, ecm)
 
 
-def rambo_xorshift(n, points, seed, ecm=ECM):
-    """convert/src/record.rs:17-52 restated: the reference's own generator, point by point."""
-    state = (seed | 1) & 0xFFFFFFFFFFFFFFFF
+def record_state(seed):
+    """the xorshift state a seed starts: record.rs takes seed | 1 (so seeds 2 and 3 would share a
+    stream); here the seed goes through the splitmix64 finaliser first, then | 1"""
+    z = (seed + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return (z ^ (z >> 31)) | 1
+
+
+def rambo_at(n, seed, ids, ecm=ECM):
+    """the RAMBO points number `ids` of the counter-based stream `seed` (as rambo() draws them)"""
+    ids = np.asarray(ids, dtype=np.uint64)
+    per = np.uint64(4 * (n - 2))
+    pos = ids[:, None] * per + np.arange(int(per), dtype=np.uint64)[None, :]
+    u = uniform_at(seed, pos.reshape(-1)).reshape(len(ids), n - 2, 4)
+    return _rambo_from(u, n, ecm)
+
+
+def min_pt_and_dr(mom):
+    """per point: the smallest pT and the smallest Delta R among the outgoing partons"""
+    out = mom[:, 2:]
+    px, py, pz = out[..., 1], out[..., 2], out[..., 3]
+    pt = np.hypot(px, py)
+    p = np.sqrt(px * px + py * py + pz * pz)
+    eta = 0.5 * np.log((p + pz) / (p - pz))
+    phi = np.arctan2(py, px)
+    dr = np.full(len(mom), np.inf)
+    m = out.shape[1]
+    for i in range(m):
+        for j in range(i + 1, m):
+            dphi = np.abs(phi[:, i] - phi[:, j])
+            dphi = np.where(dphi > math.pi, 2 * math.pi - dphi, dphi)
+            dr = np.minimum(dr, np.hypot(eta[:, i] - eta[:, j], dphi))
+    return pt.min(axis=1), dr
+
+
+def rambo_xorshift(n, points, seed, ecm=ECM, state=None):
+    """convert/src/record.rs:17-52 restated: the reference's own generator, point by point, from
+    state seed | 1 (record.rs) or the given state."""
+    state = (seed | 1) & 0xFFFFFFFFFFFFFFFF if state is None else state
     mask = 0xFFFFFFFFFFFFFFFF
 
     def uniform():

@@ -266,8 +266,10 @@ class Unweighting:
         clock[key] = clock.get(key, 0.0) + time.perf_counter() - t
         return out
 
-    def plain(self, candidates, seed=4, chunk=1 << 24, q=0.99999, accept_seed=21):
-        """plain unweighting of `candidates` points: dict of counts, times and cross sections"""
+    def plain(self, candidates, seed=4, chunk=1 << 24, q=0.99999, accept_seed=21, keep_events=False):
+        """plain unweighting of `candidates` points: dict of counts, times and cross sections (with
+        keep_events: the accepted candidates' ids and weights, and every passing candidate's id and
+        exact weight, for distributions)"""
         clock, hist = {}, self.dev.array(np.zeros(BINS, dtype=np.uint32))
         kept, sums = [], [0.0, 0.0, 0.0]  # per chunk (w, ids, n, first); sum w, sum w^2, max w
         passed = 0
@@ -290,16 +292,26 @@ class Unweighting:
         h = hist.download(BINS, np.uint32)
         hist.free()
         w_thr = quantile_threshold(h, q)
-        acc_w = []
+        acc_w, events = [], {"event_ids": [], "all_ids": [], "all_w": []}
         for w, idx, n, start in kept:
             a_idx, a_val, m = self._timed(clock, "accept", self.sel.select, n, w=w, ids=idx, first=start,
                                           threshold=w_thr, seed=accept_seed, values=True)
             acc_w.append(a_val.download(m))
+            if keep_events:
+                pos = a_idx.download(m, np.uint32)
+                all_ids = idx.download(n, np.uint32).astype(np.int64) + start
+                events["event_ids"].append(all_ids[pos])
+                events["all_ids"].append(all_ids)
+                events["all_w"].append(w.download(n))
             for a in (a_idx, a_val, w, idx):
                 a.free()
         acc_w = np.concatenate(acc_w)
         x = w_thr * np.maximum(1.0, acc_w / w_thr)
-        return {"candidates": candidates, "passed_cuts": passed, "exact_evaluations": passed,
+        extra = {}
+        if keep_events:
+            extra = {k: np.concatenate(v) for k, v in events.items()}
+            extra["event_weights"] = x
+        return {**extra, "candidates": candidates, "passed_cuts": passed, "exact_evaluations": passed,
                 "events": len(acc_w), "overweight_events": int(np.sum(acc_w > w_thr)), "threshold": w_thr,
                 "estimate": self._estimate(x, candidates), "weighted": self._weighted(sums, candidates),
                 "seconds": sum(clock.values()), "times_s": clock}
@@ -427,7 +439,8 @@ class Unweighting:
         acc = ps.uniform_at(stage2_seed, ids) * r_thr < r
         clock["stage2"] = clock.get("stage2", 0.0) + time.perf_counter() - t
         x = s_thr * r_thr * np.maximum(1.0, s1[acc] / s_thr) * np.maximum(1.0, r[acc] / r_thr)
-        extra = {"stage1_ids": ids, "stage1_s": s1} if keep_ids else {}
+        extra = {"stage1_ids": ids, "stage1_s": s1, "stage1_w": w1, "event_ids": ids[acc], "event_weights": x} \
+            if keep_ids else {}
         if network:
             extra["envelope_accepted"] = n_env
         return {**extra, "candidates": candidates, "network_evaluations": n_net,
