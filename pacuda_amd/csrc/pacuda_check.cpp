@@ -1,7 +1,15 @@
 // SPDX-License-Identifier: 0BSD
 //
 // pacuda-check FILE.pacuda --runner LIBRARY [--kernel f64|df64|mixed|mixed64] [--tile N]
-//              [--host-reduce] [--dump]
+//              [--host-reduce] [--dump] [--examples EXAMPLES.json]
+// pacuda-check FILE.pacuda --write-examples EXAMPLES.json
+//
+// Reference points come from the recording, or from an examples.json in pac2json's format
+// (convert/src/pac2json.rs:774-804, read back as :1137-1190): {"description": ..., "programs":
+// [{"process": key, "colour": id, "points": [{"momenta": [[E, px, py, pz] per leg, before the
+// crossing], "outputs": [per helicity, group factor applied], "total": their sum}]}]}, one
+// program per (process, colour) in the recording's order; --write-examples writes the
+// recording's points in that format (numbers in their shortest round-trip form).
 //
 // Replays the reference points of every (process, colour) of a .pacuda v2 recording through a
 // runner library's C ABI and compares them with the recorded CPU values, as
@@ -20,6 +28,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <charconv>
+#include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -166,6 +177,257 @@ File read_file(const std::vector<uint8_t> &bytes) {
     return f;
 }
 
+// ---- examples.json ----------------------------------------------------------------------------
+
+// A JSON value: null, bool, number, string, array or object (keys in file order).
+struct Json {
+    enum Kind { NUL, BOOL, NUM, STR, ARR, OBJ } kind = NUL;
+    bool b = false;
+    double num = 0;
+    std::string str;
+    std::vector<Json> items;
+    std::vector<std::pair<std::string, Json>> fields;
+    const Json &at(const std::string &key) const {
+        for (const auto &f : fields)
+            if (f.first == key) return f.second;
+        throw std::runtime_error("examples: no \"" + key + "\"");
+    }
+    const std::vector<Json> &array() const {
+        if (kind != ARR) throw std::runtime_error("examples: an array expected");
+        return items;
+    }
+    double number() const {
+        if (kind == NUL) return NAN;  // (pac2json writes non-finite numbers as null)
+        if (kind != NUM) throw std::runtime_error("examples: a number expected");
+        return num;
+    }
+    const std::string &text() const {
+        if (kind != STR) throw std::runtime_error("examples: a string expected");
+        return str;
+    }
+};
+
+class JsonParser {
+  public:
+    explicit JsonParser(const std::string &t) : t_(t) {}
+    Json document() {
+        Json v = value();
+        space();
+        if (at_ != t_.size()) fail("trailing characters");
+        return v;
+    }
+
+  private:
+    const std::string &t_;
+    size_t at_ = 0;
+    [[noreturn]] void fail(const char *what) {
+        throw std::runtime_error(std::string("examples: ") + what + " at byte " + std::to_string(at_));
+    }
+    void space() {
+        while (at_ < t_.size() && (t_[at_] == ' ' || t_[at_] == '\n' || t_[at_] == '\r' || t_[at_] == '\t')) at_++;
+    }
+    bool eat(char c) {
+        space();
+        if (at_ < t_.size() && t_[at_] == c) {
+            at_++;
+            return true;
+        }
+        return false;
+    }
+    void expect(char c) {
+        if (!eat(c)) fail("unexpected character");
+    }
+    std::string string() {
+        expect('"');
+        std::string out;
+        while (at_ < t_.size() && t_[at_] != '"') {
+            char c = t_[at_++];
+            if (c != '\\') {
+                out.push_back(c);
+                continue;
+            }
+            if (at_ >= t_.size()) fail("a bad escape");
+            c = t_[at_++];
+            switch (c) {
+            case 'n': out.push_back('\n'); break;
+            case 't': out.push_back('\t'); break;
+            case 'r': out.push_back('\r'); break;
+            case 'b': out.push_back('\b'); break;
+            case 'f': out.push_back('\f'); break;
+            case 'u': {
+                if (at_ + 4 > t_.size()) fail("a bad escape");
+                unsigned cp = (unsigned)std::stoul(t_.substr(at_, 4), nullptr, 16);
+                at_ += 4;
+                // UTF-8 (the descriptions hold a few non-ASCII characters; no surrogate pairs)
+                if (cp < 0x80) {
+                    out.push_back((char)cp);
+                } else if (cp < 0x800) {
+                    out.push_back((char)(0xc0 | cp >> 6));
+                    out.push_back((char)(0x80 | (cp & 0x3f)));
+                } else {
+                    out.push_back((char)(0xe0 | cp >> 12));
+                    out.push_back((char)(0x80 | ((cp >> 6) & 0x3f)));
+                    out.push_back((char)(0x80 | (cp & 0x3f)));
+                }
+                break;
+            }
+            default: out.push_back(c); break;
+            }
+        }
+        if (at_ >= t_.size()) fail("an unterminated string");
+        at_++;
+        return out;
+    }
+    Json value() {
+        space();
+        if (at_ >= t_.size()) fail("a value expected");
+        Json v;
+        const char c = t_[at_];
+        if (c == '{') {
+            at_++;
+            v.kind = Json::OBJ;
+            if (eat('}')) return v;
+            do {
+                std::string key = string();
+                expect(':');
+                v.fields.emplace_back(key, value());
+            } while (eat(','));
+            expect('}');
+        } else if (c == '[') {
+            at_++;
+            v.kind = Json::ARR;
+            if (eat(']')) return v;
+            do v.items.push_back(value());
+            while (eat(','));
+            expect(']');
+        } else if (c == '"') {
+            v.kind = Json::STR;
+            v.str = string();
+        } else if (t_.compare(at_, 4, "null") == 0) {
+            at_ += 4;
+        } else if (t_.compare(at_, 4, "true") == 0) {
+            at_ += 4;
+            v.kind = Json::BOOL;
+            v.b = true;
+        } else if (t_.compare(at_, 5, "false") == 0) {
+            at_ += 5;
+            v.kind = Json::BOOL;
+        } else {
+            // strtod rounds correctly: the shortest form a writer prints reads back to its bits
+            char *end = nullptr;
+            v.kind = Json::NUM;
+            v.num = strtod(t_.c_str() + at_, &end);
+            if (end == t_.c_str() + at_) fail("a value expected");
+            at_ = end - t_.c_str();
+        }
+        return v;
+    }
+};
+
+std::string read_text(const char *path) {
+    FILE *fp = fopen(path, "rb");
+    if (!fp) throw std::runtime_error(std::string(path) + ": cannot open");
+    std::string text;
+    char buf[1 << 16];
+    size_t got;
+    while ((got = fread(buf, 1, sizeof buf, fp)) > 0) text.append(buf, got);
+    fclose(fp);
+    return text;
+}
+
+// the recording's reference points replaced by those of an examples.json
+void take_examples(File &f, const Json &doc) {
+    const std::vector<Json> &programs = doc.at("programs").array();
+    size_t n = 0;
+    for (Process &p : f.processes)
+        for (Colour &c : p.colours) {
+            if (n >= programs.size()) throw std::runtime_error("examples: fewer programs than the recording");
+            const Json &e = programs[n++];
+            if (e.at("process").text() != p.key || e.at("colour").text() != c.id)
+                throw std::runtime_error("examples: program " + std::to_string(n - 1) + " is not " + p.key + " " + c.id);
+            c.momenta.clear();
+            c.values.clear();
+            c.points = 0;
+            for (const Json &point : e.at("points").array()) {
+                const std::vector<Json> &legs = point.at("momenta").array();
+                const std::vector<Json> &outputs = point.at("outputs").array();
+                if (legs.size() != p.legs || outputs.size() != p.helicities)
+                    throw std::runtime_error("examples: a point of " + p.key + " " + c.id + " of the wrong size");
+                for (const Json &m : legs) {
+                    if (m.array().size() != 4) throw std::runtime_error("examples: a momentum of the wrong size");
+                    for (const Json &x : m.array()) c.momenta.push_back(x.number());
+                }
+                for (const Json &x : outputs) c.values.push_back(x.number());
+                c.points++;
+            }
+        }
+    if (n != programs.size()) throw std::runtime_error("examples: more programs than the recording");
+}
+
+std::string json_number(double x) {
+    if (!std::isfinite(x)) return "null";
+    char buf[64];
+    const auto r = std::to_chars(buf, buf + sizeof buf, x);  // the shortest form that reads back to x
+    std::string s(buf, r.ptr);
+    if (s.find_first_of(".en") == std::string::npos) s += ".0";  // (a float, as pac2json prints one)
+    return s;
+}
+
+std::string json_string(const std::string &s) {
+    std::string out = "\"";
+    for (unsigned char c : s) {
+        if (c == '"' || c == '\\') {
+            out.push_back('\\');
+            out.push_back((char)c);
+        } else if (c < 0x20) {
+            char buf[8];
+            snprintf(buf, sizeof buf, "\\u%04x", c);
+            out += buf;
+        } else {
+            out.push_back((char)c);
+        }
+    }
+    return out + "\"";
+}
+
+void write_examples(const File &f, const char *path) {
+    std::string out = "{\"description\": " +
+                      json_string("per program, phase-space points and the CPU's outputs (f64): momenta[leg] = [E, px, py, "
+                                  "pz], the caller's legs before the crossing; outputs[h] per helicity, group factor "
+                                  "applied; total = the sum of outputs (in order)") +
+                      ",\n \"programs\": [";
+    bool first_program = true;
+    for (const Process &p : f.processes)
+        for (const Colour &c : p.colours) {
+            out += first_program ? "\n  " : ",\n  ";
+            first_program = false;
+            out += "{\"process\": " + json_string(p.key) + ", \"colour\": " + json_string(c.id) + ", \"points\": [";
+            for (uint32_t q = 0; q < c.points; q++) {
+                out += q ? ",\n    " : "\n    ";
+                out += "{\"momenta\": [";
+                for (uint32_t leg = 0; leg < p.legs; leg++) {
+                    out += leg ? ", [" : "[";
+                    for (int k = 0; k < 4; k++)
+                        out += (k ? ", " : "") + json_number(c.momenta[((size_t)q * p.legs + leg) * 4 + k]);
+                    out += "]";
+                }
+                out += "], \"outputs\": [";
+                double total = 0.0;
+                for (uint32_t h = 0; h < p.helicities; h++) {
+                    const double v = c.values[(size_t)q * p.helicities + h];
+                    out += (h ? ", " : "") + json_number(v);
+                    total = total + v;
+                }
+                out += "], \"total\": " + json_number(total) + "}";
+            }
+            out += "]}";
+        }
+    out += "\n]}\n";
+    FILE *fp = fopen(path, "wb");
+    if (!fp || fwrite(out.data(), 1, out.size(), fp) != out.size()) throw std::runtime_error(std::string(path) + ": cannot write");
+    fclose(fp);
+}
+
 struct Runner {
     int (*run)(const pacman_recurrence_call *);
     int (*reduce_resolved)(const pacman_recurrence_reduce_resolved *);
@@ -276,7 +538,7 @@ std::vector<double> evaluate(const Runner &rn, const std::vector<uint32_t> &map,
 }  // namespace
 
 int main(int argc, char **argv) {
-    const char *file = nullptr, *runner = nullptr, *kernel = "f64";
+    const char *file = nullptr, *runner = nullptr, *kernel = "f64", *examples = nullptr, *write = nullptr;
     size_t tile = 0;
     bool host_reduce = false, dump = false;
     for (int k = 1; k < argc; k++) {
@@ -286,15 +548,18 @@ int main(int argc, char **argv) {
         else if (a == "--tile" && k + 1 < argc) tile = strtoul(argv[++k], nullptr, 10);
         else if (a == "--host-reduce") host_reduce = true;
         else if (a == "--dump") dump = true;
+        else if (a == "--examples" && k + 1 < argc) examples = argv[++k];
+        else if (a == "--write-examples" && k + 1 < argc) write = argv[++k];
         else if (!file) file = argv[k];
         else {
             fprintf(stderr, "pacuda-check: unknown argument %s\n", argv[k]);
             return 2;
         }
     }
-    if (!file || (!runner && !dump)) {
+    if (!file || (!runner && !dump && !write)) {
         fprintf(stderr, "usage: pacuda-check FILE.pacuda --runner LIBRARY [--kernel f64|df64|mixed|mixed64] [--tile N] "
-                        "[--host-reduce] [--dump]\n");
+                        "[--host-reduce] [--dump] [--examples EXAMPLES.json]\n"
+                        "       pacuda-check FILE.pacuda --write-examples EXAMPLES.json\n");
         return 2;
     }
     try {
@@ -306,6 +571,14 @@ int main(int argc, char **argv) {
         while ((got = fread(buf, 1, sizeof buf, fp)) > 0) bytes.insert(bytes.end(), buf, buf + got);
         fclose(fp);
         File f = read_file(bytes);
+        if (write) {
+            write_examples(f, write);
+            if (!runner && !dump) return 0;
+        }
+        if (examples) {
+            const std::string text = read_text(examples);
+            take_examples(f, JsonParser(text).document());
+        }
         if (dump) {
             printf("kernel %u batch %u symjit %zu processes %zu\n", f.kernel, f.batch, f.sha256.size(), f.processes.size());
             for (const Process &p : f.processes)

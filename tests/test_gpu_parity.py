@@ -167,10 +167,15 @@ def test_v1_other_kernels_in_fresh_processes(gg_ggg, tmp_path):
 
 def test_recording_replays_bit_exact_on_gpu():
     """tests/golden/gg_ggg_qqbar.pacuda through pacuda-check on the HIP library: every f64 value
-    equals the recorded CPU value (device and host reductions)"""
+    equals the recorded CPU value (device and host reductions, and the reference points read
+    back from an examples.json the tool writes)"""
     from pacuda_amd import build
     golden = os.path.join(os.path.dirname(__file__), "golden", "gg_ggg_qqbar.pacuda")
-    for args in [(), ("--host-reduce", "--tile", "3")]:
+    examples = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"examples_{os.getpid()}.json")
+    p = subprocess.run([build.CHECK_TOOL, golden, "--write-examples", examples], capture_output=True, text=True,
+                       timeout=60)
+    assert p.returncode == 0, p.stderr
+    for args in [(), ("--host-reduce", "--tile", "3"), ("--examples", examples)]:
         print("pacuda-check", args, flush=True)
         p = subprocess.run([build.CHECK_TOOL, golden, "--runner", abi.LIBRARY, *args], capture_output=True, text=True,
                            timeout=120)

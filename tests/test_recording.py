@@ -60,3 +60,48 @@ def test_malformed_files(tmp_path):
         assert code == 1, name
         with pytest.raises(ValueError):
             pacuda_file.read(data)
+
+
+def test_examples_json_round_trip(tmp_path):
+    """pacuda-check writes the recording's reference points as pac2json's examples.json
+    (convert/src/pac2json.rs:774-804) and replays from such a file: every number reads back to
+    its bits, the replay through the oracle agrees bit for bit, a value changed in its last bit is
+    found, and a file whose programs do not match the recording's is refused"""
+    import json
+    ex = tmp_path / "examples.json"
+    p = subprocess.run([build.CHECK_TOOL, GOLDEN, "--write-examples", str(ex)], capture_output=True, text=True,
+                       timeout=60)
+    assert p.returncode == 0, p.stderr
+    doc = json.loads(ex.read_text())
+    f = pacuda_file.read(open(GOLDEN, "rb").read())
+    programs = doc["programs"]
+    assert [(e["process"], e["colour"]) for e in programs] == \
+        [(pr["key"], c["id"]) for pr in f["processes"] for c in pr["colours"]]
+    n = 0
+    for pr in f["processes"]:
+        for c in pr["colours"]:
+            e = programs[n]
+            n += 1
+            mom = np.array([pt["momenta"] for pt in e["points"]], dtype=np.float64).reshape(-1)
+            val = np.array([pt["outputs"] for pt in e["points"]], dtype=np.float64).reshape(-1)
+            assert np.array_equal(mom.view(np.uint64), np.asarray(c["momenta"], dtype=np.float64).view(np.uint64))
+            assert np.array_equal(val.view(np.uint64), np.asarray(c["values"], dtype=np.float64).view(np.uint64))
+            for pt in e["points"]:
+                total = 0.0
+                for v in pt["outputs"]:
+                    total = total + v
+                assert pt["total"] == total
+    code, out = check(GOLDEN, refcpu.ABI_LIBRARY, "--examples", str(ex))
+    assert code == 0 and out.count("0 of 256 values differ") == 1 and out.count("0 of 128 values differ") == 1, out
+    # one output in its last bit
+    doc["programs"][0]["points"][3]["outputs"][5] = float(np.nextafter(doc["programs"][0]["points"][3]["outputs"][5],
+                                                                     np.inf))
+    bad = tmp_path / "bad.json"
+    bad.write_text(json.dumps(doc))
+    code, out = check(GOLDEN, refcpu.ABI_LIBRARY, "--examples", str(bad))
+    assert code == 1 and "1 of 256 values differ" in out
+    # the programs must be the recording's, in order
+    doc["programs"][0]["colour"] = "2"
+    bad.write_text(json.dumps(doc))
+    code, out = check(GOLDEN, refcpu.ABI_LIBRARY, "--examples", str(bad))
+    assert code == 1 and "is not g g > g g g 1" in out
