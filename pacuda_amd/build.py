@@ -38,16 +38,33 @@ def hipcc():
     return "hipcc"
 
 
-def build_library(verbose=False):
+GEN_MARK = "// Generated by pacuda-gen from plane-kernel files"
+
+
+def build_library(verbose=False, kernels_header=None, out=None):
+    """librecurrence.so; `kernels_header`: a pacuda-gen header to build in instead of the one the
+    build generates from pacuda_amd/kernels (a directory holding it as plane_kernels.hpp goes
+    first on the include path)"""
+    import shutil
+    import tempfile
     os.makedirs(LIB_DIR, exist_ok=True)
-    tmp = LIBRARY + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
+    target = out or LIBRARY
+    tmp = target + ".tmp"
+    include = []
+    if kernels_header:
+        with open(kernels_header) as f:
+            if not f.readline().startswith(GEN_MARK):
+                raise ValueError(f"{kernels_header}: not a pacuda-gen HIP header (a CUDA one cannot be built for gfx950)")
+        kdir = tempfile.mkdtemp(prefix="pacuda_kernels_")
+        shutil.copy(kernels_header, os.path.join(kdir, "plane_kernels.hpp"))
+        include = ["-I", kdir]
+    cmd = [hipcc(), *HIPCC_FLAGS, *include, "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
            *[os.path.join(HERE, "csrc", s) for s in SOURCES], "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIBRARY)
-    return LIBRARY
+    os.replace(tmp, target)
+    return target
 
 
 CHECK_TOOL = os.path.join(LIB_DIR, "pacuda-check")

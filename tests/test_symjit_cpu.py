@@ -170,3 +170,12 @@ def test_invalid_bindings_are_refused(case):
     a = plan.arrays()
     with pytest.raises(RuntimeError):
         refcpu.evaluate(a, harness.plan_counts(plan), np.zeros((12, 1)), 1, kernel=1)
+
+
+def test_build_refuses_a_cuda_kernel_header(tmp_path):
+    """pacuda_amd/build.sh's PACUDA_KERNELS takes pacuda-gen's HIP headers only; the reference's
+    generator writes CUDA device code, which does not build for gfx950"""
+    h = tmp_path / "kernels.cuh"
+    h.write_text("// Generated by pacuda-gen from pyAmpliCol's symjit plane kernels.\n")
+    with pytest.raises(ValueError):
+        build.build_library(kernels_header=str(h), out=str(tmp_path / "lib.so"))
