@@ -9,9 +9,9 @@ sum) of every point, momenta already in HBM, |M|^2 per point back into HBM.
 
 The JSON line also carries: the roofline of the fused f64 kernel (its time from HIP events the
 library records on the kernel's stream); the CPU oracle (the same plan interpreter, OpenMP)
-timed on a bounded sample of the same points; df64/mixed/mixed64 throughput and error against
-f64; g g > g g g g (2^21 points, f64 and mixed, the persistent grid tuned by a sweep); the
-p p > j j j batch (every subprocess with at most one quark line in one persistent launch, Zipf
+timed on a bounded sample of the same points; df64/mixed/mixed64/compensated(64) throughput and error
+against f64; g g > g g g g (2^21 points, every mode, the persistent grid tuned by a sweep); the
+p p > j j j batch (all 181 subprocesses, four-quark ones included, in one persistent launch, Zipf
 point counts summing to 2^22); and surrogate vs plain unweighting of g g > g g g g."""
 import argparse
 import itertools
@@ -233,8 +233,8 @@ def main():
 
     modes = {}
     if not args.no_modes:
-        for name in ("df64", "mixed", "mixed64"):
-            k = abi.KERNELS[name]
+        for name in ("df64", "mixed", "mixed64", "compensated", "compensated64"):
+            k = abi.FUSED_KERNELS[name]
             evaluate(lib, jobs, k, timed=True)
             _, ms = timed_steps(lib, jobs, k, 3)
             modes[name] = {"points_per_s": n / (ms * 1e-3), "kernel_ms": ms, **mode_errors(d_tot.download(n), total64)}
@@ -281,8 +281,8 @@ def main():
         res6 = {"points": n6, "plan_rows": len(p6.rows), "flops_per_point": f6flops, "tuned": best, "sweep_ms": sweep,
                 "f64_points_per_s": n6 / (ms64 * 1e-3),
                 "f64_tflops": f6flops * n6 / (ms64 * 1e-3) / 1e12}
-        for name in ("mixed", "mixed64", "df64"):
-            k = abi.KERNELS[name]
+        for name in ("mixed", "mixed64", "compensated", "compensated64", "df64"):
+            k = abi.FUSED_KERNELS[name]
             (kw, kb), _ = autotune(lib, j6, k)
             _, ms = timed_steps(lib, j6, k, 3, kw, kb)
             res6[name] = {"points_per_s": n6 / (ms * 1e-3), "tuned": f"{kw}x{kb}",
@@ -308,13 +308,14 @@ def main():
         refs = [j[5].download(j[2]) for j in pjobs]
         pp = {"subprocesses": len(subs), "points": int(counts.sum()), "f64_points_per_s": int(counts.sum()) / (ms_p * 1e-3),
               "counts_min_max": [int(counts.min()), int(counts.max())]}
-        for name in ("mixed", "mixed64"):
-            k = abi.KERNELS[name]
+        for name in ("mixed", "mixed64", "compensated", "compensated64"):
+            k = abi.FUSED_KERNELS[name]
             evaluate(lib, pjobs, k, timed=True)
             _, ms = timed_steps(lib, pjobs, k, 3)
-            errs = {sp.name: mode_errors(j[5].download(j[2]), r)["max_err_rel_to_max_f64"]
-                    for sp, j, r in zip(subs, pjobs, refs)}
+            per = {sp.name: mode_errors(j[5].download(j[2]), r) for sp, j, r in zip(subs, pjobs, refs)}
+            errs = {n: e["max_err_rel_to_max_f64"] for n, e in per.items()}
             pp[name] = {"points_per_s": int(counts.sum()) / (ms * 1e-3), "worst_subprocess_err": max(errs.values()),
+                        "worst_subprocess_median_rel_err": max(e["median_rel_err_vs_f64"] for e in per.values()),
                         "err_rel_to_max_f64_per_subprocess": errs}
         extra["pp_jjj"] = pp
         log(f"pp>jjj f64 {pp['f64_points_per_s']:.4g} points/s over {len(subs)} subprocesses")

@@ -105,11 +105,27 @@ typedef struct {
     double *total;
 } pacuda_item;
 
+/*
+ * The compensated mixed policy of the fused entry points (no counterpart in the reference; the
+ * v1 configure call keeps the reference's four): f32 currents, df64 denominators and amplitude
+ * sums as PACMAN_RECURRENCE_KERNEL_MIXED, and the arithmetic of every row (vertices, the sum of a
+ * destination's contributions, propagators, identity rows, the closures' products) in df64 from
+ * the f32 values, so that a current is rounded to f32 once, when it is stored.
+ */
+#define PACUDA_KERNEL_COMPENSATED 4u
+/*
+ * The same with f64 arithmetic (no counterpart in the reference): f32 currents, each rounded to
+ * f32 once when stored, and the arithmetic of every row, the forms, constants, denominators and
+ * amplitude sums in f64 (native on gfx950, where df64 takes about ten f32 operations for one).
+ * Equal to PACUDA_KERNEL_COMPENSATED to df64's accuracy.
+ */
+#define PACUDA_KERNEL_COMPENSATED64 5u
+
 typedef struct {
     uint32_t abi_version;   /* PACMAN_RECURRENCE_ABI */
     uint32_t struct_size;   /* sizeof(pacuda_launch) */
-    uint32_t kernel;        /* PACMAN_RECURRENCE_KERNEL_* */
-    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 12 (f64), 16 (df64, mixed) */
+    uint32_t kernel;        /* PACMAN_RECURRENCE_KERNEL_* or PACUDA_KERNEL_COMPENSATED(64) */
+    uint32_t waves_per_cu;  /* persistent wavefronts per compute unit, 0: 12 (f64), 16 (the others) */
     uint32_t block;         /* threads per workgroup, a multiple of 64 up to 256, 0: 256 */
     uint32_t reserved;
     void *stream;           /* hipStream_t to queue on, or NULL: the library's stream */

@@ -155,11 +155,22 @@ struct Arena {
     size_t ld;
 };
 
-// the precision policies (runner/recurrence_runner.cu:178-190)
-struct KF64 { typedef double real; typedef double den; typedef double amp; };
-struct KDF64 { typedef D real; typedef D den; typedef D amp; };
-struct KMixed { typedef float real; typedef D den; typedef D amp; };
-struct KMixed64 { typedef float real; typedef double den; typedef D amp; };
+// the precision policies (runner/recurrence_runner.cu:178-190); acc: the vertex arithmetic and
+// the accumulation of a destination's contributions, the current type R in the reference's four
+struct KF64 { typedef double real; typedef double den; typedef double amp; typedef double acc; };
+struct KDF64 { typedef D real; typedef D den; typedef D amp; typedef D acc; };
+struct KMixed { typedef float real; typedef D den; typedef D amp; typedef float acc; };
+struct KMixed64 { typedef float real; typedef double den; typedef D amp; typedef float acc; };
+// the library's compensated mixed policy (no counterpart in the reference): KMixed with the
+// arithmetic of every row (vertices and the sum of a destination's contributions, propagators,
+// identity rows, the closures' products) in df64 from the f32 currents, the df64 forms and the
+// df64 constants; a current is rounded to f32 (fl(hi + lo)) once, when it is stored, and a
+// forwarded one as if it had been stored
+struct KComp { typedef float real; typedef D den; typedef D amp; typedef D acc; };
+// the library's compensated policy with f64 arithmetic (no counterpart in the reference): the
+// same f32 currents and roundings with every other value and operation in f64, which KComp's
+// df64 follows to about 2^-44 per operation
+struct KComp64 { typedef float real; typedef double den; typedef double amp; typedef double acc; };
 
 inline float to_float(float x) { return x; }
 inline float to_float(double x) { return (float)x; }
@@ -175,6 +186,12 @@ template <class S> struct ToReal<float, S> { static float of(S x) { return to_fl
 inline D widen(float x) { return D(x, 0.0f); }
 inline double widen(double x) { return x; }
 inline D widen(D x) { return x; }
+
+// R <-> acc: the exact widening of an f32 to df64, and a df64 rounded to the nearest f32
+template <class T, class S> struct Conv { static T of(S x) { return x; } };
+template <> struct Conv<D, float> { static D of(float x) { return D(x, 0.0f); } };
+template <> struct Conv<float, D> { static float of(D x) { return x.hi + x.lo; } };
+template <class T, class S> inline Cx<T> cvt(Cx<S> c) { return cx(Conv<T, S>::of(c.re), Conv<T, S>::of(c.im)); }
 
 template <class R> inline R narrow(double x);
 template <> inline double narrow<double>(double x) { return x; }
@@ -704,13 +721,15 @@ inline double f64_of(D x) { return (double)x.hi + (double)x.lo; }
 // a scale, else the row factor; bit 3 which parent or form, bit 4 the imaginary part, bits 5 and
 // up the component or slot), the kernel, then each output pair times the factor (re fr - im fi,
 // re fi + im fr) added to the destination component unless the row initialises it, or stored
-// unscaled when the binding is direct and the factor is exactly (1, +0); narrowed to R.
+// unscaled when the binding is direct and the factor is exactly (1, +0); narrowed to R. (The
+// scales and factor are read as the policy's acc type: R in the reference's policies.)
 template <class K> void symjit_row(const Op &o, const Arena &a, size_t i0, size_t i1) {
     typedef typename K::real R;
     typedef typename K::den Den;
+    typedef typename K::acc T;
     const uint32_t kernel = o.bind[0], n_in = o.bind[1], pairs = o.bind[2], direct = o.bind[3];
     const uint32_t parent[2] = {o.f[1], o.f[2]}, dst = o.f[3], form[2] = {o.f[4], o.f[5]}, x = o.f[6];
-    const double fr = f64_of(ldr<R>(a.fre, x)), fi = f64_of(ldr<R>(a.fim, x));
+    const double fr = f64_of(ldr<T>(a.fre, x)), fi = f64_of(ldr<T>(a.fim, x));
     uint64_t frb, fib;
     memcpy(&frb, &fr, 8);
     memcpy(&fib, &fi, 8);
@@ -726,7 +745,7 @@ template <class K> void symjit_row(const Op &o, const Arena &a, size_t i0, size_
             } else if (kind == 1) {
                 v = f64_of(ldr<Den>(a.mom, ((size_t)form[which] * 4 + n) * a.ld + i));
             } else if (kind == 3) {
-                v = f64_of(ldr<R>(a.sre, n));
+                v = f64_of(ldr<T>(a.sre, n));
             } else {
                 v = im ? fi : fr;
             }
@@ -763,86 +782,91 @@ void run_program(const Op *ops, uint32_t count, const Arena &a, size_t i0, size_
     typedef typename K::real R;
     typedef typename K::den Den;
     typedef typename K::amp Amp;
+    typedef typename K::acc T;
     typedef Cx<R> C;
+    typedef Cx<T> CT;
     size_t npts = i1 - i0;
     // the destination accumulators of a run of rows (F_FIRST .. F_LAST), per point
-    std::vector<C> acc(6 * npts);
+    std::vector<CT> acc(6 * npts);
     for (uint32_t n = 0; n < count; n++) {
         const Op &o = ops[n];
         if (o.kind == K_CONTRIB) {
             int c0, c1, cd;
             contribution_shape(o.op, &c0, &c1, &cd);
             uint32_t slot = o.f[1], p0 = o.f[2], p1 = o.f[3], dst = o.f[4], f0 = o.f[5], f1 = o.f[6], x = o.f[7];
-            C scale = mul(cx(ldr<R>(a.sre, slot), ldr<R>(a.sim, slot)), cx(ldr<R>(a.fre, x), ldr<R>(a.fim, x)));
+            CT scale = mul(cx(ldr<T>(a.sre, slot), ldr<T>(a.sim, slot)), cx(ldr<T>(a.fre, x), ldr<T>(a.fim, x)));
             for (size_t i = i0; i < i1; i++) {
-                C l[6], r[4], v[6];
-                R q0[4], q1[4];
-                for (int k = 0; k < c0; k++) l[k] = ld_field<R>(a, o, 2, k, o.z[0], k, i);
-                for (int k = 0; k < c1; k++) r[k] = ld_field<R>(a, o, 3, k, o.z[1], k, i);
+                CT l[6], r[4], v[6];
+                T q0[4], q1[4];
+                for (int k = 0; k < c0; k++) l[k] = cvt<T>(ld_field<R>(a, o, 2, k, o.z[0], k, i));
+                for (int k = 0; k < c1; k++) r[k] = cvt<T>(ld_field<R>(a, o, 3, k, o.z[1], k, i));
                 if (o.op == 6)
                     for (int k = 0; k < 4; k++) {
-                        q0[k] = ld_form<R, Den>(a.mom, (f0 * 4 + k) * a.ld + i);
-                        q1[k] = ld_form<R, Den>(a.mom, (f1 * 4 + k) * a.ld + i);
+                        q0[k] = ld_form<T, Den>(a.mom, (f0 * 4 + k) * a.ld + i);
+                        q1[k] = ld_form<T, Den>(a.mom, (f1 * 4 + k) * a.ld + i);
                     }
                 contribution_values(o.op, l, r, q0, q1, v);
-                C *ac = &acc[6 * (i - i0)];
+                CT *ac = &acc[6 * (i - i0)];
                 for (int k = 0; k < cd; k++) {
-                    C c = mul(v[k], scale);
+                    CT c = mul(v[k], scale);
                     if (!(o.flags & F_FIRST)) ac[k] = add(ac[k], c);
                     else if (o.flags & F_INIT) ac[k] = c;
-                    else ac[k] = add(ld_field<R>(a, o, 4, k, o.z[2], k, i), c);
+                    else ac[k] = add(cvt<T>(ld_field<R>(a, o, 4, k, o.z[2], k, i)), c);
                 }
                 if ((o.flags & F_LAST) && !(o.flags & F_HOLD))
                     for (int k = 0; k < cd; k++)
-                        st_field<R>(a, o, 4, k, o.z[3], k, ac[k], i);
+                        st_field<R>(a, o, 4, k, o.z[3], k, cvt<R>(ac[k]), i);
             }
         } else if (o.kind == K_CLEAR) {
             for (uint32_t k = 0; k < o.f[1]; k++)
                 for (size_t i = i0; i < i1; i++) st_field<R>(a, o, 0, k, 0, 0, cx(R(0.0), R(0.0)), i);
         } else if (o.kind == K_PROP) {
+            // (in T; an input taken from the accumulators is first rounded to R, the value a
+            // stored current would have, so forwarding never changes a result)
             int m = o.op == 9 ? 4 : 2;
             uint32_t base = o.f[0], form = o.f[1], x = o.f[2];
-            C rf = cx(ldr<R>(a.fre, x), ldr<R>(a.fim, x));
+            CT rf = cx(ldr<T>(a.fre, x), ldr<T>(a.fim, x));
             for (size_t i = i0; i < i1; i++) {
-                R p[4];
+                T p[4];
                 Den q[4];
                 for (int k = 0; k < 4; k++) {
                     size_t at = (form * 4 + k) * a.ld + i;
                     q[k] = ldr<Den>(a.mom, at);
-                    p[k] = ld_form<R, Den>(a.mom, at);
+                    p[k] = ld_form<T, Den>(a.mom, at);
                 }
-                R reciprocal = ToReal<R, Den>::of(reciprocal_minkowski_norm(q[0], q[1], q[2], q[3]));
-                C c[4], v[4];
-                C *ac = &acc[6 * (i - i0)];
+                T reciprocal = ToReal<T, Den>::of(reciprocal_minkowski_norm(q[0], q[1], q[2], q[3]));
+                CT c[4], v[4];
+                CT *ac = &acc[6 * (i - i0)];
                 for (int k = 0; k < m; k++)
-                    c[k] = (o.flags & F_FROMACC) ? ac[k] : ld_field<R>(a, o, 0, k, o.z[0], k, i);
+                    c[k] = cvt<T>((o.flags & F_FROMACC) ? cvt<R>(ac[k]) : ld_field<R>(a, o, 0, k, o.z[0], k, i));
                 propagator_values(o.op, c, p, reciprocal, rf, v);
                 for (int k = 0; k < m; k++) {
                     if (o.flags & F_HOLD) ac[k] = v[k];
-                    else st_field<R>(a, o, 0, k, o.z[3], k, v[k], i);
+                    else st_field<R>(a, o, 0, k, o.z[3], k, cvt<R>(v[k]), i);
                 }
             }
         } else if (o.kind == K_IDENTITY) {
-            R fr = ldr<R>(a.fre, o.f[2]), fi = ldr<R>(a.fim, o.f[2]);
+            T fr = ldr<T>(a.fre, o.f[2]), fi = ldr<T>(a.fim, o.f[2]);
             for (uint32_t k = 0; k < o.f[1]; k++)
                 for (size_t i = i0; i < i1; i++) {
-                    C *ac = &acc[6 * (i - i0)];
+                    CT *ac = &acc[6 * (i - i0)];
                     const int b = k < 16 ? (int)k : 31;  // (no mask bit beyond 16 components)
-                    C v = identity_value(fr, fi, (o.flags & F_FROMACC) ? ac[k] : ld_field<R>(a, o, 0, k, o.z[0], b, i));
+                    CT v = identity_value(
+                        fr, fi, cvt<T>((o.flags & F_FROMACC) ? cvt<R>(ac[k]) : ld_field<R>(a, o, 0, k, o.z[0], b, i)));
                     if (o.flags & F_HOLD) ac[k] = v;
-                    else st_field<R>(a, o, 0, k, o.z[3], b, v, i);
+                    else st_field<R>(a, o, 0, k, o.z[3], b, cvt<R>(v), i);
                 }
         } else if (o.kind == K_CLOSURE) {
             uint32_t p0 = o.f[0], p1 = o.f[1], amp = o.f[2], x = o.f[3], first = o.f[4], cnt = o.f[5];
-            R rfr = ldr<R>(a.fre, x), rfi = ldr<R>(a.fim, x);
+            T rfr = ldr<T>(a.fre, x), rfi = ldr<T>(a.fim, x);
             for (size_t i = i0; i < i1; i++) {
                 size_t at = amp * a.ld + i;
                 Amp dr = ldr<Amp>(a.are, at), di = ldr<Amp>(a.aim, at);
                 for (uint32_t k = 0; k < cnt; k++) {
                     const int b = k < 8 ? (int)k : 31;
-                    C v = closure_increment(ld_field<R>(a, o, 0, k, o.z[0], b, i),
-                                            ld_field<R>(a, o, 1, k, o.z[1], b, i),
-                                            ldr<R>(a.fre, first + k), ldr<R>(a.fim, first + k), rfr, rfi);
+                    CT v = closure_increment(cvt<T>(ld_field<R>(a, o, 0, k, o.z[0], b, i)),
+                                             cvt<T>(ld_field<R>(a, o, 1, k, o.z[1], b, i)),
+                                             ldr<T>(a.fre, first + k), ldr<T>(a.fim, first + k), rfr, rfi);
                     dr = dr + widen(v.re);
                     di = di + widen(v.im);
                 }
@@ -881,6 +905,8 @@ void to_f64_planes(double *p, size_t planes, size_t stride, size_t points) {
 // ---- the v1 call on host memory (runner/recurrence_runner.cu:886-1487) --------------------
 
 enum { KERNEL_DF64 = 0, KERNEL_F64 = 1, KERNEL_MIXED = 2, KERNEL_MIXED64 = 3 };
+// the library's fused path also takes its compensated mixed policies (KComp, KComp64)
+enum { KERNEL_COMPENSATED = 4, KERNEL_COMPENSATED64 = 5 };
 struct Options {
     int kernel = KERNEL_DF64;
     bool df64 = true, df64_forms = true;
@@ -1290,8 +1316,10 @@ int evaluate_ops(uint32_t kernel, const std::vector<Op> &ops, uint32_t sources, 
                  uint32_t outputs, const uint32_t *offsets, const uint32_t *planes, const double *weights,
                  double numerator, double denominator, const double *external, uint32_t points, double *resolved,
                  double *total, uint32_t block, int threads) {
-    bool df = kernel != KERNEL_F64, df_forms = kernel == KERNEL_DF64 || kernel == KERNEL_MIXED;
-    bool mixed = kernel == KERNEL_MIXED || kernel == KERNEL_MIXED64;
+    bool df = kernel != KERNEL_F64 && kernel != KERNEL_COMPENSATED64,
+         df_forms = kernel == KERNEL_DF64 || kernel == KERNEL_MIXED || kernel == KERNEL_COMPENSATED;
+    bool mixed = kernel == KERNEL_MIXED || kernel == KERNEL_MIXED64 || kernel == KERNEL_COMPENSATED ||
+                 kernel == KERNEL_COMPENSATED64;
     // constants: f64 for the sources, df64 layout for the program when df
     std::vector<double> small64(2 * ((size_t)scale_count + factor_count) + 1);
     double *s_re = small64.data(), *s_im = s_re + scale_count, *f_re = s_im + scale_count, *f_im = f_re + factor_count;
@@ -1334,6 +1362,8 @@ int evaluate_ops(uint32_t kernel, const std::vector<Op> &ops, uint32_t sources, 
             case KERNEL_F64: run_program<KF64>(prog, count, a, 0, n); break;
             case KERNEL_MIXED: run_program<KMixed>(prog, count, a, 0, n); break;
             case KERNEL_MIXED64: run_program<KMixed64>(prog, count, a, 0, n); break;
+            case KERNEL_COMPENSATED: run_program<KComp>(prog, count, a, 0, n); break;
+            case KERNEL_COMPENSATED64: run_program<KComp64>(prog, count, a, 0, n); break;
             default: run_program<KDF64>(prog, count, a, 0, n); break;
             }
             for (size_t i = 0; i < n; i++) {
@@ -1369,7 +1399,7 @@ int refcpu_evaluate(uint32_t kernel, const uint64_t *words, uint32_t word_count,
                     const double *external, uint32_t points, double *resolved, double *total,
                     uint32_t block, int threads) {
     (void)external_count;
-    if (kernel > KERNEL_MIXED64 || block == 0) return PACMAN_RECURRENCE_INVALID;
+    if (kernel > KERNEL_COMPENSATED64 || block == 0) return PACMAN_RECURRENCE_INVALID;
     std::vector<Row> rows = decode(words, word_count, width);
     sort_contribution_runs(rows);
     std::vector<Op> ops;
@@ -1398,7 +1428,7 @@ int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t cou
                             uint32_t outputs, const uint32_t *offsets, const uint32_t *planes, const double *weights,
                             double numerator, double denominator, const double *external, uint32_t points,
                             double *resolved, double *total, uint32_t block, int threads) {
-    if (kernel > KERNEL_MIXED64 || block == 0 || !instr) return PACMAN_RECURRENCE_INVALID;
+    if (kernel > KERNEL_COMPENSATED64 || block == 0 || !instr) return PACMAN_RECURRENCE_INVALID;
     const uint32_t LDS = 1u << 24;
     auto plane = [&](uint32_t p) { return p >= LDS ? slot_planes + (p - LDS) : p; };
     // plane field k of op o from word w: a slot plane, an on-chip slot, or a slot plane with an

@@ -57,7 +57,9 @@ def _p(a, t):
 
 def evaluate(plan_arrays, counts, external, points, kernel=1, block=64, threads=1, resolved=True, total=True,
              numerator=1.0, denominator=1.0):
-    """The whole plan run per point on the CPU: (resolved [points][outputs], total [points])."""
+    """The whole plan run per point on the CPU: (resolved [points][outputs], total [points]).
+    kernel: 0 df64, 1 f64, 2 mixed, 3 mixed64 (the v1 ids), 4 and 5 the library's compensated
+    mixed policies (df64 and f64 arithmetic over f32 currents)."""
     lib = load()
     a = plan_arrays
     outputs = len(a["offsets"]) - 1

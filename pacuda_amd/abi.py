@@ -17,6 +17,11 @@ OUTPUT_ZERO = 1
 CAP_SOURCES, CAP_FORMS, CAP_SYMJIT, CAP_FUSED = 1, 2, 4, 8
 KERNEL_DF64, KERNEL_F64, KERNEL_MIXED, KERNEL_MIXED64 = 0, 1, 2, 3
 KERNELS = {"df64": KERNEL_DF64, "f64": KERNEL_F64, "mixed": KERNEL_MIXED, "mixed64": KERNEL_MIXED64}
+# the fused entry points' compensated mixed policies (PACUDA_KERNEL_COMPENSATED: df64 arithmetic,
+# PACUDA_KERNEL_COMPENSATED64: f64 arithmetic, both over f32 currents); the v1 configure call
+# takes the four above
+KERNEL_COMPENSATED, KERNEL_COMPENSATED64 = 4, 5
+FUSED_KERNELS = dict(KERNELS, compensated=KERNEL_COMPENSATED, compensated64=KERNEL_COMPENSATED64)
 W_P, W_A, W_F, W_X, W_S, W_K = range(6)
 
 u32p = C.POINTER(C.c_uint32)

@@ -137,13 +137,53 @@ PCD double to_raw(D v) { return __builtin_bit_cast(double, flt2{v.hi, v.lo}); }
 PCD double to_raw(float v) { return __builtin_bit_cast(double, flt2{v, 0.0f}); }
 
 // Precision policies: R the values, Den the propagator denominators and their forms, Amp the
-// amplitude accumulation (runner/recurrence_runner.cu:178-190).
+// amplitude accumulation (runner/recurrence_runner.cu:178-190), Acc the vertex arithmetic and
+// the accumulation of a destination's contributions (R in the reference's four policies).
 // forwarding: whether the kernel honours F_HOLD / F_FROMACC (every kernel does; a kernel without
 // it would run the plan's program without forwarding and LDS residency)
-struct KF64 { typedef double real; typedef double den; typedef double amp; static constexpr bool forwarding = true; };
-struct KDF64 { typedef D real; typedef D den; typedef D amp; static constexpr bool forwarding = true; };
-struct KMixed { typedef float real; typedef D den; typedef D amp; static constexpr bool forwarding = true; };
-struct KMixed64 { typedef float real; typedef double den; typedef D amp; static constexpr bool forwarding = true; };
+struct KF64 {
+    typedef double real; typedef double den; typedef double amp; typedef double acc;
+    static constexpr bool forwarding = true;
+};
+struct KDF64 {
+    typedef D real; typedef D den; typedef D amp; typedef D acc;
+    static constexpr bool forwarding = true;
+};
+struct KMixed {
+    typedef float real; typedef D den; typedef D amp; typedef float acc;
+    static constexpr bool forwarding = true;
+};
+struct KMixed64 {
+    typedef float real; typedef double den; typedef D amp; typedef float acc;
+    static constexpr bool forwarding = true;
+};
+// The compensated mixed policy (no counterpart in the reference): f32 currents and df64
+// denominators and amplitude sums as in KMixed, and the arithmetic of every row -- the vertices
+// (ops 2-6) and the sum of a destination's contributions, propagators, identity rows and the
+// closures' products -- in df64 from the f32 currents, the df64 forms and the df64 constants, so
+// that a current is rounded to f32 once, when it is stored (a forwarded one is rounded as if it
+// had been stored). Symjit rows run in f64 as in every policy.
+struct KComp {
+    typedef float real; typedef D den; typedef D amp; typedef D acc;
+    static constexpr bool forwarding = true;
+};
+// The compensated policy with f64 arithmetic (no counterpart in the reference): KComp's f32
+// currents and single rounding per stored current, with every row's arithmetic, the forms, the
+// constants, the denominators and the amplitude sums in f64 -- native on this GPU (f64 vector
+// operations at half the f32 rate), where df64 takes about ten f32 operations per operation.
+// It equals KComp to df64's accuracy.
+struct KComp64 {
+    typedef float real; typedef double den; typedef double amp; typedef double acc;
+    static constexpr bool forwarding = true;
+};
+template <class K> constexpr bool compensated() { return !__is_same(typename K::real, typename K::acc); }
+
+// conversions between a policy's R and Acc: exact widening, and the rounding of a df64 value
+// to the nearest f32 (fl(hi + lo))
+template <class T, class S> struct Conv { static PCD T of(S x) { return x; } };
+template <> struct Conv<D, float> { static PCD D of(float x) { return D(x, 0.0f); } };
+template <> struct Conv<float, D> { static PCD float of(D x) { return x.hi + x.lo; } };
+template <class T, class S> PCD Cx<T> cvt(Cx<S> c) { return cx(Conv<T, S>::of(c.re), Conv<T, S>::of(c.im)); }
 
 PCD float to_float(float x) { return x; }
 PCD float to_float(double x) { return (float)x; }

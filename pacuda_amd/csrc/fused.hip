@@ -93,12 +93,14 @@ __global__ void reset_counter(uint32_t *counter) {
 }
 
 // The kernels' register tiers: TIER 2 keeps REG_PLANES current planes and four forms in
-// registers, built for two wavefronts per SIMD in the f64 and df64 policies (up to 256 VGPRs)
-// and three in the mixed ones (168 VGPRs); TIER 1 keeps only the FORM_REGS forms, for three
-// wavefronts (f64, df64) or four (mixed); TIER 0 keeps neither, for four (128 VGPRs).
+// registers, built for two wavefronts per SIMD in the f64, df64 and compensated policies (up to
+// 256 VGPRs) and three in the mixed ones (168 VGPRs); TIER 1 keeps only the FORM_REGS forms, for
+// three wavefronts (f64, df64, compensated) or four (mixed); TIER 0 keeps neither, for four (128
+// VGPRs). (The compensated policy's df64 vertices and accumulators need the wide budgets.)
 template <class K> constexpr bool f32_currents() { return __is_same(typename K::real, float); }
+template <class K> constexpr bool narrow_registers() { return f32_currents<K>() && !compensated<K>(); }
 template <class K, int TIER> constexpr int waves_per_simd() {
-    return TIER == 2 ? (f32_currents<K>() ? 3 : 2) : TIER == 1 ? (f32_currents<K>() ? 4 : 3) : 4;
+    return TIER == 2 ? (narrow_registers<K>() ? 3 : 2) : TIER == 1 ? (narrow_registers<K>() ? 4 : 3) : 4;
 }
 template <class K, int TIER> constexpr int register_forms() {
     return TIER == 2 ? 4 : TIER == 1 ? (int)FORM_REGS : 0;
@@ -110,7 +112,10 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
              uint32_t *counter, double *scratch, size_t slot_doubles, uint32_t lds_per_wave, uint32_t lds_variant,
              uint32_t lds_cap) {
     typedef typename K::real R;
-    constexpr bool DF = !__is_same(R, double) || !__is_same(typename K::den, double);
+    // df64 pairs for the program's constants (all but the f64-arithmetic policies) and for the
+    // amplitude planes
+    constexpr bool DF = !__is_same(typename K::acc, double);
+    constexpr bool DF_AMP = !__is_same(typename K::amp, double);
     constexpr bool DF_FORMS = __is_same(typename K::den, D);
     const uint32_t lane = threadIdx.x & (LANES - 1);
     const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / LANES) + threadIdx.x / LANES);
@@ -250,7 +255,7 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
             double acc = 0.0;
             for (uint32_t k = cnst(pl.offsets)[o]; k < cnst(pl.offsets)[o + 1]; k++) {
                 const Cx<double> amp = a.template amp<double>(cnst(pl.planes)[k]);
-                double re = amplitude_value<DF>(amp.re), im = amplitude_value<DF>(amp.im);
+                double re = amplitude_value<DF_AMP>(amp.re), im = amplitude_value<DF_AMP>(amp.im);
                 acc = acc + cnst(pl.weights)[k] * __builtin_fma(re, re, im * im);
             }
             if (it.resolved && active) it.resolved[(size_t)i * pl.outputs + o] = acc;
@@ -1569,12 +1574,18 @@ struct Schedule {
 // The LDS variant of a launch, its LDS elements per wavefront and bytes per workgroup. A variant
 // qualifies when every plan has it and the workgroups a CU must hold for the requested
 // wavefronts fit in its LDS; the largest one is taken, else none (per_wave 0).
+// the kernels whose currents are f32
+bool f32_kernel(uint32_t kernel) {
+    return kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64 ||
+           kernel == PACUDA_KERNEL_COMPENSATED || kernel == PACUDA_KERNEL_COMPENSATED64;
+}
+
 void lds_shape(uint32_t kernel, const Schedule &sc, uint32_t *per_wave, size_t *bytes, uint32_t *variant) {
     *per_wave = 0;
     *bytes = 0;
     *variant = 0;
     // current bytes per plane and lane: f64 and df64 (a float pair) 8, the mixed kernels' f32 4
-    const size_t elem = (kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) ? 4 : 8;
+    const size_t elem = f32_kernel(kernel) ? 4 : 8;
     const uint32_t waves_per_block = sc.block / LANES;
     const uint32_t blocks_per_cu = (sc.per_cu + waves_per_block - 1) / waves_per_block;
     for (int v = LDS_VARIANTS - 1; v >= 0; v--) {
@@ -1591,7 +1602,7 @@ void lds_shape(uint32_t kernel, const Schedule &sc, uint32_t *per_wave, size_t *
 
 int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, Schedule &sc) {
     if (!l || l->abi_version != PACMAN_RECURRENCE_ABI || l->struct_size != sizeof(pacuda_launch) ||
-        l->kernel > PACMAN_RECURRENCE_KERNEL_MIXED64 || (l->block % LANES) != 0 || l->block > 256 ||
+        l->kernel > PACUDA_KERNEL_COMPENSATED64 || (l->block % LANES) != 0 || l->block > 256 ||
         l->waves_per_cu > 64 || (count && !items))
         return PACMAN_RECURRENCE_INVALID;
     std::vector<uint32_t> order(count);
@@ -1655,8 +1666,9 @@ template <class K> int tier_of(uint32_t per_cu) {
                                                              : 0;
 }
 int tier_of(uint32_t kernel, uint32_t per_cu) {
-    const bool mixed = kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64;
-    return mixed ? tier_of<KMixed>(per_cu) : tier_of<KF64>(per_cu);
+    if (kernel == PACUDA_KERNEL_COMPENSATED) return tier_of<KComp>(per_cu);
+    if (kernel == PACUDA_KERNEL_COMPENSATED64) return tier_of<KComp64>(per_cu);
+    return f32_kernel(kernel) ? tier_of<KMixed>(per_cu) : tier_of<KF64>(per_cu);
 }
 
 template <class K> bool allow_lds(int tier, bool sj) {
@@ -1695,6 +1707,8 @@ bool lds_ready(uint32_t kernel, const Schedule &sc) {
     if (kernel == PACMAN_RECURRENCE_KERNEL_F64) return allow_lds<KF64>(tier, sc.symjit);
     if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED) return allow_lds<KMixed>(tier, sc.symjit);
     if (kernel == PACMAN_RECURRENCE_KERNEL_MIXED64) return allow_lds<KMixed64>(tier, sc.symjit);
+    if (kernel == PACUDA_KERNEL_COMPENSATED) return allow_lds<KComp>(tier, sc.symjit);
+    if (kernel == PACUDA_KERNEL_COMPENSATED64) return allow_lds<KComp64>(tier, sc.symjit);
     return allow_lds<KDF64>(tier, sc.symjit);
 }
 
@@ -1712,6 +1726,10 @@ bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const Item
         return launch<KMixed>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
     case PACMAN_RECURRENCE_KERNEL_MIXED64:
         return launch<KMixed64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
+    case PACUDA_KERNEL_COMPENSATED:
+        return launch<KComp>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
+    case PACUDA_KERNEL_COMPENSATED64:
+        return launch<KComp64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
     default:
         return launch<KDF64>(sc, s, tier, bytes, items, starts, count, counter, scratch, per_wave, variant);
     }

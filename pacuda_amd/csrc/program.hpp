@@ -62,29 +62,38 @@ template <class A> PCD uint32_t zbits(uint32_t w, int shift, uint32_t width) {
     return A::zero_masks ? (w >> shift) & ((1u << width) - 1u) : 0u;
 }
 
+// A contribution row in the policy's vertex arithmetic T (K::acc): the parents widened to T
+// (exactly), the forms and the constants as T, the destination's sum in T, and the stored value
+// rounded to R once.
 template <class K, int OP, class A>
-PCD void contribution(const A &a, const Instr &o, int flags, Cx<typename K::real> *acc) {
+PCD void contribution(const A &a, const Instr &o, int flags, Cx<typename K::acc> *acc) {
     typedef typename K::real R;
     typedef typename K::den Den;
-    typedef Cx<R> C;
+    typedef typename K::acc T;
     constexpr int c0 = Shape<OP>::c0, c1 = Shape<OP>::c1, cd = Shape<OP>::cd;
     const uint32_t dst = o.w[4], slot = field16<A>(o.w[1]), x = field16<A>(o.w[7]);
-    C scale = mul(cx(ldc<R>(a.sre(), slot), ldc<R>(a.sim(), slot)), cx(ldc<R>(a.fre(), x), ldc<R>(a.fim(), x)));
-    C l[c0], r[c1], v[cd];
-    R q0[4], q1[4];
+    const Cx<T> scale =
+        mul(cx(ldc<T>(a.sre(), slot), ldc<T>(a.sim(), slot)), cx(ldc<T>(a.fre(), x), ldc<T>(a.fim(), x)));
+    Cx<R> l[c0], r[c1];
+    Cx<T> wl[c0], wr[c1], v[cd];
+    T q0[4], q1[4];
     a.template curs<R, c0>(o.w[2], l, zbits<A>(o.w[0], 22, 6));
     a.template curs<R, c1>(o.w[3], r, zbits<A>(o.w[0], 28, 4));
+#pragma unroll
+    for (int k = 0; k < c0; k++) wl[k] = cvt<T>(l[k]);
+#pragma unroll
+    for (int k = 0; k < c1; k++) wr[k] = cvt<T>(r[k]);
     if constexpr (OP == 6) {
         double f0[4], f1[4];
         a.form(o.w[5], f0);
         a.form(o.w[6], f1);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            q0[k] = Form<R, Den>::of(f0[k]);
-            q1[k] = Form<R, Den>::of(f1[k]);
+            q0[k] = Form<T, Den>::of(f0[k]);
+            q1[k] = Form<T, Den>::of(f1[k]);
         }
     }
-    contribution_values<OP>(l, r, q0, q1, v);
+    contribution_values<OP>(wl, wr, q0, q1, v);
     if (!(flags & F_FIRST)) {
 #pragma unroll
         for (int k = 0; k < cd; k++) acc[k] = add(acc[k], mul(v[k], scale));
@@ -92,44 +101,57 @@ PCD void contribution(const A &a, const Instr &o, int flags, Cx<typename K::real
 #pragma unroll
         for (int k = 0; k < cd; k++) acc[k] = mul(v[k], scale);
     } else {
-        C old[cd];
+        Cx<R> old[cd];
         a.template curs<R, cd>(dst, old, zbits<A>(o.w[1], 16, 6));
 #pragma unroll
-        for (int k = 0; k < cd; k++) acc[k] = add(old[k], mul(v[k], scale));
+        for (int k = 0; k < cd; k++) acc[k] = add(cvt<T>(old[k]), mul(v[k], scale));
     }
-    if ((flags & F_LAST) && !(flags & F_HOLD)) a.template puts<R, cd>(dst, acc, zbits<A>(o.w[7], 16, 6));
+    if ((flags & F_LAST) && !(flags & F_HOLD)) {
+        Cx<R> out[cd];
+#pragma unroll
+        for (int k = 0; k < cd; k++) out[k] = cvt<R>(acc[k]);
+        a.template puts<R, cd>(dst, out, zbits<A>(o.w[7], 16, 6));
+    }
 }
 
+// A propagator row in T: an input taken from the accumulators is first rounded to R (the value
+// the stored current would have), so forwarding never changes a result.
 template <class K, int OP, class A>
-PCD void propagator(const A &a, const Instr &o, int flags, Cx<typename K::real> *acc) {
+PCD void propagator(const A &a, const Instr &o, int flags, Cx<typename K::acc> *acc) {
     typedef typename K::real R;
     typedef typename K::den Den;
-    typedef Cx<R> C;
+    typedef typename K::acc T;
     constexpr int m = OP == 9 ? 4 : 2;
     const uint32_t base = o.w[1];
-    R p[4];
+    T p[4];
     Den q[4];
     double f[4];
     a.form(o.w[2], f);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         q[k] = from_raw<Den>(f[k]);
-        p[k] = Form<R, Den>::of(f[k]);
+        p[k] = Form<T, Den>::of(f[k]);
     }
-    R reciprocal = ToReal<R, Den>::of(reciprocal_minkowski_norm(q[0], q[1], q[2], q[3]));
-    C c[m], v[m];
+    const T reciprocal = ToReal<T, Den>::of(reciprocal_minkowski_norm(q[0], q[1], q[2], q[3]));
+    Cx<R> c[m];
+    Cx<T> wc[m], v[m];
     if (flags & F_FROMACC) {
 #pragma unroll
-        for (int k = 0; k < m; k++) c[k] = acc[k];
+        for (int k = 0; k < m; k++) c[k] = cvt<R>(acc[k]);
     } else {
         a.template curs<R, m>(base, c, zbits<A>(o.w[4], 0, 4));
     }
-    propagator_values<OP>(c, p, reciprocal, cx(ldc<R>(a.fre(), o.w[3]), ldc<R>(a.fim(), o.w[3])), v);
+#pragma unroll
+    for (int k = 0; k < m; k++) wc[k] = cvt<T>(c[k]);
+    propagator_values<OP>(wc, p, reciprocal, cx(ldc<T>(a.fre(), o.w[3]), ldc<T>(a.fim(), o.w[3])), v);
     if (flags & F_HOLD) {
 #pragma unroll
         for (int k = 0; k < m; k++) acc[k] = v[k];
     } else {
-        a.template puts<R, m>(base, v, zbits<A>(o.w[4], 8, 4));
+        Cx<R> out[m];
+#pragma unroll
+        for (int k = 0; k < m; k++) out[k] = cvt<R>(v[k]);
+        a.template puts<R, m>(base, out, zbits<A>(o.w[4], 8, 4));
     }
 }
 
@@ -150,10 +172,11 @@ template <class K, class A> __device__ __attribute__((always_inline)) inline voi
                                                                                            int flags) {
     typedef typename K::real R;
     typedef typename K::den Den;
+    typedef typename K::acc T;  // the scales and factor as the policy's acc type (R but in KComp)
     const uint32_t *b = a.bind() + o.w[6];
     const uint32_t kernel = cnst(b)[0], inputs = cnst(b)[1], pairs = cnst(b)[2], direct = cnst(b)[3];
     const uint32_t dst = o.w[3];
-    const double fr = as_f64(ldc<R>(a.fre(), o.w[5])), fi = as_f64(ldc<R>(a.fim(), o.w[5]));
+    const double fr = as_f64(ldc<T>(a.fre(), o.w[5])), fi = as_f64(ldc<T>(a.fim(), o.w[5]));
     double in[PACUDA_MAX_IN], out[PACUDA_MAX_OUT];
 #pragma unroll
     for (uint32_t k = 0; k < PACUDA_MAX_IN; k++) {
@@ -173,7 +196,7 @@ template <class K, class A> __device__ __attribute__((always_inline)) inline voi
             a.form(q / 4, raw);
             v = as_f64(from_raw<Den>(raw[q % 4]));
         } else if (kind == SB_SCALE) {
-            v = as_f64(ldc<R>(a.sre(), index));
+            v = as_f64(ldc<T>(a.sre(), index));
         } else {
             v = imag ? fi : fr;
         }
@@ -213,8 +236,9 @@ template <class K, bool SYMJIT = false, class A>
 __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count, const A &a) {
     typedef typename K::real R;
     typedef typename K::amp Amp;
+    typedef typename K::acc T;
     typedef Cx<R> C;
-    C acc[6];
+    Cx<T> acc[6];
     if (count == 0) return;
     // the next instruction is fetched while this one runs (its scalar load overlaps the row's
     // vector loads instead of preceding them)
@@ -237,7 +261,8 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
             else if (op == 7) propagator<K, 7>(a, o, flags, acc);
             else propagator<K, 8>(a, o, flags, acc);
         } else if (kind == K_CLOSURE) {
-            R rfr = ldc<R>(a.fre(), o.w[4]), rfi = ldc<R>(a.fim(), o.w[4]);
+            // (the closure's products in T: the compensated policy's are df64)
+            T rfr = ldc<T>(a.fre(), o.w[4]), rfi = ldc<T>(a.fim(), o.w[4]);
             Amp dr, di;
             if (flags & F_ZERO) {
                 dr = Amp(0.0);
@@ -254,55 +279,61 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
                 a.template curs<R, 4>(o.w[2], r, zr);
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    C v = closure_increment(l[k], r[k], ldc<R>(a.fre(), o.w[5] + k), ldc<R>(a.fim(), o.w[5] + k), rfr, rfi);
+                    Cx<T> v = closure_increment(cvt<T>(l[k]), cvt<T>(r[k]), ldc<T>(a.fre(), o.w[5] + k),
+                                                ldc<T>(a.fim(), o.w[5] + k), rfr, rfi);
                     dr = dr + widen(v.re);
                     di = di + widen(v.im);
                 }
             } else {
                 for (uint32_t k = 0; k < o.w[6]; k++) {
-                    C v = closure_increment(a.template cur<R>(o.w[1] + k, (zl >> (k & 7)) & (k < 8)),
-                                            a.template cur<R>(o.w[2] + k, (zr >> (k & 7)) & (k < 8)),
-                                            ldc<R>(a.fre(), o.w[5] + k), ldc<R>(a.fim(), o.w[5] + k), rfr, rfi);
+                    Cx<T> v = closure_increment(cvt<T>(a.template cur<R>(o.w[1] + k, (zl >> (k & 7)) & (k < 8))),
+                                                cvt<T>(a.template cur<R>(o.w[2] + k, (zr >> (k & 7)) & (k < 8))),
+                                                ldc<T>(a.fre(), o.w[5] + k), ldc<T>(a.fim(), o.w[5] + k), rfr, rfi);
                     dr = dr + widen(v.re);
                     di = di + widen(v.im);
                 }
             }
             a.template put_amp<Amp>(o.w[3], cx(dr, di));
         } else if (kind == K_IDENTITY) {
-            R fr = ldc<R>(a.fre(), o.w[3]), fi = ldc<R>(a.fim(), o.w[3]);
+            // (in T; an input from the accumulators rounded to R first, as in the propagator)
+            T fr = ldc<T>(a.fre(), o.w[3]), fi = ldc<T>(a.fim(), o.w[3]);
             const uint32_t count = o.w[2], zin = zbits<A>(o.w[4], 0, 16), zout = zbits<A>(o.w[4], 16, 16);
             if (count == 4) {  // four-plane currents: whole-object accesses
-                C u[4], v[4];
+                C u[4], out[4];
+                Cx<T> v[4];
                 if (flags & F_FROMACC) {
 #pragma unroll
-                    for (int k = 0; k < 4; k++) v[k] = identity_value(fr, fi, acc[k]);
+                    for (int k = 0; k < 4; k++) u[k] = cvt<R>(acc[k]);
                 } else {
                     a.template curs<R, 4>(o.w[1], u, zin);
-#pragma unroll
-                    for (int k = 0; k < 4; k++) v[k] = identity_value(fr, fi, u[k]);
                 }
+#pragma unroll
+                for (int k = 0; k < 4; k++) v[k] = identity_value(fr, fi, cvt<T>(u[k]));
                 if (flags & F_HOLD) {
 #pragma unroll
                     for (int k = 0; k < 4; k++) acc[k] = v[k];
                 } else {
-                    a.template puts<R, 4>(o.w[1], v, zout);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) out[k] = cvt<R>(v[k]);
+                    a.template puts<R, 4>(o.w[1], out, zout);
                 }
             } else if (flags & (F_FROMACC | F_HOLD)) {  // at most 6 components (the accumulators)
 #pragma unroll
                 for (int k = 0; k < 6; k++) {
                     if ((uint32_t)k < count) {
                         C u;
-                        if (flags & F_FROMACC) u = acc[k];
+                        if (flags & F_FROMACC) u = cvt<R>(acc[k]);
                         else u = a.template cur<R>(o.w[1] + k, (zin >> k) & 1);
-                        const C v = identity_value(fr, fi, u);
+                        const Cx<T> v = identity_value(fr, fi, cvt<T>(u));
                         if (flags & F_HOLD) acc[k] = v;
-                        else a.template put<R>(o.w[1] + k, v, (zout >> k) & 1);
+                        else a.template put<R>(o.w[1] + k, cvt<R>(v), (zout >> k) & 1);
                     }
                 }
             } else {
                 for (uint32_t k = 0; k < count; k++) {
                     const uint32_t zi = k < 16 ? (zin >> k) & 1 : 0, zo = k < 16 ? (zout >> k) & 1 : 0;
-                    a.template put<R>(o.w[1] + k, identity_value(fr, fi, a.template cur<R>(o.w[1] + k, zi)), zo);
+                    const Cx<T> v = identity_value(fr, fi, cvt<T>(a.template cur<R>(o.w[1] + k, zi)));
+                    a.template put<R>(o.w[1] + k, cvt<R>(v), zo);
                 }
             }
         } else if (kind == K_SYMJIT) {

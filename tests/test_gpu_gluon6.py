@@ -1,6 +1,7 @@
 # SPDX-License-Identifier: 0BSD
 """GPU parity of the longest plan, g g > g g g g (6 gluons, 64 helicity outputs), through the
-fused path in every precision mode and at every launch shape the library and the autotuner use.
+fused path in every precision mode (the reference's four and the two compensated mixed
+policies) and at every launch shape the library and the autotuner use.
 
 The 6-gluon program is the one that exercises the on-chip allocator under pressure (eviction,
 write-back of evicted currents to the slot, gluon sources made again), so each (kernel, waves per
@@ -50,10 +51,10 @@ def _want(plan, ext, kernel):
     return _oracle[kernel]
 
 
-@pytest.mark.parametrize("kernel", ["f64", "df64", "mixed", "mixed64"])
+@pytest.mark.parametrize("kernel", ["f64", "df64", "mixed", "mixed64", "compensated", "compensated64"])
 def test_gluon6_every_mode_and_shape_bitwise(gg_gggg, kernel):
     lib, plan, ext, fp, d_ext = gg_gggg
-    k = abi.KERNELS[kernel]
+    k = abi.FUSED_KERNELS[kernel]
     outputs = plan.amplitude_planes
     assert outputs == 64
     want_res, want_tot = _want(plan, ext, k)
@@ -79,8 +80,8 @@ def test_gluon6_default_and_autotuned_shape_bitwise(gg_gggg):
     lib, plan, ext, fp, d_ext = gg_gggg
     d_tot = DeviceArray(lib, POINTS * 8)
     try:
-        for kernel in ("f64", "df64", "mixed", "mixed64"):
-            k = abi.KERNELS[kernel]
+        for kernel in ("f64", "df64", "mixed", "mixed64", "compensated", "compensated64"):
+            k = abi.FUSED_KERNELS[kernel]
             _, want_tot = _want(plan, ext, k)
             jobs = [(fp, d_ext, POINTS, ext.shape[1], None, d_tot)]
             (wpc, block), _ = autotune(lib, jobs, k, repeats=1)

@@ -96,10 +96,10 @@ def test_v1_f64_gluon5_matches_oracle(lib, gg_ggg):
         assert bits_equal(got, want), f"points {points}: max |diff| {np.max(np.abs(got - want))}"
 
 
-@pytest.mark.parametrize("kernel", ["f64", "df64", "mixed", "mixed64"])
+@pytest.mark.parametrize("kernel", ["f64", "df64", "mixed", "mixed64", "compensated", "compensated64"])
 def test_fused_gluon5_matches_oracle(lib, gg_ggg, kernel):
     plan, ext = gg_ggg
-    k = abi.KERNELS[kernel]
+    k = abi.FUSED_KERNELS[kernel]
     fp = FusedPlan(lib, plan)
     points = 16384 - 37  # a ragged last unit
     d_ext = DeviceArray(lib, ext.nbytes)
@@ -212,13 +212,13 @@ def test_fused_colour_orderings_match_oracle(lib, gg_ggg):
     fp.destroy()
 
 
-@pytest.mark.parametrize("kernel", ["f64", "df64", "mixed"])
+@pytest.mark.parametrize("kernel", ["f64", "df64", "mixed", "compensated", "compensated64"])
 def test_fused_launch_shapes_bitwise(lib, gg_ggg, kernel):
     """every LDS variant (40, 26, 20, 13 and 10 LDS-resident planes per wavefront, picked from
     the requested occupancy and block) and the launch without LDS residency give the oracle's
     bits"""
     plan, ext = gg_ggg
-    k = abi.KERNELS[kernel]
+    k = abi.FUSED_KERNELS[kernel]
     fp = FusedPlan(lib, plan)
     points = 5000 + 3
     d_ext = DeviceArray(lib, ext.nbytes)
@@ -237,7 +237,7 @@ def test_fused_launch_shapes_bitwise(lib, gg_ggg, kernel):
 
 def test_fused_pp_jjj_subprocesses_bitwise(lib):
     """every p p > j j j subprocess plan (quark lines: ops 2, 3, 7, 8) in one persistent launch,
-    uneven point counts, f64 and mixed, equals the oracle's plan run bit for bit"""
+    uneven point counts, f64, mixed and both compensated policies, equals the oracle's plan run bit for bit"""
     from pacuda_amd import processes
     subs = processes.subprocesses()
     jobs, devs, fps, exts = [], [], [], []
@@ -253,8 +253,8 @@ def test_fused_pp_jjj_subprocesses_bitwise(lib):
         devs += [d_ext, d_tot]
         fps.append(fp)
         exts.append(ext)
-    for kernel in ("f64", "mixed"):
-        k = abi.KERNELS[kernel]
+    for kernel in ("f64", "mixed", "compensated", "compensated64"):
+        k = abi.FUSED_KERNELS[kernel]
         print("pp>jjj launch", kernel, flush=True)
         fused_evaluate(lib, jobs, k)
         for sp, (fp, _, points, _, _, d_tot), ext in zip(subs, jobs, exts):
