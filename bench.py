@@ -139,6 +139,19 @@ def measured_traffic(kernel, points, rows, waves_per_cu):
     return 1024.0 * (t["fetch_size_kib"] * t["fetch_scale"] + t["write_size_kib"] * t["write_scale"])
 
 
+def slot_summary(lib, plan):
+    """the f64 default launch's program (12 wavefronts per CU: register tier 1 with six register
+    forms, 13 LDS planes, sources made again): the slot's current planes (its live peak) and what
+    the program moves through the slot per point"""
+    from pacuda_amd.program_stats import slot_traffic
+    from pacuda_amd.runtime import plan_program
+    prog, sources, planes, lds = plan_program(lib, plan, 13)
+    st = slot_traffic(prog, sources, register_forms=6)
+    return {"program": "f64, 13 LDS planes, 6 register forms, sources made again", "slot_planes": planes,
+            "slot_kib_per_wavefront": planes * 64 * 16 / 1024, "on_chip_planes": lds,
+            "slot_bytes_per_point": st["bytes"], **{k: v for k, v in st.items() if k != "bytes"}}
+
+
 def unweighting_summary(u):
     """the unweighting figures of the JSON line (None when the unweighting did not run)"""
     if u is None:
@@ -356,6 +369,7 @@ def main():
         log(f"cpu {cpu['value']:.4g} points/s")
 
     traffic = measured_traffic("fused_kernel<KF64>", n, len(plan.rows), args.waves_per_cu or 12)  # (the f64 default)
+    slot = slot_summary(lib, plan)
     alg_bytes = plan.legs * 4 * 8 + 8  # the momenta in, |M|^2 out, per point
     line = {
         "metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
@@ -380,6 +394,7 @@ def main():
                      "traffic_tb_per_s": traffic / (kernel_ms * 1e-3) / 1e12 if traffic else None,
                      "traffic_frac_of_hbm_peak": traffic / (kernel_ms * 1e-3) / 1e12 / HBM_PEAK_TBS if traffic else None,
                      "kernel": "fused_kernel<KF64>", "kernel_ms": kernel_ms,
+                     "slot": slot,
                      "compute": {"pipe": "fp64 VALU", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                                  "frac": achieved / FP64_PEAK_TFLOPS, "flops_per_point": flops,
                                  # the program rows (the reference's operation order) contain no fused

@@ -643,6 +643,105 @@ std::vector<Instr> schedule_rows(const std::vector<Instr> &ops, uint32_t sources
     return out;
 }
 
+// Demand-driven scheduling: the closures (the program's sinks) are taken in the order `sink_key`
+// gives, and each is preceded by its not yet scheduled predecessor blocks, depth first, the
+// predecessor with the most unscheduled ancestors first (Sethi-Ullman: the larger subtree first
+// keeps fewer results waiting). Values then live from their producer to consumers that follow
+// soon after, so the peak of live currents -- the slot's size -- and the reuse distances drop.
+// Dependence-honouring, so every value keeps its bits. sink_key 0: the closures in program
+// order, 1: reversed, 2: amplitude planes in Gray-code order, 3: in bit-reversed order.
+std::vector<Instr> schedule_demand(const std::vector<Instr> &ops, uint32_t sources, uint32_t planes, int sink_key) {
+    const BlockGraph g = block_graph(ops, sources, planes);
+    if (!g.ok) return {};
+    const auto &blocks = g.blocks;
+    const uint32_t B = (uint32_t)blocks.size();
+    std::vector<uint8_t> done(B, 0);
+    std::vector<Instr> out;
+    out.reserve(ops.size());
+    // unscheduled ancestors of b (the blocks an emission of b would pull in)
+    std::vector<uint32_t> mark(B, 0);
+    uint32_t epoch = 0;
+    auto ancestors = [&](uint32_t b) {
+        epoch++;
+        uint32_t n = 0;
+        std::vector<uint32_t> stack{b};
+        mark[b] = epoch;
+        while (!stack.empty()) {
+            const uint32_t x = stack.back();
+            stack.pop_back();
+            for (uint32_t p : g.pred[x])
+                if (!done[p] && mark[p] != epoch) {
+                    mark[p] = epoch;
+                    n++;
+                    stack.push_back(p);
+                }
+        }
+        return n;
+    };
+    // iterative depth-first emission of b after its unscheduled predecessors
+    auto visit = [&](uint32_t root) {
+        if (done[root]) return;
+        std::vector<std::pair<uint32_t, std::vector<uint32_t>>> stack;
+        auto children = [&](uint32_t b) {
+            std::vector<std::pair<uint32_t, uint32_t>> c;
+            for (uint32_t p : g.pred[b])
+                if (!done[p]) c.push_back({ancestors(p), p});
+            std::sort(c.begin(), c.end(), [](const auto &x, const auto &y) {
+                return x.first != y.first ? x.first < y.first : x.second > y.second;
+            });
+            std::vector<uint32_t> v;  // popped from the back: the largest subtree first
+            for (const auto &x : c) v.push_back(x.second);
+            return v;
+        };
+        stack.push_back({root, children(root)});
+        while (!stack.empty()) {
+            auto &top = stack.back();
+            while (!top.second.empty() && done[top.second.back()]) top.second.pop_back();
+            if (top.second.empty()) {
+                const uint32_t b = top.first;
+                stack.pop_back();
+                if (done[b]) continue;
+                done[b] = 1;
+                const auto &x = blocks[b];
+                for (uint32_t r = 0; r < x.count; r++) out.push_back(ops[x.first + r]);
+                continue;
+            }
+            const uint32_t next = top.second.back();
+            top.second.pop_back();
+            stack.push_back({next, children(next)});
+        }
+    };
+    for (uint32_t b = 0; b < g.lead; b++) visit(b);
+    std::vector<uint32_t> sinks;
+    for (uint32_t b = g.lead; b < B; b++)
+        if (g.succ[b].empty()) sinks.push_back(b);
+    auto gray_rank = [](uint32_t a) {  // the position of a in the Gray-code sequence
+        uint32_t r = 0;
+        for (; a; a >>= 1) r ^= a;
+        return r;
+    };
+    auto bit_reverse = [](uint32_t a) {
+        uint32_t r = 0;
+        for (int k = 0; k < 16; k++) r |= ((a >> k) & 1u) << (15 - k);
+        return r;
+    };
+    auto key = [&](uint32_t b) -> uint64_t {
+        const int amp = blocks[b].amplitude;
+        const uint64_t a = amp < 0 ? 0xffffu : (uint32_t)amp;
+        switch (sink_key) {
+        case 1: return ((uint64_t)(B - b));
+        case 2: return ((uint64_t)gray_rank((uint32_t)a) << 32) | b;
+        case 3: return ((uint64_t)bit_reverse((uint32_t)a) << 32) | b;
+        default: return b;
+        }
+    };
+    std::stable_sort(sinks.begin(), sinks.end(), [&](uint32_t x, uint32_t y) { return key(x) < key(y); });
+    for (uint32_t b : sinks) visit(b);
+    for (uint32_t b = g.lead; b < B; b++) visit(b);
+    if (out.size() != ops.size()) return {};
+    return out;
+}
+
 // Structural zeros. A real value of the program is known to be zero for every input when it is
 // a product with a known zero or a sum of known zeros: a massless vector source's time component
 // (ext_gluon writes 0 there) and the imaginary part of its last component, a constant 0, and
@@ -925,10 +1024,12 @@ uint32_t assign_lds(std::vector<Instr> &ops, uint32_t capacity, uint32_t base = 
 // on the same f64 inputs: the same bits), so the slot load disappears; reads that find no room
 // still load the slot's copy, and evicting such a current never needs a store. A
 // replay of the rewritten program checks that every read finds the value last written (in the
-// slot only its unmasked components). Returns false with the program unchanged when an access
-// does not name a whole object or the replay fails.
+// slot only its unmasked components). With `keep_copies`, a current made again that is created
+// on chip also stores to the slot (else a read that finds no room after the current has left
+// finds no slot copy, and the replay fails). Returns false with the program unchanged when an
+// access does not name a whole object or the replay fails.
 bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, uint32_t *lds_used,
-                      uint32_t *reg_used, bool recompute = false) {
+                      uint32_t *reg_used, bool recompute = false, bool keep_copies = false) {
     struct Acc {
         uint32_t row;
         int word;
@@ -1092,6 +1193,10 @@ bool allocate_on_chip(std::vector<Instr> &ops, uint32_t lds_cap, uint32_t reg, u
                 if (remake) {  // (else the read loads the slot's copy)
                     remakes.push_back({(uint32_t)i, o, (uint32_t)best});
                     x.field = LDS_PLANE + (uint32_t)best;
+                } else if (x.create && again[o] && keep_copies) {
+                    // (a current made again also keeps its slot copy, for the reads that find no
+                    // room after it has left)
+                    x.field = dual_field(DUAL_THROUGH, p, (uint32_t)best);
                 } else {
                     x.field = x.rd ? dual_field(DUAL_FILL, p, (uint32_t)best) : LDS_PLANE + (uint32_t)best;
                 }
@@ -1396,6 +1501,10 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
             std::vector<Instr> o = schedule_rows(built, hp.sources, d->current_planes, window, w[0], w[1]);
             if (!o.empty()) orders.push_back(std::move(o));
         }
+    for (int key = 0; key < 4; key++) {
+        std::vector<Instr> o = schedule_demand(built, hp.sources, d->current_planes, key);
+        if (!o.empty()) orders.push_back(std::move(o));
+    }
     uint64_t best[2][2][LDS_VARIANTS];  // [sources made again][tier][variant]
     bool have[2][2][LDS_VARIANTS] = {};
     uint32_t lds_of[2][2][LDS_VARIANTS] = {};
@@ -1411,7 +1520,10 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
                     const uint32_t reg = tier ? REG_PLANES : 0;
                     std::vector<Instr> prog = ops;
                     uint32_t lds_used = 0, reg_used = 0;
-                    if (!allocate_on_chip(prog, LDS_CAPS[v], reg, &lds_used, &reg_used, again)) {
+                    // (programs that make sources again: first without slot copies of the made
+                    // currents, then, where the replay needs them, with)
+                    if (!allocate_on_chip(prog, LDS_CAPS[v], reg, &lds_used, &reg_used, again) &&
+                        !(again && allocate_on_chip(prog, LDS_CAPS[v], reg, &lds_used, &reg_used, true, true))) {
                         if (have[again][tier][v]) continue;
                         prog = ops;
                         lds_used = assign_lds(prog, LDS_CAPS[v]);
