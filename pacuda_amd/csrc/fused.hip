@@ -644,13 +644,15 @@ std::vector<Instr> schedule_rows(const std::vector<Instr> &ops, uint32_t sources
 }
 
 // Demand-driven scheduling: the closures (the program's sinks) are taken in the order `sink_key`
-// gives, and each is preceded by its not yet scheduled predecessor blocks, depth first, the
-// predecessor with the most unscheduled ancestors first (Sethi-Ullman: the larger subtree first
-// keeps fewer results waiting). Values then live from their producer to consumers that follow
-// soon after, so the peak of live currents -- the slot's size -- and the reuse distances drop.
+// gives, and each is preceded by its not yet scheduled predecessor blocks, depth first, in the
+// order `child_key` gives. Values then live from their producer to consumers that follow soon
+// after, so the peak of live currents -- the slot's size -- and the reuse distances drop.
 // Dependence-honouring, so every value keeps its bits. sink_key 0: the closures in program
-// order, 1: reversed, 2: amplitude planes in Gray-code order, 3: in bit-reversed order.
-std::vector<Instr> schedule_demand(const std::vector<Instr> &ops, uint32_t sources, uint32_t planes, int sink_key) {
+// order, 1: reversed, 2: amplitude planes in Gray-code order, 3: in bit-reversed order;
+// child_key 0: the largest subtree first (Sethi-Ullman), 1: the smallest first, 2: the
+// earliest block first, 3: the latest first.
+std::vector<Instr> schedule_demand(const std::vector<Instr> &ops, uint32_t sources, uint32_t planes, int sink_key,
+                                   int child_key = 0) {
     const BlockGraph g = block_graph(ops, sources, planes);
     if (!g.ok) return {};
     const auto &blocks = g.blocks;
@@ -683,9 +685,12 @@ std::vector<Instr> schedule_demand(const std::vector<Instr> &ops, uint32_t sourc
         if (done[root]) return;
         std::vector<std::pair<uint32_t, std::vector<uint32_t>>> stack;
         auto children = [&](uint32_t b) {
-            std::vector<std::pair<uint32_t, uint32_t>> c;
+            std::vector<std::pair<int64_t, uint32_t>> c;
             for (uint32_t p : g.pred[b])
-                if (!done[p]) c.push_back({ancestors(p), p});
+                if (!done[p]) {
+                    const int64_t a = child_key < 2 ? (int64_t)ancestors(p) : 0;
+                    c.push_back({child_key == 0 ? a : child_key == 1 ? -a : child_key == 2 ? -(int64_t)p : (int64_t)p, p});
+                }
             std::sort(c.begin(), c.end(), [](const auto &x, const auto &y) {
                 return x.first != y.first ? x.first < y.first : x.second > y.second;
             });
@@ -1501,8 +1506,10 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
             std::vector<Instr> o = schedule_rows(built, hp.sources, d->current_planes, window, w[0], w[1]);
             if (!o.empty()) orders.push_back(std::move(o));
         }
-    for (int key = 0; key < 4; key++) {
-        std::vector<Instr> o = schedule_demand(built, hp.sources, d->current_planes, key);
+    // (the closures in bit-reversed amplitude order were the best sink order on the gluon plans,
+    // ahead of program, reversed and Gray-code orders, under each predecessor order)
+    for (int child = 0; child < 4; child++) {
+        std::vector<Instr> o = schedule_demand(built, hp.sources, d->current_planes, 3, child);
         if (!o.empty()) orders.push_back(std::move(o));
     }
     uint64_t best[2][2][LDS_VARIANTS];  // [sources made again][tier][variant]
