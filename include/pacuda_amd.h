@@ -201,6 +201,17 @@ int pacuda_mlp_forward(pacuda_mlp *mlp, const float *x, uint32_t batch, float *y
 int pacuda_mlp_backward(pacuda_mlp *mlp, const float *y, const float *target, uint32_t batch, float *loss);
 /* one Adam step (step counts from 1) on the master weights, then the bf16 copies */
 int pacuda_mlp_adam(pacuda_mlp *mlp, float lr, float beta1, float beta2, float eps, uint32_t step);
+/* `steps` training steps, each forward (train), backward and Adam of one batch: batch s takes
+ * x + s batch inputs and target + s batch (device). Step first_step + s (0-based) of a run of
+ * total_steps uses the learning rate (float)(lr 0.5 (1 + cos(pi (first_step + s) / total_steps)))
+ * and Adam step number first_step + s + 1: the same arithmetic as the three calls above made in
+ * that order with those arguments. loss (host, may be NULL) receives the last batch's loss.
+ * Queued on the library's stream; returns after the last step when loss is set. */
+int pacuda_mlp_train_steps(pacuda_mlp *mlp, const float *x, const float *target, uint32_t batch, uint32_t steps,
+                           double lr, uint32_t first_step, uint32_t total_steps, float beta1, float beta2, float eps,
+                           float *loss);
+/* waits for the queued work and copies the loss of the last backward (or training step) */
+int pacuda_mlp_last_loss(pacuda_mlp *mlp, float *loss);
 
 /*
  * RAMBO phase space of 2 -> legs-2 massless partons in f64 on the device (convert/src/record.rs:

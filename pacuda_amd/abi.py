@@ -132,6 +132,9 @@ EXTENSION_SYMBOLS = [
     ("pacuda_mlp_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
     ("pacuda_mlp_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]),
     ("pacuda_mlp_adam", C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32]),
+    ("pacuda_mlp_last_loss", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    ("pacuda_mlp_train_steps", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_double,
+                                         C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)]),
     ("pacuda_sweep_create", C.c_int, [C.POINTER(Item), C.c_uint32, C.POINTER(Launch), C.POINTER(C.c_void_p)]),
     ("pacuda_sweep_run", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     ("pacuda_sweep_destroy", None, [C.c_void_p]),

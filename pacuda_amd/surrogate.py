@@ -73,6 +73,26 @@ class MLP:
         if self.lib.pacuda_mlp_adam(self.handle, lr, beta1, beta2, eps, self.step) != 0:
             raise RuntimeError("pacuda_mlp_adam failed")
 
+    def train_steps(self, d_x, d_t, batch, steps, lr, first_step, total_steps, beta1=0.9, beta2=0.999, eps=1e-8,
+                    want_loss=True):
+        """`steps` forward/backward/Adam steps on consecutive batches of d_x, d_t in one native
+        call (the learning rate of step s: lr 0.5 (1 + cos(pi s / total_steps))); returns the last
+        batch's loss when want_loss (a device synchronisation), else None"""
+        loss = C.c_float()
+        st = self.lib.pacuda_mlp_train_steps(self.handle, d_x.ptr, d_t.ptr, batch, steps, lr, first_step, total_steps,
+                                             beta1, beta2, eps, C.byref(loss) if want_loss else None)
+        if st != 0:
+            raise RuntimeError(f"pacuda_mlp_train_steps: status {st}")
+        self.step = first_step + steps
+        return loss.value if want_loss else None
+
+    def last_loss(self):
+        """the loss of the last backward or training step (waits for the queued work)"""
+        loss = C.c_float()
+        if self.lib.pacuda_mlp_last_loss(self.handle, C.byref(loss)) != 0:
+            raise RuntimeError("pacuda_mlp_last_loss failed")
+        return loss.value
+
     def destroy(self):
         if self.handle:
             self.lib.pacuda_mlp_destroy(self.handle)

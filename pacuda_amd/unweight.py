@@ -226,34 +226,31 @@ class Unweighting:
         d_tall = self.dev.array(t)
         d_x = self.dev.empty(n * w, np.float32)
         d_t = self.dev.empty(n, np.float32)
-        d_y = self.dev.empty(self.batch, np.float32)
         d_idx = self.dev.empty(n, np.uint32)
         dummy = self.dev.empty(1)
         history = []
         batches = n // self.batch
         steps = epochs * batches
         step = 0
+        order = rng.permutation(len(t))[:n].astype(np.uint32)
         for epoch in range(epochs):
-            order = rng.permutation(len(t))[:n].astype(np.uint32)
-            self.dev.synchronize()  # the previous epoch's steps are done with d_x, d_t
             d_idx.upload(order)
             for src, width, dst in ((d_xall, w, d_x), (d_tall, 1, d_t)):
                 st = lib.pacuda_gather(dummy.ptr, 1, 0, d_idx.ptr, n, dummy.ptr, src.ptr, width, dst.ptr)
                 if st != 0:
                     raise RuntimeError(f"pacuda_gather: status {st}")
             self.dev.synchronize()  # (the gather runs on the runner's stream, training on the network's)
-            for b in range(batches):
-                xb = DeviceArray.view(d_x, b * self.batch * w * 4, self.batch * w * 4)
-                tb = DeviceArray.view(d_t, b * self.batch * 4, self.batch * 4)
-                self.mlp.forward(xb, self.batch, d_y, train=True)
-                loss = self.mlp.backward(d_y, tb, self.batch, want_loss=b == batches - 1)
-                self.mlp.adam(lr * 0.5 * (1 + math.cos(math.pi * step / steps)))
-                step += 1
-            history.append(float(loss))
+            # one native call queues the epoch's steps (forward, backward and Adam per batch); the
+            # next epoch's permutation is drawn on the host meanwhile
+            self.mlp.train_steps(d_x, d_t, self.batch, batches, lr, step, steps, want_loss=False)
+            step += batches
+            if epoch + 1 < epochs:
+                order = rng.permutation(len(t))[:n].astype(np.uint32)
+            history.append(self.mlp.last_loss())  # (the steps are done with d_x, d_t after this)
             if log:
                 log(f"epoch {epoch}: loss {history[-1]:.4f}")
         self.dev.synchronize()
-        for d in (d_xall, d_tall, d_x, d_t, d_y, d_idx, dummy):
+        for d in (d_xall, d_tall, d_x, d_t, d_idx, dummy):
             d.free()
         return history
 

@@ -80,6 +80,42 @@ def test_training_reduces_loss():
     net.destroy()
 
 
+@pytest.mark.parametrize("inputs,hidden,layers,batch", [(14, 256, 4, 4096), (5, 128, 3, 192)])
+def test_native_training_loop_equals_the_calls(inputs, hidden, layers, batch):
+    """pacuda_mlp_train_steps gives the parameters of forward(train) + backward + adam per batch,
+    with the cosine learning rate, bit for bit"""
+    import math
+    rng = np.random.default_rng(hidden + layers)
+    dev = Device()
+    steps, total = 5, 12
+    x = rng.uniform(0, 1, (steps * batch, inputs)).astype(np.float32)
+    t = rng.normal(0, 1, steps * batch).astype(np.float32)
+    d_x, d_t, d_y = dev.array(x), dev.array(t), dev.empty(batch, np.float32)
+    p0 = mlp_ref.init(inputs, hidden, layers, seed=6)
+    nets = [surrogate.MLP(inputs, hidden, layers, max_batch=batch, lib=dev.lib) for _ in range(2)]
+    from pacuda_amd.runtime import DeviceArray
+    lr = 2e-3
+    first = 3
+    for net in nets:
+        net.set_params(p0)
+    a, b = nets
+    a.step = first
+    for s in range(steps):
+        xb = DeviceArray.view(d_x, s * batch * inputs * 4)
+        tb = DeviceArray.view(d_t, s * batch * 4)
+        a.forward(xb, batch, d_y, train=True)
+        loss_a = a.backward(d_y, tb, batch)
+        a.adam(lr * 0.5 * (1 + math.cos(math.pi * (first + s) / total)))
+    loss_b = b.train_steps(d_x, d_t, batch, steps, lr, first, total)
+    assert np.array_equal(a.params(), b.params())
+    assert np.array_equal(a.grads(), b.grads())
+    assert loss_a == loss_b
+    for d in (d_x, d_t, d_y):
+        d.free()
+    for net in nets:
+        net.destroy()
+
+
 @pytest.mark.parametrize("layers,batch", [(1, 64), (2, 192), (4, 4096 + 64), (2, 32768)])
 def test_inference_kernel(layers, batch):
     """the inference kernel (hidden 256, forward without train) against the oracle and against
