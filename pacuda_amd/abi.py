@@ -93,7 +93,7 @@ class Item(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("struct_size", C.c_uint32), ("kernel", C.c_uint32),
-                ("waves_per_cu", C.c_uint32), ("block", C.c_uint32), ("reserved", C.c_uint32),
+                ("waves_per_cu", C.c_uint32), ("block", C.c_uint32), ("unit_waves", C.c_uint32),
                 ("stream", C.c_void_p), ("kernel_ms", C.POINTER(C.c_float))]
 
 
@@ -112,6 +112,7 @@ V1_SYMBOLS = [
 EXTENSION_SYMBOLS = [
     ("pacman_recurrence_v1_test_sources", C.c_int, [u32p, f64p, f64p, C.c_uint32, f64p, f64p]),
     ("pacuda_plan_program", C.c_int, [C.POINTER(PlanDesc), C.c_uint32, u32p, C.c_uint32, u32p, u32p, u32p, u32p]),
+    ("pacuda_plan_coop_program", C.c_int, [C.POINTER(PlanDesc), C.c_uint32, C.c_uint32, u32p, C.c_uint32, u32p, u32p, u32p]),
     ("pacuda_plan_create", C.c_int, [C.POINTER(PlanDesc), C.POINTER(C.c_void_p)]),
     ("pacuda_plan_destroy", None, [C.c_void_p]),
     ("pacuda_plan_info", C.c_int, [C.c_void_p, u32p, u64p]),

@@ -390,8 +390,12 @@ template <int N> struct FormRegs {
 // REG: the lane has a register tier; an on-chip slot at or above lds_cap is in it (only whole
 // four-plane currents go there, so only the four-plane accessors look). NF: the lane also holds
 // the first NF register-resident forms (a form field marked with a later slot, or any marked
-// field when NF is 0, is read from the slot's copy).
-template <class R, bool REG = false, int NF = 0> struct SlotArena {
+// field when NF is 0, is read from the slot's copy). COOP: the arena of a cooperative kernel,
+// whose slot and LDS region belong to the workgroup (one unit, shared by its wavefronts): a form
+// field with FORM_LDS set names an LDS plane pair (components 0-1 in the first plane, 2-3 in
+// the second), an amplitude field at or above LDS_PLANE an LDS plane.
+constexpr uint32_t FORM_LDS = 1u << 23;
+template <class R, bool REG = false, int NF = 0, bool COOP = false> struct SlotArena {
     typedef typename Packed<R>::T V;
     typedef Buffer<sizeof(V)> B;
     static constexpr bool forwarding = true;
@@ -515,6 +519,15 @@ template <class R, bool REG = false, int NF = 0> struct SlotArena {
     }
     // form field f: a register slot when bits 24 and up are set (REG kernels), else a slot form
     PCD void form(uint32_t f, double *raw) const {
+        if constexpr (COOP) {
+            static_assert(sizeof(V) == 16, "LDS forms are two 16-byte planes");
+            if (f & FORM_LDS) {
+                const uint32_t o = lds + (f & (FORM_LDS - 1)) * PLANE;
+                const dbl2 x = __builtin_bit_cast(dbl2, *lds_at(o)), y = __builtin_bit_cast(dbl2, *lds_at(o + PLANE));
+                raw[0] = x.x; raw[1] = x.y; raw[2] = y.x; raw[3] = y.y;
+                return;
+            }
+        }
         if constexpr (NF > 0) {
             if ((f >> FORM_SLOT_SHIFT) && (f >> FORM_SLOT_SHIFT) <= (uint32_t)NF) {
                 fregs->get((f >> FORM_SLOT_SHIFT) - 1, raw);
@@ -527,14 +540,34 @@ template <class R, bool REG = false, int NF = 0> struct SlotArena {
         raw[0] = x.x; raw[1] = x.y; raw[2] = y.x; raw[3] = y.y;
     }
     PCD void put_form(uint32_t f, const double *raw) const {
+        if constexpr (COOP) {
+            if (f & FORM_LDS) {
+                const uint32_t o = lds + (f & (FORM_LDS - 1)) * PLANE;
+                *lds_at(o) = __builtin_bit_cast(V, dbl2{raw[0], raw[1]});
+                *lds_at(o + PLANE) = __builtin_bit_cast(V, dbl2{raw[2], raw[3]});
+                return;
+            }
+        }
         Buffer<16>::store(dbl2{raw[0], raw[1]}, slot, idx * 32u, form_base + f * 2048u);
         Buffer<16>::store(dbl2{raw[2], raw[3]}, slot, idx * 32u + 16u, form_base + f * 2048u);
     }
     template <class T> PCD Cx<T> amp(uint32_t q) const {
+        if constexpr (COOP) {
+            if (q >= LDS_PLANE) {
+                const dbl2 v = __builtin_bit_cast(dbl2, *lds_at(lds + (q - LDS_PLANE) * PLANE));
+                return cx(from_raw<T>(v.x), from_raw<T>(v.y));
+            }
+        }
         const dbl2 v = Buffer<16>::load<dbl2>(slot, idx * 16u, amp_base + q * 1024u);
         return cx(from_raw<T>(v.x), from_raw<T>(v.y));
     }
     template <class T> PCD void put_amp(uint32_t q, Cx<T> v) const {
+        if constexpr (COOP) {
+            if (q >= LDS_PLANE) {
+                *lds_at(lds + (q - LDS_PLANE) * PLANE) = __builtin_bit_cast(V, dbl2{to_raw(v.re), to_raw(v.im)});
+                return;
+            }
+        }
         Buffer<16>::store(dbl2{to_raw(v.re), to_raw(v.im)}, slot, idx * 16u, amp_base + q * 1024u);
     }
     PCD const double *sre() const { return sre_; }

@@ -29,6 +29,7 @@
 #include "pacman_recurrence.h"
 #include "pacuda_amd.h"
 #include "plan_compiler.hpp"
+#include "coop.hpp"
 #include "program.hpp"
 #include "runtime.hpp"
 
@@ -37,6 +38,13 @@ using namespace pacuda;
 namespace {
 
 constexpr int LDS_VARIANTS = 5;
+// the cooperative shapes (wavefronts per unit, units per compute unit): one 64-point unit per
+// workgroup, LDS_BYTES / units per CU of LDS for it (its top COOP_CONTROL bytes hold the unit
+// number), so (LDS_BYTES / units - COOP_CONTROL) / 1 KB planes of 64 f64 values
+constexpr int COOP_SHAPES = 3;
+constexpr uint32_t COOP_WAVES[COOP_SHAPES] = {16, 8, 8};
+constexpr uint32_t COOP_UNITS[COOP_SHAPES] = {1, 1, 2};
+constexpr uint32_t COOP_CONTROL = 256;
 
 struct PlanDev {
     const Instr *prog_lds[LDS_VARIANTS];  // with forwarding and LDS-resident currents (LDS_CAPS[v] planes)
@@ -60,6 +68,12 @@ struct PlanDev {
     const uint32_t *bindings;             // the program's symjit bindings
     double numerator, denominator;
     uint32_t factor, pad;
+    // the cooperative programs per shape (coop_prog null: none for the plan)
+    const Instr *coop_prog[COOP_SHAPES];     // the wavefronts' streams, one after another
+    const uint32_t *coop_start[COOP_SHAPES];  // their offsets (wavefronts + 1)
+    const uint32_t *coop_form[COOP_SHAPES];   // per form: FORM_LDS | LDS plane, or the slot form
+    const uint32_t *coop_amp[COOP_SHAPES];    // per amplitude plane: LDS_PLANE + LDS plane, or the slot amplitude
+    uint32_t coop_slot[COOP_SHAPES];          // slot current planes
 };
 
 struct ItemDev {
@@ -266,6 +280,108 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
     }
 }
 
+// The cooperative kernel: each workgroup takes one 64-point unit at a time off the same work
+// queue, and its W wavefronts share it: they compute the unit's momentum forms (form f by
+// wavefront f mod W) into the unit's LDS region or slot, meet at a barrier, run their streams of
+// the plan's cooperative program (whose barrier rows separate its phases; currents, forms and
+// amplitudes in LDS or in the workgroup's slot), meet again, and wavefront 0 reduces the
+// amplitudes as fused_kernel does. f64 only: LDS planes hold 16 bytes per lane.
+template <class K, int W, int UNITS>
+__global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W * UNITS / 4)))
+coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start, uint32_t item_count,
+            uint32_t *counter, double *scratch, size_t slot_doubles, uint32_t shape, uint32_t lds_bytes) {
+    typedef typename K::real R;
+    static_assert(__is_same(R, double) && __is_same(typename K::den, double) && __is_same(typename K::amp, double),
+                  "cooperative programs run in f64");
+    extern __shared__ __attribute__((aligned(16))) char pacuda_lds[];
+    const uint32_t lane = threadIdx.x & (LANES - 1);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / LANES);
+    double *slot = scratch + (size_t)blockIdx.x * slot_doubles;
+    volatile uint32_t *control = reinterpret_cast<volatile uint32_t *>(pacuda_lds + lds_bytes - COOP_CONTROL);
+    const uint32_t units = cnst(unit_start)[item_count];
+    for (;;) {
+        // (the barrier also ends the previous unit's reduction before its LDS is reused)
+        if (threadIdx.x == 0) control[0] = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t unit = __builtin_amdgcn_readfirstlane(control[0]);
+        if (unit >= units) break;
+        uint32_t lo = 0, hi = item_count;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) / 2;
+            if (cnst(unit_start)[mid] <= unit) lo = mid;
+            else hi = mid;
+        }
+        const ItemDev it = cnst(items)[lo];
+        const PlanDev *plan = it.plan;
+        const uint32_t i = (unit - cnst(unit_start)[lo]) * LANES + lane;
+        const bool active = i < it.points;
+        const size_t ie = active ? i : it.points - 1;  // inactive lanes repeat the last point
+        const uint32_t A = cnst(plan)->amplitude_planes, F = cnst(plan)->form_count;
+        // (selects, not an index: a runtime index into the plan's arrays would put them in scratch)
+        const Instr *prog = shape == 0 ? cnst(plan)->coop_prog[0] : shape == 1 ? cnst(plan)->coop_prog[1] : cnst(plan)->coop_prog[2];
+        const uint32_t *starts = shape == 0 ? cnst(plan)->coop_start[0] : shape == 1 ? cnst(plan)->coop_start[1] : cnst(plan)->coop_start[2];
+        const uint32_t *form_field = shape == 0 ? cnst(plan)->coop_form[0] : shape == 1 ? cnst(plan)->coop_form[1] : cnst(plan)->coop_form[2];
+        const uint32_t *amp_field = shape == 0 ? cnst(plan)->coop_amp[0] : shape == 1 ? cnst(plan)->coop_amp[1] : cnst(plan)->coop_amp[2];
+        const uint32_t C = shape == 0 ? cnst(plan)->coop_slot[0] : shape == 1 ? cnst(plan)->coop_slot[1] : cnst(plan)->coop_slot[2];
+        SlotArena<R, false, 0, true> a;
+        a.regs = nullptr;
+        a.fregs = nullptr;
+        a.lds_cap = 0;
+        a.slot = __builtin_amdgcn_make_buffer_rsrc(slot, (short)0, (int)(slot_doubles * sizeof(double)), BUFFER_FORMAT);
+        a.amp_base = C * 1024u;
+        a.form_base = a.amp_base + A * 1024u;
+        a.idx = lane;
+        a.bind_ = nullptr;
+        a.point_ = i;
+        a.points_ = it.points;
+        a.lds = lane * 16u;
+        const double *c64 = cnst(plan)->c64;
+        const uint32_t S = cnst(plan)->scale_count, X = cnst(plan)->factor_count;
+        a.sre_ = c64; a.sim_ = c64 + S; a.fre_ = c64 + 2 * S; a.fim_ = c64 + 2 * S + X;
+        // the momentum forms in f64 (runner/recurrence_runner.cu:745-764), form f by wavefront f mod W
+        const uint32_t *form_terms = cnst(plan)->form_terms, *term_slots = cnst(plan)->term_slots;
+        const double *coefficients = cnst(plan)->coefficients;
+        for (uint32_t f = wave; f < F; f += W) {
+            const uint32_t first = cnst(form_terms)[2 * f], n = cnst(form_terms)[2 * f + 1];
+            double v[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                v[k] = 0.0;
+                for (uint32_t t = 0; t < n; t++) {
+                    double term = cnst(coefficients)[first + t] *
+                                  it.external[((size_t)cnst(term_slots)[first + t] * 4 + k) * it.ext_stride + ie];
+                    v[k] = t == 0 ? 0.0 + term : v[k] + term;
+                }
+            }
+            a.put_form(cnst(form_field)[f], v);
+        }
+        __syncthreads();
+        const uint32_t first = cnst(starts)[wave];
+        interpret<K, false, true>(prog + first, cnst(starts)[wave + 1] - first, a);
+        __syncthreads();
+        if (wave == 0) {
+            // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
+            const uint32_t outputs = cnst(plan)->outputs;
+            const uint32_t *offsets = cnst(plan)->offsets, *planes = cnst(plan)->planes;
+            const double *weights = cnst(plan)->weights;
+            const bool factor = cnst(plan)->factor;
+            const double numerator = cnst(plan)->numerator, denominator = cnst(plan)->denominator;
+            double t = 0.0;
+            for (uint32_t o = 0; o < outputs; o++) {
+                double acc = 0.0;
+                for (uint32_t k = cnst(offsets)[o]; k < cnst(offsets)[o + 1]; k++) {
+                    const Cx<double> amp = a.template amp<double>(cnst(amp_field)[cnst(planes)[k]]);
+                    acc = acc + cnst(weights)[k] * __builtin_fma(amp.re, amp.re, amp.im * amp.im);
+                }
+                if (it.resolved && active) it.resolved[(size_t)i * outputs + o] = acc;
+                double v = factor ? (acc * numerator) / denominator : acc;
+                t = t + v;
+            }
+            if (it.total && active) it.total[i] = t;
+        }
+    }
+}
+
 }  // namespace
 
 struct pacuda_plan {
@@ -273,6 +389,7 @@ struct pacuda_plan {
     PlanDev *device = nullptr;
     std::vector<void *> allocations;
     uint64_t slot_doubles = 0;
+    uint64_t coop_slot_doubles[COOP_SHAPES] = {};  // a workgroup's slot per cooperative shape (0: no program)
     bool symjit = false;  // the program holds symjit rows
 };
 
@@ -1369,6 +1486,7 @@ struct Compiled {
     bool zero_fill = true;
     std::vector<uint32_t> bindings;
     bool symjit = false;
+    CoopProgram coop[COOP_SHAPES];  // the cooperative programs (ok false: none)
 };
 
 // F_ZERO on the first closure of every amplitude plane (closures are the only rows that touch
@@ -1496,6 +1614,13 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
     forward_in_place(hp.ops);
     c.zero_fill = mark_first_closures(hp.ops, d->amplitude_planes);
     c.fwd = hp.ops;
+    {
+        std::vector<uint8_t> used(d->amplitude_planes, 0);
+        for (uint32_t k = 0; k < count; k++) used[d->planes[k]] = 1;
+        for (int s = 0; s < COOP_SHAPES; s++)
+            c.coop[s] = coop_program(c.fwd, c.planes, d->amplitude_planes, d->form_count, used, COOP_WAVES[s],
+                                     (LDS_BYTES / COOP_UNITS[s] - COOP_CONTROL) / 1024);
+    }
     // the on-chip programs: the passes over the rows as built and over each candidate order, the
     // allocator's placement (the static one where it gives up), keeping per variant and tier the
     // program that leaves the fewest slot accesses
@@ -1594,6 +1719,29 @@ extern "C" int pacuda_plan_program(const pacuda_plan_desc *d, uint32_t variant, 
     return PACMAN_RECURRENCE_OK;
 }
 
+extern "C" int pacuda_plan_coop_program(const pacuda_plan_desc *d, uint32_t shape, uint32_t serial, uint32_t *out,
+                                        uint32_t capacity, uint32_t *count, uint32_t *info, uint32_t *starts) {
+    if (!count || shape >= (uint32_t)COOP_SHAPES) return PACMAN_RECURRENCE_INVALID;
+    Compiled c;
+    const int st = compile_plan(d, c);
+    if (st) return st;
+    const CoopProgram &q = c.coop[shape];
+    if (!q.ok) return PACMAN_RECURRENCE_INVALID;
+    const std::vector<Instr> &ops = serial ? q.serial : q.ops;
+    *count = (uint32_t)ops.size();
+    if (info) {
+        const uint32_t v[6] = {q.waves, q.phases, q.lds_planes, q.slot_planes, (uint32_t)q.slot_accesses,
+                               (uint32_t)q.lds_accesses};
+        memcpy(info, v, sizeof v);
+    }
+    if (starts) memcpy(starts, q.start.data(), q.start.size() * sizeof(uint32_t));
+    if (out) {
+        if (capacity < ops.size()) return PACMAN_RECURRENCE_INVALID;
+        memcpy(out, ops.data(), ops.size() * sizeof(Instr));
+    }
+    return PACMAN_RECURRENCE_OK;
+}
+
 extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) {
     if (!out) return PACMAN_RECURRENCE_INVALID;
     *out = nullptr;
@@ -1651,6 +1799,17 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     h.numerator = d->numerator;
     h.denominator = d->denominator;
     h.factor = d->numerator != d->denominator;
+    for (int s = 0; s < COOP_SHAPES; s++) {
+        const CoopProgram &q = cp.coop[s];
+        if (!q.ok) continue;
+        h.coop_prog[s] = upload(p, q.ops.data(), q.ops.size(), &ok);
+        h.coop_start[s] = upload(p, q.start.data(), q.start.size(), &ok);
+        h.coop_form[s] = upload(p, q.form_field.data(), q.form_field.size(), &ok);
+        h.coop_amp[s] = upload(p, q.amp_field.data(), q.amp_field.size(), &ok);
+        h.coop_slot[s] = q.slot_planes;
+        p->coop_slot_doubles[s] =
+            (uint64_t)LANES * (2ull * q.slot_planes + 2ull * d->amplitude_planes + 4ull * d->form_count);
+    }
     p->device = upload(p, &h, 1, &ok);
     p->slot_doubles = (uint64_t)LANES * (2ull * planes + 2ull * d->amplitude_planes + 4ull * d->form_count);
     if (!ok) {
@@ -1688,6 +1847,9 @@ struct Schedule {
     uint32_t lds_planes[LDS_VARIANTS] = {};  // per variant, the most any item's plan uses
     bool lds_all[LDS_VARIANTS] = {true, true, true, true, true};  // every plan has the variant
     bool symjit = false;  // some item's plan holds symjit rows
+    int coop = -1;        // the cooperative shape, or -1
+    uint64_t slots = 0;   // scratch slots: one per wavefront, or per workgroup in a cooperative launch
+    uint32_t slots_per_block() const { return coop >= 0 ? 1u : block / LANES; }
 };
 
 // The LDS variant of a launch, its LDS elements per wavefront and bytes per workgroup. A variant
@@ -1721,9 +1883,19 @@ void lds_shape(uint32_t kernel, const Schedule &sc, uint32_t *per_wave, size_t *
 
 int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, Schedule &sc) {
     if (!l || l->abi_version != PACMAN_RECURRENCE_ABI || l->struct_size != sizeof(pacuda_launch) ||
-        l->kernel > PACUDA_KERNEL_COMPENSATED64 || (l->block % LANES) != 0 || l->block > 256 ||
+        l->kernel > PACUDA_KERNEL_COMPENSATED64 || (!l->unit_waves && ((l->block % LANES) != 0 || l->block > 256)) ||
         l->waves_per_cu > 64 || (count && !items))
         return PACMAN_RECURRENCE_INVALID;
+    if (l->unit_waves) {  // a cooperative launch: its shape
+        const uint32_t per_cu = l->waves_per_cu ? l->waves_per_cu : 16;
+        for (int k = 0; k < COOP_SHAPES; k++)
+            if (COOP_WAVES[k] == l->unit_waves && COOP_WAVES[k] * COOP_UNITS[k] == per_cu) sc.coop = k;
+        if (sc.coop < 0 || l->kernel != PACMAN_RECURRENCE_KERNEL_F64) {
+            fprintf(stderr, "pacuda: no cooperative launch of kernel %u with %u wavefronts per unit and %u per CU\n",
+                    l->kernel, l->unit_waves, per_cu);
+            return PACMAN_RECURRENCE_INVALID;
+        }
+    }
     std::vector<uint32_t> order(count);
     std::vector<uint64_t> cost(count);
     for (uint32_t k = 0; k < count; k++) {
@@ -1746,7 +1918,11 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
         d.ext_stride = it.ext_stride;
         sc.starts[j] = (uint32_t)sc.units;
         sc.units += (it.points + LANES - 1) / LANES;
-        sc.slot = std::max<uint64_t>(sc.slot, it.plan->slot_doubles);
+        if (sc.coop >= 0 && !it.plan->coop_slot_doubles[sc.coop]) {
+            fprintf(stderr, "pacuda: a plan of the launch has no cooperative program (symjit rows)\n");
+            return PACMAN_RECURRENCE_INVALID;
+        }
+        sc.slot = std::max<uint64_t>(sc.slot, sc.coop >= 0 ? it.plan->coop_slot_doubles[sc.coop] : it.plan->slot_doubles);
         sc.symjit = sc.symjit || it.plan->symjit;
         for (int v = 0; v < LDS_VARIANTS; v++) {
             sc.lds_planes[v] = std::max(sc.lds_planes[v], it.plan->host.lds_planes[v]);
@@ -1756,6 +1932,14 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
     // (a slot is addressed by 32-bit buffer offsets)
     if (sc.units >= 0xffffffffull || sc.slot * sizeof(double) > 0x7fffffffull) return PACMAN_RECURRENCE_INVALID;
     sc.starts[count] = (uint32_t)sc.units;
+    if (sc.coop >= 0) {  // one slot per workgroup, COOP_UNITS workgroups per CU
+        sc.block = COOP_WAVES[sc.coop] * LANES;
+        sc.per_cu = COOP_WAVES[sc.coop] * COOP_UNITS[sc.coop];
+        sc.grid = (uint32_t)std::min<uint64_t>((uint64_t)compute_units() * COOP_UNITS[sc.coop], std::max<uint64_t>(sc.units, 1));
+        sc.waves = (uint64_t)sc.grid * COOP_WAVES[sc.coop];
+        sc.slots = sc.grid;
+        return PACMAN_RECURRENCE_OK;
+    }
     sc.block = l->block ? l->block : 256;
     // default occupancy per kernel (measured on g g > g g g and g g > g g g g)
     const uint32_t per_cu = l->waves_per_cu                                  ? l->waves_per_cu
@@ -1767,6 +1951,7 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
     uint64_t waves = std::min<uint64_t>((uint64_t)compute_units() * per_cu, std::max<uint64_t>(sc.units, 1));
     sc.grid = (uint32_t)((waves + waves_per_block - 1) / waves_per_block);
     sc.waves = (uint64_t)sc.grid * waves_per_block;
+    sc.slots = sc.waves;
     return PACMAN_RECURRENCE_OK;
 }
 
@@ -1816,8 +2001,36 @@ bool launch(const Schedule &sc, hipStream_t s, int tier, size_t bytes, const Ite
     return true;
 }
 
+template <int W, int U> bool allow_coop_lds() {
+    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(coop_kernel<KF64, W, U>),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES / U),
+                                       "cooperative kernel LDS");
+    return ok;
+}
+bool allow_coop_lds(int shape) {
+    return shape == 0 ? allow_coop_lds<16, 1>() : shape == 1 ? allow_coop_lds<8, 1>() : allow_coop_lds<8, 2>();
+}
+static_assert(COOP_SHAPES == 3 && COOP_WAVES[0] == 16 && COOP_UNITS[0] == 1 && COOP_WAVES[1] == 8 && COOP_UNITS[1] == 1 &&
+                  COOP_WAVES[2] == 8 && COOP_UNITS[2] == 2,
+              "the cooperative kernels instantiated");
+
+bool launch_coop(const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts, uint32_t count,
+                 uint32_t *counter, double *scratch) {
+    if (!allow_coop_lds(sc.coop)) return false;
+    const uint32_t bytes = LDS_BYTES / COOP_UNITS[sc.coop];
+    reset_counter<<<1, 64, 0, s>>>(counter);
+    if (sc.coop == 0)
+        coop_kernel<KF64, 16, 1><<<sc.grid, 1024, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, 0, bytes);
+    else if (sc.coop == 1)
+        coop_kernel<KF64, 8, 1><<<sc.grid, 512, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, 1, bytes);
+    else
+        coop_kernel<KF64, 8, 2><<<sc.grid, 512, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, 2, bytes);
+    return true;
+}
+
 // the LDS attribute of the kernel a launch of `sc` needs, set once (before any graph capture)
 bool lds_ready(uint32_t kernel, const Schedule &sc) {
+    if (sc.coop >= 0) return allow_coop_lds(sc.coop);
     uint32_t per_wave, variant;
     size_t bytes;
     lds_shape(kernel, sc, &per_wave, &bytes, &variant);
@@ -1833,6 +2046,7 @@ bool lds_ready(uint32_t kernel, const Schedule &sc) {
 
 bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
                   uint32_t count, uint32_t *counter, double *scratch) {
+    if (sc.coop >= 0) return launch_coop(sc, s, items, starts, count, counter, scratch);
     uint32_t per_wave, variant;
     size_t bytes;
     lds_shape(kernel, sc, &per_wave, &bytes, &variant);
@@ -1862,7 +2076,7 @@ extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const p
     if (st) return st;
     if (count == 0 || sc.units == 0) return PACMAN_RECURRENCE_OK;
     std::lock_guard<std::mutex> guard(fs.lock);
-    const size_t need = (size_t)sc.waves * sc.slot;
+    const size_t need = (size_t)sc.slots * sc.slot;
     hipStream_t s = l->stream ? (hipStream_t)l->stream : runner_stream();
     if (!s) return PACMAN_RECURRENCE_DEVICE_ERROR;
     // the previous launch (on any stream) reads the item table and uses the counter and slots
@@ -1967,19 +2181,21 @@ extern "C" int pacuda_evaluate_static(const pacuda_item *items, uint32_t count, 
         return PACMAN_RECURRENCE_INVALID;
     }
     if (count == 0 || sc.units == 0) return PACMAN_RECURRENCE_OK;
-    // the resident wavefronts whose slots fit: at most the launch's, at least one workgroup
+    // the resident workgroups whose slots fit: at most the launch's, at least one (a slot per
+    // wavefront, or per workgroup in a cooperative launch)
     const size_t slot_bytes = (size_t)sc.slot * sizeof(double);
     const uint64_t fit = (STATIC_BYTES - STATIC_SLOTS_AT) / std::max<size_t>(slot_bytes, 1);
-    const uint32_t waves_per_block = sc.block / LANES;
-    if (fit < waves_per_block) {
+    const uint32_t slots_per_block = sc.slots_per_block();
+    if (fit < slots_per_block) {
         fprintf(stderr,
-                "pacuda: pacuda_evaluate_static: %u wavefronts of %.1f MB slots exceed the static scratch of %zu MB\n",
-                waves_per_block, slot_bytes / 1e6, STATIC_BYTES >> 20);
+                "pacuda: pacuda_evaluate_static: %u slots of %.1f MB exceed the static scratch of %zu MB\n",
+                slots_per_block, slot_bytes / 1e6, STATIC_BYTES >> 20);
         return PACMAN_RECURRENCE_INVALID;
     }
-    if (sc.waves > fit) {
-        sc.grid = (uint32_t)(fit / waves_per_block);
-        sc.waves = (uint64_t)sc.grid * waves_per_block;
+    if (sc.slots > fit) {
+        sc.grid = (uint32_t)(fit / slots_per_block);
+        sc.waves = (uint64_t)sc.grid * (sc.block / LANES);
+        sc.slots = (uint64_t)sc.grid * slots_per_block;
     }
     std::lock_guard<std::mutex> guard(static_lock);
     int device = 0;
@@ -2043,7 +2259,7 @@ extern "C" int pacuda_sweep_create(const pacuda_item *items, uint32_t count, con
     }
     w->kernel = l->kernel;
     w->count = count;
-    const size_t need = (size_t)w->sc.waves * w->sc.slot;
+    const size_t need = (size_t)w->sc.slots * w->sc.slot;
     bool ok = !hip_failed(hipMalloc((void **)&w->items, count * sizeof(ItemDev)), "sweep") &&
               !hip_failed(hipMalloc((void **)&w->starts, (count + 1) * sizeof(uint32_t)), "sweep") &&
               !hip_failed(hipMalloc((void **)&w->counter, 64), "sweep") &&

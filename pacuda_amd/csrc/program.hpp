@@ -16,7 +16,7 @@
 
 namespace pacuda {
 
-enum { K_CLEAR = 1, K_CONTRIB, K_PROP, K_IDENTITY, K_CLOSURE, K_SOURCE, K_SYMJIT };
+enum { K_CLEAR = 1, K_CONTRIB, K_PROP, K_IDENTITY, K_CLOSURE, K_SOURCE, K_SYMJIT, K_BARRIER };
 // a symjit binding's input code: bits 0-2 the kind below, bit 3 which parent or form, bit 4 the
 // imaginary part, bits 5 and up the component (or the scale slot) (runner/recurrence_runner.cu:
 // 582, 631-684)
@@ -35,7 +35,8 @@ enum { F_INIT = 1, F_FIRST = 2, F_LAST = 4, F_HOLD = 8, F_FROMACC = 16, F_ZERO =
 // component factor, w[6] count; source: w[1] destination, w[2] form, w[3] factor, w[4] mass
 // slot, w[5] template; symjit: w[1] parent 0, w[2] parent 1, w[3] destination, w[4] form 0 |
 // form 1 << 16, w[5] factor, w[6] the binding's offset in the binding table (kernel, inputs,
-// output pairs, direct, then the input codes and the output components).
+// output pairs, direct, then the input codes and the output components); barrier (cooperative
+// programs only): no operands, the wavefront waits for its workgroup.
 struct alignas(32) Instr {
     uint32_t w[8];
 };
@@ -231,8 +232,10 @@ template <class R, class A> __device__ inline void source_row(const Instr &o, co
 
 // SYMJIT: whether the program may hold symjit rows. The symjit row is an out-of-line call with
 // stack arrays, so kernels instantiated with it allocate registers worse everywhere; programs
-// without symjit rows run an instance without the branch.
-template <class K, bool SYMJIT = false, class A>
+// without symjit rows run an instance without the branch. COOP: the program is one wavefront's
+// share of a cooperative program, whose barrier rows end its phases (every wavefront of the
+// workgroup runs the same number of them).
+template <class K, bool SYMJIT = false, bool COOP = false, class A>
 __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count, const A &a) {
     typedef typename K::real R;
     typedef typename K::amp Amp;
@@ -338,6 +341,8 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
             }
         } else if (kind == K_SYMJIT) {
             if constexpr (SYMJIT) symjit_row<K>(a, o, flags);
+        } else if (kind == K_BARRIER) {
+            if constexpr (COOP) __syncthreads();
         } else if (kind == K_SOURCE) {  // a source row made again (programs of the f64 kernel only)
             if constexpr (__is_same(R, double) && __is_same(typename K::den, double)) source_row<R>(o, a);
         } else if (kind == K_CLEAR) {

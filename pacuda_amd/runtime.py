@@ -98,6 +98,28 @@ def plan_desc(plan, arrays, no_sort=0):
     return d
 
 
+# the cooperative shapes (pacuda_plan_coop_program): (wavefronts per unit, waves per CU)
+COOP_SHAPES = ((16, 16), (8, 8), (8, 16))
+
+
+def coop_program(lib, plan, shape, serial=False):
+    """a plan's cooperative program for shape index `shape` (pacuda_plan_coop_program, host
+    only): (uint32 [count][8], info dict, stream starts), or None when the plan has none"""
+    a = plan.arrays()
+    d = plan_desc(plan, a)
+    n, info, starts = C.c_uint32(), (C.c_uint32 * 6)(), (C.c_uint32 * 17)()
+    if lib.pacuda_plan_coop_program(C.byref(d), shape, int(serial), None, 0, C.byref(n), info, starts) != 0:
+        return None
+    out = np.zeros((n.value, 8), dtype=np.uint32)
+    st = lib.pacuda_plan_coop_program(C.byref(d), shape, int(serial), out.ctypes.data_as(u32p), n.value, C.byref(n),
+                                      info, starts)
+    if st != 0:
+        raise RuntimeError(f"pacuda_plan_coop_program: status {st}")
+    keys = ("waves", "phases", "lds_planes", "slot_planes", "slot_accesses", "lds_accesses")
+    info = dict(zip(keys, list(info)))
+    return out, info, list(starts)[:info["waves"] + 1]
+
+
 def plan_program(lib, plan, variant):
     """the fused path's instruction stream of a plan (pacuda_plan_program, host only): (uint32
     [count][8], leading sources, slot planes, LDS planes)"""
@@ -147,10 +169,11 @@ class FusedPlan:
             self.handle = None
 
 
-def evaluate(lib, jobs, kernel, waves_per_cu=0, block=0, timed=False):
+def evaluate(lib, jobs, kernel, waves_per_cu=0, block=0, timed=False, unit_waves=0):
     """jobs: [(FusedPlan, DeviceArray external, points, ext_stride, DeviceArray|None resolved,
-    DeviceArray|None total)]; one persistent launch. Returns the kernel's time in ms (HIP events)
-    when timed, else waits for the device and returns 0."""
+    DeviceArray|None total)]; one persistent launch (cooperative when unit_waves is 16 or 8: that
+    many wavefronts share each 64-point unit). Returns the kernel's time in ms (HIP events) when
+    timed, else waits for the device and returns 0."""
     items = (abi.Item * max(1, len(jobs)))()
     for k, (fp, ext, points, stride, res, tot) in enumerate(jobs):
         items[k].plan = fp.handle
@@ -160,7 +183,7 @@ def evaluate(lib, jobs, kernel, waves_per_cu=0, block=0, timed=False):
         items[k].resolved = res.ptr if res is not None else None
         items[k].total = tot.ptr if tot is not None else None
     ms = C.c_float(0)
-    l = abi.Launch(abi.ABI, C.sizeof(abi.Launch), kernel, waves_per_cu, block, 0, None,
+    l = abi.Launch(abi.ABI, C.sizeof(abi.Launch), kernel, waves_per_cu, block, unit_waves, None,
                    C.pointer(ms) if timed else None)
     st = lib.pacuda_evaluate(items, len(jobs), C.byref(l))
     if st != 0:
@@ -174,7 +197,7 @@ def evaluate(lib, jobs, kernel, waves_per_cu=0, block=0, timed=False):
 class Sweep:
     """A batch of items captured once into a HIP graph (pacuda_sweep_create) and replayed"""
 
-    def __init__(self, lib, jobs, kernel, waves_per_cu=0, block=0):
+    def __init__(self, lib, jobs, kernel, waves_per_cu=0, block=0, unit_waves=0):
         self.lib = lib
         items = (abi.Item * len(jobs))()
         for k, (fp, ext, points, stride, res, tot) in enumerate(jobs):
@@ -184,7 +207,7 @@ class Sweep:
             items[k].ext_stride = stride
             items[k].resolved = res.ptr if res is not None else None
             items[k].total = tot.ptr if tot is not None else None
-        l = abi.Launch(abi.ABI, C.sizeof(abi.Launch), kernel, waves_per_cu, block, 0, None, None)
+        l = abi.Launch(abi.ABI, C.sizeof(abi.Launch), kernel, waves_per_cu, block, unit_waves, None, None)
         h = C.c_void_p()
         st = lib.pacuda_sweep_create(items, len(jobs), C.byref(l), C.byref(h))
         if st != 0:

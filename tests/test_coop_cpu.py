@@ -1,0 +1,102 @@
+# SPDX-License-Identifier: 0BSD
+"""Cooperative programs on the CPU (pacuda_plan_coop_program; no GPU).
+
+A cooperative program spreads one 64-point unit's rows over the wavefronts of a workgroup in
+phases separated by barriers, with the unit's currents, forms and amplitudes placed in the
+workgroup's LDS or slot. Its serial order (phase by phase, wavefront by wavefront) run through the
+oracle's fused-program interpreter (oracle/refcpu.cpp, LDS planes mapped after the slot's) must
+equal the oracle's run of the plan bit for bit: that pins the dependences the phases honour and
+the placement (a clobbered current changes values). The streams' structure (the same barriers in
+every wavefront, the same rows as the serial order) is what the kernel relies on to neither
+deadlock nor skip work."""
+import numpy as np
+import pytest
+
+from pacuda_amd import abi, plans, processes, phase_space as ps
+from pacuda_amd.runtime import coop_program, COOP_SHAPES
+import harness
+import refcpu
+
+K_BARRIER = 8
+LDS_CAP = {0: 159, 1: 159, 2: 79}
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return abi.load()
+
+
+def _plans():
+    subs = processes.subprocesses()
+    return [("gg>ggg", plans.gluon_plan(5)), ("gg>gggg", plans.gluon_plan(6)), ("qline5", plans.quark_line_plan(5)),
+            (subs[7].name, subs[7].plan), (subs[25].name, subs[25].plan)]
+
+
+@pytest.mark.parametrize("shape", range(len(COOP_SHAPES)))
+@pytest.mark.parametrize("which", range(5))
+def test_serial_order_equals_the_plan_run(lib, shape, which):
+    name, plan = _plans()[which]
+    got = coop_program(lib, plan, shape, serial=True)
+    assert got is not None, name
+    ser, info, _ = got
+    assert info["waves"] == COOP_SHAPES[shape][0]
+    assert info["lds_planes"] <= LDS_CAP[shape]
+    n = 4 * 64 + 13
+    mom, _ = ps.generate(plan.legs, n, seed=40 + which)
+    ext = ps.external_planes(mom, plan.crossing)
+    counts = harness.plan_counts(plan)
+    want_res, want_tot = refcpu.evaluate(plan.arrays(), counts, ext, n, kernel=1)
+    res, tot = refcpu.evaluate_program(ser, 0, info["slot_planes"], info["lds_planes"], plan.arrays(), counts, ext, n,
+                                       kernel=1, numerator=plan.numerator, denominator=plan.denominator)
+    assert np.array_equal(res.view(np.uint64), want_res.view(np.uint64)), name
+    assert np.array_equal(tot.view(np.uint64), want_tot.view(np.uint64)), name
+
+
+@pytest.mark.parametrize("shape", range(len(COOP_SHAPES)))
+def test_streams_meet_the_same_barriers(lib, shape):
+    for name, plan in _plans():
+        dev, info, starts = coop_program(lib, plan, shape)
+        ser, _, _ = coop_program(lib, plan, shape, serial=True)
+        W, P = info["waves"], info["phases"]
+        assert len(starts) == W + 1 and starts[0] == 0 and starts[-1] == len(dev)
+        assert all(a <= b for a, b in zip(starts, starts[1:]))
+        kinds = dev[:, 0] & 0xff
+        for w in range(W):
+            assert np.count_nonzero(kinds[starts[w]:starts[w + 1]] == K_BARRIER) == P - 1, (name, w)
+        # every row of the plan's program once (the serial order holds them in phase order)
+        assert len(dev) - W * (P - 1) == len(ser), name
+        assert sorted(dev[kinds != K_BARRIER, 0].tolist()) == sorted(ser[:, 0].tolist()), name
+
+
+def test_more_lds_leaves_fewer_slot_accesses(lib):
+    plan = plans.gluon_plan(5)
+    small = coop_program(lib, plan, 2)[1]
+    large = coop_program(lib, plan, 0)[1]
+    assert large["slot_accesses"] < small["slot_accesses"]
+    assert large["slot_accesses"] + large["lds_accesses"] == small["slot_accesses"] + small["lds_accesses"]
+    # g g > g g g: most current accesses stay in the unit's LDS at 159 planes
+    assert large["lds_accesses"] > 2 * large["slot_accesses"]
+
+
+def test_plans_with_symjit_rows_have_none(lib):
+    sub = processes.four_quark_subprocesses()[0]
+    assert coop_program(lib, sub.plan, 0) is None
+
+
+def test_the_oracle_sees_a_broken_order(lib):
+    """the parity check has teeth: the serial rows with two dependent phases swapped (the
+    closures before the currents they read) give other values"""
+    plan = plans.gluon_plan(5)
+    ser, info, _ = coop_program(lib, plan, 0, serial=True)
+    kinds = ser[:, 0] & 0xff
+    closures = ser[kinds == 5]
+    rest = ser[kinds != 5]
+    broken = np.concatenate([closures, rest])
+    n = 64
+    mom, _ = ps.generate(5, n, seed=3)
+    ext = ps.external_planes(mom, plan.crossing)
+    counts = harness.plan_counts(plan)
+    _, want = refcpu.evaluate(plan.arrays(), counts, ext, n, kernel=1)
+    _, got = refcpu.evaluate_program(broken, 0, info["slot_planes"], info["lds_planes"], plan.arrays(), counts, ext, n,
+                                     kernel=1)
+    assert not np.array_equal(got.view(np.uint64), want.view(np.uint64))
