@@ -26,7 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from pacuda_amd import abi, plans, processes, phase_space as ps  # noqa: E402
-from pacuda_amd.runtime import Device, FusedPlan, autotune, evaluate  # noqa: E402
+from pacuda_amd.runtime import (Device, FusedPlan, autotune, evaluate, shape_name, TUNE_SHAPES,  # noqa: E402
+                                COOP_TUNE_SHAPES)
 from pacuda_amd.unweight import Generator, Unweighting  # noqa: E402
 
 METRIC = "|M|^2 points/s (f64 bit-exact) + unweighted events/s, surrogate vs plain"
@@ -77,12 +78,13 @@ def host_threads():
     return max(1, min(n, int(omp))) if omp.isdigit() and int(omp) > 0 else max(1, n)
 
 
-def timed_steps(lib, jobs, kernel, steps, waves_per_cu=0, block=0):
+def timed_steps(lib, jobs, kernel, steps, waves_per_cu=0, block=0, unit_waves=0):
     """wall seconds of `steps` launches (synchronised) and the mean kernel ms (HIP events)"""
     ms = []
     t = time.perf_counter()
     for _ in range(steps):
-        ms.append(evaluate(lib, jobs, kernel, waves_per_cu=waves_per_cu, block=block, timed=True))
+        ms.append(evaluate(lib, jobs, kernel, waves_per_cu=waves_per_cu, block=block, timed=True,
+                           unit_waves=unit_waves))
     lib.pacuda_synchronize()
     return time.perf_counter() - t, float(np.mean(ms))
 
@@ -126,25 +128,47 @@ def mode_errors(got, ref):
             "max_err_rel_to_max_f64": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))}
 
 
-def measured_traffic(kernel, points, rows, waves_per_cu):
+def kernel_name(shape):
+    """the f64 kernel a launch shape (waves_per_cu, block, unit_waves) runs"""
+    wpc, _, uw = shape
+    if uw:
+        return f"coop_kernel<KF64, {uw}, {(wpc or 16) // uw}>"
+    return "fused_kernel<KF64>"
+
+
+def measured_traffic(kernel, points, rows, launch):
     """HBM bytes per dispatch from the committed rocprofv3 counters (profiles/traffic.json),
-    scaled by the copy-kernel calibration, when they were taken on this workload; else None"""
+    scaled by the copy-kernel calibration, when they were taken on this workload and launch;
+    else None"""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             t = json.load(f).get(kernel)
     except (OSError, ValueError):
         return None
-    if not t or (t["points"], t["plan_rows"], t["waves_per_cu"]) != (points, rows, waves_per_cu):
+    if not t or (t["points"], t["plan_rows"], t["launch"]) != (points, rows, launch):
         return None
     return 1024.0 * (t["fetch_size_kib"] * t["fetch_scale"] + t["write_size_kib"] * t["write_scale"])
 
 
-def slot_summary(lib, plan):
-    """the f64 default launch's program (12 wavefronts per CU: register tier 1 with six register
-    forms, 13 LDS planes, sources made again): the slot's current planes (its live peak) and what
-    the program moves through the slot per point"""
+def slot_summary(lib, plan, shape):
+    """what the launch's program keeps on chip and moves through its scratch slot per point. The
+    f64 default launch (12 wavefronts per CU: register tier 1 with six register forms, 13 LDS
+    planes, sources made again): the slot's current planes (its live peak) and the program's slot
+    accesses. A cooperative launch: its phases, LDS and slot planes per unit, and the current
+    components read or written in LDS and in the slot per point."""
     from pacuda_amd.program_stats import slot_traffic
-    from pacuda_amd.runtime import plan_program
+    from pacuda_amd.runtime import plan_program, coop_program, COOP_SHAPES
+    wpc, _, uw = shape
+    if uw:
+        k = COOP_SHAPES.index((uw, wpc or 16))
+        _, info, _ = coop_program(lib, plan, k)
+        slot_planes = info["slot_planes"] + plan.amplitude_planes + 2 * len(plan.forms)
+        return {"program": f"cooperative, {uw} wavefronts per unit, {(wpc or 16) // uw} units per CU",
+                "phases": info["phases"], "lds_planes": info["lds_planes"],
+                "slot_current_planes": info["slot_planes"], "slot_kib_per_workgroup": slot_planes * 64 * 16 / 1024,
+                "lds_current_components_per_point": info["lds_accesses"],
+                "slot_current_components_per_point": info["slot_accesses"],
+                "slot_current_bytes_per_point": 16 * info["slot_accesses"]}
     prog, sources, planes, lds = plan_program(lib, plan, 13)
     st = slot_traffic(prog, sources, register_forms=6)
     return {"program": "f64, 13 LDS planes, 6 register forms, sources made again", "slot_planes": planes,
@@ -195,6 +219,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--points", type=int, default=1 << 22)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--launch", default="auto",
+                    help="the f64 headline launch: auto (the fastest of the one-wavefront-per-unit default and the "
+                         "cooperative shapes, chosen before the warm-up), fused (one wavefront per unit, "
+                         "--waves-per-cu), or coop:U:W (U wavefronts per unit, W per CU)")
     ap.add_argument("--stream", choices=("counter", "record"), default="counter",
                     help="headline inputs: the GPU's counter-based RAMBO stream, or the reference's own "
                          "xorshift stream (convert/src/record.rs) generated on the host")
@@ -236,9 +264,20 @@ def main():
     flops, divs = plans.plan_flops(plan)
     jobs = [(fp, d_ext, n, n, None, d_tot)]
     k64 = abi.KERNEL_F64
+    tuned = None
+    if args.launch == "fused":
+        shape = (args.waves_per_cu, 0, 0)
+    elif args.launch.startswith("coop:"):
+        _, u, w = args.launch.split(":")
+        shape = (int(w), 0, int(u))
+    else:
+        shape, tuned = autotune(lib, jobs, k64, shapes=((args.waves_per_cu, 0),) + COOP_TUNE_SHAPES)
+        shape = tuple(shape) + (0,) * (3 - len(shape))
+        log(f"f64 launch: {shape_name(shape)} ({', '.join(f'{k} {v:.2f} ms' for k, v in tuned.items())})")
+    wpc, _, uw = shape
     for _ in range(args.warmup):
-        evaluate(lib, jobs, k64, waves_per_cu=args.waves_per_cu, timed=True)
-    wall, kernel_ms = timed_steps(lib, jobs, k64, args.steps, args.waves_per_cu)
+        evaluate(lib, jobs, k64, waves_per_cu=wpc, timed=True, unit_waves=uw)
+    wall, kernel_ms = timed_steps(lib, jobs, k64, args.steps, wpc, 0, uw)
     total64 = d_tot.download(n)
     value = n * args.steps / wall
     achieved = flops * n / (kernel_ms * 1e-3) / 1e12
@@ -270,9 +309,9 @@ def main():
             lc_jobs.append((fp, e, n, n, None, t))
             lc_hold += [e, t]
         ident.free()
-        (lw, lb), _ = autotune(lib, lc_jobs, k64, key="lc5")
-        _, ms_lc = timed_steps(lib, lc_jobs, k64, 3, lw, lb)
-        extra["gg_ggg_lc_sum"] = {"orderings": len(orders), "points": n, "tuned": f"{lw}x{lb}",
+        lshape, _ = autotune(lib, lc_jobs, k64, shapes=TUNE_SHAPES + COOP_TUNE_SHAPES, key="lc5")
+        _, ms_lc = timed_steps(lib, lc_jobs, k64, 3, *(tuple(lshape) + (0,))[:3])
+        extra["gg_ggg_lc_sum"] = {"orderings": len(orders), "points": n, "tuned": shape_name(lshape),
                                   "kernel_ms": ms_lc, "points_per_s": n / (ms_lc * 1e-3),
                                   "plan_runs_per_s": len(orders) * n / (ms_lc * 1e-3)}
         log(f"gg>ggg LC sum over {len(orders)} orderings: {n / (ms_lc * 1e-3):.4g} points/s")
@@ -286,9 +325,9 @@ def main():
         f6 = FusedPlan(lib, p6)
         t6 = dev.empty(n6)
         j6 = [(f6, e6, n6, n6, None, t6)]
-        (wpc, block), sweep = autotune(lib, j6, k64)
-        best = f"{wpc}x{block}"
-        _, ms64 = timed_steps(lib, j6, k64, 3, wpc, block)
+        shape6, sweep = autotune(lib, j6, k64, shapes=TUNE_SHAPES + COOP_TUNE_SHAPES)
+        best = shape_name(shape6)
+        _, ms64 = timed_steps(lib, j6, k64, 3, *(tuple(shape6) + (0,))[:3])
         ref6 = t6.download(n6)
         f6flops = plans.plan_flops(p6)[0]
         res6 = {"points": n6, "plan_rows": len(p6.rows), "flops_per_point": f6flops, "tuned": best, "sweep_ms": sweep,
@@ -368,8 +407,9 @@ def main():
         cpu = cpu_baseline(plan, ext_host, total64, sample)
         log(f"cpu {cpu['value']:.4g} points/s")
 
-    traffic = measured_traffic("fused_kernel<KF64>", n, len(plan.rows), args.waves_per_cu or 12)  # (the f64 default)
-    slot = slot_summary(lib, plan)
+    kname = kernel_name(shape)
+    traffic = measured_traffic(kname, n, len(plan.rows), shape_name(shape if uw else (wpc or 12, 256)))
+    slot = slot_summary(lib, plan, shape)
     alg_bytes = plan.legs * 4 * 8 + 8  # the momenta in, |M|^2 out, per point
     line = {
         "metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
@@ -379,7 +419,8 @@ def main():
         "config": {"workload": f"g g > g g g LC plan run (one colour ordering, {len(plan.outputs)} helicities summed), "
                                f"f64, {n} RAMBO points, sqrt(s)=1 TeV, pT>20 GeV, |eta|<2.5, dR>0.4",
                    "points": n, "plan_rows": len(plan.rows), "current_planes": plan.current_planes,
-                   "flops_per_point": flops, "divides_per_point": divs, "parallelism": "1 GPU, persistent fused kernel"},
+                   "flops_per_point": flops, "divides_per_point": divs, "parallelism": "1 GPU, persistent fused kernel",
+                   "launch": shape_name(shape if uw else (wpc or 12, 256)), "launch_sweep_ms": tuned},
         # The fused f64 kernel's rocprofv3 counters (profiles/) show it bound by the traffic of its
         # scratch slots beyond L2 (about 78% of the HBM peak, VALU busy about half the time), so the
         # roofline is the HBM one: achieved = the algorithmic bytes (momenta in, |M|^2 out) per
@@ -393,7 +434,7 @@ def main():
                      "algorithmic_bytes_per_point": alg_bytes,
                      "traffic_tb_per_s": traffic / (kernel_ms * 1e-3) / 1e12 if traffic else None,
                      "traffic_frac_of_hbm_peak": traffic / (kernel_ms * 1e-3) / 1e12 / HBM_PEAK_TBS if traffic else None,
-                     "kernel": "fused_kernel<KF64>", "kernel_ms": kernel_ms,
+                     "kernel": kname, "kernel_ms": kernel_ms,
                      "slot": slot,
                      "compute": {"pipe": "fp64 VALU", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                                  "frac": achieved / FP64_PEAK_TFLOPS, "flops_per_point": flops,

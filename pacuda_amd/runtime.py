@@ -230,22 +230,35 @@ class Sweep:
 # (waves per CU, block) pairs the autotuner tries: 2 to 6 persistent waves per SIMD, blocks of
 # one or four waves
 TUNE_SHAPES = ((6, 64), (8, 128), (8, 256), (12, 256), (16, 256), (16, 64))
+# ... and the cooperative launches (waves per CU, block ignored, wavefronts per unit), which only
+# the f64 kernel runs, for plans without symjit rows
+COOP_TUNE_SHAPES = tuple((per_cu, 0, uw) for uw, per_cu in COOP_SHAPES)
 _tuned = {}
 
 
+def shape_name(shape):
+    """"WxB" for (waves_per_cu, block), "coop UxW" for a cooperative launch (wavefronts per unit
+    U, waves per CU W)"""
+    return f"coop {shape[2]}x{shape[0]}" if len(shape) > 2 and shape[2] else f"{shape[0]}x{shape[1]}"
+
+
 def autotune(lib, jobs, kernel, shapes=TUNE_SHAPES, repeats=2, key=None):
-    """the (waves_per_cu, block) of `shapes` with the shortest persistent launch of `jobs` (the
-    minimum of `repeats` timed launches after one warm-up each); cached under `key` when given.
-    Returns ((waves_per_cu, block), {"WxB": ms})."""
+    """the launch shape of `shapes` -- (waves_per_cu, block) or (waves_per_cu, block, unit_waves)
+    -- with the shortest persistent launch of `jobs` (the minimum of `repeats` timed launches
+    after one warm-up each); cached under `key` when given. Returns (shape, {name: ms})."""
     if key is not None and key in _tuned:
         return _tuned[key]
-    table = {}
-    for wpc, block in shapes:
-        evaluate(lib, jobs, kernel, waves_per_cu=wpc, block=block, timed=True)
-        table[f"{wpc}x{block}"] = min(evaluate(lib, jobs, kernel, waves_per_cu=wpc, block=block, timed=True)
-                                      for _ in range(repeats))
+    table, by_name = {}, {}
+    for shape in shapes:
+        wpc, block = shape[0], shape[1]
+        uw = shape[2] if len(shape) > 2 else 0
+        evaluate(lib, jobs, kernel, waves_per_cu=wpc, block=block, timed=True, unit_waves=uw)
+        name = shape_name(shape)
+        table[name] = min(evaluate(lib, jobs, kernel, waves_per_cu=wpc, block=block, timed=True, unit_waves=uw)
+                          for _ in range(repeats))
+        by_name[name] = tuple(shape)
     best = min(table, key=table.get)
-    out = (tuple(int(v) for v in best.split("x")), table)
+    out = (by_name[best], table)
     if key is not None:
         _tuned[key] = out
     return out
