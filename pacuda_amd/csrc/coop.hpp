@@ -155,8 +155,12 @@ inline int first_fit(std::vector<std::vector<uint8_t>> &occ, const Obj &o, bool 
 // wavefronts and `lds_cap` LDS planes of 1 KB; `used_amps` marks the amplitude planes the
 // reduction reads. ok is false when a row or an access is not one the pass models (symjit rows,
 // accesses to parts of a current), or an amplitude the reduction reads has no closure.
+// `late`: each task but the sources goes to the last phase before its first reader instead of the
+// first after its last producer (the same number of phases; values read far from where they are
+// made live shorter). `rank`: the LDS ranking's lifetime weight (0: square root, 1: linear, 2: none).
 inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, uint32_t amplitudes, uint32_t forms,
-                                const std::vector<uint8_t> &used_amps, uint32_t waves, uint32_t lds_cap) {
+                                const std::vector<uint8_t> &used_amps, uint32_t waves, uint32_t lds_cap, bool late = false,
+                                int rank = 0) {
     using namespace coop_detail;
     CoopProgram cp;
     cp.waves = waves;
@@ -262,6 +266,13 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
         for (uint32_t d : dep[t]) phase[t] = std::max(phase[t], phase[d] + 1);
         P = std::max(P, phase[t]);
     }
+    if (late) {  // as late as the readers allow (a task with no reader: the last phase), sources first
+        std::vector<uint32_t> latest(T, P);
+        for (uint32_t t = T; t-- > 0;)
+            for (uint32_t d : dep[t]) latest[d] = std::min(latest[d], latest[t] - 1);
+        for (uint32_t t = 0; t < T; t++)
+            if ((ops[task_first[t]].w[0] & 0xff) != K_SOURCE) phase[t] = latest[t];
+    }
     cp.phases = P;
     // wavefronts: longest task first to the least-loaded wavefront, per phase
     std::vector<uint32_t> cost(T, 0), wave_of(T, 0);
@@ -339,7 +350,8 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
     for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
     auto density = [&](uint32_t i) {
         const Obj &o = objs[i];
-        return (double)o.uses / o.size / std::sqrt((double)(o.hi - o.lo + 1));
+        const double life = (double)(o.hi - o.lo + 1);
+        return (double)o.uses / o.size / (rank == 0 ? std::sqrt(life) : rank == 1 ? life : 1.0);
     };
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return density(a) > density(b); });
     std::vector<std::vector<uint8_t>> lds(P + 2, std::vector<uint8_t>(lds_cap, 0));

@@ -428,7 +428,7 @@ template <class R, bool REG = false, int NF = 0, bool COOP = false> struct SlotA
     template <class Q> PCD Cx<R> cur(uint32_t p, uint32_t z = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) {
-            if (p & DUAL) {
+            if (!COOP && (p & DUAL)) {  // (cooperative programs have no dual fields)
                 V *c = lds_at(lds + dual_chip(p) * PLANE);
                 if (!(p & DUAL_FILL)) return Packed<R>::get(*c);
                 const V x = mem_ld(p & (LDS_PLANE - 1), z);
@@ -442,7 +442,7 @@ template <class R, bool REG = false, int NF = 0, bool COOP = false> struct SlotA
     template <class Q> PCD void put(uint32_t p, Cx<R> v, uint32_t z = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) {
-            if (p & DUAL) {
+            if (!COOP && (p & DUAL)) {  // (cooperative programs have no dual fields)
                 *lds_at(lds + dual_chip(p) * PLANE) = Packed<R>::set(v);
                 if (p & DUAL_THROUGH) mem_st(p & (LDS_PLANE - 1), Packed<R>::set(v), z);
                 return;
@@ -479,7 +479,7 @@ template <class R, bool REG = false, int NF = 0, bool COOP = false> struct SlotA
     template <class Q, int N> PCD void curs(uint32_t p, Cx<R> *v, uint32_t zmask = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) {
-            if (p & DUAL) {
+            if (!COOP && (p & DUAL)) {  // (cooperative programs have no dual fields)
                 if (p & DUAL_FILL) {  // (copies go to LDS: the register tier takes objects on writes)
                     const uint32_t o = lds + dual_chip(p) * PLANE;
 #pragma unroll
@@ -502,7 +502,7 @@ template <class R, bool REG = false, int NF = 0, bool COOP = false> struct SlotA
     template <class Q, int N> PCD void puts(uint32_t p, const Cx<R> *v, uint32_t zmask = 0) const {
         static_assert(__is_same(Q, R), "one value type per slot");
         if (p >= LDS_PLANE) {
-            if (p & DUAL) {
+            if (!COOP && (p & DUAL)) {  // (cooperative programs have no dual fields)
                 chip_set<N>(dual_chip(p), v);
                 if (p & DUAL_THROUGH) {
 #pragma unroll

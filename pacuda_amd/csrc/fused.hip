@@ -41,9 +41,9 @@ constexpr int LDS_VARIANTS = 5;
 // the cooperative shapes (wavefronts per unit, units per compute unit): one 64-point unit per
 // workgroup, LDS_BYTES / units per CU of LDS for it (its top COOP_CONTROL bytes hold the unit
 // number), so (LDS_BYTES / units - COOP_CONTROL) / 1 KB planes of 64 f64 values
-constexpr int COOP_SHAPES = 3;
-constexpr uint32_t COOP_WAVES[COOP_SHAPES] = {16, 8, 8};
-constexpr uint32_t COOP_UNITS[COOP_SHAPES] = {1, 1, 2};
+constexpr int COOP_SHAPES = 4;
+constexpr uint32_t COOP_WAVES[COOP_SHAPES] = {16, 8, 8, 4};
+constexpr uint32_t COOP_UNITS[COOP_SHAPES] = {1, 1, 2, 4};
 constexpr uint32_t COOP_CONTROL = 256;
 
 struct PlanDev {
@@ -286,10 +286,10 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
 // the plan's cooperative program (whose barrier rows separate its phases; currents, forms and
 // amplitudes in LDS or in the workgroup's slot), meet again, and wavefront 0 reduces the
 // amplitudes as fused_kernel does. f64 only: LDS planes hold 16 bytes per lane.
-template <class K, int W, int UNITS>
+template <class K, int SHAPE, int W = COOP_WAVES[SHAPE], int UNITS = COOP_UNITS[SHAPE]>
 __global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W * UNITS / 4)))
 coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start, uint32_t item_count,
-            uint32_t *counter, double *scratch, size_t slot_doubles, uint32_t shape, uint32_t lds_bytes) {
+            uint32_t *counter, double *scratch, size_t slot_doubles, uint32_t lds_bytes) {
     typedef typename K::real R;
     static_assert(__is_same(R, double) && __is_same(typename K::den, double) && __is_same(typename K::amp, double),
                   "cooperative programs run in f64");
@@ -317,12 +317,11 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
         const bool active = i < it.points;
         const size_t ie = active ? i : it.points - 1;  // inactive lanes repeat the last point
         const uint32_t A = cnst(plan)->amplitude_planes, F = cnst(plan)->form_count;
-        // (selects, not an index: a runtime index into the plan's arrays would put them in scratch)
-        const Instr *prog = shape == 0 ? cnst(plan)->coop_prog[0] : shape == 1 ? cnst(plan)->coop_prog[1] : cnst(plan)->coop_prog[2];
-        const uint32_t *starts = shape == 0 ? cnst(plan)->coop_start[0] : shape == 1 ? cnst(plan)->coop_start[1] : cnst(plan)->coop_start[2];
-        const uint32_t *form_field = shape == 0 ? cnst(plan)->coop_form[0] : shape == 1 ? cnst(plan)->coop_form[1] : cnst(plan)->coop_form[2];
-        const uint32_t *amp_field = shape == 0 ? cnst(plan)->coop_amp[0] : shape == 1 ? cnst(plan)->coop_amp[1] : cnst(plan)->coop_amp[2];
-        const uint32_t C = shape == 0 ? cnst(plan)->coop_slot[0] : shape == 1 ? cnst(plan)->coop_slot[1] : cnst(plan)->coop_slot[2];
+        const Instr *prog = cnst(plan)->coop_prog[SHAPE];
+        const uint32_t *starts = cnst(plan)->coop_start[SHAPE];
+        const uint32_t *form_field = cnst(plan)->coop_form[SHAPE];
+        const uint32_t *amp_field = cnst(plan)->coop_amp[SHAPE];
+        const uint32_t C = cnst(plan)->coop_slot[SHAPE];
         SlotArena<R, false, 0, true> a;
         a.regs = nullptr;
         a.fregs = nullptr;
@@ -1617,9 +1616,16 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
     {
         std::vector<uint8_t> used(d->amplitude_planes, 0);
         for (uint32_t k = 0; k < count; k++) used[d->planes[k]] = 1;
+        // per shape, the candidate (phases as early or as late as possible, three LDS rankings)
+        // that leaves the fewest current accesses in the slot
         for (int s = 0; s < COOP_SHAPES; s++)
-            c.coop[s] = coop_program(c.fwd, c.planes, d->amplitude_planes, d->form_count, used, COOP_WAVES[s],
-                                     (LDS_BYTES / COOP_UNITS[s] - COOP_CONTROL) / 1024);
+            for (int late = 0; late < 2; late++)
+                for (int rank = 0; rank < 3; rank++) {
+                    CoopProgram q = coop_program(c.fwd, c.planes, d->amplitude_planes, d->form_count, used,
+                                                 COOP_WAVES[s], (LDS_BYTES / COOP_UNITS[s] - COOP_CONTROL) / 1024,
+                                                 late, rank);
+                    if (q.ok && (!c.coop[s].ok || q.slot_accesses < c.coop[s].slot_accesses)) c.coop[s] = std::move(q);
+                }
     }
     // the on-chip programs: the passes over the rows as built and over each candidate order, the
     // allocator's placement (the static one where it gives up), keeping per variant and tier the
@@ -2001,30 +2007,37 @@ bool launch(const Schedule &sc, hipStream_t s, int tier, size_t bytes, const Ite
     return true;
 }
 
-template <int W, int U> bool allow_coop_lds() {
-    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(coop_kernel<KF64, W, U>),
-                                                           hipFuncAttributeMaxDynamicSharedMemorySize, LDS_BYTES / U),
+template <int SHAPE> bool allow_coop_lds() {
+    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(coop_kernel<KF64, SHAPE>),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                           LDS_BYTES / COOP_UNITS[SHAPE]),
                                        "cooperative kernel LDS");
     return ok;
 }
+static_assert(COOP_SHAPES == 4, "the cooperative kernels instantiated below");
 bool allow_coop_lds(int shape) {
-    return shape == 0 ? allow_coop_lds<16, 1>() : shape == 1 ? allow_coop_lds<8, 1>() : allow_coop_lds<8, 2>();
+    return shape == 0 ? allow_coop_lds<0>() : shape == 1 ? allow_coop_lds<1>() : shape == 2 ? allow_coop_lds<2>()
+                                                                                             : allow_coop_lds<3>();
 }
-static_assert(COOP_SHAPES == 3 && COOP_WAVES[0] == 16 && COOP_UNITS[0] == 1 && COOP_WAVES[1] == 8 && COOP_UNITS[1] == 1 &&
-                  COOP_WAVES[2] == 8 && COOP_UNITS[2] == 2,
-              "the cooperative kernels instantiated");
+
+template <int SHAPE>
+void launch_coop_shape(const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts, uint32_t count,
+                       uint32_t *counter, double *scratch) {
+    const uint32_t bytes = LDS_BYTES / COOP_UNITS[SHAPE];
+    coop_kernel<KF64, SHAPE><<<sc.grid, COOP_WAVES[SHAPE] * LANES, bytes, s>>>(items, starts, count, counter, scratch,
+                                                                           sc.slot, bytes);
+}
 
 bool launch_coop(const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts, uint32_t count,
                  uint32_t *counter, double *scratch) {
     if (!allow_coop_lds(sc.coop)) return false;
-    const uint32_t bytes = LDS_BYTES / COOP_UNITS[sc.coop];
     reset_counter<<<1, 64, 0, s>>>(counter);
-    if (sc.coop == 0)
-        coop_kernel<KF64, 16, 1><<<sc.grid, 1024, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, 0, bytes);
-    else if (sc.coop == 1)
-        coop_kernel<KF64, 8, 1><<<sc.grid, 512, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, 1, bytes);
-    else
-        coop_kernel<KF64, 8, 2><<<sc.grid, 512, bytes, s>>>(items, starts, count, counter, scratch, sc.slot, 2, bytes);
+    switch (sc.coop) {
+    case 0: launch_coop_shape<0>(sc, s, items, starts, count, counter, scratch); break;
+    case 1: launch_coop_shape<1>(sc, s, items, starts, count, counter, scratch); break;
+    case 2: launch_coop_shape<2>(sc, s, items, starts, count, counter, scratch); break;
+    default: launch_coop_shape<3>(sc, s, items, starts, count, counter, scratch); break;
+    }
     return true;
 }
 

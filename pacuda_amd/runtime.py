@@ -99,7 +99,7 @@ def plan_desc(plan, arrays, no_sort=0):
 
 
 # the cooperative shapes (pacuda_plan_coop_program): (wavefronts per unit, waves per CU)
-COOP_SHAPES = ((16, 16), (8, 8), (8, 16))
+COOP_SHAPES = ((16, 16), (8, 8), (8, 16), (4, 16))
 
 
 def coop_program(lib, plan, shape, serial=False):

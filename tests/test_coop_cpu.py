@@ -18,7 +18,7 @@ import harness
 import refcpu
 
 K_BARRIER = 8
-LDS_CAP = {0: 159, 1: 159, 2: 79}
+LDS_CAP = {0: 159, 1: 159, 2: 79, 3: 39}
 
 
 @pytest.fixture(scope="module")

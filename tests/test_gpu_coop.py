@@ -155,7 +155,7 @@ def test_coop_refusals(lib):
         assert status(fp, F64, 16, 16) == 2
         assert status(good, abi.FUSED_KERNELS["mixed"], 16, 16) == 2
         assert status(good, F64, 12, 16) == 2
-        assert status(good, F64, 16, 4) == 2
+        assert status(good, F64, 16, 2) == 2
     finally:
         d.free()
         d_tot.free()
