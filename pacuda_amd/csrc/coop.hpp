@@ -26,8 +26,11 @@
 //     the writes). The rest go to the workgroup's slot the same way, without a limit.
 //
 // Each wavefront's stream holds its tasks phase by phase with a barrier row after each phase but
-// the last, so all wavefronts meet the same barriers. The forms phase, the barrier after it, the
-// barrier before the reduction and the reduction itself are the kernel's.
+// the last, so all wavefronts meet the same barriers, and one stream holds a fetch row that takes
+// the workgroup's next unit off the work queue. The forms phase, the barrier after it, the
+// barrier before the reduction and the reduction itself are the kernel's; the kernel computes the
+// next unit's forms while one wavefront reduces this unit's amplitudes, so an amplitude's LDS room
+// is never a form's.
 #pragma once
 #include <stdint.h>
 
@@ -58,7 +61,9 @@ struct CoopProgram {
 
 namespace coop_detail {
 
-// a row's rough cost in arithmetic (for balancing the wavefronts of a phase)
+// a row's rough cost in arithmetic (for balancing the wavefronts of a phase), and the charges per
+// current component a row moves through LDS or the slot
+constexpr uint32_t COST_LDS = 2, COST_SLOT = 16;
 inline uint32_t row_cost(const Instr &o) {
     const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff;
     switch (kind) {
@@ -129,8 +134,11 @@ struct Obj {
     int kind = 0;           // 0 current, 1 amplitude, 2 form
 };
 
-// first-fit placement of [lo, hi] x size in occupancy occ[phase][plane] (capacity occ[0].size())
+// first-fit placement of [lo, hi] x size in occupancy occ[phase][plane] (capacity occ[0].size());
+// an object that lives into the reduction (phase P + 1 = occ.size() - 1) also holds its room in
+// phase 0: the kernel computes the next unit's forms while it reduces this unit's amplitudes
 inline int first_fit(std::vector<std::vector<uint8_t>> &occ, const Obj &o, bool grow) {
+    const bool wrap = o.hi + 1 == occ.size();
     for (uint32_t s = 0;; s++) {
         if (s + o.size > occ[0].size()) {
             if (!grow) return -1;
@@ -140,9 +148,12 @@ inline int first_fit(std::vector<std::vector<uint8_t>> &occ, const Obj &o, bool 
         for (uint32_t p = o.lo; p <= o.hi && free; p++)
             for (uint32_t q = s; q < s + o.size && free; q++)
                 if (occ[p][q]) free = false;
+        for (uint32_t q = s; wrap && q < s + o.size && free; q++)
+            if (occ[0][q]) free = false;
         if (free) {
             for (uint32_t p = o.lo; p <= o.hi; p++)
                 for (uint32_t q = s; q < s + o.size; q++) occ[p][q] = 1;
+            for (uint32_t q = s; wrap && q < s + o.size; q++) occ[0][q] = 1;
             return (int)s;
         }
     }
@@ -274,21 +285,6 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
             if ((ops[task_first[t]].w[0] & 0xff) != K_SOURCE) phase[t] = latest[t];
     }
     cp.phases = P;
-    // wavefronts: longest task first to the least-loaded wavefront, per phase
-    std::vector<uint32_t> cost(T, 0), wave_of(T, 0);
-    for (size_t n = 0; n < N; n++) cost[task_of[n]] += row_cost(ops[n]);
-    for (uint32_t p = 1; p <= P; p++) {
-        std::vector<uint32_t> ts;
-        for (uint32_t t = 0; t < T; t++)
-            if (phase[t] == p) ts.push_back(t);
-        std::stable_sort(ts.begin(), ts.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
-        std::vector<uint64_t> load(waves, 0);
-        for (uint32_t t : ts) {
-            const uint32_t w = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
-            load[w] += cost[t];
-            wave_of[t] = w;
-        }
-    }
     // lifetimes of the currents
     for (uint32_t id = 0; id < current_objs; id++) {
         Obj &ob = objs[id];
@@ -370,6 +366,43 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
     }
     for (uint32_t id = 0; id < current_objs; id++)
         (objs[id].room >= 0 ? cp.lds_accesses : cp.slot_accesses) += objs[id].uses;
+    // wavefronts: longest task first to the least-loaded wavefront, per phase; the fetch of the
+    // next unit goes to the least-loaded wavefront of the phase where that one's slack (the phase's
+    // largest load minus its own) is largest
+    // (a task's cost: its arithmetic, plus a charge per current component it moves through the
+    // slot or LDS, the slot's being its wait on L2 or beyond)
+    std::vector<uint32_t> cost(T, 0), wave_of(T, 0);
+    for (size_t n = 0; n < N; n++) {
+        cost[task_of[n]] += row_cost(ops[n]);
+        Field f[3];
+        const int nf = current_fields(ops[n], f);
+        for (int k = 0; k < nf; k++) {
+            const int id = fobj[n][f[k].word];
+            const uint32_t moved = (f[k].read ? f[k].len : 0) + (f[k].write ? f[k].len : 0);
+            cost[task_of[n]] += moved * (objs[id].room >= 0 ? COST_LDS : COST_SLOT);
+        }
+    }
+    uint32_t fetch_phase = 1, fetch_wave = 0;
+    uint64_t best_slack = 0;
+    for (uint32_t p = 1; p <= P; p++) {
+        std::vector<uint32_t> ts;
+        for (uint32_t t = 0; t < T; t++)
+            if (phase[t] == p) ts.push_back(t);
+        std::stable_sort(ts.begin(), ts.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+        std::vector<uint64_t> load(waves, 0);
+        for (uint32_t t : ts) {
+            const uint32_t w = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[w] += cost[t];
+            wave_of[t] = w;
+        }
+        const uint32_t least = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        const uint64_t slack = *std::max_element(load.begin(), load.end()) - load[least];
+        if (p == 1 || slack > best_slack) {
+            best_slack = slack;
+            fetch_phase = p;
+            fetch_wave = least;
+        }
+    }
     // the rewritten rows: device fields, and the oracle's
     auto current_field = [&](int id, bool oracle) {
         const Obj &o = objs[id];
@@ -407,13 +440,16 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
         ora[n] = r;
     }
     // the streams
-    Instr barrier;
+    Instr barrier, fetch;
     memset(&barrier, 0, sizeof barrier);
     barrier.w[0] = K_BARRIER;
+    memset(&fetch, 0, sizeof fetch);
+    fetch.w[0] = K_FETCH;
     cp.start.assign(waves + 1, 0);
     for (uint32_t w = 0; w < waves; w++) {
         cp.start[w] = (uint32_t)cp.ops.size();
         for (uint32_t p = 1; p <= P; p++) {
+            if (p == fetch_phase && w == fetch_wave) cp.ops.push_back(fetch);
             for (uint32_t t = 0; t < T; t++)
                 if (phase[t] == p && wave_of[t] == w)
                     for (uint32_t r = 0; r < task_rows[t]; r++) cp.ops.push_back(dev[task_first[t] + r]);
@@ -426,13 +462,20 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
             for (uint32_t t = 0; t < T; t++)
                 if (phase[t] == p && wave_of[t] == w)
                     for (uint32_t r = 0; r < task_rows[t]; r++) cp.serial.push_back(ora[task_first[t] + r]);
-    // checks: every wavefront meets P - 1 barriers; the placement keeps one object per room and
-    // phase (an object never shares a room with one live in any of its phases)
+    // checks: every wavefront meets P - 1 barriers, and the streams hold one fetch; the placement
+    // keeps one object per room and phase (an object never shares a room with one live in any of
+    // its phases)
+    uint32_t fetches = 0;
     for (uint32_t w = 0; w < waves; w++) {
         uint32_t b = 0;
-        for (uint32_t n = cp.start[w]; n < cp.start[w + 1]; n++) b += (cp.ops[n].w[0] & 0xff) == K_BARRIER;
+        for (uint32_t n = cp.start[w]; n < cp.start[w + 1]; n++) {
+            b += (cp.ops[n].w[0] & 0xff) == K_BARRIER;
+            fetches += (cp.ops[n].w[0] & 0xff) == K_FETCH;
+        }
         if (b != P - 1) return cp;
     }
+    if (fetches != 1) return cp;
+    // (an amplitude's room is also held in phase 0, where the next unit's forms are written)
     std::vector<std::vector<int>> held(P + 2, std::vector<int>(std::max(cp.lds_planes, 1u), -1));
     std::vector<std::vector<int>> held_slot(P + 2, std::vector<int>(std::max(cp.slot_planes, 1u), -1));
     for (uint32_t i = 0; i < objs.size(); i++) {
@@ -444,6 +487,11 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
             for (uint32_t q = s; q < s + o.size; q++) {
                 if (q >= h[p].size() || h[p][q] >= 0) return cp;
                 h[p][q] = (int)i;
+            }
+        if (o.hi == P + 1 && o.room >= 0)
+            for (uint32_t q = s; q < s + o.size; q++) {
+                if (h[0][q] >= 0) return cp;
+                h[0][q] = (int)i;
             }
     }
     if (cp.lds_planes > lds_cap) return cp;

@@ -576,6 +576,12 @@ template <class R, bool REG = false, int NF = 0, bool COOP = false> struct SlotA
     PCD const double *fim() const { return fim_; }
     const uint32_t *bind_;  // the plan's symjit bindings
     uint32_t point_, points_;  // the lane's point in its item, the item's points
+    // (COOP) the work counter and the LDS word that receives the workgroup's next unit
+    uint32_t *counter;
+    volatile uint32_t *next_unit;
+    PCD void fetch() const {
+        if (idx == 0) *next_unit = atomicAdd(counter, 1u);
+    }
     PCD const uint32_t *bind() const { return bind_; }
     PCD uint32_t point() const { return point_; }
     PCD uint32_t points() const { return points_; }

@@ -73,6 +73,7 @@ struct PlanDev {
     const uint32_t *coop_start[COOP_SHAPES];  // their offsets (wavefronts + 1)
     const uint32_t *coop_form[COOP_SHAPES];   // per form: FORM_LDS | LDS plane, or the slot form
     const uint32_t *coop_amp[COOP_SHAPES];    // per amplitude plane: LDS_PLANE + LDS plane, or the slot amplitude
+    const uint32_t *coop_red[COOP_SHAPES];    // per term of the reduction (`planes`): its amplitude's field
     uint32_t coop_slot[COOP_SHAPES];          // slot current planes
 };
 
@@ -281,11 +282,15 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
 }
 
 // The cooperative kernel: each workgroup takes one 64-point unit at a time off the same work
-// queue, and its W wavefronts share it: they compute the unit's momentum forms (form f by
-// wavefront f mod W) into the unit's LDS region or slot, meet at a barrier, run their streams of
-// the plan's cooperative program (whose barrier rows separate its phases; currents, forms and
-// amplitudes in LDS or in the workgroup's slot), meet again, and wavefront 0 reduces the
-// amplitudes as fused_kernel does. f64 only: LDS planes hold 16 bytes per lane.
+// queue, and its W wavefronts share it. Per unit: the momentum forms (form f by wavefront f mod W,
+// into the unit's LDS region or slot), a barrier, each wavefront's stream of the plan's
+// cooperative program (whose barrier rows separate its phases; currents, forms and amplitudes in
+// LDS or in the workgroup's slot), a barrier, then wavefront 0 reduces the amplitudes as
+// fused_kernel does while the others (and then wavefront 0) compute the next unit's forms: the
+// cooperative programs never give an amplitude a form's LDS room, and when the next unit belongs
+// to another plan a barrier ends the reduction first. A fetch row in one wavefront's stream takes
+// the next unit off the queue while the current one runs. f64 only: LDS planes hold 16 bytes per
+// lane.
 template <class K, int SHAPE, int W = COOP_WAVES[SHAPE], int UNITS = COOP_UNITS[SHAPE]>
 __global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W * UNITS / 4)))
 coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start, uint32_t item_count,
@@ -296,49 +301,40 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
     extern __shared__ __attribute__((aligned(16))) char pacuda_lds[];
     const uint32_t lane = threadIdx.x & (LANES - 1);
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / LANES);
-    double *slot = scratch + (size_t)blockIdx.x * slot_doubles;
     volatile uint32_t *control = reinterpret_cast<volatile uint32_t *>(pacuda_lds + lds_bytes - COOP_CONTROL);
     const uint32_t units = cnst(unit_start)[item_count];
-    for (;;) {
-        // (the barrier also ends the previous unit's reduction before its LDS is reused)
-        if (threadIdx.x == 0) control[0] = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t unit = __builtin_amdgcn_readfirstlane(control[0]);
-        if (unit >= units) break;
+    SlotArena<R, false, 0, true> a;
+    a.regs = nullptr;
+    a.fregs = nullptr;
+    a.lds_cap = 0;
+    a.slot = __builtin_amdgcn_make_buffer_rsrc(scratch + (size_t)blockIdx.x * slot_doubles, (short)0,
+                                               (int)(slot_doubles * sizeof(double)), BUFFER_FORMAT);
+    a.idx = lane;
+    a.bind_ = nullptr;
+    a.lds = lane * 16u;
+    a.counter = counter;
+    a.next_unit = control + 1;
+    // the item of a unit (uniform binary search) and the lane's point in it
+    auto locate = [&](uint32_t u, ItemDev &it, uint32_t &i) {
         uint32_t lo = 0, hi = item_count;
         while (hi - lo > 1) {
             uint32_t mid = (lo + hi) / 2;
-            if (cnst(unit_start)[mid] <= unit) lo = mid;
+            if (cnst(unit_start)[mid] <= u) lo = mid;
             else hi = mid;
         }
-        const ItemDev it = cnst(items)[lo];
+        it = cnst(items)[lo];
+        i = (u - cnst(unit_start)[lo]) * LANES + lane;
+    };
+    // the arena's regions for a unit of `plan`, then the unit's momentum forms in f64
+    // (runner/recurrence_runner.cu:745-764), form f by wavefront f mod W
+    auto forms = [&](const ItemDev &it, uint32_t i) {
         const PlanDev *plan = it.plan;
-        const uint32_t i = (unit - cnst(unit_start)[lo]) * LANES + lane;
-        const bool active = i < it.points;
-        const size_t ie = active ? i : it.points - 1;  // inactive lanes repeat the last point
-        const uint32_t A = cnst(plan)->amplitude_planes, F = cnst(plan)->form_count;
-        const Instr *prog = cnst(plan)->coop_prog[SHAPE];
-        const uint32_t *starts = cnst(plan)->coop_start[SHAPE];
-        const uint32_t *form_field = cnst(plan)->coop_form[SHAPE];
-        const uint32_t *amp_field = cnst(plan)->coop_amp[SHAPE];
-        const uint32_t C = cnst(plan)->coop_slot[SHAPE];
-        SlotArena<R, false, 0, true> a;
-        a.regs = nullptr;
-        a.fregs = nullptr;
-        a.lds_cap = 0;
-        a.slot = __builtin_amdgcn_make_buffer_rsrc(slot, (short)0, (int)(slot_doubles * sizeof(double)), BUFFER_FORMAT);
-        a.amp_base = C * 1024u;
-        a.form_base = a.amp_base + A * 1024u;
-        a.idx = lane;
-        a.bind_ = nullptr;
-        a.point_ = i;
-        a.points_ = it.points;
-        a.lds = lane * 16u;
-        const double *c64 = cnst(plan)->c64;
-        const uint32_t S = cnst(plan)->scale_count, X = cnst(plan)->factor_count;
-        a.sre_ = c64; a.sim_ = c64 + S; a.fre_ = c64 + 2 * S; a.fim_ = c64 + 2 * S + X;
-        // the momentum forms in f64 (runner/recurrence_runner.cu:745-764), form f by wavefront f mod W
+        a.amp_base = cnst(plan)->coop_slot[SHAPE] * 1024u;
+        a.form_base = a.amp_base + cnst(plan)->amplitude_planes * 1024u;
+        const size_t ie = i < it.points ? i : it.points - 1;  // inactive lanes repeat the last point
+        const uint32_t F = cnst(plan)->form_count;
         const uint32_t *form_terms = cnst(plan)->form_terms, *term_slots = cnst(plan)->term_slots;
+        const uint32_t *form_field = cnst(plan)->coop_form[SHAPE];
         const double *coefficients = cnst(plan)->coefficients;
         for (uint32_t f = wave; f < F; f += W) {
             const uint32_t first = cnst(form_terms)[2 * f], n = cnst(form_terms)[2 * f + 1];
@@ -354,22 +350,48 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
             }
             a.put_form(cnst(form_field)[f], v);
         }
-        __syncthreads();
+    };
+    if (threadIdx.x == 0) control[0] = atomicAdd(counter, 1u);
+    __syncthreads();
+    uint32_t unit = __builtin_amdgcn_readfirstlane(control[0]);
+    ItemDev it;
+    uint32_t i = 0;
+    if (unit < units) {
+        locate(unit, it, i);
+        forms(it, i);
+    }
+    __syncthreads();
+    while (unit < units) {
+        const PlanDev *plan = it.plan;
+        a.point_ = i;
+        a.points_ = it.points;
+        const double *c64 = cnst(plan)->c64;
+        const uint32_t S = cnst(plan)->scale_count, X = cnst(plan)->factor_count;
+        a.sre_ = c64; a.sim_ = c64 + S; a.fre_ = c64 + 2 * S; a.fim_ = c64 + 2 * S + X;
+        const Instr *prog = cnst(plan)->coop_prog[SHAPE];
+        const uint32_t *starts = cnst(plan)->coop_start[SHAPE];
         const uint32_t first = cnst(starts)[wave];
+        // (one wavefront's stream takes the next unit off the queue: a fetch row in the phase that
+        // hides the atomic's round trip best)
         interpret<K, false, true>(prog + first, cnst(starts)[wave + 1] - first, a);
         __syncthreads();
+        const uint32_t next = __builtin_amdgcn_readfirstlane(control[1]);
+        ItemDev nit = it;
+        uint32_t ni = 0;
+        if (next < units) locate(next, nit, ni);
         if (wave == 0) {
             // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
             const uint32_t outputs = cnst(plan)->outputs;
-            const uint32_t *offsets = cnst(plan)->offsets, *planes = cnst(plan)->planes;
+            const uint32_t *offsets = cnst(plan)->offsets, *fields = cnst(plan)->coop_red[SHAPE];
             const double *weights = cnst(plan)->weights;
             const bool factor = cnst(plan)->factor;
             const double numerator = cnst(plan)->numerator, denominator = cnst(plan)->denominator;
+            const bool active = i < it.points;
             double t = 0.0;
             for (uint32_t o = 0; o < outputs; o++) {
                 double acc = 0.0;
                 for (uint32_t k = cnst(offsets)[o]; k < cnst(offsets)[o + 1]; k++) {
-                    const Cx<double> amp = a.template amp<double>(cnst(amp_field)[cnst(planes)[k]]);
+                    const Cx<double> amp = a.template amp<double>(cnst(fields)[k]);
                     acc = acc + cnst(weights)[k] * __builtin_fma(amp.re, amp.re, amp.im * amp.im);
                 }
                 if (it.resolved && active) it.resolved[(size_t)i * outputs + o] = acc;
@@ -378,6 +400,14 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
             }
             if (it.total && active) it.total[i] = t;
         }
+        if (next < units) {
+            if (nit.plan != plan) __syncthreads();  // (another plan's rooms: after the reduction)
+            forms(nit, ni);
+        }
+        __syncthreads();
+        unit = next;
+        it = nit;
+        i = ni;
     }
 }
 
@@ -1812,6 +1842,9 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
         h.coop_start[s] = upload(p, q.start.data(), q.start.size(), &ok);
         h.coop_form[s] = upload(p, q.form_field.data(), q.form_field.size(), &ok);
         h.coop_amp[s] = upload(p, q.amp_field.data(), q.amp_field.size(), &ok);
+        std::vector<uint32_t> red(count);
+        for (uint32_t k = 0; k < count; k++) red[k] = q.amp_field[d->planes[k]];
+        h.coop_red[s] = upload(p, red.data(), red.size(), &ok);
         h.coop_slot[s] = q.slot_planes;
         p->coop_slot_doubles[s] =
             (uint64_t)LANES * (2ull * q.slot_planes + 2ull * d->amplitude_planes + 4ull * d->form_count);

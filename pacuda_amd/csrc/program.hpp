@@ -16,7 +16,7 @@
 
 namespace pacuda {
 
-enum { K_CLEAR = 1, K_CONTRIB, K_PROP, K_IDENTITY, K_CLOSURE, K_SOURCE, K_SYMJIT, K_BARRIER };
+enum { K_CLEAR = 1, K_CONTRIB, K_PROP, K_IDENTITY, K_CLOSURE, K_SOURCE, K_SYMJIT, K_BARRIER, K_FETCH };
 // a symjit binding's input code: bits 0-2 the kind below, bit 3 which parent or form, bit 4 the
 // imaginary part, bits 5 and up the component (or the scale slot) (runner/recurrence_runner.cu:
 // 582, 631-684)
@@ -36,7 +36,8 @@ enum { F_INIT = 1, F_FIRST = 2, F_LAST = 4, F_HOLD = 8, F_FROMACC = 16, F_ZERO =
 // slot, w[5] template; symjit: w[1] parent 0, w[2] parent 1, w[3] destination, w[4] form 0 |
 // form 1 << 16, w[5] factor, w[6] the binding's offset in the binding table (kernel, inputs,
 // output pairs, direct, then the input codes and the output components); barrier (cooperative
-// programs only): no operands, the wavefront waits for its workgroup.
+// programs only): no operands, the wavefront waits for its workgroup; fetch (cooperative
+// programs only, once per program): the wavefront takes the workgroup's next unit off the queue.
 struct alignas(32) Instr {
     uint32_t w[8];
 };
@@ -343,6 +344,8 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
             if constexpr (SYMJIT) symjit_row<K>(a, o, flags);
         } else if (kind == K_BARRIER) {
             if constexpr (COOP) __syncthreads();
+        } else if (kind == K_FETCH) {
+            if constexpr (COOP) a.fetch();
         } else if (kind == K_SOURCE) {  // a source row made again (programs of the f64 kernel only)
             if constexpr (__is_same(R, double) && __is_same(typename K::den, double)) source_row<R>(o, a);
         } else if (kind == K_CLEAR) {

@@ -7,8 +7,8 @@ workgroup's LDS or slot. Its serial order (phase by phase, wavefront by wavefron
 oracle's fused-program interpreter (oracle/refcpu.cpp, LDS planes mapped after the slot's) must
 equal the oracle's run of the plan bit for bit: that pins the dependences the phases honour and
 the placement (a clobbered current changes values). The streams' structure (the same barriers in
-every wavefront, the same rows as the serial order) is what the kernel relies on to neither
-deadlock nor skip work."""
+every wavefront, one fetch of the next unit, the same rows as the serial order) is what the kernel
+relies on to neither deadlock nor skip work."""
 import numpy as np
 import pytest
 
@@ -17,7 +17,7 @@ from pacuda_amd.runtime import coop_program, COOP_SHAPES
 import harness
 import refcpu
 
-K_BARRIER = 8
+K_BARRIER, K_FETCH = 8, 9
 LDS_CAP = {0: 159, 1: 159, 2: 79, 3: 39}
 
 
@@ -63,9 +63,12 @@ def test_streams_meet_the_same_barriers(lib, shape):
         kinds = dev[:, 0] & 0xff
         for w in range(W):
             assert np.count_nonzero(kinds[starts[w]:starts[w + 1]] == K_BARRIER) == P - 1, (name, w)
+        # one wavefront takes the next unit off the queue, once
+        assert np.count_nonzero(kinds == K_FETCH) == 1, name
         # every row of the plan's program once (the serial order holds them in phase order)
-        assert len(dev) - W * (P - 1) == len(ser), name
-        assert sorted(dev[kinds != K_BARRIER, 0].tolist()) == sorted(ser[:, 0].tolist()), name
+        assert len(dev) - W * (P - 1) - 1 == len(ser), name
+        rows = (kinds != K_BARRIER) & (kinds != K_FETCH)
+        assert sorted(dev[rows, 0].tolist()) == sorted(ser[:, 0].tolist()), name
 
 
 def test_more_lds_leaves_fewer_slot_accesses(lib):
