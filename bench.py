@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 from pacuda_amd import abi, plans, processes, phase_space as ps  # noqa: E402
 from pacuda_amd.runtime import (Device, FusedPlan, autotune, evaluate, shape_name, TUNE_SHAPES,  # noqa: E402
-                                COOP_TUNE_SHAPES)
+                                COOP_TUNE_SHAPES, COOP_TUNE_SHAPES_ANY)
 from pacuda_amd.unweight import Generator, Unweighting  # noqa: E402
 
 METRIC = "|M|^2 points/s (f64 bit-exact) + unweighted events/s, surrogate vs plain"
@@ -287,10 +287,11 @@ def main():
     if not args.no_modes:
         for name in ("df64", "mixed", "mixed64", "compensated", "compensated64"):
             k = abi.FUSED_KERNELS[name]
-            evaluate(lib, jobs, k, timed=True)
-            _, ms = timed_steps(lib, jobs, k, 3)
-            modes[name] = {"points_per_s": n / (ms * 1e-3), "kernel_ms": ms, **mode_errors(d_tot.download(n), total64)}
-            log(f"{name} {modes[name]['points_per_s']:.4g} points/s")
+            mshape, _ = autotune(lib, jobs, k, shapes=((0, 0),) + COOP_TUNE_SHAPES_ANY, repeats=1)
+            _, ms = timed_steps(lib, jobs, k, 3, *(tuple(mshape) + (0,))[:3])
+            modes[name] = {"points_per_s": n / (ms * 1e-3), "kernel_ms": ms, "launch": shape_name(mshape),
+                           **mode_errors(d_tot.download(n), total64)}
+            log(f"{name} {modes[name]['points_per_s']:.4g} points/s ({shape_name(mshape)})")
 
     extra = {}
     if not args.no_extra and not args.only_unweighting:
@@ -335,9 +336,9 @@ def main():
                 "f64_tflops": f6flops * n6 / (ms64 * 1e-3) / 1e12}
         for name in ("mixed", "mixed64", "compensated", "compensated64", "df64"):
             k = abi.FUSED_KERNELS[name]
-            (kw, kb), _ = autotune(lib, j6, k)
-            _, ms = timed_steps(lib, j6, k, 3, kw, kb)
-            res6[name] = {"points_per_s": n6 / (ms * 1e-3), "tuned": f"{kw}x{kb}",
+            kshape, _ = autotune(lib, j6, k, shapes=TUNE_SHAPES + COOP_TUNE_SHAPES_ANY)
+            _, ms = timed_steps(lib, j6, k, 3, *(tuple(kshape) + (0,))[:3])
+            res6[name] = {"points_per_s": n6 / (ms * 1e-3), "tuned": shape_name(kshape),
                           **mode_errors(t6.download(n6), ref6)}
         extra["gg_gggg"] = res6
         log(f"gg>gggg f64 {res6['f64_points_per_s']:.4g} points/s tuned {best}")

@@ -86,7 +86,8 @@ int pacuda_plan_program(const pacuda_plan_desc *desc, uint32_t variant, uint32_t
                         uint32_t *count, uint32_t *sources, uint32_t *planes, uint32_t *lds_planes);
 /*
  * The cooperative program of a plan for cooperative shape `shape` (0: 16 wavefronts per unit, one
- * unit per CU; 1: 8 and one; 2: 8 and two; 3: 4 and four). serial 0: the wavefronts' streams one after another
+ * unit per CU; 1: 8 and one; 2: 8 and two; 3: 4 and four) with 16-byte currents (f64, df64), or
+ * shape + 4 for the f32 currents of the mixed and compensated kernels (shapes 0 and 2). serial 0: the wavefronts' streams one after another
  * (starts, may be NULL, receives wavefronts + 1 offsets), each a share of the rows with barrier
  * rows (kind 8) between its phases, current fields at or above 2^24 in LDS, form fields with bit
  * 23 set an LDS plane pair, amplitude fields at or above 2^24 an LDS plane; serial 1: the same rows
@@ -150,11 +151,11 @@ typedef struct {
  * Evaluates every item in one persistent launch: each wavefront takes 64-point units of the
  * items off a device work queue and runs forms, sources, program and reduction for them with its
  * arena in an on-device scratch slot. Results equal the v1 path bit for bit for the same kernel.
- * Cooperative launch (unit_waves 16, 8 or 4, f64 kernel only): each workgroup of unit_waves
- * wavefronts takes one unit at a time and its wavefronts share it, running the plan's
- * cooperative program (its rows split into phases between workgroup barriers, the unit's
- * currents in the workgroup's LDS and slot); waves_per_cu 0 means 16, and waves_per_cu /
- * unit_waves units share a compute unit's LDS (16 / 16, 8 / 8, 16 / 8 or 16 / 4); block is ignored.
+ * Cooperative launch (unit_waves 16, 8 or 4): each workgroup of unit_waves wavefronts takes one
+ * unit at a time and its wavefronts share it, running the plan's cooperative program (its rows
+ * split into phases between workgroup barriers, the unit's currents in the workgroup's LDS and
+ * slot); waves_per_cu 0 means 16, and waves_per_cu / unit_waves units share a compute unit's LDS
+ * (f64: 16 / 16, 8 / 8, 16 / 8 or 16 / 4; the other kernels 16 / 16 or 16 / 8); block is ignored.
  * The same results bit for bit. Status 2 when a plan has no cooperative program (symjit rows).
  * Asynchronous on the stream unless kernel_ms is set. Calls may go to different streams: the
  * item table, work counter and slots are shared by all calls, so each call first waits on the

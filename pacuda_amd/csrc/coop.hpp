@@ -50,6 +50,7 @@ namespace pacuda {
 struct CoopProgram {
     std::vector<Instr> ops;            // wavefront 0's stream, then wavefront 1's, ...
     std::vector<uint32_t> start;       // W + 1 stream offsets
+    std::vector<uint32_t> split;       // per wavefront: where its first phase (sources and clears) ends in `ops`
     std::vector<Instr> serial;         // the same rows in one order (phase by phase, wavefront by wavefront), forms and amplitudes
                                        // under their plan indices and LDS planes after the slot's: for the CPU oracle
     std::vector<uint32_t> form_field;  // per form: FORM_LDS | LDS plane, or the slot form
@@ -162,21 +163,22 @@ inline int first_fit(std::vector<std::vector<uint8_t>> &occ, const Obj &o, bool 
 }  // namespace coop_detail
 
 // The cooperative program of `ops` (a fused program with forwarding and F_ZERO closures, over
-// `planes` current planes, the leading `sources` rows its source rows) for W = `waves`
-// wavefronts and `lds_cap` LDS planes of 1 KB; `used_amps` marks the amplitude planes the
-// reduction reads. ok is false when a row or an access is not one the pass models (symjit rows,
+// `planes` current planes) for W = `waves` wavefronts and `lds_cap` LDS planes of `plane_bytes`
+// (64 current values: 1024 for 16-byte currents, 512 for the f32 ones; a form takes 2 KB of LDS,
+// an amplitude plane 1 KB); `used_amps` marks the amplitude planes the reduction reads. ok is false when a row or an access is not one the pass models (symjit rows,
 // accesses to parts of a current), or an amplitude the reduction reads has no closure.
 // `late`: each task but the sources goes to the last phase before its first reader instead of the
 // first after its last producer (the same number of phases; values read far from where they are
 // made live shorter). `rank`: the LDS ranking's lifetime weight (0: square root, 1: linear, 2: none).
 inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, uint32_t amplitudes, uint32_t forms,
-                                const std::vector<uint8_t> &used_amps, uint32_t waves, uint32_t lds_cap, bool late = false,
-                                int rank = 0) {
+                                const std::vector<uint8_t> &used_amps, uint32_t waves, uint32_t lds_cap,
+                                uint32_t plane_bytes, bool late = false, int rank = 0) {
     using namespace coop_detail;
     CoopProgram cp;
     cp.waves = waves;
     const size_t N = ops.size();
-    if (waves == 0 || N == 0 || forms >= FORM_LDS) return cp;
+    if (waves == 0 || N == 0 || forms >= FORM_LDS || (plane_bytes != 1024 && plane_bytes != 512)) return cp;
+    const uint32_t amp_planes = 1024 / plane_bytes, form_planes = 2048 / plane_bytes;
     // tasks
     std::vector<uint32_t> task_of(N), task_first, task_rows;
     bool open_run = false, hold = false;
@@ -308,13 +310,14 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
         if (first_closure[q] == UINT32_MAX) continue;
         Obj ob;
         ob.kind = 1;
-        ob.size = 1;
+        ob.size = amp_planes;
         ob.lo = first_closure[q];
         ob.hi = P + 1;
         ob.slot = q;
         for (size_t n = 0; n < N; n++)
-            if ((ops[n].w[0] & 0xff) == K_CLOSURE && ops[n].w[3] == q) ob.uses += ((ops[n].w[0] >> 16) & F_ZERO) ? 1 : 2;
-        ob.uses += used ? 1 : 0;
+            if ((ops[n].w[0] & 0xff) == K_CLOSURE && ops[n].w[3] == q)
+                ob.uses += (((ops[n].w[0] >> 16) & F_ZERO) ? 1 : 2) * amp_planes;
+        ob.uses += used ? amp_planes : 0;
         amp_obj[q] = (int)objs.size();
         objs.push_back(ob);
     }
@@ -323,7 +326,7 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
     for (uint32_t f = 0; f < forms; f++) {
         Obj ob;
         ob.kind = 2;
-        ob.size = 2;
+        ob.size = form_planes;
         ob.lo = 0;
         ob.hi = 0;
         ob.slot = f;
@@ -338,7 +341,7 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
             if (f >= forms) return cp;
             Obj &ob = objs[form_obj[f]];
             ob.hi = std::max(ob.hi, phase[task_of[n]]);
-            ob.uses += 2;
+            ob.uses += form_planes;
         }
     }
     // LDS placement by density, then the slot for the currents left
@@ -475,6 +478,19 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
         if (b != P - 1) return cp;
     }
     if (fetches != 1) return cp;
+    // the first phase holds only sources and clears (every other row reads a current), so a kernel
+    // may run it with other constants than the rest; it ends at each stream's first barrier
+    if (P < 2) return cp;
+    cp.split.assign(waves, 0);
+    for (uint32_t w = 0; w < waves; w++) {
+        uint32_t n = cp.start[w];
+        while ((cp.ops[n].w[0] & 0xff) != K_BARRIER) {
+            const int kind = cp.ops[n].w[0] & 0xff;
+            if (kind != K_SOURCE && kind != K_CLEAR && kind != K_FETCH) return cp;
+            n++;
+        }
+        cp.split[w] = n;
+    }
     // (an amplitude's room is also held in phase 0, where the next unit's forms are written)
     std::vector<std::vector<int>> held(P + 2, std::vector<int>(std::max(cp.lds_planes, 1u), -1));
     std::vector<std::vector<int>> held_slot(P + 2, std::vector<int>(std::max(cp.slot_planes, 1u), -1));

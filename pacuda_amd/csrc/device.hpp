@@ -392,8 +392,9 @@ template <int N> struct FormRegs {
 // the first NF register-resident forms (a form field marked with a later slot, or any marked
 // field when NF is 0, is read from the slot's copy). COOP: the arena of a cooperative kernel,
 // whose slot and LDS region belong to the workgroup (one unit, shared by its wavefronts): a form
-// field with FORM_LDS set names an LDS plane pair (components 0-1 in the first plane, 2-3 in
-// the second), an amplitude field at or above LDS_PLANE an LDS plane.
+// field with FORM_LDS set names the LDS planes from (field & (FORM_LDS - 1)) on that hold its 2 KB
+// (a lane's components 0-1 at 16 bytes per lane, then 2-3 1 KB further), an amplitude field at or
+// above LDS_PLANE the planes from (field - LDS_PLANE) on that hold its 1 KB (16 bytes per lane).
 constexpr uint32_t FORM_LDS = 1u << 23;
 template <class R, bool REG = false, int NF = 0, bool COOP = false> struct SlotArena {
     typedef typename Packed<R>::T V;
@@ -520,10 +521,10 @@ template <class R, bool REG = false, int NF = 0, bool COOP = false> struct SlotA
     // form field f: a register slot when bits 24 and up are set (REG kernels), else a slot form
     PCD void form(uint32_t f, double *raw) const {
         if constexpr (COOP) {
-            static_assert(sizeof(V) == 16, "LDS forms are two 16-byte planes");
             if (f & FORM_LDS) {
-                const uint32_t o = lds + (f & (FORM_LDS - 1)) * PLANE;
-                const dbl2 x = __builtin_bit_cast(dbl2, *lds_at(o)), y = __builtin_bit_cast(dbl2, *lds_at(o + PLANE));
+                const uint32_t o = (f & (FORM_LDS - 1)) * PLANE + idx * 16u;
+                const dbl2 x = *reinterpret_cast<const dbl2 *>(lds_at(o));
+                const dbl2 y = *reinterpret_cast<const dbl2 *>(lds_at(o + 1024u));
                 raw[0] = x.x; raw[1] = x.y; raw[2] = y.x; raw[3] = y.y;
                 return;
             }
@@ -542,9 +543,9 @@ template <class R, bool REG = false, int NF = 0, bool COOP = false> struct SlotA
     PCD void put_form(uint32_t f, const double *raw) const {
         if constexpr (COOP) {
             if (f & FORM_LDS) {
-                const uint32_t o = lds + (f & (FORM_LDS - 1)) * PLANE;
-                *lds_at(o) = __builtin_bit_cast(V, dbl2{raw[0], raw[1]});
-                *lds_at(o + PLANE) = __builtin_bit_cast(V, dbl2{raw[2], raw[3]});
+                const uint32_t o = (f & (FORM_LDS - 1)) * PLANE + idx * 16u;
+                *reinterpret_cast<dbl2 *>(lds_at(o)) = dbl2{raw[0], raw[1]};
+                *reinterpret_cast<dbl2 *>(lds_at(o + 1024u)) = dbl2{raw[2], raw[3]};
                 return;
             }
         }
@@ -554,7 +555,7 @@ template <class R, bool REG = false, int NF = 0, bool COOP = false> struct SlotA
     template <class T> PCD Cx<T> amp(uint32_t q) const {
         if constexpr (COOP) {
             if (q >= LDS_PLANE) {
-                const dbl2 v = __builtin_bit_cast(dbl2, *lds_at(lds + (q - LDS_PLANE) * PLANE));
+                const dbl2 v = *reinterpret_cast<const dbl2 *>(lds_at((q - LDS_PLANE) * PLANE + idx * 16u));
                 return cx(from_raw<T>(v.x), from_raw<T>(v.y));
             }
         }
@@ -564,7 +565,7 @@ template <class R, bool REG = false, int NF = 0, bool COOP = false> struct SlotA
     template <class T> PCD void put_amp(uint32_t q, Cx<T> v) const {
         if constexpr (COOP) {
             if (q >= LDS_PLANE) {
-                *lds_at(lds + (q - LDS_PLANE) * PLANE) = __builtin_bit_cast(V, dbl2{to_raw(v.re), to_raw(v.im)});
+                *reinterpret_cast<dbl2 *>(lds_at((q - LDS_PLANE) * PLANE + idx * 16u)) = dbl2{to_raw(v.re), to_raw(v.im)};
                 return;
             }
         }

@@ -45,6 +45,9 @@ constexpr int COOP_SHAPES = 4;
 constexpr uint32_t COOP_WAVES[COOP_SHAPES] = {16, 8, 8, 4};
 constexpr uint32_t COOP_UNITS[COOP_SHAPES] = {1, 1, 2, 4};
 constexpr uint32_t COOP_CONTROL = 256;
+// the cooperative kernels built: f64 in every shape, the other policies in shapes 0 and 2 (per
+// class of current, 0 for 16-byte currents and 1 for the f32 ones, the programs they need)
+constexpr bool coop_built(int cls, int shape) { return cls == 0 || shape == 0 || shape == 2; }
 
 struct PlanDev {
     const Instr *prog_lds[LDS_VARIANTS];  // with forwarding and LDS-resident currents (LDS_CAPS[v] planes)
@@ -68,13 +71,14 @@ struct PlanDev {
     const uint32_t *bindings;             // the program's symjit bindings
     double numerator, denominator;
     uint32_t factor, pad;
-    // the cooperative programs per shape (coop_prog null: none for the plan)
-    const Instr *coop_prog[COOP_SHAPES];     // the wavefronts' streams, one after another
-    const uint32_t *coop_start[COOP_SHAPES];  // their offsets (wavefronts + 1)
-    const uint32_t *coop_form[COOP_SHAPES];   // per form: FORM_LDS | LDS plane, or the slot form
-    const uint32_t *coop_amp[COOP_SHAPES];    // per amplitude plane: LDS_PLANE + LDS plane, or the slot amplitude
-    const uint32_t *coop_red[COOP_SHAPES];    // per term of the reduction (`planes`): its amplitude's field
-    uint32_t coop_slot[COOP_SHAPES];          // slot current planes
+    // the cooperative programs per class of current (0: 16 bytes, f64 and df64; 1: 8, the f32
+    // ones) and shape (coop_prog null: none for the plan)
+    const Instr *coop_prog[2][COOP_SHAPES];     // the wavefronts' streams, one after another
+    const uint32_t *coop_start[2][COOP_SHAPES];  // their offsets (wavefronts + 1)
+    const uint32_t *coop_split[2][COOP_SHAPES];  // per wavefront: where its first phase ends
+    const uint32_t *coop_red[2][COOP_SHAPES];    // per term of the reduction (`planes`): its amplitude's field
+    const uint32_t *coop_form[2][COOP_SHAPES];   // per form: FORM_LDS | LDS plane, or the slot form
+    uint32_t coop_slot[2][COOP_SHAPES];          // slot current planes
 };
 
 struct ItemDev {
@@ -289,15 +293,19 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
 // fused_kernel does while the others (and then wavefront 0) compute the next unit's forms: the
 // cooperative programs never give an amplitude a form's LDS room, and when the next unit belongs
 // to another plan a barrier ends the reduction first. A fetch row in one wavefront's stream takes
-// the next unit off the queue while the current one runs. f64 only: LDS planes hold 16 bytes per
-// lane.
+// the next unit off the queue while the current one runs. Every precision policy: the first
+// phase (the sources) runs with the f64 constants, as fused_kernel runs its sources, and the
+// policies whose denominators are df64 then rewrite the forms in df64 between two barriers.
 template <class K, int SHAPE, int W = COOP_WAVES[SHAPE], int UNITS = COOP_UNITS[SHAPE]>
 __global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W * UNITS / 4)))
 coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start, uint32_t item_count,
             uint32_t *counter, double *scratch, size_t slot_doubles, uint32_t lds_bytes) {
     typedef typename K::real R;
-    static_assert(__is_same(R, double) && __is_same(typename K::den, double) && __is_same(typename K::amp, double),
-                  "cooperative programs run in f64");
+    typedef typename Packed<R>::T V;
+    constexpr int CLASS = sizeof(V) == 16 ? 0 : 1;  // the programs of 16-byte or 8-byte currents
+    constexpr bool DF = !__is_same(typename K::acc, double);    // df64 constants after the sources
+    constexpr bool DF_AMP = !__is_same(typename K::amp, double);
+    constexpr bool DF_FORMS = __is_same(typename K::den, D);
     extern __shared__ __attribute__((aligned(16))) char pacuda_lds[];
     const uint32_t lane = threadIdx.x & (LANES - 1);
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / LANES);
@@ -311,7 +319,7 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
                                                (int)(slot_doubles * sizeof(double)), BUFFER_FORMAT);
     a.idx = lane;
     a.bind_ = nullptr;
-    a.lds = lane * 16u;
+    a.lds = lane * (uint32_t)sizeof(V);
     a.counter = counter;
     a.next_unit = control + 1;
     // the item of a unit (uniform binary search) and the lane's point in it
@@ -329,12 +337,12 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
     // (runner/recurrence_runner.cu:745-764), form f by wavefront f mod W
     auto forms = [&](const ItemDev &it, uint32_t i) {
         const PlanDev *plan = it.plan;
-        a.amp_base = cnst(plan)->coop_slot[SHAPE] * 1024u;
+        a.amp_base = cnst(plan)->coop_slot[CLASS][SHAPE] * 1024u;
         a.form_base = a.amp_base + cnst(plan)->amplitude_planes * 1024u;
         const size_t ie = i < it.points ? i : it.points - 1;  // inactive lanes repeat the last point
         const uint32_t F = cnst(plan)->form_count;
         const uint32_t *form_terms = cnst(plan)->form_terms, *term_slots = cnst(plan)->term_slots;
-        const uint32_t *form_field = cnst(plan)->coop_form[SHAPE];
+        const uint32_t *form_field = cnst(plan)->coop_form[CLASS][SHAPE];
         const double *coefficients = cnst(plan)->coefficients;
         for (uint32_t f = wave; f < F; f += W) {
             const uint32_t first = cnst(form_terms)[2 * f], n = cnst(form_terms)[2 * f + 1];
@@ -365,15 +373,33 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
         const PlanDev *plan = it.plan;
         a.point_ = i;
         a.points_ = it.points;
-        const double *c64 = cnst(plan)->c64;
+        const double *c64 = cnst(plan)->c64, *cons = DF ? cnst(plan)->cdf : c64;
         const uint32_t S = cnst(plan)->scale_count, X = cnst(plan)->factor_count;
         a.sre_ = c64; a.sim_ = c64 + S; a.fre_ = c64 + 2 * S; a.fim_ = c64 + 2 * S + X;
-        const Instr *prog = cnst(plan)->coop_prog[SHAPE];
-        const uint32_t *starts = cnst(plan)->coop_start[SHAPE];
-        const uint32_t first = cnst(starts)[wave];
+        const Instr *prog = cnst(plan)->coop_prog[CLASS][SHAPE];
+        const uint32_t first = cnst(cnst(plan)->coop_start[CLASS][SHAPE])[wave];
+        const uint32_t end = cnst(cnst(plan)->coop_start[CLASS][SHAPE])[wave + 1];
+        const uint32_t mid = cnst(cnst(plan)->coop_split[CLASS][SHAPE])[wave];  // (a barrier row)
         // (one wavefront's stream takes the next unit off the queue: a fetch row in the phase that
         // hides the atomic's round trip best)
-        interpret<K, false, true>(prog + first, cnst(starts)[wave + 1] - first, a);
+        interpret<K, false, true>(prog + first, mid - first, a);  // the sources, in f64
+        a.sre_ = cons; a.sim_ = cons + S; a.fre_ = cons + 2 * S; a.fim_ = cons + 2 * S + X;
+        if constexpr (DF_FORMS) {  // the forms in df64 once no source reads them
+            __syncthreads();  // (the barrier row at `mid`)
+            const uint32_t F = cnst(plan)->form_count;
+            const uint32_t *form_field = cnst(plan)->coop_form[CLASS][SHAPE];
+            for (uint32_t f = wave; f < F; f += W) {
+                double raw[4];
+                a.form(cnst(form_field)[f], raw);
+#pragma unroll
+                for (int c = 0; c < 4; c++) raw[c] = to_raw(narrow<D>(raw[c]));
+                a.put_form(cnst(form_field)[f], raw);
+            }
+            __syncthreads();
+            interpret<K, false, true>(prog + mid + 1, end - mid - 1, a);
+        } else {
+            interpret<K, false, true>(prog + mid, end - mid, a);
+        }
         __syncthreads();
         const uint32_t next = __builtin_amdgcn_readfirstlane(control[1]);
         ItemDev nit = it;
@@ -382,7 +408,7 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
         if (wave == 0) {
             // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
             const uint32_t outputs = cnst(plan)->outputs;
-            const uint32_t *offsets = cnst(plan)->offsets, *fields = cnst(plan)->coop_red[SHAPE];
+            const uint32_t *offsets = cnst(plan)->offsets, *fields = cnst(plan)->coop_red[CLASS][SHAPE];
             const double *weights = cnst(plan)->weights;
             const bool factor = cnst(plan)->factor;
             const double numerator = cnst(plan)->numerator, denominator = cnst(plan)->denominator;
@@ -392,7 +418,8 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
                 double acc = 0.0;
                 for (uint32_t k = cnst(offsets)[o]; k < cnst(offsets)[o + 1]; k++) {
                     const Cx<double> amp = a.template amp<double>(cnst(fields)[k]);
-                    acc = acc + cnst(weights)[k] * __builtin_fma(amp.re, amp.re, amp.im * amp.im);
+                    const double re = amplitude_value<DF_AMP>(amp.re), im = amplitude_value<DF_AMP>(amp.im);
+                    acc = acc + cnst(weights)[k] * __builtin_fma(re, re, im * im);
                 }
                 if (it.resolved && active) it.resolved[(size_t)i * outputs + o] = acc;
                 double v = factor ? (acc * numerator) / denominator : acc;
@@ -418,7 +445,7 @@ struct pacuda_plan {
     PlanDev *device = nullptr;
     std::vector<void *> allocations;
     uint64_t slot_doubles = 0;
-    uint64_t coop_slot_doubles[COOP_SHAPES] = {};  // a workgroup's slot per cooperative shape (0: no program)
+    uint64_t coop_slot_doubles[2][COOP_SHAPES] = {};  // a workgroup's slot per class and shape (0: no program)
     bool symjit = false;  // the program holds symjit rows
 };
 
@@ -1515,7 +1542,7 @@ struct Compiled {
     bool zero_fill = true;
     std::vector<uint32_t> bindings;
     bool symjit = false;
-    CoopProgram coop[COOP_SHAPES];  // the cooperative programs (ok false: none)
+    CoopProgram coop[2][COOP_SHAPES];  // the cooperative programs per class of current (ok false: none)
 };
 
 // F_ZERO on the first closure of every amplitude plane (closures are the only rows that touch
@@ -1648,14 +1675,19 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
         for (uint32_t k = 0; k < count; k++) used[d->planes[k]] = 1;
         // per shape, the candidate (phases as early or as late as possible, three LDS rankings)
         // that leaves the fewest current accesses in the slot
-        for (int s = 0; s < COOP_SHAPES; s++)
-            for (int late = 0; late < 2; late++)
-                for (int rank = 0; rank < 3; rank++) {
-                    CoopProgram q = coop_program(c.fwd, c.planes, d->amplitude_planes, d->form_count, used,
-                                                 COOP_WAVES[s], (LDS_BYTES / COOP_UNITS[s] - COOP_CONTROL) / 1024,
-                                                 late, rank);
-                    if (q.ok && (!c.coop[s].ok || q.slot_accesses < c.coop[s].slot_accesses)) c.coop[s] = std::move(q);
-                }
+        for (int cls = 0; cls < 2; cls++)
+            for (int s = 0; s < COOP_SHAPES; s++) {
+                if (!coop_built(cls, s)) continue;
+                const uint32_t plane_bytes = cls ? 512 : 1024;
+                for (int late = 0; late < 2; late++)
+                    for (int rank = 0; rank < 3; rank++) {
+                        CoopProgram q = coop_program(c.fwd, c.planes, d->amplitude_planes, d->form_count, used,
+                                                     COOP_WAVES[s], (LDS_BYTES / COOP_UNITS[s] - COOP_CONTROL) / plane_bytes,
+                                                     plane_bytes, late, rank);
+                        CoopProgram &best = c.coop[cls][s];
+                        if (q.ok && (!best.ok || q.slot_accesses < best.slot_accesses)) best = std::move(q);
+                    }
+            }
     }
     // the on-chip programs: the passes over the rows as built and over each candidate order, the
     // allocator's placement (the static one where it gives up), keeping per variant and tier the
@@ -1757,11 +1789,11 @@ extern "C" int pacuda_plan_program(const pacuda_plan_desc *d, uint32_t variant, 
 
 extern "C" int pacuda_plan_coop_program(const pacuda_plan_desc *d, uint32_t shape, uint32_t serial, uint32_t *out,
                                         uint32_t capacity, uint32_t *count, uint32_t *info, uint32_t *starts) {
-    if (!count || shape >= (uint32_t)COOP_SHAPES) return PACMAN_RECURRENCE_INVALID;
+    if (!count || shape >= 2u * COOP_SHAPES) return PACMAN_RECURRENCE_INVALID;
     Compiled c;
     const int st = compile_plan(d, c);
     if (st) return st;
-    const CoopProgram &q = c.coop[shape];
+    const CoopProgram &q = c.coop[shape / COOP_SHAPES][shape % COOP_SHAPES];
     if (!q.ok) return PACMAN_RECURRENCE_INVALID;
     const std::vector<Instr> &ops = serial ? q.serial : q.ops;
     *count = (uint32_t)ops.size();
@@ -1835,20 +1867,22 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
     h.numerator = d->numerator;
     h.denominator = d->denominator;
     h.factor = d->numerator != d->denominator;
-    for (int s = 0; s < COOP_SHAPES; s++) {
-        const CoopProgram &q = cp.coop[s];
-        if (!q.ok) continue;
-        h.coop_prog[s] = upload(p, q.ops.data(), q.ops.size(), &ok);
-        h.coop_start[s] = upload(p, q.start.data(), q.start.size(), &ok);
-        h.coop_form[s] = upload(p, q.form_field.data(), q.form_field.size(), &ok);
-        h.coop_amp[s] = upload(p, q.amp_field.data(), q.amp_field.size(), &ok);
-        std::vector<uint32_t> red(count);
-        for (uint32_t k = 0; k < count; k++) red[k] = q.amp_field[d->planes[k]];
-        h.coop_red[s] = upload(p, red.data(), red.size(), &ok);
-        h.coop_slot[s] = q.slot_planes;
-        p->coop_slot_doubles[s] =
-            (uint64_t)LANES * (2ull * q.slot_planes + 2ull * d->amplitude_planes + 4ull * d->form_count);
-    }
+    for (int cls = 0; cls < 2; cls++)
+        for (int s = 0; s < COOP_SHAPES; s++) {
+            const CoopProgram &q = cp.coop[cls][s];
+            if (!q.ok) continue;
+            h.coop_prog[cls][s] = upload(p, q.ops.data(), q.ops.size(), &ok);
+            h.coop_start[cls][s] = upload(p, q.start.data(), q.start.size(), &ok);
+            h.coop_split[cls][s] = upload(p, q.split.data(), q.split.size(), &ok);
+            h.coop_form[cls][s] = upload(p, q.form_field.data(), q.form_field.size(), &ok);
+            std::vector<uint32_t> red(count);
+            for (uint32_t k = 0; k < count; k++) red[k] = q.amp_field[d->planes[k]];
+            h.coop_red[cls][s] = upload(p, red.data(), red.size(), &ok);
+            h.coop_slot[cls][s] = q.slot_planes;
+            // (the slot's current planes at 1 KB whatever the class, as fused_kernel lays them out)
+            p->coop_slot_doubles[cls][s] =
+                (uint64_t)LANES * (2ull * q.slot_planes + 2ull * d->amplitude_planes + 4ull * d->form_count);
+        }
     p->device = upload(p, &h, 1, &ok);
     p->slot_doubles = (uint64_t)LANES * (2ull * planes + 2ull * d->amplitude_planes + 4ull * d->form_count);
     if (!ok) {
@@ -1887,6 +1921,7 @@ struct Schedule {
     bool lds_all[LDS_VARIANTS] = {true, true, true, true, true};  // every plan has the variant
     bool symjit = false;  // some item's plan holds symjit rows
     int coop = -1;        // the cooperative shape, or -1
+    int coop_class = 0;   // its programs' class of current (0: 16 bytes, 1: the f32 kernels' 8)
     uint64_t slots = 0;   // scratch slots: one per wavefront, or per workgroup in a cooperative launch
     uint32_t slots_per_block() const { return coop >= 0 ? 1u : block / LANES; }
 };
@@ -1929,7 +1964,8 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
         const uint32_t per_cu = l->waves_per_cu ? l->waves_per_cu : 16;
         for (int k = 0; k < COOP_SHAPES; k++)
             if (COOP_WAVES[k] == l->unit_waves && COOP_WAVES[k] * COOP_UNITS[k] == per_cu) sc.coop = k;
-        if (sc.coop < 0 || l->kernel != PACMAN_RECURRENCE_KERNEL_F64) {
+        sc.coop_class = f32_kernel(l->kernel) ? 1 : 0;
+        if (sc.coop < 0 || (l->kernel != PACMAN_RECURRENCE_KERNEL_F64 && !coop_built(1, sc.coop))) {
             fprintf(stderr, "pacuda: no cooperative launch of kernel %u with %u wavefronts per unit and %u per CU\n",
                     l->kernel, l->unit_waves, per_cu);
             return PACMAN_RECURRENCE_INVALID;
@@ -1957,11 +1993,12 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
         d.ext_stride = it.ext_stride;
         sc.starts[j] = (uint32_t)sc.units;
         sc.units += (it.points + LANES - 1) / LANES;
-        if (sc.coop >= 0 && !it.plan->coop_slot_doubles[sc.coop]) {
+        if (sc.coop >= 0 && !it.plan->coop_slot_doubles[sc.coop_class][sc.coop]) {
             fprintf(stderr, "pacuda: a plan of the launch has no cooperative program (symjit rows)\n");
             return PACMAN_RECURRENCE_INVALID;
         }
-        sc.slot = std::max<uint64_t>(sc.slot, sc.coop >= 0 ? it.plan->coop_slot_doubles[sc.coop] : it.plan->slot_doubles);
+        sc.slot = std::max<uint64_t>(sc.slot, sc.coop >= 0 ? it.plan->coop_slot_doubles[sc.coop_class][sc.coop]
+                                                           : it.plan->slot_doubles);
         sc.symjit = sc.symjit || it.plan->symjit;
         for (int v = 0; v < LDS_VARIANTS; v++) {
             sc.lds_planes[v] = std::max(sc.lds_planes[v], it.plan->host.lds_planes[v]);
@@ -2040,43 +2077,63 @@ bool launch(const Schedule &sc, hipStream_t s, int tier, size_t bytes, const Ite
     return true;
 }
 
-template <int SHAPE> bool allow_coop_lds() {
-    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(coop_kernel<KF64, SHAPE>),
+template <class K, int SHAPE> bool allow_coop_lds() {
+    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(coop_kernel<K, SHAPE>),
                                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                                            LDS_BYTES / COOP_UNITS[SHAPE]),
                                        "cooperative kernel LDS");
     return ok;
 }
-static_assert(COOP_SHAPES == 4, "the cooperative kernels instantiated below");
-bool allow_coop_lds(int shape) {
-    return shape == 0 ? allow_coop_lds<0>() : shape == 1 ? allow_coop_lds<1>() : shape == 2 ? allow_coop_lds<2>()
-                                                                                             : allow_coop_lds<3>();
-}
 
-template <int SHAPE>
-void launch_coop_shape(const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts, uint32_t count,
-                       uint32_t *counter, double *scratch) {
+template <class K, int SHAPE>
+bool launch_coop_shape(const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts, uint32_t count,
+                       uint32_t *counter, double *scratch, bool run) {
+    if (!allow_coop_lds<K, SHAPE>()) return false;
+    if (!run) return true;
     const uint32_t bytes = LDS_BYTES / COOP_UNITS[SHAPE];
-    coop_kernel<KF64, SHAPE><<<sc.grid, COOP_WAVES[SHAPE] * LANES, bytes, s>>>(items, starts, count, counter, scratch,
-                                                                           sc.slot, bytes);
+    reset_counter<<<1, 64, 0, s>>>(counter);
+    coop_kernel<K, SHAPE><<<sc.grid, COOP_WAVES[SHAPE] * LANES, bytes, s>>>(items, starts, count, counter, scratch,
+                                                                         sc.slot, bytes);
+    return true;
 }
 
-bool launch_coop(const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts, uint32_t count,
-                 uint32_t *counter, double *scratch) {
-    if (!allow_coop_lds(sc.coop)) return false;
-    reset_counter<<<1, 64, 0, s>>>(counter);
-    switch (sc.coop) {
-    case 0: launch_coop_shape<0>(sc, s, items, starts, count, counter, scratch); break;
-    case 1: launch_coop_shape<1>(sc, s, items, starts, count, counter, scratch); break;
-    case 2: launch_coop_shape<2>(sc, s, items, starts, count, counter, scratch); break;
-    default: launch_coop_shape<3>(sc, s, items, starts, count, counter, scratch); break;
+// the policies other than f64 run shapes 0 and 2 only (coop_built)
+template <class K>
+bool launch_coop_policy(const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
+                        uint32_t count, uint32_t *counter, double *scratch, bool run) {
+    if (sc.coop == 0) return launch_coop_shape<K, 0>(sc, s, items, starts, count, counter, scratch, run);
+    if (sc.coop == 2) return launch_coop_shape<K, 2>(sc, s, items, starts, count, counter, scratch, run);
+    return false;
+}
+static_assert(COOP_SHAPES == 4, "the cooperative kernels instantiated below");
+
+// a cooperative launch of `kernel` (run false: only the LDS attribute, set before any graph capture)
+bool launch_coop(uint32_t kernel, const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
+                 uint32_t count, uint32_t *counter, double *scratch, bool run = true) {
+    switch (kernel) {
+    case PACMAN_RECURRENCE_KERNEL_F64:
+        switch (sc.coop) {
+        case 0: return launch_coop_shape<KF64, 0>(sc, s, items, starts, count, counter, scratch, run);
+        case 1: return launch_coop_shape<KF64, 1>(sc, s, items, starts, count, counter, scratch, run);
+        case 2: return launch_coop_shape<KF64, 2>(sc, s, items, starts, count, counter, scratch, run);
+        default: return launch_coop_shape<KF64, 3>(sc, s, items, starts, count, counter, scratch, run);
+        }
+    case PACMAN_RECURRENCE_KERNEL_MIXED:
+        return launch_coop_policy<KMixed>(sc, s, items, starts, count, counter, scratch, run);
+    case PACMAN_RECURRENCE_KERNEL_MIXED64:
+        return launch_coop_policy<KMixed64>(sc, s, items, starts, count, counter, scratch, run);
+    case PACUDA_KERNEL_COMPENSATED:
+        return launch_coop_policy<KComp>(sc, s, items, starts, count, counter, scratch, run);
+    case PACUDA_KERNEL_COMPENSATED64:
+        return launch_coop_policy<KComp64>(sc, s, items, starts, count, counter, scratch, run);
+    default:
+        return launch_coop_policy<KDF64>(sc, s, items, starts, count, counter, scratch, run);
     }
-    return true;
 }
 
 // the LDS attribute of the kernel a launch of `sc` needs, set once (before any graph capture)
 bool lds_ready(uint32_t kernel, const Schedule &sc) {
-    if (sc.coop >= 0) return allow_coop_lds(sc.coop);
+    if (sc.coop >= 0) return launch_coop(kernel, sc, nullptr, nullptr, nullptr, 0, nullptr, nullptr, false);
     uint32_t per_wave, variant;
     size_t bytes;
     lds_shape(kernel, sc, &per_wave, &bytes, &variant);
@@ -2092,7 +2149,7 @@ bool lds_ready(uint32_t kernel, const Schedule &sc) {
 
 bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
                   uint32_t count, uint32_t *counter, double *scratch) {
-    if (sc.coop >= 0) return launch_coop(sc, s, items, starts, count, counter, scratch);
+    if (sc.coop >= 0) return launch_coop(kernel, sc, s, items, starts, count, counter, scratch);
     uint32_t per_wave, variant;
     size_t bytes;
     lds_shape(kernel, sc, &per_wave, &bytes, &variant);

@@ -346,8 +346,8 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
             if constexpr (COOP) __syncthreads();
         } else if (kind == K_FETCH) {
             if constexpr (COOP) a.fetch();
-        } else if (kind == K_SOURCE) {  // a source row made again (programs of the f64 kernel only)
-            if constexpr (__is_same(R, double) && __is_same(typename K::den, double)) source_row<R>(o, a);
+        } else if (kind == K_SOURCE) {  // a source row made again (programs of the f64 kernel), or a cooperative program's
+            if constexpr (COOP || (__is_same(R, double) && __is_same(typename K::den, double))) source_row<R>(o, a);
         } else if (kind == K_CLEAR) {
             if (o.w[2] == 4) {
                 const C z[4] = {cx(R(0.0), R(0.0)), cx(R(0.0), R(0.0)), cx(R(0.0), R(0.0)), cx(R(0.0), R(0.0))};

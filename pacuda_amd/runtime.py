@@ -102,9 +102,11 @@ def plan_desc(plan, arrays, no_sort=0):
 COOP_SHAPES = ((16, 16), (8, 8), (8, 16), (4, 16))
 
 
-def coop_program(lib, plan, shape, serial=False):
+def coop_program(lib, plan, shape, serial=False, f32=False):
     """a plan's cooperative program for shape index `shape` (pacuda_plan_coop_program, host
-    only): (uint32 [count][8], info dict, stream starts), or None when the plan has none"""
+    only), for 16-byte currents or (f32) the f32 ones: (uint32 [count][8], info dict, stream
+    starts), or None when the plan has none"""
+    shape = shape + (4 if f32 else 0)
     a = plan.arrays()
     d = plan_desc(plan, a)
     n, info, starts = C.c_uint32(), (C.c_uint32 * 6)(), (C.c_uint32 * 17)()
@@ -230,9 +232,10 @@ class Sweep:
 # (waves per CU, block) pairs the autotuner tries: 2 to 6 persistent waves per SIMD, blocks of
 # one or four waves
 TUNE_SHAPES = ((6, 64), (8, 128), (8, 256), (12, 256), (16, 256), (16, 64))
-# ... and the cooperative launches (waves per CU, block ignored, wavefronts per unit), which only
-# the f64 kernel runs, for plans without symjit rows
+# ... and the cooperative launches (waves per CU, block ignored, wavefronts per unit), for plans
+# without symjit rows: the f64 kernel's, and the two the other kernels run
 COOP_TUNE_SHAPES = tuple((per_cu, 0, uw) for uw, per_cu in COOP_SHAPES)
+COOP_TUNE_SHAPES_ANY = ((16, 0, 16), (16, 0, 8))
 _tuned = {}
 
 

@@ -32,31 +32,43 @@ def _plans():
             (subs[7].name, subs[7].plan), (subs[25].name, subs[25].plan)]
 
 
-@pytest.mark.parametrize("shape", range(len(COOP_SHAPES)))
+# (class, shape, policy): the 16-byte currents' programs in every shape under f64 and df64, the
+# f32 currents' in the shapes the other kernels run under mixed and compensated64
+CASES = [(False, k, "f64") for k in range(len(COOP_SHAPES))] + [(False, 2, "df64"), (True, 0, "mixed"),
+                                                                (True, 2, "mixed"), (True, 2, "compensated64")]
+
+
+@pytest.mark.parametrize("f32,shape,policy", CASES)
 @pytest.mark.parametrize("which", range(5))
-def test_serial_order_equals_the_plan_run(lib, shape, which):
+def test_serial_order_equals_the_plan_run(lib, f32, shape, policy, which):
     name, plan = _plans()[which]
-    got = coop_program(lib, plan, shape, serial=True)
+    got = coop_program(lib, plan, shape, serial=True, f32=f32)
     assert got is not None, name
     ser, info, _ = got
     assert info["waves"] == COOP_SHAPES[shape][0]
-    assert info["lds_planes"] <= LDS_CAP[shape]
+    assert info["lds_planes"] <= LDS_CAP[shape] * (2 if f32 else 1)
     n = 4 * 64 + 13
     mom, _ = ps.generate(plan.legs, n, seed=40 + which)
     ext = ps.external_planes(mom, plan.crossing)
     counts = harness.plan_counts(plan)
-    want_res, want_tot = refcpu.evaluate(plan.arrays(), counts, ext, n, kernel=1)
-    res, tot = refcpu.evaluate_program(ser, 0, info["slot_planes"], info["lds_planes"], plan.arrays(), counts, ext, n,
-                                       kernel=1, numerator=plan.numerator, denominator=plan.denominator)
+    k = abi.FUSED_KERNELS[policy]
+    # the first phase is the sources (in f64, before the policy's forms and constants), which
+    # the oracle runs as its sources phase
+    kinds = ser[:, 0] & 0xff
+    lead = int(np.argmax(kinds != 6))
+    assert lead > 0 and not np.any(kinds[lead:] == 6)
+    want_res, want_tot = refcpu.evaluate(plan.arrays(), counts, ext, n, kernel=k)
+    res, tot = refcpu.evaluate_program(ser, lead, info["slot_planes"], info["lds_planes"], plan.arrays(), counts, ext, n,
+                                       kernel=k, numerator=plan.numerator, denominator=plan.denominator)
     assert np.array_equal(res.view(np.uint64), want_res.view(np.uint64)), name
     assert np.array_equal(tot.view(np.uint64), want_tot.view(np.uint64)), name
 
 
-@pytest.mark.parametrize("shape", range(len(COOP_SHAPES)))
-def test_streams_meet_the_same_barriers(lib, shape):
+@pytest.mark.parametrize("f32,shape", [(False, k) for k in range(len(COOP_SHAPES))] + [(True, 0), (True, 2)])
+def test_streams_meet_the_same_barriers(lib, f32, shape):
     for name, plan in _plans():
-        dev, info, starts = coop_program(lib, plan, shape)
-        ser, _, _ = coop_program(lib, plan, shape, serial=True)
+        dev, info, starts = coop_program(lib, plan, shape, f32=f32)
+        ser, _, _ = coop_program(lib, plan, shape, serial=True, f32=f32)
         W, P = info["waves"], info["phases"]
         assert len(starts) == W + 1 and starts[0] == 0 and starts[-1] == len(dev)
         assert all(a <= b for a, b in zip(starts, starts[1:]))
@@ -84,6 +96,15 @@ def test_more_lds_leaves_fewer_slot_accesses(lib):
 def test_plans_with_symjit_rows_have_none(lib):
     sub = processes.four_quark_subprocesses()[0]
     assert coop_program(lib, sub.plan, 0) is None
+    assert coop_program(lib, sub.plan, 0, f32=True) is None
+
+
+def test_f32_currents_fit_twice_the_planes(lib):
+    """the f32 currents' programs place planes of 512 bytes: twice as many in the same LDS, so
+    fewer slot accesses"""
+    plan = plans.gluon_plan(5)
+    wide, narrow = coop_program(lib, plan, 2)[1], coop_program(lib, plan, 2, f32=True)[1]
+    assert narrow["lds_planes"] > wide["lds_planes"] and narrow["slot_accesses"] < wide["slot_accesses"]
 
 
 def test_the_oracle_sees_a_broken_order(lib):

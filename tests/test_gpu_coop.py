@@ -27,10 +27,11 @@ def lib():
     return abi.load()
 
 
-def _run_plan(lib, plan, points, seed, shapes=COOP_SHAPES):
+def _run_plan(lib, plan, points, seed, shapes=COOP_SHAPES, kernel="f64"):
     mom, _ = ps.generate(plan.legs, points, seed=seed)
     ext = ps.external_planes(mom, plan.crossing)
-    want_res, want_tot = refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), ext, points, kernel=1, threads=8)
+    k = abi.FUSED_KERNELS[kernel]
+    want_res, want_tot = refcpu.evaluate(plan.arrays(), harness.plan_counts(plan), ext, points, kernel=k, threads=8)
     outputs = want_res.shape[1]
     fp = FusedPlan(lib, plan)
     d_ext = DeviceArray(lib, ext.nbytes)
@@ -39,13 +40,15 @@ def _run_plan(lib, plan, points, seed, shapes=COOP_SHAPES):
     d_tot = DeviceArray(lib, points * 8)
     try:
         for unit_waves, per_cu in shapes:
-            print(plan.name, points, "points: cooperative", unit_waves, "wavefronts per unit,", per_cu, "per CU", flush=True)
-            fused_evaluate(lib, [(fp, d_ext, points, ext.shape[1], d_res, d_tot)], F64, waves_per_cu=per_cu,
+            print(plan.name, kernel, points, "points: cooperative", unit_waves, "wavefronts per unit,", per_cu, "per CU",
+                  flush=True)
+            fused_evaluate(lib, [(fp, d_ext, points, ext.shape[1], d_res, d_tot)], k, waves_per_cu=per_cu,
                            unit_waves=unit_waves)
             res = d_res.download((points, outputs))
             tot = d_tot.download(points)
-            assert bits_equal(res, want_res), f"{plan.name} {unit_waves}/{per_cu}: {np.count_nonzero(res != want_res)} values differ"
-            assert bits_equal(tot, want_tot), f"{plan.name} {unit_waves}/{per_cu}: totals differ"
+            assert bits_equal(res, want_res), \
+                f"{plan.name} {kernel} {unit_waves}/{per_cu}: {np.count_nonzero(res != want_res)} values differ"
+            assert bits_equal(tot, want_tot), f"{plan.name} {kernel} {unit_waves}/{per_cu}: totals differ"
     finally:
         d_ext.free()
         d_res.free()
@@ -63,6 +66,20 @@ def test_coop_gluon6_bitwise(lib):
 
 def test_coop_quark_line_bitwise(lib):
     _run_plan(lib, plans.quark_line_plan(5), 9000 + 63, seed=63)
+
+
+# the shapes the kernels other than f64 run
+ANY_SHAPES = ((16, 16), (8, 16))
+
+
+@pytest.mark.parametrize("kernel", ["df64", "mixed", "mixed64", "compensated", "compensated64"])
+def test_coop_other_policies_bitwise(lib, kernel):
+    """every precision policy's cooperative kernel (the sources in f64, then the policy's forms and
+    constants) equals the oracle's emulation of the policy bit for bit: g g > g g g and, in the
+    f32-current policies, g g > g g g g, whose currents overflow LDS into the slot"""
+    _run_plan(lib, plans.gluon_plan(5), 6000 + 17, seed=65, shapes=ANY_SHAPES, kernel=kernel)
+    if kernel in ("mixed", "mixed64", "compensated64"):
+        _run_plan(lib, plans.gluon_plan(6), 3000 + 41, seed=66, shapes=ANY_SHAPES, kernel=kernel)
 
 
 def test_coop_one_launch_of_many_plans(lib):
@@ -138,8 +155,8 @@ def test_coop_sweep_replays_and_static_entry_point(lib):
 
 
 def test_coop_refusals(lib):
-    """host-side refusals (nothing is launched): a plan with symjit rows, a kernel other than
-    f64, a shape that does not exist"""
+    """host-side refusals (nothing is launched): a plan with symjit rows, a shape only the f64
+    kernel runs asked of another, a shape that does not exist"""
     sub = processes.four_quark_subprocesses()[0]
     fp = FusedPlan(lib, sub.plan)
     good = FusedPlan(lib, plans.gluon_plan(5))
@@ -153,7 +170,7 @@ def test_coop_refusals(lib):
             l = abi.Launch(abi.ABI, C.sizeof(abi.Launch), kernel, per_cu, 0, unit_waves, None, None)
             return lib.pacuda_evaluate(items, 1, C.byref(l))
         assert status(fp, F64, 16, 16) == 2
-        assert status(good, abi.FUSED_KERNELS["mixed"], 16, 16) == 2
+        assert status(good, abi.FUSED_KERNELS["mixed"], 8, 8) == 2  # (a shape only f64 runs)
         assert status(good, F64, 12, 16) == 2
         assert status(good, F64, 16, 2) == 2
     finally:
