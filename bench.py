@@ -129,10 +129,13 @@ def mode_errors(got, ref):
 
 
 def kernel_name(shape):
-    """the f64 kernel a launch shape (waves_per_cu, block, unit_waves) runs"""
+    """the f64 kernel a launch shape (waves_per_cu, block, unit_waves) runs, as rocprofv3 names
+    its template (the cooperative kernel: shape index, wavefronts per unit, units per CU)"""
+    from pacuda_amd.runtime import COOP_SHAPES
     wpc, _, uw = shape
     if uw:
-        return f"coop_kernel<KF64, {uw}, {(wpc or 16) // uw}>"
+        k = COOP_SHAPES.index((uw, wpc or 16))
+        return f"coop_kernel<KF64, {k}, {uw}, {(wpc or 16) // uw}>"
     return "fused_kernel<KF64>"
 
 
@@ -422,27 +425,28 @@ def main():
                    "points": n, "plan_rows": len(plan.rows), "current_planes": plan.current_planes,
                    "flops_per_point": flops, "divides_per_point": divs, "parallelism": "1 GPU, persistent fused kernel",
                    "launch": shape_name(shape if uw else (wpc or 12, 256)), "launch_sweep_ms": tuned},
-        # The fused f64 kernel's rocprofv3 counters (profiles/) show it bound by the traffic of its
-        # scratch slots beyond L2 (about 78% of the HBM peak, VALU busy about half the time), so the
-        # roofline is the HBM one: achieved = the algorithmic bytes (momenta in, |M|^2 out) per
-        # second, traffic = the measured HBM bytes per dispatch. By its flops the same kernel sits
-        # far above the HBM ridge; "compute" states that figure against the FP64 VALU peak.
-        "roofline": {"bound": "hbm", "achieved": alg_bytes * n / (kernel_ms * 1e-3) / 1e9, "peak": HBM_PEAK_TBS * 1e3,
-                     "unit": "GB/s", "frac": alg_bytes * n / (kernel_ms * 1e-3) / 1e9 / (HBM_PEAK_TBS * 1e3),
-                     "traffic": traffic,
-                     "traffic_unit": "bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                     "algorithmic_bytes_per_dispatch": alg_bytes * n,
-                     "algorithmic_bytes_per_point": alg_bytes,
-                     "traffic_tb_per_s": traffic / (kernel_ms * 1e-3) / 1e12 if traffic else None,
-                     "traffic_frac_of_hbm_peak": traffic / (kernel_ms * 1e-3) / 1e12 / HBM_PEAK_TBS if traffic else None,
+        # The f64 kernel by its algorithm: 39,448 flops per point against 168 algorithmic bytes
+        # (momenta in, |M|^2 out), far above the HBM ridge (78.6 TFLOP/s / 8 TB/s), so the
+        # roofline is the FP64 one (the VALU's: the kernel issues no MFMA; MI355X's FP64 vector
+        # and matrix peaks are both 78.6 TFLOP/s). Its measured limiters are stated beside it:
+        # the HBM traffic of its scratch slots (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/) and
+        # the FMA-free ceiling of its rows.
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "traffic_unit": "HBM bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "flops_per_point": flops,
+                     # the program rows (the reference's operation order) contain no fused
+                     # multiply-adds, so they issue one FP64 instruction per flop: half the
+                     # FMA-counted peak is their ceiling; only the reduction uses an fma
+                     "fma_free_ceiling_frac": 0.5,
                      "kernel": kname, "kernel_ms": kernel_ms,
-                     "slot": slot,
-                     "compute": {"pipe": "fp64 VALU", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                                 "frac": achieved / FP64_PEAK_TFLOPS, "flops_per_point": flops,
-                                 # the program rows (the reference's operation order) contain no fused
-                                 # multiply-adds, so they issue one FP64 instruction per flop: half the
-                                 # FMA-counted peak is their ceiling; only the reduction uses an fma
-                                 "fma_free_ceiling_frac": 0.5}},
+                     "hbm": {"achieved": alg_bytes * n / (kernel_ms * 1e-3) / 1e9, "peak": HBM_PEAK_TBS * 1e3,
+                             "unit": "GB/s", "frac": alg_bytes * n / (kernel_ms * 1e-3) / 1e9 / (HBM_PEAK_TBS * 1e3),
+                             "algorithmic_bytes_per_point": alg_bytes, "algorithmic_bytes_per_dispatch": alg_bytes * n,
+                             "traffic_tb_per_s": traffic / (kernel_ms * 1e-3) / 1e12 if traffic else None,
+                             "traffic_frac_of_hbm_peak":
+                                 traffic / (kernel_ms * 1e-3) / 1e12 / HBM_PEAK_TBS if traffic else None},
+                     "slot": slot},
         "cpu_baseline": cpu,
         "modes": modes,
         "unweighted_events_per_s": unweighting_summary(extra.get("unweighting")),
