@@ -155,10 +155,15 @@ class Unweighting:
         self.flux = 1.0 / (2.0 * ps.ECM ** 2)
 
     # ---- the exact integrand --------------------------------------------------------------
+    # (the f64 launch the plain method and the second stage use alike: the cooperative one with
+    # 16 wavefronts per 64-point unit, the fastest for g g > g g g g)
+    EXACT_LAUNCH = dict(waves_per_cu=16, unit_waves=16)
+
     def exact(self, ext, count, kernel=abi.KERNEL_F64):
         tot = self.dev.empty(max(count, 1))
         if count:
-            evaluate(self.dev.lib, [(self.fused, ext, count, count, None, tot)], kernel)
+            launch = self.EXACT_LAUNCH if kernel == abi.KERNEL_F64 else {}
+            evaluate(self.dev.lib, [(self.fused, ext, count, count, None, tot)], kernel, **launch)
         return tot
 
     def surrogate_weights(self, d, count, network=True):
