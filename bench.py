@@ -359,18 +359,19 @@ def main():
             t = dev.empty(int(c))
             pjobs.append((f, e, int(c), int(c), None, t))
             holds.append((f, e, t))
-        evaluate(lib, pjobs, k64, timed=True)
-        _, ms_p = timed_steps(lib, pjobs, k64, 3)
+        pshape, _ = autotune(lib, pjobs, k64, shapes=((0, 0),) + COOP_TUNE_SHAPES_ANY, repeats=1)
+        _, ms_p = timed_steps(lib, pjobs, k64, 3, *(tuple(pshape) + (0,))[:3])
         refs = [j[5].download(j[2]) for j in pjobs]
         pp = {"subprocesses": len(subs), "points": int(counts.sum()), "f64_points_per_s": int(counts.sum()) / (ms_p * 1e-3),
-              "counts_min_max": [int(counts.min()), int(counts.max())]}
+              "f64_launch": shape_name(pshape), "counts_min_max": [int(counts.min()), int(counts.max())]}
         for name in ("mixed", "mixed64", "compensated", "compensated64"):
             k = abi.FUSED_KERNELS[name]
-            evaluate(lib, pjobs, k, timed=True)
-            _, ms = timed_steps(lib, pjobs, k, 3)
+            kshape, _ = autotune(lib, pjobs, k, shapes=((0, 0),) + COOP_TUNE_SHAPES_ANY, repeats=1)
+            _, ms = timed_steps(lib, pjobs, k, 3, *(tuple(kshape) + (0,))[:3])
             per = {sp.name: mode_errors(j[5].download(j[2]), r) for sp, j, r in zip(subs, pjobs, refs)}
             errs = {n: e["max_err_rel_to_max_f64"] for n, e in per.items()}
-            pp[name] = {"points_per_s": int(counts.sum()) / (ms * 1e-3), "worst_subprocess_err": max(errs.values()),
+            pp[name] = {"points_per_s": int(counts.sum()) / (ms * 1e-3), "launch": shape_name(kshape),
+                        "worst_subprocess_err": max(errs.values()),
                         "worst_subprocess_median_rel_err": max(e["median_rel_err_vs_f64"] for e in per.values()),
                         "err_rel_to_max_f64_per_subprocess": errs}
         extra["pp_jjj"] = pp

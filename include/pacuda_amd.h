@@ -99,6 +99,13 @@ int pacuda_plan_program(const pacuda_plan_desc *desc, uint32_t variant, uint32_t
  */
 int pacuda_plan_coop_program(const pacuda_plan_desc *desc, uint32_t shape, uint32_t serial, uint32_t *out,
                              uint32_t capacity, uint32_t *count, uint32_t *info, uint32_t *starts);
+/*
+ * The binding table a plan's programs index (symjit rows: w[6] is an offset into it; per binding
+ * the kernel, inputs, output pairs, direct, then the input codes and the output components): out
+ * (may be NULL) receives up to `capacity` words, *count their number. For tests and inspection
+ * (the CPU oracle runs the programs of plans with symjit rows with it); no reference counterpart.
+ */
+int pacuda_plan_bindings(const pacuda_plan_desc *desc, uint32_t *out, uint32_t capacity, uint32_t *count);
 /* 0 OK, 2 invalid description, 3 device error */
 int pacuda_plan_create(const pacuda_plan_desc *desc, pacuda_plan **plan);
 void pacuda_plan_destroy(pacuda_plan *plan);
@@ -156,7 +163,7 @@ typedef struct {
  * split into phases between workgroup barriers, the unit's currents in the workgroup's LDS and
  * slot); waves_per_cu 0 means 16, and waves_per_cu / unit_waves units share a compute unit's LDS
  * (f64: 16 / 16, 8 / 8, 16 / 8 or 16 / 4; the other kernels 16 / 16 or 16 / 8); block is ignored.
- * The same results bit for bit. Status 2 when a plan has no cooperative program (symjit rows).
+ * The same results bit for bit. Plans with symjit rows run in the 16 / 16 and 16 / 8 shapes only.
  * Asynchronous on the stream unless kernel_ms is set. Calls may go to different streams: the
  * item table, work counter and slots are shared by all calls, so each call first waits on the
  * host until the previous call's launch has ended. 0 OK, 2 invalid, 3 device error.

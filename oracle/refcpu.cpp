@@ -1417,7 +1417,8 @@ int refcpu_evaluate(uint32_t kernel, const uint64_t *words, uint32_t word_count,
 // `slot_planes` planes, as are the on-chip copies of dual fields (bits 30-31 set: a slot plane in
 // the low 24 bits whose copy sits at on-chip slot (plane + bits 24-29) & 63, read and written as
 // the op's dual flags say), form fields keep their low 24 bits (the bits above name the register
-// slot a fused kernel keeps the form in), and the known-zero masks in spare bits are honoured
+// slot a fused kernel keeps the form in), symjit rows take their bindings from `bindings` (the
+// plan's binding table, pacuda_plan_bindings), and the known-zero masks in spare bits are honoured
 // as the kernels honour them in the slot (see Op). Lets the tests check, on the CPU, that the library's
 // program passes leave every value unchanged.
 int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t count, uint32_t sources,
@@ -1427,7 +1428,8 @@ int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t cou
                             const uint32_t *form_terms, const uint32_t *term_slots, const double *term_coefficients,
                             uint32_t outputs, const uint32_t *offsets, const uint32_t *planes, const double *weights,
                             double numerator, double denominator, const double *external, uint32_t points,
-                            double *resolved, double *total, uint32_t block, int threads) {
+                            double *resolved, double *total, uint32_t block, int threads, const uint32_t *bindings,
+                            uint32_t binding_words) {
     if (kernel > KERNEL_COMPENSATED64 || block == 0 || !instr) return PACMAN_RECURRENCE_INVALID;
     const uint32_t LDS = 1u << 24;
     auto plane = [&](uint32_t p) { return p >= LDS ? slot_planes + (p - LDS) : p; };
@@ -1477,6 +1479,18 @@ int refcpu_evaluate_program(uint32_t kernel, const uint32_t *instr, uint32_t cou
             field(o, 0, w[1]); o.f[1] = form(w[2]); o.f[2] = w[3]; o.f[3] = w[4]; o.f[4] = w[5];
             o.z[3] = w[6] & 15;
             break;
+        case K_SYMJIT: {  // w[1], w[2] parents, w[3] destination, w[4] forms, w[5] factor, w[6] binding offset
+            const size_t at = w[6];
+            if (!bindings || at + 4 > binding_words) return PACMAN_RECURRENCE_INVALID;
+            const size_t words = 4 + (size_t)bindings[at + 1] + bindings[at + 2];
+            if (at + words > binding_words || words > (size_t)BIND_WORDS || bindings[at] >= PACUDA_KERNEL_COUNT)
+                return PACMAN_RECURRENCE_INVALID;
+            o.flags &= F_INIT;
+            o.f[1] = plane(w[1]); o.f[2] = plane(w[2]); o.f[3] = plane(w[3]);
+            o.f[4] = w[4] & 0xffff; o.f[5] = w[4] >> 16; o.f[6] = w[5];
+            memcpy(o.bind, bindings + at, words * sizeof(uint32_t));
+            break;
+        }
         default: return PACMAN_RECURRENCE_INVALID;
         }
     }

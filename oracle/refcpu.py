@@ -37,7 +37,7 @@ def load():
             C.c_uint32, u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
             f64p, f64p, C.c_uint32, f64p, f64p, C.c_uint32,
             u32p, u32p, f64p, C.c_uint32, u32p, u32p, f64p, C.c_double, C.c_double,
-            f64p, C.c_uint32, f64p, f64p, C.c_uint32, C.c_int]
+            f64p, C.c_uint32, f64p, f64p, C.c_uint32, C.c_int, u32p, C.c_uint32]
         lib.refcpu_test_sources.restype = C.c_int
         lib.refcpu_test_sources.argtypes = [u32p, f64p, f64p, C.c_uint32, f64p, f64p]
         lib.refcpu_plane_kernel.restype = C.c_int
@@ -81,22 +81,25 @@ def evaluate(plan_arrays, counts, external, points, kernel=1, block=64, threads=
 
 
 def evaluate_program(instr, sources, slot_planes, lds_planes, plan_arrays, counts, external, points, kernel=1,
-                     block=64, threads=1, numerator=1.0, denominator=1.0):
+                     block=64, threads=1, numerator=1.0, denominator=1.0, bindings=None):
     """The whole plan run per point from an instruction stream of the library's fused path
-    (uint32 [count][8]): (resolved [points][outputs], total [points])."""
+    (uint32 [count][8]; symjit rows index `bindings`, the plan's binding table): (resolved
+    [points][outputs], total [points])."""
     lib = load()
     a = plan_arrays
     outputs = len(a["offsets"]) - 1
     res, tot = np.zeros((points, outputs)), np.zeros(points)
     ins = np.ascontiguousarray(instr, dtype=np.uint32)
     ext = np.ascontiguousarray(external, dtype=np.float64)
+    bind = None if bindings is None else np.ascontiguousarray(bindings, dtype=np.uint32)
     st = lib.refcpu_evaluate_program(
         kernel, _p(ins, u32p), len(ins), sources, slot_planes, lds_planes, counts["amplitude_planes"],
         counts["form_count"], _p(a["scale_re"], f64p), _p(a["scale_im"], f64p), len(a["scale_re"]),
         _p(a["factor_re"], f64p), _p(a["factor_im"], f64p), len(a["factor_re"]), _p(a["form_terms"], u32p),
         _p(a["term_slots"], u32p), _p(a["term_coefficients"], f64p), outputs, _p(a["offsets"], u32p),
         _p(a["planes"], u32p), _p(a["weights"], f64p), numerator, denominator, _p(ext, f64p), points,
-        _p(res, f64p), _p(tot, f64p), block, threads)
+        _p(res, f64p), _p(tot, f64p), block, threads, _p(bind, u32p) if bind is not None else None,
+        0 if bind is None else len(bind))
     if st != 0:
         raise RuntimeError(f"refcpu_evaluate_program: status {st}")
     return res, tot

@@ -113,6 +113,7 @@ EXTENSION_SYMBOLS = [
     ("pacman_recurrence_v1_test_sources", C.c_int, [u32p, f64p, f64p, C.c_uint32, f64p, f64p]),
     ("pacuda_plan_program", C.c_int, [C.POINTER(PlanDesc), C.c_uint32, u32p, C.c_uint32, u32p, u32p, u32p, u32p]),
     ("pacuda_plan_coop_program", C.c_int, [C.POINTER(PlanDesc), C.c_uint32, C.c_uint32, u32p, C.c_uint32, u32p, u32p, u32p]),
+    ("pacuda_plan_bindings", C.c_int, [C.POINTER(PlanDesc), u32p, C.c_uint32, u32p]),
     ("pacuda_plan_create", C.c_int, [C.POINTER(PlanDesc), C.POINTER(C.c_void_p)]),
     ("pacuda_plan_destroy", None, [C.c_void_p]),
     ("pacuda_plan_info", C.c_int, [C.c_void_p, u32p, u64p]),

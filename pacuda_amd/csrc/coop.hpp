@@ -74,6 +74,7 @@ inline uint32_t row_cost(const Instr &o) {
     case K_CLOSURE: return 8 * o.w[6] + 4;
     case K_SOURCE: return 60;
     case K_CLEAR: return o.w[2] + 4;
+    case K_SYMJIT: return 60;
     default: return 10;
     }
 }
@@ -116,6 +117,46 @@ inline int current_fields(const Instr &o, Field *f) {
         return 2;
     default: return -1;
     }
+}
+
+// the current fields of a symjit row (a binding: kernel, inputs, output pairs, direct, the input
+// codes, the output components; the codes' layout is the reference's, runner/recurrence_runner.cu:
+// 582, 631-684): each parent a code reads is read whole (len 0: the object at the field, whatever
+// its size), the destination is created when the row initialises it and its outputs are the
+// components 0..m-1 (and no code reads it), else read and written whole; forms the codes read are
+// reported in `forms` (their 16-bit fields cannot name LDS). -1 when the binding is out of the table.
+inline int symjit_fields(const Instr &o, const std::vector<uint32_t> &bindings, Field *f, uint32_t *forms, int *nforms) {
+    const size_t at = o.w[6];
+    if (at + 4 > bindings.size()) return -1;
+    const uint32_t inputs = bindings[at + 1], pairs = bindings[at + 2];
+    if (at + 4 + inputs + pairs > bindings.size()) return -1;
+    bool parent[2] = {false, false}, dest_read = false;
+    *nforms = 0;
+    for (uint32_t k = 0; k < inputs; k++) {
+        const uint32_t code = bindings[at + 4 + k], which = (code >> 3) & 1, index = code >> 5;
+        switch (code & 7) {
+        case SB_PARENT: parent[which] = true; break;
+        case SB_DESTINATION: dest_read = true; break;
+        case SB_MOMENTUM:
+            forms[(*nforms)++] = ((which ? o.w[4] >> 16 : o.w[4] & 0xffff) * 4 + index) / 4;
+            break;
+        default: break;
+        }
+    }
+    uint32_t m = 0, mask = 0;
+    bool distinct = true;
+    for (uint32_t j = 0; j < pairs; j++) {
+        const uint32_t c = bindings[at + 4 + inputs + j];
+        if (c >= 32 || (mask >> c) & 1) distinct = false;
+        else mask |= 1u << c;
+        m = std::max(m, c + 1);
+    }
+    const bool create = ((o.w[0] >> 16) & F_INIT) && !dest_read && distinct && pairs > 0 && mask == (1ull << m) - 1;
+    int nf = 0;
+    if (parent[0]) f[nf++] = {1, 0, true, false, false};
+    if (parent[1]) f[nf++] = {2, 0, true, false, false};
+    f[nf++] = create ? Field{3, m, false, true, true} : Field{3, 0, true, true, false};
+    return nf;
 }
 
 // the form fields of row o (word indices)
@@ -163,16 +204,19 @@ inline int first_fit(std::vector<std::vector<uint8_t>> &occ, const Obj &o, bool 
 }  // namespace coop_detail
 
 // The cooperative program of `ops` (a fused program with forwarding and F_ZERO closures, over
-// `planes` current planes) for W = `waves` wavefronts and `lds_cap` LDS planes of `plane_bytes`
+// `planes` current planes, `bindings` its symjit rows' binding table) for W = `waves` wavefronts
+// and `lds_cap` LDS planes of `plane_bytes`
 // (64 current values: 1024 for 16-byte currents, 512 for the f32 ones; a form takes 2 KB of LDS,
-// an amplitude plane 1 KB); `used_amps` marks the amplitude planes the reduction reads. ok is false when a row or an access is not one the pass models (symjit rows,
-// accesses to parts of a current), or an amplitude the reduction reads has no closure.
+// an amplitude plane 1 KB); `used_amps` marks the amplitude planes the reduction reads. ok is
+// false when a row or an access is not one the pass models (accesses to parts of a current), an
+// amplitude the reduction reads has no closure, or one's first closure does not start from zero.
 // `late`: each task but the sources goes to the last phase before its first reader instead of the
 // first after its last producer (the same number of phases; values read far from where they are
 // made live shorter). `rank`: the LDS ranking's lifetime weight (0: square root, 1: linear, 2: none).
 inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, uint32_t amplitudes, uint32_t forms,
                                 const std::vector<uint8_t> &used_amps, uint32_t waves, uint32_t lds_cap,
-                                uint32_t plane_bytes, bool late = false, int rank = 0) {
+                                uint32_t plane_bytes, const std::vector<uint32_t> &bindings, bool late = false,
+                                int rank = 0) {
     using namespace coop_detail;
     CoopProgram cp;
     cp.waves = waves;
@@ -205,6 +249,7 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
     std::vector<Obj> objs;
     std::vector<int> owner(planes, -1);
     std::vector<std::array<int, 8>> fobj(N);  // object of each current field word
+    std::vector<std::array<uint32_t, 8>> moved(N);  // current components each field word reads and writes
     std::vector<int> amp_obj(amplitudes, -1);
     // dependences
     std::vector<std::vector<uint32_t>> dep(T);
@@ -215,16 +260,28 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
         if (d >= 0 && (uint32_t)d != t) dep[t].push_back((uint32_t)d);
     };
     std::vector<std::vector<uint32_t>> obj_tasks;  // tasks that access each object
+    std::vector<uint8_t> slot_forms(forms, 0);     // forms a symjit row reads: they stay in the slot
     for (size_t n = 0; n < N; n++) {
         const Instr &o = ops[n];
         const uint32_t t = task_of[n];
         const int kind = o.w[0] & 0xff;
         fobj[n].fill(-1);
         Field f[3];
-        const int nf = current_fields(o, f);
+        uint32_t sforms[2 * 64];
+        int nsforms = 0;
+        const int nf = kind == K_SYMJIT ? symjit_fields(o, bindings, f, sforms, &nsforms) : current_fields(o, f);
         if (nf < 0) return cp;
+        for (int k = 0; k < nsforms; k++) {
+            if (sforms[k] >= forms) return cp;
+            slot_forms[sforms[k]] = 1;
+        }
         for (int k = 0; k < nf; k++) {
-            const uint32_t base = o.w[f[k].word], len = f[k].len;
+            const uint32_t base = o.w[f[k].word];
+            if (!f[k].create && f[k].len == 0) {  // a symjit row's whole object at the field
+                if (base >= planes || owner[base] < 0) return cp;
+                f[k].len = objs[owner[base]].size;
+            }
+            const uint32_t len = f[k].len;
             if (len == 0 || base >= planes || len > planes - base) return cp;
             int id;
             if (f[k].create) {
@@ -252,6 +309,9 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
                 objs[id].uses += f[k].len;
             }
         }
+        moved[n].fill(0);
+        for (int k = 0; k < nf; k++)
+            moved[n][f[k].word] = (f[k].read ? f[k].len : 0) + (f[k].write ? f[k].len : 0);
         for (int k = 0; k < nf; k++) {
             const int id = fobj[n][f[k].word];
             if (f[k].read) readers[id].push_back(t);
@@ -267,6 +327,9 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
         if (kind == K_CLOSURE) {
             const uint32_t q = o.w[3];
             if (q >= amplitudes) return cp;
+            // (an amplitude starts from zero in its first closure: the unit's LDS and slot are
+            // not cleared)
+            if (last_closure[q] < 0 && !((o.w[0] >> 16) & F_ZERO)) return cp;
             add_dep(t, last_closure[q]);
             last_closure[q] = (int)t;
         }
@@ -358,6 +421,7 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
     for (uint32_t i : order) {
         Obj &o = objs[i];
         if (o.uses == 0) continue;  // (a form no row reads)
+        if (o.kind == 2 && slot_forms[o.slot]) continue;
         if (lds_cap) o.room = first_fit(lds, o, false);
         if (o.room >= 0) cp.lds_planes = std::max(cp.lds_planes, (uint32_t)o.room + o.size);
     }
@@ -377,13 +441,8 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
     std::vector<uint32_t> cost(T, 0), wave_of(T, 0);
     for (size_t n = 0; n < N; n++) {
         cost[task_of[n]] += row_cost(ops[n]);
-        Field f[3];
-        const int nf = current_fields(ops[n], f);
-        for (int k = 0; k < nf; k++) {
-            const int id = fobj[n][f[k].word];
-            const uint32_t moved = (f[k].read ? f[k].len : 0) + (f[k].write ? f[k].len : 0);
-            cost[task_of[n]] += moved * (objs[id].room >= 0 ? COST_LDS : COST_SLOT);
-        }
+        for (int w = 1; w < 8; w++)
+            if (fobj[n][w] >= 0) cost[task_of[n]] += moved[n][w] * (objs[fobj[n][w]].room >= 0 ? COST_LDS : COST_SLOT);
     }
     uint32_t fetch_phase = 1, fetch_wave = 0;
     uint64_t best_slack = 0;

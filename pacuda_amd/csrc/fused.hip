@@ -296,7 +296,7 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
 // the next unit off the queue while the current one runs. Every precision policy: the first
 // phase (the sources) runs with the f64 constants, as fused_kernel runs its sources, and the
 // policies whose denominators are df64 then rewrite the forms in df64 between two barriers.
-template <class K, int SHAPE, int W = COOP_WAVES[SHAPE], int UNITS = COOP_UNITS[SHAPE]>
+template <class K, int SHAPE, bool SYMJIT, int W = COOP_WAVES[SHAPE], int UNITS = COOP_UNITS[SHAPE]>
 __global__ void __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W * UNITS / 4)))
 coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit_start, uint32_t item_count,
             uint32_t *counter, double *scratch, size_t slot_doubles, uint32_t lds_bytes) {
@@ -373,6 +373,7 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
         const PlanDev *plan = it.plan;
         a.point_ = i;
         a.points_ = it.points;
+        a.bind_ = cnst(plan)->bindings;
         const double *c64 = cnst(plan)->c64, *cons = DF ? cnst(plan)->cdf : c64;
         const uint32_t S = cnst(plan)->scale_count, X = cnst(plan)->factor_count;
         a.sre_ = c64; a.sim_ = c64 + S; a.fre_ = c64 + 2 * S; a.fim_ = c64 + 2 * S + X;
@@ -382,7 +383,7 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
         const uint32_t mid = cnst(cnst(plan)->coop_split[CLASS][SHAPE])[wave];  // (a barrier row)
         // (one wavefront's stream takes the next unit off the queue: a fetch row in the phase that
         // hides the atomic's round trip best)
-        interpret<K, false, true>(prog + first, mid - first, a);  // the sources, in f64
+        interpret<K, SYMJIT, true>(prog + first, mid - first, a);  // the sources, in f64
         a.sre_ = cons; a.sim_ = cons + S; a.fre_ = cons + 2 * S; a.fim_ = cons + 2 * S + X;
         if constexpr (DF_FORMS) {  // the forms in df64 once no source reads them
             __syncthreads();  // (the barrier row at `mid`)
@@ -396,9 +397,9 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
                 a.put_form(cnst(form_field)[f], raw);
             }
             __syncthreads();
-            interpret<K, false, true>(prog + mid + 1, end - mid - 1, a);
+            interpret<K, SYMJIT, true>(prog + mid + 1, end - mid - 1, a);
         } else {
-            interpret<K, false, true>(prog + mid, end - mid, a);
+            interpret<K, SYMJIT, true>(prog + mid, end - mid, a);
         }
         __syncthreads();
         const uint32_t next = __builtin_amdgcn_readfirstlane(control[1]);
@@ -1604,6 +1605,28 @@ void mark_form_regs(std::vector<Instr> &ops, const uint32_t *reg_forms, uint32_t
     }
 }
 
+// The cooperative programs of `prog` (over `planes` current planes): per class of current and
+// shape, the candidate (phases as early or as late as possible, three LDS rankings) that leaves
+// the fewest current accesses in the slot
+void build_coop(Compiled &c, const std::vector<Instr> &prog, uint32_t planes, const pacuda_plan_desc *d) {
+    const uint32_t count = d->offsets[d->outputs];
+    std::vector<uint8_t> used(d->amplitude_planes, 0);
+    for (uint32_t k = 0; k < count; k++) used[d->planes[k]] = 1;
+    for (int cls = 0; cls < 2; cls++)
+        for (int s = 0; s < COOP_SHAPES; s++) {
+            if (!coop_built(cls, s)) continue;
+            const uint32_t plane_bytes = cls ? 512 : 1024;
+            for (int late = 0; late < 2; late++)
+                for (int rank = 0; rank < 3; rank++) {
+                    CoopProgram q = coop_program(prog, planes, d->amplitude_planes, d->form_count, used, COOP_WAVES[s],
+                                                 (LDS_BYTES / COOP_UNITS[s] - COOP_CONTROL) / plane_bytes, plane_bytes,
+                                                 c.bindings, late, rank);
+                    CoopProgram &best = c.coop[cls][s];
+                    if (q.ok && (!best.ok || q.slot_accesses < best.slot_accesses)) best = std::move(q);
+                }
+        }
+}
+
 int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
     if (!d || d->abi_version != PACMAN_RECURRENCE_ABI || d->struct_size != sizeof(pacuda_plan_desc) || !d->words ||
         !d->offsets || (d->form_count && !d->form_terms) || (d->term_count && (!d->term_slots || !d->term_coefficients)) ||
@@ -1649,6 +1672,12 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
         }
         for (uint32_t &f : c.reg_forms) f = UINT32_MAX;
         c.zero_fill = true;
+        // the cooperative programs model a symjit row's accesses from its binding (whole parents,
+        // the destination created or read and written whole); the first closure of each
+        // amplitude starts from zero there
+        std::vector<Instr> coop = hp.ops;
+        mark_first_closures(coop, d->amplitude_planes);
+        build_coop(c, coop, d->current_planes, d);
         return PACMAN_RECURRENCE_OK;
     }
     if (!reads_after_writes(hp.ops, d->current_planes)) {
@@ -1670,25 +1699,7 @@ int compile_plan(const pacuda_plan_desc *d, Compiled &c) {
     forward_in_place(hp.ops);
     c.zero_fill = mark_first_closures(hp.ops, d->amplitude_planes);
     c.fwd = hp.ops;
-    {
-        std::vector<uint8_t> used(d->amplitude_planes, 0);
-        for (uint32_t k = 0; k < count; k++) used[d->planes[k]] = 1;
-        // per shape, the candidate (phases as early or as late as possible, three LDS rankings)
-        // that leaves the fewest current accesses in the slot
-        for (int cls = 0; cls < 2; cls++)
-            for (int s = 0; s < COOP_SHAPES; s++) {
-                if (!coop_built(cls, s)) continue;
-                const uint32_t plane_bytes = cls ? 512 : 1024;
-                for (int late = 0; late < 2; late++)
-                    for (int rank = 0; rank < 3; rank++) {
-                        CoopProgram q = coop_program(c.fwd, c.planes, d->amplitude_planes, d->form_count, used,
-                                                     COOP_WAVES[s], (LDS_BYTES / COOP_UNITS[s] - COOP_CONTROL) / plane_bytes,
-                                                     plane_bytes, late, rank);
-                        CoopProgram &best = c.coop[cls][s];
-                        if (q.ok && (!best.ok || q.slot_accesses < best.slot_accesses)) best = std::move(q);
-                    }
-            }
-    }
+    build_coop(c, c.fwd, c.planes, d);
     // the on-chip programs: the passes over the rows as built and over each candidate order, the
     // allocator's placement (the static one where it gives up), keeping per variant and tier the
     // program that leaves the fewest slot accesses
@@ -1806,6 +1817,19 @@ extern "C" int pacuda_plan_coop_program(const pacuda_plan_desc *d, uint32_t shap
     if (out) {
         if (capacity < ops.size()) return PACMAN_RECURRENCE_INVALID;
         memcpy(out, ops.data(), ops.size() * sizeof(Instr));
+    }
+    return PACMAN_RECURRENCE_OK;
+}
+
+extern "C" int pacuda_plan_bindings(const pacuda_plan_desc *d, uint32_t *out, uint32_t capacity, uint32_t *count) {
+    if (!count) return PACMAN_RECURRENCE_INVALID;
+    Compiled c;
+    const int st = compile_plan(d, c);
+    if (st) return st;
+    *count = (uint32_t)c.bindings.size();
+    if (out) {
+        if (capacity < c.bindings.size()) return PACMAN_RECURRENCE_INVALID;
+        memcpy(out, c.bindings.data(), c.bindings.size() * sizeof(uint32_t));
     }
     return PACMAN_RECURRENCE_OK;
 }
@@ -2005,6 +2029,10 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
             sc.lds_all[v] = sc.lds_all[v] && it.plan->host.lds_planes[v] > 0;
         }
     }
+    if (sc.coop >= 0 && sc.symjit && sc.coop != 0 && sc.coop != 2) {
+        fprintf(stderr, "pacuda: plans with symjit rows run cooperatively with 16 wavefronts per CU only\n");
+        return PACMAN_RECURRENCE_INVALID;
+    }
     // (a slot is addressed by 32-bit buffer offsets)
     if (sc.units >= 0xffffffffull || sc.slot * sizeof(double) > 0x7fffffffull) return PACMAN_RECURRENCE_INVALID;
     sc.starts[count] = (uint32_t)sc.units;
@@ -2077,24 +2105,36 @@ bool launch(const Schedule &sc, hipStream_t s, int tier, size_t bytes, const Ite
     return true;
 }
 
-template <class K, int SHAPE> bool allow_coop_lds() {
-    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(coop_kernel<K, SHAPE>),
+template <class K, int SHAPE, bool SJ> bool allow_coop_lds() {
+    static const bool ok = !hip_failed(hipFuncSetAttribute(reinterpret_cast<const void *>(coop_kernel<K, SHAPE, SJ>),
                                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                                            LDS_BYTES / COOP_UNITS[SHAPE]),
                                        "cooperative kernel LDS");
     return ok;
 }
 
-template <class K, int SHAPE>
-bool launch_coop_shape(const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts, uint32_t count,
-                       uint32_t *counter, double *scratch, bool run) {
-    if (!allow_coop_lds<K, SHAPE>()) return false;
+template <class K, int SHAPE, bool SJ>
+bool launch_coop_kernel(const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts,
+                        uint32_t count, uint32_t *counter, double *scratch, bool run) {
+    if (!allow_coop_lds<K, SHAPE, SJ>()) return false;
     if (!run) return true;
     const uint32_t bytes = LDS_BYTES / COOP_UNITS[SHAPE];
     reset_counter<<<1, 64, 0, s>>>(counter);
-    coop_kernel<K, SHAPE><<<sc.grid, COOP_WAVES[SHAPE] * LANES, bytes, s>>>(items, starts, count, counter, scratch,
-                                                                         sc.slot, bytes);
+    coop_kernel<K, SHAPE, SJ><<<sc.grid, COOP_WAVES[SHAPE] * LANES, bytes, s>>>(items, starts, count, counter,
+                                                                            scratch, sc.slot, bytes);
     return true;
+}
+
+// (the instance with symjit rows when a plan of the launch holds some: shapes 0 and 2)
+template <class K, int SHAPE>
+bool launch_coop_shape(const Schedule &sc, hipStream_t s, const ItemDev *items, const uint32_t *starts, uint32_t count,
+                       uint32_t *counter, double *scratch, bool run) {
+    if constexpr (SHAPE == 0 || SHAPE == 2) {
+        if (sc.symjit) return launch_coop_kernel<K, SHAPE, true>(sc, s, items, starts, count, counter, scratch, run);
+    } else {
+        if (sc.symjit) return false;  // (schedule refuses it)
+    }
+    return launch_coop_kernel<K, SHAPE, false>(sc, s, items, starts, count, counter, scratch, run);
 }
 
 // the policies other than f64 run shapes 0 and 2 only (coop_built)

@@ -102,6 +102,19 @@ def plan_desc(plan, arrays, no_sort=0):
 COOP_SHAPES = ((16, 16), (8, 8), (8, 16), (4, 16))
 
 
+def plan_bindings(lib, plan):
+    """the binding table a plan's programs index (pacuda_plan_bindings, host only): uint32 array"""
+    a = plan.arrays()
+    d = plan_desc(plan, a)
+    n = C.c_uint32()
+    if lib.pacuda_plan_bindings(C.byref(d), None, 0, C.byref(n)) != 0:
+        raise RuntimeError("pacuda_plan_bindings failed")
+    out = np.zeros(max(n.value, 1), dtype=np.uint32)
+    if lib.pacuda_plan_bindings(C.byref(d), out.ctypes.data_as(u32p), len(out), C.byref(n)) != 0:
+        raise RuntimeError("pacuda_plan_bindings failed")
+    return out[:n.value]
+
+
 def coop_program(lib, plan, shape, serial=False, f32=False):
     """a plan's cooperative program for shape index `shape` (pacuda_plan_coop_program, host
     only), for 16-byte currents or (f32) the f32 ones: (uint32 [count][8], info dict, stream

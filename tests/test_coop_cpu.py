@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from pacuda_amd import abi, plans, processes, phase_space as ps
-from pacuda_amd.runtime import coop_program, COOP_SHAPES
+from pacuda_amd.runtime import coop_program, plan_bindings, COOP_SHAPES
 import harness
 import refcpu
 
@@ -28,8 +28,10 @@ def lib():
 
 def _plans():
     subs = processes.subprocesses()
+    four = processes.four_quark_subprocesses()
     return [("gg>ggg", plans.gluon_plan(5)), ("gg>gggg", plans.gluon_plan(6)), ("qline5", plans.quark_line_plan(5)),
-            (subs[7].name, subs[7].plan), (subs[25].name, subs[25].plan)]
+            (subs[7].name, subs[7].plan), (subs[25].name, subs[25].plan), (four[0].name, four[0].plan),
+            (four[57].name, four[57].plan)]
 
 
 # (class, shape, policy): the 16-byte currents' programs in every shape under f64 and df64, the
@@ -39,7 +41,7 @@ CASES = [(False, k, "f64") for k in range(len(COOP_SHAPES))] + [(False, 2, "df64
 
 
 @pytest.mark.parametrize("f32,shape,policy", CASES)
-@pytest.mark.parametrize("which", range(5))
+@pytest.mark.parametrize("which", range(7))
 def test_serial_order_equals_the_plan_run(lib, f32, shape, policy, which):
     name, plan = _plans()[which]
     got = coop_program(lib, plan, shape, serial=True, f32=f32)
@@ -59,7 +61,8 @@ def test_serial_order_equals_the_plan_run(lib, f32, shape, policy, which):
     assert lead > 0 and not np.any(kinds[lead:] == 6)
     want_res, want_tot = refcpu.evaluate(plan.arrays(), counts, ext, n, kernel=k)
     res, tot = refcpu.evaluate_program(ser, lead, info["slot_planes"], info["lds_planes"], plan.arrays(), counts, ext, n,
-                                       kernel=k, numerator=plan.numerator, denominator=plan.denominator)
+                                       kernel=k, numerator=plan.numerator, denominator=plan.denominator,
+                                       bindings=plan_bindings(lib, plan))
     assert np.array_equal(res.view(np.uint64), want_res.view(np.uint64)), name
     assert np.array_equal(tot.view(np.uint64), want_tot.view(np.uint64)), name
 
@@ -93,10 +96,20 @@ def test_more_lds_leaves_fewer_slot_accesses(lib):
     assert large["lds_accesses"] > 2 * large["slot_accesses"]
 
 
-def test_plans_with_symjit_rows_have_none(lib):
+def test_symjit_rows_in_cooperative_programs(lib):
+    """a four-quark plan's symjit rows (fermion-pair currents, kernel 1 of the built-in table) are
+    modelled from their bindings: each reads both two-component parents and creates a
+    four-component destination, which the program then propagates in place"""
     sub = processes.four_quark_subprocesses()[0]
-    assert coop_program(lib, sub.plan, 0) is None
-    assert coop_program(lib, sub.plan, 0, f32=True) is None
+    dev, info, starts = coop_program(lib, sub.plan, 0)
+    kinds = dev[:, 0] & 0xff
+    assert np.count_nonzero(kinds == 7) == 16
+    assert info["slot_accesses"] < info["lds_accesses"]
+    bind = plan_bindings(lib, sub.plan)
+    for w in dev[kinds == 7]:
+        at = int(w[6])
+        inputs, pairs = int(bind[at + 1]), int(bind[at + 2])
+        assert list(bind[at + 4 + inputs:at + 4 + inputs + pairs]) == [0, 1, 2, 3]
 
 
 def test_f32_currents_fit_twice_the_planes(lib):

@@ -84,9 +84,21 @@ def test_coop_other_policies_bitwise(lib, kernel):
 
 def test_coop_one_launch_of_many_plans(lib):
     """the one-quark-line p p > j j j subprocesses and g g > g g g in one cooperative launch with
-    uneven point counts (a unit count of 1 included)"""
+    uneven point counts (a unit count of 1 included), in every f64 shape"""
     subs = [s for s in processes.subprocesses() if s.plan.name in ("5g", "qqbar+3g")]
     assert len(subs) == 29
+    _many_plans(lib, subs, COOP_SHAPES, ("f64",))
+
+
+def test_coop_pp_jjj_with_symjit_rows(lib):
+    """every p p > j j j subprocess, the 152 four-quark ones with their symjit rows among them, in
+    one cooperative launch, in the shapes and policies that run symjit rows cooperatively"""
+    subs = processes.subprocesses()
+    assert len(subs) == 181
+    _many_plans(lib, subs, ANY_SHAPES, ("f64", "mixed64", "compensated64"))
+
+
+def _many_plans(lib, subs, shapes, kernels):
     jobs, devs, fps, exts = [], [], [], []
     for k, sp in enumerate(subs):
         points = 64 * (k % 7) + 29 * k + 1
@@ -101,13 +113,15 @@ def test_coop_one_launch_of_many_plans(lib):
         fps.append(fp)
         exts.append(ext)
     try:
-        for unit_waves, per_cu in COOP_SHAPES:
-            print("pp>jjj one-line subprocesses: cooperative", unit_waves, per_cu, flush=True)
-            fused_evaluate(lib, jobs, F64, waves_per_cu=per_cu, unit_waves=unit_waves)
-            for sp, (fp, _, points, _, _, d_tot), ext in zip(subs, jobs, exts):
-                _, want = refcpu.evaluate(sp.plan.arrays(), harness.plan_counts(sp.plan), ext, points, kernel=1,
-                                          resolved=False)
-                assert bits_equal(d_tot.download(points), want), f"{unit_waves}/{per_cu} {sp.name}"
+        for kernel in kernels:
+            k = abi.FUSED_KERNELS[kernel]
+            wants = [refcpu.evaluate(sp.plan.arrays(), harness.plan_counts(sp.plan), ext, job[2], kernel=k,
+                                     resolved=False)[1] for sp, job, ext in zip(subs, jobs, exts)]
+            for unit_waves, per_cu in shapes:
+                print(len(subs), "p p > j j j subprocesses:", kernel, "cooperative", unit_waves, per_cu, flush=True)
+                fused_evaluate(lib, jobs, k, waves_per_cu=per_cu, unit_waves=unit_waves)
+                for sp, (fp, _, points, _, _, d_tot), want in zip(subs, jobs, wants):
+                    assert bits_equal(d_tot.download(points), want), f"{kernel} {unit_waves}/{per_cu} {sp.name}"
     finally:
         for d in devs:
             d.free()
@@ -155,10 +169,11 @@ def test_coop_sweep_replays_and_static_entry_point(lib):
 
 
 def test_coop_refusals(lib):
-    """host-side refusals (nothing is launched): a plan with symjit rows, a shape only the f64
-    kernel runs asked of another, a shape that does not exist"""
+    """host-side refusals (nothing is launched): a plan with symjit rows in a shape without a
+    symjit instance, a shape only the f64 kernel runs asked of another, a shape that does not
+    exist"""
     sub = processes.four_quark_subprocesses()[0]
-    fp = FusedPlan(lib, sub.plan)
+    fp = FusedPlan(lib, sub.plan)  # (symjit rows: shapes 0 and 2 only)
     good = FusedPlan(lib, plans.gluon_plan(5))
     d = DeviceArray(lib, 5 * 4 * 64 * 8)
     d_tot = DeviceArray(lib, 64 * 8)
@@ -169,7 +184,7 @@ def test_coop_refusals(lib):
             items[0].ext_stride, items[0].resolved, items[0].total = 64, None, d_tot.ptr
             l = abi.Launch(abi.ABI, C.sizeof(abi.Launch), kernel, per_cu, 0, unit_waves, None, None)
             return lib.pacuda_evaluate(items, 1, C.byref(l))
-        assert status(fp, F64, 16, 16) == 2
+        assert status(fp, F64, 8, 8) == 2
         assert status(good, abi.FUSED_KERNELS["mixed"], 8, 8) == 2  # (a shape only f64 runs)
         assert status(good, F64, 12, 16) == 2
         assert status(good, F64, 16, 2) == 2
