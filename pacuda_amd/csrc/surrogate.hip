@@ -325,6 +325,7 @@ __global__ void __launch_bounds__(256, 1) mlp_infer(InferArgs a) {
 constexpr uint32_t GT = 64;      // tile side
 constexpr uint32_t GK = 256;     // k per LDS chunk
 constexpr uint32_t GG = GK / 8;  // 16-byte granules per LDS row
+constexpr uint32_t GD = 4;       // chunks whose loads are issued together
 enum { E_PARTIAL, E_FORWARD, E_DZ };
 
 struct GemmArgs {
@@ -374,9 +375,22 @@ struct GemmTable {
 __device__ __forceinline__ void head_reduce(const GemmTable *t, uint32_t block) {
     const uint32_t v = block * 16 + (threadIdx.x >> 4), q = threadIdx.x & 15, H = t->hidden;
     float s = 0.0f;
+    // (eight loads issued before their eight additions, which keep the order tile q, q + 16, ...:
+    // the loads' latencies overlap instead of adding up)
     if (v <= H + 1)
-        for (uint32_t tile = q; tile < t->tiles; tile += 16)
-            s += v == H + 1 ? t->loss_part[tile] : t->head_part[(size_t)tile * (H + 1) + v];
+        for (uint32_t base = q; base < t->tiles; base += 8 * 16) {
+            float x[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8; j++) {
+                const uint32_t tile = base + 16 * j;
+                x[j] = tile >= t->tiles ? 0.0f
+                       : v == H + 1    ? t->loss_part[tile]
+                                       : t->head_part[(size_t)tile * (H + 1) + v];
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 8; j++)
+                if (base + 16 * j < t->tiles) s += x[j];
+        }
 #pragma unroll
     for (uint32_t m = 1; m < 16; m *= 2) s += __shfl_xor(s, m);
     if (q == 0 && v <= H + 1) t->head_total[v] = s;
@@ -385,11 +399,23 @@ __device__ __forceinline__ void head_reduce(const GemmTable *t, uint32_t block) 
 __device__ __forceinline__ void gemm_body(const GemmArgs &a, uint32_t bx, uint32_t by, uint32_t bz, uint4 *sa,
                                           uint4 *sb);
 
+// XCD-aware numbering: the hardware hands consecutive workgroups to the eight XCDs in turn, each
+// with its own L2, so workgroup b takes logical index `xcd_major(b)`: the workgroups of one XCD
+// get consecutive logical indices. The products number their workgroups m-tile fastest, then
+// n-tile, then k split, so the tiles that share one k split's operand rows (a few hundred KB per
+// layer) run on one XCD and read them once from HBM instead of once per XCD.
+constexpr uint32_t XCDS = 8;
+__device__ __forceinline__ uint32_t xcd_major(uint32_t b, uint32_t n) {
+    const uint32_t q = n / XCDS, rem = n % XCDS, x = b % XCDS, local = b / XCDS;
+    return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + local;
+}
+
 __global__ void __launch_bounds__(256) gemm_tile(GemmPair pair) {
     __shared__ uint4 sa[GT * GG], sb[GT * GG];
-    const bool second = blockIdx.x >= pair.first1;
+    const uint32_t block = xcd_major(blockIdx.x, gridDim.x);
+    const bool second = block >= pair.first1;
     const GemmArgs a = second ? pair.g[1] : pair.g[0];
-    const uint32_t local = blockIdx.x - (second ? pair.first1 : 0), tiles_m = pair.tiles_m[second],
+    const uint32_t local = block - (second ? pair.first1 : 0), tiles_m = pair.tiles_m[second],
                    tiles_n = pair.tiles_n[second];
     gemm_body(a, local % tiles_m, (local / tiles_m) % tiles_n, local / (tiles_m * tiles_n), sa, sb);
 }
@@ -397,14 +423,15 @@ __global__ void __launch_bounds__(256) gemm_tile(GemmPair pair) {
 __global__ void __launch_bounds__(256) gemm_table(const GemmTable *table) {
     __shared__ uint4 sa[GT * GG], sb[GT * GG];
     const GemmTable *t = table;
-    if (blockIdx.x >= t->first[t->count]) {
-        head_reduce(t, blockIdx.x - t->first[t->count]);
+    const uint32_t block = xcd_major(blockIdx.x, gridDim.x);
+    if (block >= t->first[t->count]) {
+        head_reduce(t, block - t->first[t->count]);
         return;
     }
     uint32_t p = 0;
-    while (p + 1 < t->count && blockIdx.x >= t->first[p + 1]) p++;
+    while (p + 1 < t->count && block >= t->first[p + 1]) p++;
     const GemmArgs a = t->g[p];
-    const uint32_t local = blockIdx.x - t->first[p], tiles_m = t->tiles_m[p], tiles_n = t->tiles_n[p];
+    const uint32_t local = block - t->first[p], tiles_m = t->tiles_m[p], tiles_n = t->tiles_n[p];
     gemm_body(a, local % tiles_m, (local / tiles_m) % tiles_n, local / (tiles_m * tiles_n), sa, sb);
 }
 
@@ -425,24 +452,36 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &a, uint32_t bx, uint32
     const uint32_t sr = tid >> 2, sg = tid & 3;
     const bool b_row = tn + sr < a.N;
     const bf16 *ap = a.A + (size_t)(tm + sr) * a.lda, *bp = a.B + (size_t)(tn + sr) * a.ldb;
-    for (uint32_t k0 = k_begin; k0 < k_end; k0 += GK) {
-        const uint32_t kc = min(GK, k_end - k0), gs = (kc + 31) / 32 * 4;  // granules staged per row
-        uint4 ra[GG / 4], rb[GG / 4];
+    // The loads of up to GD chunks are issued together, into registers, before the first is stored
+    // to LDS: a workgroup waits one round trip to memory per GD chunks rather than per chunk (the
+    // products of a chunk take far less time than a load's latency)
+    uint4 ra[GD][GG / 4], rb[GD][GG / 4];
+    for (uint32_t kg = k_begin; kg < k_end; kg += GD * GK) {
 #pragma unroll
-        for (uint32_t q = 0; q < GG / 4; q++) {
-            const uint32_t g = sg + 4 * q, k = k0 + 8 * g;
-            ra[q] = rb[q] = uint4{0u, 0u, 0u, 0u};
-            if (4 * q < gs && k < k_end) {
-                ra[q] = *reinterpret_cast<const uint4 *>(ap + k);
-                if (b_row) rb[q] = *reinterpret_cast<const uint4 *>(bp + k);
+        for (uint32_t d = 0; d < GD; d++) {
+            const uint32_t k0 = kg + d * GK;
+            const uint32_t gs = k0 < k_end ? (min(GK, k_end - k0) + 31) / 32 * 4 : 0;
+#pragma unroll
+            for (uint32_t q = 0; q < GG / 4; q++) {
+                const uint32_t g = sg + 4 * q, k = k0 + 8 * g;
+                ra[d][q] = rb[d][q] = uint4{0u, 0u, 0u, 0u};
+                if (4 * q < gs && k < k_end) {
+                    ra[d][q] = *reinterpret_cast<const uint4 *>(ap + k);
+                    if (b_row) rb[d][q] = *reinterpret_cast<const uint4 *>(bp + k);
+                }
             }
         }
+#pragma unroll
+        for (uint32_t d = 0; d < GD; d++) {
+        const uint32_t k0 = kg + d * GK;
+        if (k0 >= k_end) break;
+        const uint32_t kc = min(GK, k_end - k0), gs = (kc + 31) / 32 * 4;  // granules staged per row
         __syncthreads();  // (the previous chunk's reads are done)
 #pragma unroll
         for (uint32_t q = 0; q < GG / 4; q++)
             if (4 * q < gs) {
-                sa[gran(sr, sg + 4 * q)] = ra[q];
-                sb[gran(sr, sg + 4 * q)] = rb[q];
+                sa[gran(sr, sg + 4 * q)] = ra[d][q];
+                sb[gran(sr, sg + 4 * q)] = rb[d][q];
             }
         __syncthreads();
         if (sums)
@@ -460,6 +499,7 @@ __device__ __forceinline__ void gemm_body(const GemmArgs &a, uint32_t bx, uint32
 #pragma unroll
                 for (int j = 0; j < 2; j++) acc[i][j] = mfma32(av[i], bv[j], acc[i][j]);
         }
+    }
     }
     if (sums) {
         rs += __shfl_xor(rs, 1);
@@ -1182,6 +1222,12 @@ void forward_launches(pacuda_mlp *m, const float *x, uint32_t batch, float *y) {
                                                         n.params + n.bout_off, y);
 }
 
+// the K splits of the weight-gradient products (K = batch): a workgroup takes at least
+// SPLIT_K_MIN of K in LDS chunks of GK whose loads overlap the previous chunk's products, and the
+// few output tiles (hidden/64 x fan_in/64 per layer) times the splits still give every CU work
+constexpr uint32_t SPLIT_K_MIN = 1024;
+uint32_t weight_splits(uint32_t batch) { return std::max(1u, std::min(MAX_SPLIT, batch / SPLIT_K_MIN)); }
+
 // the backward down to the partial sums (y NULL: computed from the kept activations); returns
 // the dW splits used
 uint32_t backward_launches(pacuda_mlp *m, const float *y, const float *target, uint32_t batch) {
@@ -1195,7 +1241,7 @@ uint32_t backward_launches(pacuda_mlp *m, const float *y, const float *target, u
                                                         n.params + n.wout_off, n.dz[0], n.dzt[0]);
     // dW_l splits K (= batch) so that the few output tiles still fill the GPU; the partials and the
     // bias gradient's row sums are summed in order afterwards
-    const uint32_t splits = std::max(1u, std::min(MAX_SPLIT, batch / 256));
+    const uint32_t splits = weight_splits(batch);
     const uint32_t k_split = (batch / splits + 15) / 16 * 16;
     const uint32_t used = (batch + k_split - 1) / k_split;
     const uint64_t part_stride = (uint64_t)MAX_SPLIT * H * n.kmax;
@@ -1244,7 +1290,7 @@ TileArgs tile_args(pacuda_mlp *m, const float *x, const float *t, float *y, uint
 uint32_t weight_gradients(pacuda_mlp *m, uint32_t batch, bool upload = true) {
     Net &n = m->n;
     const uint32_t H = n.hidden, L = n.layers;
-    const uint32_t splits = std::max(1u, std::min(MAX_SPLIT, batch / 256));
+    const uint32_t splits = weight_splits(batch);
     const uint32_t k_split = (batch / splits + 15) / 16 * 16;
     const uint32_t used = (batch + k_split - 1) / k_split;
     const uint64_t part_stride = (uint64_t)MAX_SPLIT * H * n.kmax;
