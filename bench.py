@@ -268,13 +268,14 @@ def main():
     jobs = [(fp, d_ext, n, n, None, d_tot)]
     k64 = abi.KERNEL_F64
     tuned = None
+    one_wave = (args.waves_per_cu or 12, 256)  # (the one-wavefront-per-unit launch of the f64 kernel)
     if args.launch == "fused":
-        shape = (args.waves_per_cu, 0, 0)
+        shape = one_wave + (0,)
     elif args.launch.startswith("coop:"):
         _, u, w = args.launch.split(":")
         shape = (int(w), 0, int(u))
     else:
-        shape, tuned = autotune(lib, jobs, k64, shapes=((args.waves_per_cu, 0),) + COOP_TUNE_SHAPES)
+        shape, tuned = autotune(lib, jobs, k64, shapes=(one_wave,) + COOP_TUNE_SHAPES)
         shape = tuple(shape) + (0,) * (3 - len(shape))
         log(f"f64 launch: {shape_name(shape)} ({', '.join(f'{k} {v:.2f} ms' for k, v in tuned.items())})")
     wpc, _, uw = shape
@@ -290,7 +291,7 @@ def main():
     if not args.no_modes:
         for name in ("df64", "mixed", "mixed64", "compensated", "compensated64"):
             k = abi.FUSED_KERNELS[name]
-            mshape, _ = autotune(lib, jobs, k, shapes=((0, 0),) + COOP_TUNE_SHAPES_ANY, repeats=1)
+            mshape, _ = autotune(lib, jobs, k, shapes=((16, 256),) + COOP_TUNE_SHAPES_ANY, repeats=1)
             _, ms = timed_steps(lib, jobs, k, 3, *(tuple(mshape) + (0,))[:3])
             modes[name] = {"points_per_s": n / (ms * 1e-3), "kernel_ms": ms, "launch": shape_name(mshape),
                            **mode_errors(d_tot.download(n), total64)}
@@ -359,14 +360,14 @@ def main():
             t = dev.empty(int(c))
             pjobs.append((f, e, int(c), int(c), None, t))
             holds.append((f, e, t))
-        pshape, _ = autotune(lib, pjobs, k64, shapes=((0, 0),) + COOP_TUNE_SHAPES_ANY, repeats=1)
+        pshape, _ = autotune(lib, pjobs, k64, shapes=((12, 256),) + COOP_TUNE_SHAPES_ANY, repeats=1)
         _, ms_p = timed_steps(lib, pjobs, k64, 3, *(tuple(pshape) + (0,))[:3])
         refs = [j[5].download(j[2]) for j in pjobs]
         pp = {"subprocesses": len(subs), "points": int(counts.sum()), "f64_points_per_s": int(counts.sum()) / (ms_p * 1e-3),
               "f64_launch": shape_name(pshape), "counts_min_max": [int(counts.min()), int(counts.max())]}
         for name in ("mixed", "mixed64", "compensated", "compensated64"):
             k = abi.FUSED_KERNELS[name]
-            kshape, _ = autotune(lib, pjobs, k, shapes=((0, 0),) + COOP_TUNE_SHAPES_ANY, repeats=1)
+            kshape, _ = autotune(lib, pjobs, k, shapes=((16, 256),) + COOP_TUNE_SHAPES_ANY, repeats=1)
             _, ms = timed_steps(lib, pjobs, k, 3, *(tuple(kshape) + (0,))[:3])
             per = {sp.name: mode_errors(j[5].download(j[2]), r) for sp, j, r in zip(subs, pjobs, refs)}
             errs = {n: e["max_err_rel_to_max_f64"] for n, e in per.items()}

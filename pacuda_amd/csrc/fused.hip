@@ -48,6 +48,11 @@ constexpr uint32_t COOP_CONTROL = 256;
 // the cooperative kernels built: f64 in every shape, the other policies in shapes 0 and 2 (per
 // class of current, 0 for 16-byte currents and 1 for the f32 ones, the programs they need)
 constexpr bool coop_built(int cls, int shape) { return cls == 0 || shape == 0 || shape == 2; }
+// the default launch's shape for the kernels whose cooperative launch is the faster one (f64,
+// mixed, mixed64, compensated64: 8 wavefronts per unit, two units per CU; the df64-arithmetic
+// kernels are VALU-bound and keep one wavefront per unit)
+constexpr int COOP_DEFAULT = 2;
+static_assert(COOP_WAVES[COOP_DEFAULT] == 8 && COOP_UNITS[COOP_DEFAULT] == 2, "the default cooperative shape");
 
 struct PlanDev {
     const Instr *prog_lds[LDS_VARIANTS];  // with forwarding and LDS-resident currents (LDS_CAPS[v] planes)
@@ -1953,6 +1958,11 @@ struct Schedule {
 // The LDS variant of a launch, its LDS elements per wavefront and bytes per workgroup. A variant
 // qualifies when every plan has it and the workgroups a CU must hold for the requested
 // wavefronts fit in its LDS; the largest one is taken, else none (per_wave 0).
+bool coop_default(uint32_t kernel) {
+    return kernel == PACMAN_RECURRENCE_KERNEL_F64 || kernel == PACMAN_RECURRENCE_KERNEL_MIXED ||
+           kernel == PACMAN_RECURRENCE_KERNEL_MIXED64 || kernel == PACUDA_KERNEL_COMPENSATED64;
+}
+
 // the kernels whose currents are f32
 bool f32_kernel(uint32_t kernel) {
     return kernel == PACMAN_RECURRENCE_KERNEL_MIXED || kernel == PACMAN_RECURRENCE_KERNEL_MIXED64 ||
@@ -1993,6 +2003,17 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
             fprintf(stderr, "pacuda: no cooperative launch of kernel %u with %u wavefronts per unit and %u per CU\n",
                     l->kernel, l->unit_waves, per_cu);
             return PACMAN_RECURRENCE_INVALID;
+        }
+    } else if (!l->waves_per_cu && !l->block && coop_default(l->kernel)) {
+        // the default launch: cooperative, 8 wavefronts per unit and two units per CU (the
+        // fastest shape measured for these kernels), when every plan has its program
+        const int cls = f32_kernel(l->kernel) ? 1 : 0;
+        bool all = count > 0;
+        for (uint32_t k = 0; k < count && all; k++)
+            all = items[k].plan && items[k].plan->coop_slot_doubles[cls][COOP_DEFAULT];
+        if (all) {
+            sc.coop = COOP_DEFAULT;
+            sc.coop_class = cls;
         }
     }
     std::vector<uint32_t> order(count);
