@@ -125,11 +125,12 @@ inline int current_fields(const Instr &o, Field *f) {
 // its size), the destination is created when the row initialises it and its outputs are the
 // components 0..m-1 (and no code reads it), else read and written whole; forms the codes read are
 // reported in `forms` (their 16-bit fields cannot name LDS). -1 when the binding is out of the table.
+constexpr int SYMJIT_FORMS_MAX = 64;  // the momentum inputs one binding may read (`forms` holds them)
 inline int symjit_fields(const Instr &o, const std::vector<uint32_t> &bindings, Field *f, uint32_t *forms, int *nforms) {
     const size_t at = o.w[6];
     if (at + 4 > bindings.size()) return -1;
     const uint32_t inputs = bindings[at + 1], pairs = bindings[at + 2];
-    if (at + 4 + inputs + pairs > bindings.size()) return -1;
+    if (at + 4 + inputs + pairs > bindings.size() || inputs > (uint32_t)SYMJIT_FORMS_MAX) return -1;
     bool parent[2] = {false, false}, dest_read = false;
     *nforms = 0;
     for (uint32_t k = 0; k < inputs; k++) {
@@ -267,7 +268,7 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
         const int kind = o.w[0] & 0xff;
         fobj[n].fill(-1);
         Field f[3];
-        uint32_t sforms[2 * 64];
+        uint32_t sforms[SYMJIT_FORMS_MAX];
         int nsforms = 0;
         const int nf = kind == K_SYMJIT ? symjit_fields(o, bindings, f, sforms, &nsforms) : current_fields(o, f);
         if (nf < 0) return cp;
