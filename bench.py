@@ -129,14 +129,17 @@ def mode_errors(got, ref):
 
 
 def kernel_name(shape):
-    """the f64 kernel a launch shape (waves_per_cu, block, unit_waves) runs, as rocprofv3 names
-    its template (the cooperative kernel: shape index, wavefronts per unit, units per CU)"""
+    """the f64 kernel a launch shape (waves_per_cu, block, unit_waves) of a plan without symjit
+    rows runs, as rocprofv3 names its template: the one-wavefront kernel by its register tier (2
+    up to 8 wavefronts per CU, 1 up to 12, else 0), the cooperative kernel by its shape index,
+    wavefronts per unit and units per CU"""
     from pacuda_amd.runtime import COOP_SHAPES
     wpc, _, uw = shape
     if uw:
         k = COOP_SHAPES.index((uw, wpc or 16))
-        return f"coop_kernel<KF64, {k}, {uw}, {(wpc or 16) // uw}>"
-    return "fused_kernel<KF64>"
+        return f"coop_kernel<KF64, {k}, false, {uw}, {(wpc or 16) // uw}>"
+    tier = 2 if (wpc or 12) <= 8 else 1 if (wpc or 12) <= 12 else 0
+    return f"fused_kernel<KF64, {tier}, false>"
 
 
 def measured_traffic(kernel, points, rows, launch):
@@ -177,6 +180,133 @@ def slot_summary(lib, plan, shape):
     return {"program": "f64, 13 LDS planes, 6 register forms, sources made again", "slot_planes": planes,
             "slot_kib_per_wavefront": planes * 64 * 16 / 1024, "on_chip_planes": lds,
             "slot_bytes_per_point": st["bytes"], **{k: v for k, v in st.items() if k != "bytes"}}
+
+
+def v1_abi(lib, plan, ext_host, want, tiles=(1 << 18, 1 << 20, 1 << 22), repeats=3):
+    """the reference's calling pattern through the v1 entry points (umami_pacuda.c's batched
+    evaluation): per tile of points, the external momenta from pinned host memory,
+    pacman_recurrence_v1_run with the forms built on the device and the amplitudes kept (cleared
+    on the device, none downloaded), then the reduction into pinned host memory: every helicity
+    output from zero (pacman_recurrence_v1_reduce_resolved, 8 B per output and point back) or the
+    helicity-sampled totals (pacman_recurrence_v1_reduce_sample, 12 B per point back). Wall time
+    over all points of the best of `repeats` passes after one warm-up pass: host-to-device and
+    device-to-host copies included, as the entry points take and return host memory. The totals
+    are compared bit for bit with the fused f64 launch's."""
+    C = abi.C
+    opt = abi.Options(abi.ABI, C.sizeof(abi.Options), abi.KERNEL_F64, 0, 0, 0)
+    if lib.pacman_recurrence_v1_configure(C.byref(opt)) != 0:
+        raise RuntimeError("pacman_recurrence_v1_configure refused the f64 kernel")
+    a = plan.arrays()
+    n = ext_host.shape[1]
+    rows = ext_host.shape[0]
+    outputs = len(a["offsets"]) - 1
+    n_amp = plan.amplitude_planes
+    amp_runs = np.array([0, n_amp], dtype=np.uint32)
+    helicity = np.arange(outputs, dtype=np.uint32)
+    rng = np.random.default_rng(7)
+
+    def pinned(count, dtype=np.float64):
+        p = lib.pacman_recurrence_v1_host_alloc(count * np.dtype(dtype).itemsize)
+        if not p:
+            raise RuntimeError("pacman_recurrence_v1_host_alloc failed")
+        buf = (C.c_char * (count * np.dtype(dtype).itemsize)).from_address(p)
+        return p, np.frombuffer(buf, dtype=dtype)
+
+    res = {"workload": f"{plan.name}, {n} points, f64, KEEP_AMPLITUDES | FORMS_ON_DEVICE", "tiles": {}}
+    for tile in tiles:
+        tile = min(tile, n)
+        starts = list(range(0, n, tile))
+        # the tiles' momenta staged in pinned memory before the timed passes, each tile's planes of
+        # `points` values (the layout the run reads)
+        p_ext, ext = pinned(rows * n)
+        p_out, out = pinned(outputs * tile)
+        p_rand, rand = pinned(tile)
+        p_tot, tot = pinned(n)
+        p_cho, chosen = pinned(tile, np.uint32)
+        hold = []
+        try:
+            for s in starts:
+                pts = min(tile, n - s)
+                ext[rows * s:rows * (s + pts)] = ext_host[:, s:s + pts].reshape(-1)
+            rand[:] = rng.random(tile)
+            calls = []
+            for s in starts:
+                pts = min(tile, n - s)
+                c = abi.Call()
+                c.abi_version, c.struct_size = abi.ABI, C.sizeof(abi.Call)
+                c.words, c.words_key, c.word_count = abi.ptr(a["words"], abi.u64p), 0x6767676767, len(a["words"])
+                for k in range(8):
+                    c.width[k] = int(a["width"][k])
+                c.scale_re, c.scale_im = abi.ptr(a["scale_re"]), abi.ptr(a["scale_im"])
+                c.factor_re, c.factor_im = abi.ptr(a["factor_re"]), abi.ptr(a["factor_im"])
+                c.current_planes, c.amplitude_planes = plan.current_planes, n_amp
+                c.momentum_planes = 4 * len(plan.forms)
+                c.scale_count, c.factor_count = len(a["scale_re"]), len(a["factor_re"])
+                c.stride, c.points = pts, pts
+                c.zero_amplitude, c.zero_amplitude_runs = abi.ptr(amp_runs, abi.u32p), 1
+                c.flags = abi.KEEP_AMPLITUDES | abi.FORMS_ON_DEVICE
+                c.external_momenta = C.cast(p_ext + rows * s * 8, abi.f64p)
+                c.external_count = plan.legs
+                c.term_count = len(a["term_slots"])
+                c.form_terms = abi.ptr(a["form_terms"], abi.u32p)
+                c.term_slots = abi.ptr(a["term_slots"], abi.u32p)
+                c.term_coefficients = abi.ptr(a["term_coefficients"])
+                resolved = abi.ReduceResolved(abi.ABI, C.sizeof(abi.ReduceResolved), abi.ptr(a["planes"], abi.u32p),
+                                              abi.ptr(a["weights"]), abi.ptr(a["offsets"], abi.u32p),
+                                              C.cast(p_out, abi.f64p), outputs, pts, 1, outputs, abi.OUTPUT_ZERO)
+                sample = abi.ReduceSample(abi.ABI, C.sizeof(abi.ReduceSample), abi.ptr(a["planes"], abi.u32p),
+                                          abi.ptr(a["weights"]), abi.ptr(a["offsets"], abi.u32p),
+                                          abi.ptr(helicity, abi.u32p), outputs, pts, C.cast(p_rand, abi.f64p), None,
+                                          float(plan.numerator), float(plan.denominator),
+                                          C.cast(p_tot + s * 8, abi.f64p), C.cast(p_cho, abi.u32p))
+                calls.append((s, pts, c, resolved, sample))
+            hold.append(calls)
+
+            def pass_resolved(check=False):
+                exact = True
+                for s, pts, c, r, _ in calls:
+                    if lib.pacman_recurrence_v1_run(C.byref(c)) != 0 or lib.pacman_recurrence_v1_reduce_resolved(
+                            C.byref(r)) != 0:
+                        raise RuntimeError("v1 run/reduce_resolved failed")
+                    if check:
+                        v = out[:pts * outputs].reshape(pts, outputs)
+                        t = np.zeros(pts)
+                        for o in range(outputs):
+                            x = v[:, o]
+                            if plan.numerator != plan.denominator:
+                                x = (x * plan.numerator) / plan.denominator
+                            t = t + x
+                        exact &= bool(np.array_equal(t.view(np.uint64), want[s:s + pts].view(np.uint64)))
+                return exact
+
+            def pass_sample():
+                for s, pts, c, _, smp in calls:
+                    if lib.pacman_recurrence_v1_run(C.byref(c)) != 0 or lib.pacman_recurrence_v1_reduce_sample(
+                            C.byref(smp)) != 0:
+                        raise RuntimeError("v1 run/reduce_sample failed")
+
+            entry = {"calls_per_pass": len(calls)}
+            entry["resolved_bitexact_vs_fused"] = pass_resolved(check=True)
+            best = min(_wall(pass_resolved) for _ in range(repeats))
+            entry["resolved_points_per_s"] = n / best
+            pass_sample()
+            entry["sample_totals_bitexact_vs_fused"] = bool(np.array_equal(tot.view(np.uint64), want.view(np.uint64)))
+            best = min(_wall(pass_sample) for _ in range(repeats))
+            entry["sample_points_per_s"] = n / best
+            res["tiles"][str(tile)] = entry
+            log(f"v1 ABI tile {tile}: resolved {entry['resolved_points_per_s']:.4g} points/s, sample "
+                f"{entry['sample_points_per_s']:.4g} points/s (bit-exact {entry['resolved_bitexact_vs_fused']}, "
+                f"{entry['sample_totals_bitexact_vs_fused']})")
+        finally:
+            for p in (p_ext, p_out, p_rand, p_tot, p_cho):
+                lib.pacman_recurrence_v1_host_free(p)
+    return res
+
+
+def _wall(f):
+    t = time.perf_counter()
+    f()
+    return time.perf_counter() - t
 
 
 def unweighting_summary(u):
@@ -234,6 +364,7 @@ def main():
     ap.add_argument("--no-modes", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
     ap.add_argument("--only-unweighting", action="store_true", help="of the extra workloads, only the unweighting")
+    ap.add_argument("--only-v1", action="store_true", help="of the extra workloads, only the v1 entry points")
     ap.add_argument("--candidates", type=int, default=1 << 24)
     ap.add_argument("--train-points", type=int, default=1 << 20)
     ap.add_argument("--epochs", type=int, default=20)
@@ -299,6 +430,9 @@ def main():
 
     extra = {}
     if not args.no_extra and not args.only_unweighting:
+        # the same points through the reference-compatible entry points, host memory in and out
+        extra["v1_abi"] = v1_abi(lib, plan, d_ext.download((plan.legs * 4, n)), total64)
+    if not args.no_extra and not args.only_unweighting and not args.only_v1:
         # the whole leading-colour sum of g g > g g g: the 12 orderings (0, s1..s4) of the legs up
         # to reflection, each the same plan on its permutation of the external legs, in one launch
         orders = [(0,) + p for p in itertools.permutations(range(1, plan.legs)) if p[0] < p[-1]]
@@ -382,7 +516,7 @@ def main():
             t.free()
             f.destroy()
 
-    if not args.no_extra:
+    if not args.no_extra and not args.only_v1:
         # surrogate vs plain unweighting of g g > g g g g
         t = time.perf_counter()
         uw = Unweighting(legs=6, hidden=256, layers=4, batch=args.train_batch, prior=True, lib=lib)

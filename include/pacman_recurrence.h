@@ -24,7 +24,10 @@ enum { W_P, W_A, W_F, W_X, W_S, W_K, W_COUNT };
 /*
  * One plan run (runner/pacman_recurrence.h:17-51). A plane holds `stride` doubles, one per
  * point; runs are (first plane, plane count) pairs. The device arena outlives the call; only
- * the named runs move between host and device, and only their first `points` values.
+ * the named runs move between host and device, and only their first `points` values. (This
+ * library runs a keyed call that uploads only the external momenta and downloads no currents on
+ * its fused kernels, with the same amplitudes; the arena's current planes then keep what they
+ * held: pacuda_v1_fused_runs in pacuda_amd.h.)
  * struct_size: 248 (all fields), 208 (ends before external_momenta) or 192 (ends before
  * zero_amplitude); fields past struct_size read as zero.
  */

@@ -31,6 +31,18 @@ int pacman_recurrence_v1_test_sources(const uint32_t *codes, const double *masse
 /* bit 3 of pacman_recurrence_v1_capabilities(): pacuda_plan_create / pacuda_evaluate */
 #define PACUDA_CAP_FUSED 8u
 
+/*
+ * The number of pacman_recurrence_v1_run calls of this process that ran on the fused kernels (no
+ * reference counterpart; for tests and tuning). A run does when it gives the same results there:
+ * a keyed call (words_key nonzero) with PACMAN_RECURRENCE_FORMS_ON_DEVICE, no current or
+ * amplitude uploads, no current downloads, a program that writes every current plane it reads,
+ * and closures that write only planes of the zero_amplitude runs. It is then one launch of the
+ * fused kernels (forms, sources, program) that writes the amplitude planes into the device arena;
+ * the arena's current planes are not written. PACUDA_V1_FUSED=0 in the environment keeps every
+ * run on the v1 kernels.
+ */
+uint64_t pacuda_v1_fused_runs(void);
+
 typedef struct pacuda_plan pacuda_plan;
 
 /*

@@ -119,6 +119,7 @@ EXTENSION_SYMBOLS = [
     ("pacuda_plan_info", C.c_int, [C.c_void_p, u32p, u64p]),
     ("pacuda_evaluate", C.c_int, [C.POINTER(Item), C.c_uint32, C.POINTER(Launch)]),
     ("pacuda_evaluate_static", C.c_int, [C.POINTER(Item), C.c_uint32, C.POINTER(Launch)]),
+    ("pacuda_v1_fused_runs", C.c_uint64, []),
     ("pacuda_device_alloc", C.c_void_p, [C.c_size_t]),
     ("pacuda_device_free", None, [C.c_void_p]),
     ("pacuda_copy_to_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

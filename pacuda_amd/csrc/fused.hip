@@ -91,6 +91,11 @@ struct ItemDev {
     const double *external;
     double *resolved, *total;
     uint32_t points, ext_stride;
+    // a v1 run on these kernels (amp_re not null): instead of the reduction, the amplitude planes
+    // the reduction names (`planes`, one per output) go to these plane arrays, amp_stride values
+    // apart, in the kernel's amplitude layout
+    double *amp_re, *amp_im;
+    size_t amp_stride;
 };
 
 constexpr int LANES = 64;
@@ -273,6 +278,16 @@ fused_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ uni
             interpret<K, SYMJIT>(prog + pl.sources, count - pl.sources, a);
         else
             interpret<K, SYMJIT>(pl.prog_plain + pl.sources, pl.count_plain - pl.sources, a);
+        if (it.amp_re) {  // a v1 run: the amplitude planes, for the v1 reductions
+            if (active)
+                for (uint32_t k = 0; k < pl.outputs; k++) {
+                    const uint32_t q = cnst(pl.planes)[k];
+                    const Cx<double> amp = a.template amp<double>(q);
+                    it.amp_re[(size_t)q * it.amp_stride + i] = amp.re;
+                    it.amp_im[(size_t)q * it.amp_stride + i] = amp.im;
+                }
+            continue;
+        }
         // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
         double t = 0.0;
         for (uint32_t o = 0; o < pl.outputs; o++) {
@@ -411,7 +426,17 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
         ItemDev nit = it;
         uint32_t ni = 0;
         if (next < units) locate(next, nit, ni);
-        if (wave == 0) {
+        if (wave == 0 && it.amp_re) {  // a v1 run: the amplitude planes, for the v1 reductions
+            const uint32_t outputs = cnst(plan)->outputs;
+            const uint32_t *planes = cnst(plan)->planes, *fields = cnst(plan)->coop_red[CLASS][SHAPE];
+            if (i < it.points)
+                for (uint32_t k = 0; k < outputs; k++) {
+                    const Cx<double> amp = a.template amp<double>(cnst(fields)[k]);
+                    const size_t at = (size_t)cnst(planes)[k] * it.amp_stride + i;
+                    it.amp_re[at] = amp.re;
+                    it.amp_im[at] = amp.im;
+                }
+        } else if (wave == 0) {
             // the reduction (runner/recurrence_runner.cu:1511-1527) and the helicity sum
             const uint32_t outputs = cnst(plan)->outputs;
             const uint32_t *offsets = cnst(plan)->offsets, *fields = cnst(plan)->coop_red[CLASS][SHAPE];
@@ -470,14 +495,17 @@ template <class T> T *upload(pacuda_plan *p, const T *src, size_t n, bool *ok) {
 }
 
 // every current plane the program reads has been written by it before
-bool reads_after_writes(const std::vector<Instr> &ops, uint32_t planes) {
+bool reads_after_writes(const std::vector<Instr> &ops, uint32_t planes, const std::vector<uint32_t> *bindings = nullptr) {
     std::vector<char> w(planes, 0);
     auto written = [&](uint32_t base, uint32_t n) {
         for (uint32_t k = 0; k < n; k++)
-            if (!w[base + k]) return false;
+            if ((uint64_t)base + k >= planes || !w[base + k]) return false;
         return true;
     };
-    auto mark = [&](uint32_t base, uint32_t n) { for (uint32_t k = 0; k < n; k++) w[base + k] = 1; };
+    auto mark = [&](uint32_t base, uint32_t n) {
+        for (uint32_t k = 0; k < n; k++)
+            if ((uint64_t)base + k < planes) w[base + k] = 1;
+    };
     for (const Instr &o : ops) {
         const int kind = o.w[0] & 0xff, op = (o.w[0] >> 8) & 0xff, flags = o.w[0] >> 16;
         if (kind == K_SOURCE) {
@@ -497,6 +525,25 @@ bool reads_after_writes(const std::vector<Instr> &ops, uint32_t planes) {
             if (!written(o.w[1], o.w[2])) return false;
         } else if (kind == K_CLOSURE) {
             if (!written(o.w[1], o.w[6]) || !written(o.w[2], o.w[6])) return false;
+        } else if (kind == K_SYMJIT) {
+            // the components its binding's codes read, then its outputs (read first when the row
+            // does not initialise its destination)
+            if (!bindings) return false;
+            const std::vector<uint32_t> &b = *bindings;
+            const size_t at = o.w[6];
+            if (at + 4 > b.size()) return false;
+            const uint32_t inputs = b[at + 1], pairs = b[at + 2];
+            if (at + 4 + inputs + pairs > b.size()) return false;
+            for (uint32_t k = 0; k < inputs; k++) {
+                const uint32_t code = b[at + 4 + k], which = (code >> 3) & 1, index = code >> 5;
+                if ((code & 7) == SB_PARENT && !written((which ? o.w[2] : o.w[1]) + index, 1)) return false;
+                if ((code & 7) == SB_DESTINATION && !written(o.w[3] + index, 1)) return false;
+            }
+            for (uint32_t j = 0; j < pairs; j++) {
+                const uint32_t plane = o.w[3] + b[at + 4 + inputs + j];
+                if (!(flags & F_INIT) && !written(plane, 1)) return false;
+                mark(plane, 1);
+            }
         }
     }
     return true;
@@ -2234,14 +2281,14 @@ bool launch_fused(uint32_t kernel, const Schedule &sc, hipStream_t s, const Item
 
 }  // namespace
 
-extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const pacuda_launch *l) {
-    Schedule sc;
-    int st = schedule(items, count, l, sc);
-    if (st) return st;
+namespace {
+
+// the launch of a schedule of `count` items on `s` with the library's shared item table, counter
+// and slots; kernel_ms as pacuda_launch's
+int launch_schedule(const Schedule &sc, uint32_t count, uint32_t kernel, hipStream_t s, float *kernel_ms) {
     if (count == 0 || sc.units == 0) return PACMAN_RECURRENCE_OK;
     std::lock_guard<std::mutex> guard(fs.lock);
     const size_t need = (size_t)sc.slots * sc.slot;
-    hipStream_t s = l->stream ? (hipStream_t)l->stream : runner_stream();
     if (!s) return PACMAN_RECURRENCE_DEVICE_ERROR;
     // the previous launch (on any stream) reads the item table and uses the counter and slots
     // until it ends
@@ -2281,20 +2328,102 @@ extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const p
     if (hip_failed(hipMemcpy(fs.items, sc.items.data(), count * sizeof(ItemDev), hipMemcpyHostToDevice), "items") ||
         hip_failed(hipMemcpy(fs.unit_start, sc.starts.data(), (count + 1) * sizeof(uint32_t), hipMemcpyHostToDevice), "items"))
         return PACMAN_RECURRENCE_DEVICE_ERROR;
-    if (l->kernel_ms) (void)hipEventRecord(fs.start, s);
-    if (!launch_fused(l->kernel, sc, s, fs.items, fs.unit_start, count, fs.counter, fs.scratch) ||
+    if (kernel_ms) (void)hipEventRecord(fs.start, s);
+    if (!launch_fused(kernel, sc, s, fs.items, fs.unit_start, count, fs.counter, fs.scratch) ||
         hip_failed(hipGetLastError(), "fused launch") || hip_failed(hipEventRecord(fs.last, s), "event"))
         return PACMAN_RECURRENCE_DEVICE_ERROR;
     fs.last_pending = true;
-    if (l->kernel_ms) {
+    if (kernel_ms) {
         if (hip_failed(hipEventRecord(fs.stop, s), "event") || hip_failed(hipEventSynchronize(fs.stop), "kernel"))
             return PACMAN_RECURRENCE_DEVICE_ERROR;
         float ms = 0;
         if (hip_failed(hipEventElapsedTime(&ms, fs.start, fs.stop), "event")) return PACMAN_RECURRENCE_DEVICE_ERROR;
-        *l->kernel_ms = ms;
+        *kernel_ms = ms;
     }
     return PACMAN_RECURRENCE_OK;
 }
+
+}  // namespace
+
+extern "C" int pacuda_evaluate(const pacuda_item *items, uint32_t count, const pacuda_launch *l) {
+    Schedule sc;
+    int st = schedule(items, count, l, sc);
+    if (st) return st;
+    return launch_schedule(sc, count, l->kernel, l->stream ? (hipStream_t)l->stream : runner_stream(), l->kernel_ms);
+}
+
+// ---- v1 runs on the fused kernels ---------------------------------------------------------------
+
+namespace pacuda {
+
+pacuda_plan *v1_fused_plan(const pacman_recurrence_call &c, bool sort, std::vector<uint32_t> &amplitudes) {
+    amplitudes.clear();
+    if (c.momentum_planes % 4 || c.current_planes >= LDS_PLANE) return nullptr;
+    const CompiledProgram hp =
+        compile_program(c.words, c.word_count, c.width, sort, {PACUDA_KERNEL_COUNT, PACUDA_MAX_IN, PACUDA_MAX_OUT});
+    const uint32_t forms = c.momentum_planes / 4;
+    if (hp.error || !program_fits(hp.need, c.current_planes, c.amplitude_planes, forms, c.scale_count, c.factor_count) ||
+        !reads_after_writes(hp.ops, c.current_planes, &hp.bindings))
+        return nullptr;
+    std::vector<char> written(c.amplitude_planes, 0);
+    for (const Instr &o : hp.ops)
+        if ((o.w[0] & 0xff) == K_CLOSURE) written[o.w[3]] = 1;  // (in range: program_fits)
+    for (uint32_t q = 0; q < c.amplitude_planes; q++)
+        if (written[q]) amplitudes.push_back(q);
+    if (amplitudes.empty()) return nullptr;
+    // the plan's "reduction": one output per written amplitude plane, which a v1 launch stores
+    const uint32_t n = (uint32_t)amplitudes.size();
+    std::vector<uint32_t> offsets(n + 1);
+    for (uint32_t k = 0; k <= n; k++) offsets[k] = k;
+    const std::vector<double> weights(n, 1.0);
+    pacuda_plan_desc d;
+    memset(&d, 0, sizeof d);
+    d.abi_version = PACMAN_RECURRENCE_ABI;
+    d.struct_size = sizeof d;
+    d.words = c.words;
+    d.word_count = c.word_count;
+    memcpy(d.width, c.width, sizeof d.width);
+    d.current_planes = c.current_planes;
+    d.amplitude_planes = c.amplitude_planes;
+    d.form_count = forms;
+    d.scale_re = c.scale_re;
+    d.scale_im = c.scale_im;
+    d.scale_count = c.scale_count;
+    d.factor_re = c.factor_re;
+    d.factor_im = c.factor_im;
+    d.factor_count = c.factor_count;
+    d.external_count = c.external_count;
+    d.term_count = c.term_count;
+    d.form_terms = c.form_terms;
+    d.term_slots = c.term_slots;
+    d.term_coefficients = c.term_coefficients;
+    d.outputs = n;
+    d.offsets = offsets.data();
+    d.planes = amplitudes.data();
+    d.weights = weights.data();
+    d.numerator = d.denominator = 1.0;
+    d.no_sort = sort ? 0 : 1;
+    pacuda_plan *p = nullptr;
+    return pacuda_plan_create(&d, &p) == PACMAN_RECURRENCE_OK ? p : nullptr;
+}
+
+int v1_fused_run(const pacuda_plan *plan, uint32_t kernel, const double *external, uint32_t points, double *are,
+                 double *aim, size_t stride, hipStream_t s) {
+    const pacuda_item item{plan, external, points, points, nullptr, nullptr};
+    pacuda_launch l;
+    memset(&l, 0, sizeof l);
+    l.abi_version = PACMAN_RECURRENCE_ABI;
+    l.struct_size = sizeof l;
+    l.kernel = kernel;
+    Schedule sc;
+    if (const int st = schedule(&item, 1, &l, sc)) return st;
+    sc.items[0].amp_re = are;
+    sc.items[0].amp_im = aim;
+    sc.items[0].amp_stride = stride;
+    return launch_schedule(sc, 1, kernel, s, nullptr);
+}
+
+}  // namespace pacuda
 
 // ---- the evaluation without allocation ---------------------------------------------------------
 //

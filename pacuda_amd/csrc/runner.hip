@@ -364,6 +364,54 @@ class ProgramCache {
     std::unique_ptr<DeviceProgram> unkeyed_;
 };
 
+// Fused plans of keyed calls (words_key nonzero), each with the call shape and tables it was
+// built from: a call under the same key with other tables (bit for bit) builds it again.
+class FusedCache {
+  public:
+    struct Entry {
+        pacuda_plan *plan = nullptr;       // null: the key's program does not run on the fused kernels
+        std::vector<uint32_t> amplitudes;  // the amplitude planes its closures write, ascending
+        std::vector<uint64_t> shape;       // the counts, widths and sort it was built for
+        std::vector<uint64_t> numbers;     // its scales, factors and form coefficients (bits)
+        std::vector<uint32_t> indices;     // its form terms and term slots
+        bool operator==(const Entry &o) const { return shape == o.shape && numbers == o.numbers && indices == o.indices; }
+    };
+
+    const Entry &get(const pacman_recurrence_call &c, bool sort) {
+        Entry e;
+        describe(c, sort, e);
+        auto it = entries_.find(c.words_key);
+        if (it != entries_.end()) {
+            if (it->second == e) return it->second;
+            pacuda_plan_destroy(it->second.plan);  // (waits for the device: no launch still reads it)
+            entries_.erase(it);
+        }
+        e.plan = v1_fused_plan(c, sort, e.amplitudes);
+        return entries_[c.words_key] = std::move(e);
+    }
+
+  private:
+    static void bits(std::vector<uint64_t> &out, const double *v, uint32_t n) {
+        const size_t at = out.size();
+        out.resize(at + n);
+        if (n) memcpy(out.data() + at, v, n * sizeof(double));
+    }
+    static void describe(const pacman_recurrence_call &c, bool sort, Entry &e) {
+        uint64_t width = 0;
+        memcpy(&width, c.width, sizeof width);
+        e.shape = {c.word_count, c.current_planes, c.amplitude_planes, c.momentum_planes, c.scale_count,
+                   c.factor_count, c.external_count, c.term_count, width, sort ? 1u : 0u};
+        bits(e.numbers, c.scale_re, c.scale_count);
+        bits(e.numbers, c.scale_im, c.scale_count);
+        bits(e.numbers, c.factor_re, c.factor_count);
+        bits(e.numbers, c.factor_im, c.factor_count);
+        bits(e.numbers, c.term_coefficients, c.term_count);
+        e.indices.assign(c.form_terms, c.form_terms + 2 * (size_t)(c.momentum_planes / 4));
+        e.indices.insert(e.indices.end(), c.term_slots, c.term_slots + c.term_count);
+    }
+    std::unordered_map<uint64_t, Entry> entries_;
+};
+
 // The amplitudes a KEEP run left on the device for the next reduction.
 struct Kept {
     bool valid = false;
@@ -378,6 +426,8 @@ struct Runner {
     Stats stats;
     Kept kept;
     ProgramCache programs;
+    FusedCache fused;
+    uint64_t fused_runs = 0;  // runs that took the fused kernels
     // the arena: currents (f64 or df64 pairs; f32 planes beside them for the mixed kernels),
     // amplitudes, momentum forms, the constants (scales re, im, factors re, im)
     DeviceArray<double> cre, cim, are, aim, forms, constants;
@@ -425,6 +475,7 @@ class RunPlan {
     int execute() {
         const CallTimer timer(r_.stats.enabled);
         int status = check();
+        if (status == OK) choose_kernels();
         if (status == OK) status = reserve();
         if (status == OK) status = stage_inputs();
         if (status == OK) status = prepare();
@@ -432,6 +483,7 @@ class RunPlan {
         if (status == OK) status = deliver();
         if (status != OK) return status;
         if (keep_) r_.kept = Kept{true, r_.config.df64_layout(), c_.stride, c_.points, c_.amplitude_planes};
+        if (fused_) r_.fused_runs++;
         r_.account(c_.points, timer.seconds());
         return OK;
     }
@@ -443,6 +495,50 @@ class RunPlan {
     bool keep_ = false, forms_on_device_ = false;
     const DeviceProgram *program_ = nullptr;
     Arena arena_{};
+    // the fused kernels' plan when this run takes them (null: the v1 kernels), and the cleared
+    // amplitude runs they do not write
+    const FusedCache::Entry *fused_ = nullptr;
+    std::vector<uint32_t> zero_runs_;
+
+    static bool no_runs(const uint32_t *runs, uint32_t n) {
+        for (uint32_t k = 0; k < n; k++)
+            if (runs[2 * k + 1]) return false;
+        return true;
+    }
+
+    // A run takes the fused kernels (one launch: forms, sources, program, the amplitudes into the
+    // arena) when it gives the same results there: a keyed call (its program is built once) with
+    // the forms built on the device, nothing uploaded but the external momenta, no current
+    // downloaded, a program that writes every current it reads and closures that write cleared
+    // amplitude planes only. Its currents then stay on chip: the arena's current planes are not
+    // written. PACUDA_V1_FUSED=0 in the environment keeps every run on the v1 kernels.
+    void choose_kernels() {
+        zero_runs_.assign(c_.zero_amplitude, c_.zero_amplitude + 2 * (size_t)c_.zero_amplitude_runs);
+        const char *env = getenv("PACUDA_V1_FUSED");
+        if (!c_.words_key || !forms_on_device_ || (env && strcmp(env, "0") == 0) ||
+            (c_.scale_count && (!c_.scale_re || !c_.scale_im)) || (c_.factor_count && (!c_.factor_re || !c_.factor_im)) ||
+            !no_runs(c_.upload_current, c_.upload_current_runs) ||
+            !no_runs(c_.upload_amplitude, c_.upload_amplitude_runs) ||
+            !no_runs(c_.download_current, c_.download_current_runs))
+            return;
+        const FusedCache::Entry &e = r_.fused.get(c_, r_.config.sort);
+        if (!e.plan) return;
+        std::vector<char> cleared(c_.amplitude_planes, 0), written(c_.amplitude_planes, 0);
+        for (uint32_t k = 0; k < c_.zero_amplitude_runs; k++)
+            for (uint32_t q = 0; q < c_.zero_amplitude[2 * k + 1]; q++) cleared[c_.zero_amplitude[2 * k] + q] = 1;
+        for (uint32_t q : e.amplitudes) {
+            if (!cleared[q]) return;  // (a closure adds to what the plane held: the v1 kernels)
+            written[q] = 1;
+        }
+        fused_ = &e;
+        // the cleared planes the launch does not write, as runs
+        zero_runs_.clear();
+        for (uint32_t q = 0; q < c_.amplitude_planes; q++) {
+            if (!cleared[q] || written[q]) continue;
+            if (!zero_runs_.empty() && zero_runs_[zero_runs_.size() - 2] + zero_runs_.back() == q) zero_runs_.back()++;
+            else zero_runs_.insert(zero_runs_.end(), {q, 1u});
+        }
+    }
 
     size_t plane_doubles(uint32_t planes) const { return (size_t)planes * c_.stride; }
     double *scale_re() const { return r_.constants.data(); }
@@ -477,6 +573,11 @@ class RunPlan {
 
     int reserve() {
         const size_t currents = plane_doubles(c_.current_planes), amplitudes = plane_doubles(c_.amplitude_planes);
+        if (fused_)
+            return r_.are.fit(amplitudes, "amplitudes") && r_.aim.fit(amplitudes, "amplitudes") &&
+                           r_.external.fit((size_t)c_.external_count * 4 * c_.points, "external momenta")
+                       ? OK
+                       : DEVICE_ERROR;
         bool ok = r_.cre.fit(currents, "current arena") && r_.cim.fit(currents, "current arena") &&
                   r_.are.fit(amplitudes, "amplitudes") && r_.aim.fit(amplitudes, "amplitudes") &&
                   r_.forms.fit(plane_doubles(c_.momentum_planes), "momenta") &&
@@ -513,7 +614,27 @@ class RunPlan {
         return true;
     }
 
+    bool zero_amplitudes() {
+        for (size_t k = 0; k < zero_runs_.size(); k += 2) {
+            const PlaneRun run{zero_runs_[k], zero_runs_[k + 1]};
+            if (run.count == 0) continue;
+            const size_t pitch = c_.stride * sizeof(double), at = (size_t)run.first * c_.stride;
+            for (double *plane : {r_.are.data(), r_.aim.data()})
+                if (hip_failed(hipMemset2DAsync(plane + at, pitch, 0, c_.points * sizeof(double), run.count, s_), "zero"))
+                    return false;
+        }
+        return true;
+    }
+
     int stage_inputs() {
+        if (fused_) {  // the external momenta (the plan holds the tables), the cleared planes
+            const size_t external = (size_t)c_.external_count * 4 * c_.points;
+            if (!upload(r_.external.data(), c_.external_momenta, external * sizeof(double), "external momenta") ||
+                !zero_amplitudes())
+                return DEVICE_ERROR;
+            r_.stats.copied(1, external * sizeof(double));
+            return OK;
+        }
         // momentum forms: from the external momenta on the device, or the caller's planes
         const uint32_t form_count = c_.momentum_planes / 4;
         if (forms_on_device_) {
@@ -542,20 +663,14 @@ class RunPlan {
             !copy_runs(r_.cre.data(), r_.cim.data(), c_.current_re, c_.current_im, c_.upload_current,
                        c_.upload_current_runs, true) ||
             !copy_runs(r_.are.data(), r_.aim.data(), c_.amplitude_re, c_.amplitude_im, c_.upload_amplitude,
-                       c_.upload_amplitude_runs, true))
+                       c_.upload_amplitude_runs, true) ||
+            !zero_amplitudes())
             return DEVICE_ERROR;
-        for (uint32_t k = 0; k < c_.zero_amplitude_runs; k++) {
-            const PlaneRun run{c_.zero_amplitude[2 * k], c_.zero_amplitude[2 * k + 1]};
-            if (run.count == 0) continue;
-            const size_t pitch = c_.stride * sizeof(double), at = (size_t)run.first * c_.stride;
-            for (double *plane : {r_.are.data(), r_.aim.data()})
-                if (hip_failed(hipMemset2DAsync(plane + at, pitch, 0, c_.points * sizeof(double), run.count, s_), "zero"))
-                    return DEVICE_ERROR;
-        }
         return OK;
     }
 
     int prepare() {
+        if (fused_) return OK;
         program_ = r_.programs.get(c_, r_.config.sort, s_);
         if (!program_) return DEVICE_ERROR;
         // a program indexing past this call's tables would read or write outside the arena
@@ -632,6 +747,11 @@ class RunPlan {
 
     int compute() {
         const Config &cfg = r_.config;
+        if (fused_)
+            return v1_fused_run(fused_->plan, cfg.kernel, r_.external.data(), c_.points, r_.are.data(), r_.aim.data(),
+                                c_.stride, s_) == OK
+                       ? OK
+                       : DEVICE_ERROR;
         // the sources in f64, written in the kernel's current layout
         const bool sources = cfg.kernel == PACMAN_RECURRENCE_KERNEL_F64 ? launch_sources<double>()
                              : cfg.f32_currents()                      ? launch_sources<float>()
@@ -926,6 +1046,13 @@ extern "C" int pacman_recurrence_v1_configure(const pacman_recurrence_options *o
 extern "C" uint32_t pacman_recurrence_v1_capabilities(void) {
     return PACMAN_RECURRENCE_CAP_SOURCES | PACMAN_RECURRENCE_CAP_FORMS |
            (PACUDA_KERNEL_COUNT > 0 ? PACMAN_RECURRENCE_CAP_SYMJIT : 0u) | PACUDA_CAP_FUSED;
+}
+
+extern "C" uint64_t pacuda_v1_fused_runs(void) {
+    runner_lock().enter();
+    const uint64_t n = runner().fused_runs;
+    runner_lock().leave();
+    return n;
 }
 
 // The index of the generated kernel whose plane file has this SHA-256 (hex, either case), or -1.
