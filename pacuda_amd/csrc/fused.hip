@@ -2407,9 +2407,9 @@ pacuda_plan *v1_fused_plan(const pacman_recurrence_call &c, bool sort, std::vect
     return pacuda_plan_create(&d, &p) == PACMAN_RECURRENCE_OK ? p : nullptr;
 }
 
-int v1_fused_run(const pacuda_plan *plan, uint32_t kernel, const double *external, uint32_t points, double *are,
-                 double *aim, size_t stride, hipStream_t s) {
-    const pacuda_item item{plan, external, points, points, nullptr, nullptr};
+int v1_fused_run(const pacuda_plan *plan, uint32_t kernel, const double *external, uint32_t points, uint32_t ext_stride,
+                 double *are, double *aim, size_t stride, hipStream_t s) {
+    const pacuda_item item{plan, external, points, ext_stride, nullptr, nullptr};
     pacuda_launch l;
     memset(&l, 0, sizeof l);
     l.abi_version = PACMAN_RECURRENCE_ABI;

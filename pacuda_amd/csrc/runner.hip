@@ -137,18 +137,49 @@ __global__ void reduce_kernel(const double *are, const double *aim, size_t ld, c
     output[i] = reduce_terms<DF>(are, aim, ld, planes, weights, 0, count, i, output[i]);
 }
 
-// one lane per point, every output in order
+// one lane per point, every output in order (from zero when no two outputs share a place, else
+// from the output's value)
 template <bool DF>
 __global__ void reduce_resolved_kernel(const double *are, const double *aim, size_t ld, const uint32_t *__restrict__ planes,
                                        const double *__restrict__ weights, const uint32_t *__restrict__ offsets,
                                        uint32_t outputs, double *output, uint32_t points, size_t output_stride,
-                                       size_t point_stride) {
+                                       size_t point_stride, bool from_zero) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= points) return;
     for (uint32_t o = 0; o < outputs; o++) {
         size_t out = o * output_stride + i * point_stride;
-        double acc = output[out];
+        double acc = from_zero ? 0.0 : output[out];
         output[out] = reduce_terms<DF>(are, aim, ld, planes, weights, cnst(offsets)[o], cnst(offsets)[o + 1], i, acc);
+    }
+}
+
+// The same from zero into rows, output[i * outputs + o]: each wavefront's 64 points' values of
+// up to ROW_GROUP outputs at a time go through LDS, so that its stores write whole rows (a lane's
+// outputs are a row: stored straight from the lanes, a wavefront's store would touch 64 rows)
+constexpr uint32_t ROW_GROUP = 16;
+template <bool DF>
+__global__ void __launch_bounds__(256)
+reduce_rows_kernel(const double *are, const double *aim, size_t ld, const uint32_t *__restrict__ planes,
+                   const double *__restrict__ weights, const uint32_t *__restrict__ offsets, uint32_t outputs,
+                   double *output, uint32_t points) {
+    constexpr uint32_t PITCH = ROW_GROUP + 1;  // (an odd row pitch spreads the lanes' rows over the banks)
+    __shared__ double tile[4][64 * PITCH];
+    const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const size_t p0 = ((size_t)blockIdx.x * 4 + wave) * 64, i = p0 + lane;
+    const uint32_t here = p0 >= points ? 0u : points - p0 < 64 ? (uint32_t)(points - p0) : 64u;
+    double *t = tile[wave];
+    for (uint32_t g0 = 0; g0 < outputs; g0 += ROW_GROUP) {
+        const uint32_t gn = outputs - g0 < ROW_GROUP ? outputs - g0 : ROW_GROUP;
+        if (i < points)
+            for (uint32_t o = 0; o < gn; o++)
+                t[lane * PITCH + o] =
+                    reduce_terms<DF>(are, aim, ld, planes, weights, cnst(offsets)[g0 + o], cnst(offsets)[g0 + o + 1], i, 0.0);
+        __syncthreads();
+        for (uint32_t e = lane; e < here * gn; e += 64) {
+            const uint32_t q = e / gn, o = e - q * gn;
+            output[(p0 + q) * outputs + g0 + o] = t[q * PITCH + o];
+        }
+        __syncthreads();
     }
 }
 
@@ -428,6 +459,19 @@ struct Runner {
     ProgramCache programs;
     FusedCache fused;
     uint64_t fused_runs = 0;  // runs that took the fused kernels
+    // a run on the fused kernels uploads its external momenta in chunks on this stream, each
+    // chunk's launch waiting for its event, so later chunks' copies overlap earlier launches
+    hipStream_t copies = nullptr;
+    std::vector<hipEvent_t> chunk_done;
+    bool chunk_events(size_t n) {
+        if (!copies && hip_failed(hipStreamCreateWithFlags(&copies, hipStreamNonBlocking), "copy stream")) return false;
+        while (chunk_done.size() < n) {
+            hipEvent_t e = nullptr;
+            if (hip_failed(hipEventCreateWithFlags(&e, hipEventDisableTiming), "copy event")) return false;
+            chunk_done.push_back(e);
+        }
+        return true;
+    }
     // the arena: currents (f64 or df64 pairs; f32 planes beside them for the mixed kernels),
     // amplitudes, momentum forms, the constants (scales re, im, factors re, im)
     DeviceArray<double> cre, cim, are, aim, forms, constants;
@@ -481,6 +525,8 @@ class RunPlan {
         if (status == OK) status = prepare();
         if (status == OK) status = compute();
         if (status == OK) status = deliver();
+        // (a failed fused run may have left chunk copies from the caller's memory in flight)
+        if (status != OK && fused_ && r_.copies) (void)hipStreamSynchronize(r_.copies);
         if (status != OK) return status;
         if (keep_) r_.kept = Kept{true, r_.config.df64_layout(), c_.stride, c_.points, c_.amplitude_planes};
         if (fused_) r_.fused_runs++;
@@ -499,6 +545,17 @@ class RunPlan {
     // amplitude runs they do not write
     const FusedCache::Entry *fused_ = nullptr;
     std::vector<uint32_t> zero_runs_;
+    std::vector<PlaneRun> chunks_;  // (first point, points) of each chunk of a fused run
+
+    // Chunks of a fused run's points: 1 below 2^18 points, else up to 8 of at least 2^17 (whole
+    // 64-point units). With PACUDA_SIMD_LANES a symjit row's operation order depends on the
+    // point's place among the call's points, so such builds run one chunk.
+    void split_points() {
+        chunks_.clear();
+        uint32_t n = c_.points < (1u << 18) || PACUDA_SIMD_LANES > 0 ? 1u : std::min(8u, c_.points >> 17);
+        const uint32_t size = ((c_.points + n - 1) / n + 63) & ~63u;
+        for (uint32_t first = 0; first < c_.points; first += size) chunks_.push_back({first, std::min(size, c_.points - first)});
+    }
 
     static bool no_runs(const uint32_t *runs, uint32_t n) {
         for (uint32_t k = 0; k < n; k++)
@@ -627,12 +684,21 @@ class RunPlan {
     }
 
     int stage_inputs() {
-        if (fused_) {  // the external momenta (the plan holds the tables), the cleared planes
-            const size_t external = (size_t)c_.external_count * 4 * c_.points;
-            if (!upload(r_.external.data(), c_.external_momenta, external * sizeof(double), "external momenta") ||
-                !zero_amplitudes())
-                return DEVICE_ERROR;
-            r_.stats.copied(1, external * sizeof(double));
+        if (fused_) {  // the external momenta by chunks (the plan holds the tables), the cleared planes
+            split_points();
+            const size_t external = (size_t)c_.external_count * 4 * c_.points, pitch = c_.points * sizeof(double);
+            if (!r_.chunk_events(chunks_.size())) return DEVICE_ERROR;
+            for (size_t k = 0; k < chunks_.size(); k++) {
+                const PlaneRun ch = chunks_[k];
+                if (hip_failed(hipMemcpy2DAsync(r_.external.data() + ch.first, pitch, c_.external_momenta + ch.first, pitch,
+                                                ch.count * sizeof(double), (size_t)c_.external_count * 4,
+                                                hipMemcpyHostToDevice, r_.copies),
+                               "external momenta") ||
+                    hip_failed(hipEventRecord(r_.chunk_done[k], r_.copies), "copy event"))
+                    return DEVICE_ERROR;
+            }
+            if (!zero_amplitudes()) return DEVICE_ERROR;
+            r_.stats.copied(chunks_.size(), external * sizeof(double));
             return OK;
         }
         // momentum forms: from the external momenta on the device, or the caller's planes
@@ -747,11 +813,16 @@ class RunPlan {
 
     int compute() {
         const Config &cfg = r_.config;
-        if (fused_)
-            return v1_fused_run(fused_->plan, cfg.kernel, r_.external.data(), c_.points, r_.are.data(), r_.aim.data(),
-                                c_.stride, s_) == OK
-                       ? OK
-                       : DEVICE_ERROR;
+        if (fused_) {
+            for (size_t k = 0; k < chunks_.size(); k++) {
+                const PlaneRun ch = chunks_[k];
+                if (hip_failed(hipStreamWaitEvent(s_, r_.chunk_done[k], 0), "copy event") ||
+                    v1_fused_run(fused_->plan, cfg.kernel, r_.external.data() + ch.first, ch.count, c_.points,
+                                 r_.are.data() + ch.first, r_.aim.data() + ch.first, c_.stride, s_) != OK)
+                    return DEVICE_ERROR;
+            }
+            return OK;
+        }
         // the sources in f64, written in the kernel's current layout
         const bool sources = cfg.kernel == PACMAN_RECURRENCE_KERNEL_F64 ? launch_sources<double>()
                              : cfg.f32_currents()                      ? launch_sources<float>()
@@ -846,22 +917,38 @@ class Reduction {
         // the output span: every (output, point) value and whatever lies between them
         const size_t span = (size_t)(q->outputs - 1) * q->output_stride + (size_t)(q->points - 1) * q->point_stride + 1;
         const bool zero = q->flags & PACMAN_RECURRENCE_OUTPUT_ZERO;
+        // outputs that fill the span, as rows per point or planes per output: nothing between them
+        const bool rows = q->output_stride == 1 && q->point_stride == q->outputs;
+        const bool dense = rows || (q->point_stride == 1 && (q->output_stride == q->points || q->outputs == 1));
         // OUTPUT_ZERO clears the span on the device (the values between the outputs come back as
-        // zeros too); otherwise the caller's span goes up first
+        // zeros too; a dense span is all outputs, which the kernel writes from zero); otherwise
+        // the caller's span goes up first
         if (!s_ || !upload_tables(q->planes, q->weights, q->offsets[q->outputs], q->offsets, q->outputs) ||
             !r_.routput.fit(span, "reduction") ||
-            (zero ? hip_failed(hipMemsetAsync(r_.routput.data(), 0, span * sizeof(double), s_), "reduction")
+            (zero ? !dense && hip_failed(hipMemsetAsync(r_.routput.data(), 0, span * sizeof(double), s_), "reduction")
                   : !upload(r_.routput.data(), q->output, span * sizeof(double), "reduction")))
             return DEVICE_ERROR;
         const unsigned grid = (q->points + 255) / 256;
-        if (r_.kept.df64)
+        if (zero && rows) {
+            if (r_.kept.df64)
+                reduce_rows_kernel<true><<<grid, 256, 0, s_>>>(r_.are.data(), r_.aim.data(), r_.kept.stride, r_.rplanes.data(),
+                                                               r_.rweights.data(), r_.roffsets.data(), q->outputs,
+                                                               r_.routput.data(), q->points);
+            else
+                reduce_rows_kernel<false><<<grid, 256, 0, s_>>>(r_.are.data(), r_.aim.data(), r_.kept.stride,
+                                                                r_.rplanes.data(), r_.rweights.data(), r_.roffsets.data(),
+                                                                q->outputs, r_.routput.data(), q->points);
+        } else if (r_.kept.df64) {
             reduce_resolved_kernel<true><<<grid, 256, 0, s_>>>(r_.are.data(), r_.aim.data(), r_.kept.stride, r_.rplanes.data(),
                                                                r_.rweights.data(), r_.roffsets.data(), q->outputs,
-                                                               r_.routput.data(), q->points, q->output_stride, q->point_stride);
-        else
+                                                               r_.routput.data(), q->points, q->output_stride, q->point_stride,
+                                                               zero && dense);
+        } else {
             reduce_resolved_kernel<false><<<grid, 256, 0, s_>>>(r_.are.data(), r_.aim.data(), r_.kept.stride, r_.rplanes.data(),
                                                                 r_.rweights.data(), r_.roffsets.data(), q->outputs,
-                                                                r_.routput.data(), q->points, q->output_stride, q->point_stride);
+                                                                r_.routput.data(), q->points, q->output_stride, q->point_stride,
+                                                                zero && dense);
+        }
         if (hip_failed(hipGetLastError(), "reduction") ||
             !download(q->output, r_.routput.data(), span * sizeof(double), "reduction"))
             return DEVICE_ERROR;

@@ -21,10 +21,10 @@ hipStream_t runner_stream();
 // earlier call left in the arena), no closure. `amplitudes` receives the amplitude planes the
 // closures write, ascending. Destroy with pacuda_plan_destroy.
 pacuda_plan *v1_fused_plan(const pacman_recurrence_call &c, bool sort, std::vector<uint32_t> &amplitudes);
-// One launch of the fused kernels (the default launch of `kernel`) for points [0, points) of
-// the external momenta (planes of `points` values, on the device): the plan's amplitude planes,
+// One launch of the fused kernels (the default launch of `kernel`) for `points` points of the
+// external momenta on the device (planes ext_stride values apart): the plan's amplitude planes,
 // each from zero, into are / aim (planes `stride` values apart) in the kernel's amplitude layout,
 // queued on `s`. 0 OK, else a pacman_recurrence status.
-int v1_fused_run(const pacuda_plan *plan, uint32_t kernel, const double *external, uint32_t points, double *are,
-                 double *aim, size_t stride, hipStream_t s);
+int v1_fused_run(const pacuda_plan *plan, uint32_t kernel, const double *external, uint32_t points, uint32_t ext_stride,
+                 double *are, double *aim, size_t stride, hipStream_t s);
 }  // namespace pacuda

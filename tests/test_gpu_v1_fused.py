@@ -3,9 +3,10 @@
 process, tests/v1_fused_driver.py), a sequence of runs and reductions (g g > g g g with a stride
 beyond the points and cleared planes no closure writes, g g > g g g g with helicity sampling, a
 quark line with the amplitudes downloaded, a four-quark subprocess with symjit rows, a key whose
-scales change, and the calls that must stay on the v1 kernels: a written plane not cleared, no
-key, an uploaded current run, PACUDA_V1_FUSED=0) equals the oracle's emulation of the v1 runner
-bit for bit, and exactly the qualifying calls take the fused kernels."""
+scales change, a run of 2^18 + 4099 points uploaded in chunks, and the calls that must stay on
+the v1 kernels: a written plane not cleared, no key, an uploaded current run, PACUDA_V1_FUSED=0)
+equals the oracle's emulation of the v1 runner bit for bit, and exactly the qualifying calls take
+the fused kernels."""
 import os
 import subprocess
 import sys

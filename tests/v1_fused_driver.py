@@ -194,6 +194,12 @@ def main():
     # 10. and on again, after a run on the v1 kernels
     step("g g > g g g, fused again", True,
          lambda c: (c.run(g5, a5, e5, n5, s5, 0x5001, True, everything(g5)), c.resolved(a5, n5))[1])
+    # 11. enough points for the run's upload to go in chunks, each launched when it has arrived
+    nb = (1 << 18) + 4099
+    eb = inputs(g5, nb, 75)
+    rb, sb = rng.random(nb), np.ones(nb)
+    step("g g > g g g, 2^18 + 4099 points (chunked upload), kept, reduce_sample", True,
+         lambda c: (c.run(g5, a5, eb, nb, nb + 1, 0x5001, True, everything(g5)), c.sample(g5, a5, nb, rb, sb))[1])
     print(f"{kernel}: all equal", flush=True)
 
 
