@@ -12,7 +12,8 @@ library records on the kernel's stream); the CPU oracle (the same plan interpret
 timed on a bounded sample of the same points; df64/mixed/mixed64/compensated(64) throughput and error
 against f64; g g > g g g g (2^21 points, every mode, the persistent grid tuned by a sweep); the
 p p > j j j batch (all 181 subprocesses, four-quark ones included, in one persistent launch, Zipf
-point counts summing to 2^22); and surrogate vs plain unweighting of g g > g g g g."""
+point counts summing to 2^22); the headline points through the reference's v1 entry points
+(host memory in and out); and surrogate vs plain unweighting of g g > g g g g."""
 import argparse
 import itertools
 import json
@@ -191,7 +192,8 @@ def v1_abi(lib, plan, ext_host, want, tiles=(1 << 18, 1 << 20, 1 << 22), repeats
     helicity-sampled totals (pacman_recurrence_v1_reduce_sample, 12 B per point back). Wall time
     over all points of the best of `repeats` passes after one warm-up pass: host-to-device and
     device-to-host copies included, as the entry points take and return host memory. The totals
-    are compared bit for bit with the fused f64 launch's."""
+    are compared bit for bit with the fused f64 launch's. The keyed calls run on the fused
+    kernels; the largest tile is measured again on the v1 kernels."""
     C = abi.C
     opt = abi.Options(abi.ABI, C.sizeof(abi.Options), abi.KERNEL_F64, 0, 0, 0)
     if lib.pacman_recurrence_v1_configure(C.byref(opt)) != 0:
@@ -213,8 +215,10 @@ def v1_abi(lib, plan, ext_host, want, tiles=(1 << 18, 1 << 20, 1 << 22), repeats
         return p, np.frombuffer(buf, dtype=dtype)
 
     res = {"workload": f"{plan.name}, {n} points, f64, KEEP_AMPLITUDES | FORMS_ON_DEVICE", "tiles": {}}
-    for tile in tiles:
-        tile = min(tile, n)
+    # the tiles on the fused kernels, then the largest again on the v1 kernels (PACUDA_V1_FUSED=0)
+    for tile in list(tiles) + ["v1_kernels"]:
+        v1_kernels = tile == "v1_kernels"
+        tile = min(tiles[-1] if v1_kernels else tile, n)
         starts = list(range(0, n, tile))
         # the tiles' momenta staged in pinned memory before the timed passes, each tile's planes of
         # `points` values (the layout the run reads)
@@ -225,6 +229,8 @@ def v1_abi(lib, plan, ext_host, want, tiles=(1 << 18, 1 << 20, 1 << 22), repeats
         p_cho, chosen = pinned(tile, np.uint32)
         hold = []
         try:
+            if v1_kernels:
+                os.environ["PACUDA_V1_FUSED"] = "0"
             for s in starts:
                 pts = min(tile, n - s)
                 ext[rows * s:rows * (s + pts)] = ext_host[:, s:s + pts].reshape(-1)
@@ -293,11 +299,15 @@ def v1_abi(lib, plan, ext_host, want, tiles=(1 << 18, 1 << 20, 1 << 22), repeats
             entry["sample_totals_bitexact_vs_fused"] = bool(np.array_equal(tot.view(np.uint64), want.view(np.uint64)))
             best = min(_wall(pass_sample) for _ in range(repeats))
             entry["sample_points_per_s"] = n / best
-            res["tiles"][str(tile)] = entry
-            log(f"v1 ABI tile {tile}: resolved {entry['resolved_points_per_s']:.4g} points/s, sample "
-                f"{entry['sample_points_per_s']:.4g} points/s (bit-exact {entry['resolved_bitexact_vs_fused']}, "
-                f"{entry['sample_totals_bitexact_vs_fused']})")
+            if v1_kernels:
+                res["v1_kernels"] = dict(entry, tile=tile)
+            else:
+                res["tiles"][str(tile)] = entry
+            log(f"v1 ABI tile {tile}{' (v1 kernels)' if v1_kernels else ''}: resolved "
+                f"{entry['resolved_points_per_s']:.4g} points/s, sample {entry['sample_points_per_s']:.4g} points/s "
+                f"(bit-exact {entry['resolved_bitexact_vs_fused']}, {entry['sample_totals_bitexact_vs_fused']})")
         finally:
+            os.environ.pop("PACUDA_V1_FUSED", None)
             for p in (p_ext, p_out, p_rand, p_tot, p_cho):
                 lib.pacman_recurrence_v1_host_free(p)
     return res
