@@ -468,13 +468,21 @@ class Unweighting:
 
     VARIANTS = {"network": "two_stage", "network_dense": "two_stage_dense", "prior": "two_stage_prior_only"}
 
-    def run(self, candidates, seed=4, chunk=1 << 24, q=0.99999, q2=0.999, variants=("network", "prior")):
+    def run(self, candidates, seed=4, chunk=1 << 24, q=0.99999, q2=0.999, variants=("network", "prior"), repeats=3):
         """plain and two-stage unweighting on the same candidates; `variants` of the surrogate:
         "network" (the trained network with the prior, evaluated inside the envelope only),
         "network_dense" (the same network evaluated on every candidate) and "prior" (the prior
         alone, no network). Pulls: against the plain estimate (errors added in quadrature) and
-        against the weighted Monte Carlo estimate of the same candidates (the method's error)."""
-        out = {"plain": self.plain(candidates, seed, chunk, q)}
+        against the weighted Monte Carlo estimate of the same candidates (the method's error).
+        Each method runs `repeats` times on the same streams (the same events each time) and
+        reports its fastest run, with every run's seconds in `seconds_all`."""
+        def best(f):
+            runs = [f() for _ in range(max(1, repeats))]
+            r = min(runs, key=lambda x: x["seconds"])
+            r["seconds_all"] = [x["seconds"] for x in runs]
+            return r
+
+        out = {"plain": best(lambda: self.plain(candidates, seed, chunk, q))}
         p = out["plain"]
         rate_p = p["events"] / p["seconds"]
         p["events_per_s"] = rate_p
@@ -488,9 +496,9 @@ class Unweighting:
                 else:
                     continue
             if v in ("network", "prior"):
-                t = self.two_stage_envelope(candidates, seed, chunk, q, q2, network=(v == "network"))
+                t = best(lambda: self.two_stage_envelope(candidates, seed, chunk, q, q2, network=(v == "network")))
             else:
-                t = self.two_stage(candidates, seed, chunk, q, q2, network=True)
+                t = best(lambda: self.two_stage(candidates, seed, chunk, q, q2, network=True))
             t["events_per_s"] = t["events"] / t["seconds"]
             t["gain"] = t["events_per_s"] / rate_p if rate_p else None
             sig, err = t["estimate"]
