@@ -1,7 +1,7 @@
 // SPDX-License-Identifier: 0BSD
 //
 // pacuda-check FILE.pacuda --runner LIBRARY [--kernel f64|df64|mixed|mixed64] [--tile N]
-//              [--host-reduce] [--dump] [--examples EXAMPLES.json]
+//              [--host-reduce] [--key] [--dump] [--examples EXAMPLES.json]
 // pacuda-check FILE.pacuda --write-examples EXAMPLES.json
 //
 // Reference points come from the recording, or from an examples.json in pac2json's format
@@ -18,7 +18,8 @@
 // --host-reduce), the resolved reduction into [point][helicity] (or the host's), the group
 // factor; then the count of values differing in bits, the worst error relative to the point's
 // total and the worst relative error of a point's total. With the f64 kernel any differing
-// value is a failure (exit 1). The file grammar is convert/src/format.rs:6-29 (little endian,
+// value is a failure (exit 1). replay.rs's calls carry no key; with --key each colour's calls
+// carry one (the FNV-1a hash of its words and widths), as a caller that keeps its programs does. The file grammar is convert/src/format.rs:6-29 (little endian,
 // no padding, counted arrays, u32-length strings); the reader rejects trailing bytes and the
 // inconsistencies umami/umami_pacuda.c:253-257 rejects.
 #include <dlfcn.h>
@@ -452,10 +453,22 @@ std::vector<uint64_t> remap_kernels(const std::vector<uint64_t> &words, const st
     return out;
 }
 
+// a nonzero key for a program: the 64-bit FNV-1a hash of its words and widths
+uint64_t program_key(const std::vector<uint64_t> &words, const uint8_t *width) {
+    uint64_t hash = 0xcbf29ce484222325ull;
+    auto mix = [&](const void *data, size_t bytes) {
+        for (size_t k = 0; k < bytes; k++) hash = (hash ^ ((const uint8_t *)data)[k]) * 0x100000001b3ull;
+    };
+    mix(words.data(), words.size() * sizeof(uint64_t));
+    mix(width, 8);
+    return hash ? hash : 1;
+}
+
 std::vector<double> evaluate(const Runner &rn, const std::vector<uint32_t> &map, const Process &p, const Colour &c,
-                             size_t tile, bool host_reduce) {
+                             size_t tile, bool host_reduce, bool keyed) {
     const size_t n = p.legs, h = p.helicities, points = c.points;
     std::vector<uint64_t> words = remap_kernels(c.words, map);
+    const uint64_t key = keyed ? program_key(words, c.width) : 0;
     std::vector<double> values(points * h, 0.0);
     const size_t stride = tile;
     std::vector<double> cre(c.current_planes * stride), cim(cre.size()), are(c.amplitude_planes * stride), aim(are.size()),
@@ -473,6 +486,7 @@ std::vector<double> evaluate(const Runner &rn, const std::vector<uint32_t> &map,
         call.abi_version = PACMAN_RECURRENCE_ABI;
         call.struct_size = sizeof call;
         call.words = words.data();
+        call.words_key = key;
         call.word_count = (uint32_t)words.size();
         memcpy(call.width, c.width, 8);
         call.current_re = cre.data();
@@ -540,13 +554,14 @@ std::vector<double> evaluate(const Runner &rn, const std::vector<uint32_t> &map,
 int main(int argc, char **argv) {
     const char *file = nullptr, *runner = nullptr, *kernel = "f64", *examples = nullptr, *write = nullptr;
     size_t tile = 0;
-    bool host_reduce = false, dump = false;
+    bool host_reduce = false, dump = false, keyed = false;
     for (int k = 1; k < argc; k++) {
         std::string a = argv[k];
         if (a == "--runner" && k + 1 < argc) runner = argv[++k];
         else if (a == "--kernel" && k + 1 < argc) kernel = argv[++k];
         else if (a == "--tile" && k + 1 < argc) tile = strtoul(argv[++k], nullptr, 10);
         else if (a == "--host-reduce") host_reduce = true;
+        else if (a == "--key") keyed = true;
         else if (a == "--dump") dump = true;
         else if (a == "--examples" && k + 1 < argc) examples = argv[++k];
         else if (a == "--write-examples" && k + 1 < argc) write = argv[++k];
@@ -558,7 +573,7 @@ int main(int argc, char **argv) {
     }
     if (!file || (!runner && !dump && !write)) {
         fprintf(stderr, "usage: pacuda-check FILE.pacuda --runner LIBRARY [--kernel f64|df64|mixed|mixed64] [--tile N] "
-                        "[--host-reduce] [--dump] [--examples EXAMPLES.json]\n"
+                        "[--host-reduce] [--key] [--dump] [--examples EXAMPLES.json]\n"
                         "       pacuda-check FILE.pacuda --write-examples EXAMPLES.json\n");
         return 2;
     }
@@ -614,7 +629,7 @@ int main(int argc, char **argv) {
         for (const Process &p : f.processes)
             for (const Colour &c : p.colours) {
                 size_t t = tile ? tile : (f.batch ? f.batch : c.tile);
-                std::vector<double> got = evaluate(rn, map, p, c, t, host_reduce);
+                std::vector<double> got = evaluate(rn, map, p, c, t, host_reduce, keyed);
                 size_t differ = 0;
                 double worst = 0, worst_total = 0;
                 const size_t h = std::max<uint32_t>(1, p.helicities);
@@ -636,6 +651,9 @@ int main(int argc, char **argv) {
                 total_differ += differ;
                 total_values += got.size();
             }
+        // (a library that runs some calls on other kernels says how many: this library's fused ones)
+        if (auto fused = (uint64_t (*)(void))dlsym(lib, "pacuda_v1_fused_runs"))
+            printf("runs on the fused kernels: %llu\n", (unsigned long long)fused());
         if (k == "f64" && total_differ) {
             fprintf(stderr, "pacuda-check: %zu of %zu f64 values differ from the recording\n", total_differ, total_values);
             return 1;

@@ -167,20 +167,23 @@ def test_v1_other_kernels_in_fresh_processes(gg_ggg, tmp_path):
 
 def test_recording_replays_bit_exact_on_gpu():
     """tests/golden/gg_ggg_qqbar.pacuda through pacuda-check on the HIP library: every f64 value
-    equals the recorded CPU value (device and host reductions, and the reference points read
-    back from an examples.json the tool writes)"""
+    equals the recorded CPU value (device and host reductions, the reference points read back
+    from an examples.json the tool writes, and keyed calls, which run on the fused kernels)"""
     from pacuda_amd import build
     golden = os.path.join(os.path.dirname(__file__), "golden", "gg_ggg_qqbar.pacuda")
     examples = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"examples_{os.getpid()}.json")
     p = subprocess.run([build.CHECK_TOOL, golden, "--write-examples", examples], capture_output=True, text=True,
                        timeout=60)
     assert p.returncode == 0, p.stderr
-    for args in [(), ("--host-reduce", "--tile", "3"), ("--examples", examples)]:
+    for args in [(), ("--host-reduce", "--tile", "3"), ("--examples", examples), ("--key",),
+                 ("--key", "--host-reduce", "--tile", "5")]:
         print("pacuda-check", args, flush=True)
         p = subprocess.run([build.CHECK_TOOL, golden, "--runner", abi.LIBRARY, *args], capture_output=True, text=True,
                            timeout=120)
         assert p.returncode == 0, p.stdout + p.stderr
         assert "0 of 256 values differ" in p.stdout and "0 of 128 values differ" in p.stdout
+        fused = "runs on the fused kernels: 0" not in p.stdout
+        assert fused == ("--key" in args), p.stdout
 
 
 def test_fused_colour_orderings_match_oracle(lib, gg_ggg):
