@@ -563,8 +563,8 @@ class RunPlan {
         return true;
     }
 
-    // A run takes the fused kernels (one launch: forms, sources, program, the amplitudes into the
-    // arena) when it gives the same results there: a keyed call (its program is built once) with
+    // A run takes the fused kernels (launches that run forms, sources and program and store the
+    // amplitudes into the arena) when it gives the same results there: a keyed call (its program is built once) with
     // the forms built on the device, nothing uploaded but the external momenta, no current
     // downloaded, a program that writes every current it reads and closures that write cleared
     // amplitude planes only. Its currents then stay on chip: the arena's current planes are not
