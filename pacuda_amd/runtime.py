@@ -60,7 +60,7 @@ class Device:
 
     def array(self, a):
         a = np.ascontiguousarray(a)
-        d = DeviceArray(self.lib, a.nbytes)
+        d = self.empty(a.nbytes, np.uint8)
         d.upload(a)
         return d
 
@@ -70,6 +70,43 @@ class Device:
     def synchronize(self):
         if self.lib.pacuda_synchronize() != 0:
             raise RuntimeError("device synchronisation failed")
+
+
+class PooledDevice(Device):
+    """A Device whose buffers come from a pool and go back to it when freed, for drivers that
+    allocate and free large buffers on every call. Only for buffers whose every use is queued on
+    the library's stream (the library's kernels and copies run there): a buffer freed while work
+    that uses it is still queued may be handed out again at once, and the stream orders the new
+    work after the old. A buffer is reused for a request of at least half its size. close()
+    releases the pool (after the device has finished with it)."""
+
+    def __init__(self, lib=None):
+        super().__init__(lib)
+        self._free = []  # (nbytes, base DeviceArray) of the buffers not handed out
+
+    def empty(self, count, dtype=np.float64):
+        nbytes = max(1, int(count) * np.dtype(dtype).itemsize)
+        fits = [k for k, (size, _) in enumerate(self._free) if nbytes <= size <= 2 * nbytes + (1 << 20)]
+        if fits:
+            k = min(fits, key=lambda j: self._free[j][0])
+            _, base = self._free.pop(k)
+        else:
+            base = DeviceArray(self.lib, nbytes)
+        out = DeviceArray.view(base, 0, nbytes)
+        pool = self._free
+
+        def give_back(a=out, b=base):
+            if a.ptr is not None:
+                pool.append((b.nbytes, b))
+                a.ptr = None
+        out.free = give_back
+        return out
+
+    def close(self):
+        self.synchronize()
+        for _, b in self._free:
+            b.free()
+        self._free = []
 
 
 def plan_desc(plan, arrays, no_sort=0):

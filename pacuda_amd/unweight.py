@@ -30,7 +30,7 @@ import time
 import numpy as np
 
 from . import abi, plans, phase_space as ps
-from .runtime import Device, DeviceArray, FusedPlan, evaluate
+from .runtime import DeviceArray, FusedPlan, PooledDevice, evaluate
 from .surrogate import MLP, he_init
 
 GEV2_TO_PB = 0.3893793721e9
@@ -142,7 +142,9 @@ def quantile_threshold(hist, q):
 
 class Unweighting:
     def __init__(self, legs=6, hidden=256, layers=4, batch=4096, prior=True, lib=None):
-        self.dev = Device(lib)
+        # (every buffer of the methods is used on the library's stream only: pooled, so that a
+        # method's repeated allocations of candidate-sized buffers do not time the allocator)
+        self.dev = PooledDevice(lib)
         self.plan = plans.gluon_plan(legs)
         self.fused = FusedPlan(self.dev.lib, self.plan)
         self.gen = Generator(self.dev, self.plan)
@@ -511,3 +513,4 @@ class Unweighting:
         self.sel.free()
         self.mlp.destroy()
         self.fused.destroy()
+        self.dev.close()
