@@ -61,7 +61,9 @@ __global__ void __launch_bounds__(256) rambo_kernel(uint64_t state, uint64_t fir
         for (int r = 0; r < 4; r++) u[j][r] = uniform(state, base + 4 * j + r);
         const double c = 2.0 * u[j][0] - 1.0, s = sqrt(1.0 - c * c), f = 2.0 * M_PI * u[j][1];
         const double e = -log(u[j][2] * u[j][3]);
-        q[j][0] = e; q[j][1] = e * s * cos(f); q[j][2] = e * s * sin(f); q[j][3] = e * c;
+        double sf, cf;
+        sincos(f, &sf, &cf);
+        q[j][0] = e; q[j][1] = e * s * cf; q[j][2] = e * s * sf; q[j][3] = e * c;
         t0 += q[j][0]; t1 += q[j][1]; t2 += q[j][2]; t3 += q[j][3];
     }
     const double mass = sqrt(t0 * t0 - t1 * t1 - t2 * t2 - t3 * t3);
@@ -120,8 +122,10 @@ __global__ void __launch_bounds__(256) rambo_kernel(uint64_t state, uint64_t fir
             for (int k = 0; k < 4; k++) external[((size_t)l * 4 + k) * stride + i] = cr.sign[l] * xl[l][k];
     if (log_prior) {
         // log(sum_{a<b} s_ab^4 / prod_l s_{l,l+1}) over the plan's legs in order (cyclic),
-        // s_ab = |2 p_a.p_b|: the helicity-summed MHV Parke-Taylor structure of the ordering
-        double num = 0.0, log_den = 0.0;
+        // s_ab = |2 p_a.p_b|: the helicity-summed MHV Parke-Taylor structure of the ordering (the
+        // product of the adjacent invariants stays within f64's range: each lies between about
+        // 10^1 and s = 10^6 GeV^2 for points that pass the cuts)
+        double num = 0.0, den = 1.0;
 #pragma unroll
         for (int a = 0; a < LEGS; a++)
 #pragma unroll
@@ -130,9 +134,9 @@ __global__ void __launch_bounds__(256) rambo_kernel(uint64_t state, uint64_t fir
                                                xl[a][3] * xl[c][3]));
                 const double s2 = sab * sab;
                 num += s2 * s2;
-                if (c == a + 1 || (a == 0 && c == LEGS - 1)) log_den += log(sab);
+                if (c == a + 1 || (a == 0 && c == LEGS - 1)) den *= sab;
             }
-        log_prior[i] = log(num) - log_den;
+        log_prior[i] = log(num) - log(den);
     }
     if (features) {
         // per outgoing parton: the uniforms of its direction and the CDF of its energy product

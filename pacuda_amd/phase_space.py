@@ -60,8 +60,8 @@ def log_prior(mom, crossing):
         x, y = p[:, a], p[:, b]
         return np.abs(2 * (x[:, 0] * y[:, 0] - x[:, 1] * y[:, 1] - x[:, 2] * y[:, 2] - x[:, 3] * y[:, 3]))
     num = sum(s(a, b) ** 4 for a in range(legs) for b in range(a + 1, legs))
-    den = sum(np.log(s(a, (a + 1) % legs)) for a in range(legs))
-    return np.log(num) - den
+    den = np.prod([s(a, (a + 1) % legs) for a in range(legs)], axis=0)
+    return np.log(num) - np.log(den)
 
 
 def _rambo_from(u, n, ecm):
