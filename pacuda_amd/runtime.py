@@ -77,8 +77,8 @@ class PooledDevice(Device):
     allocate and free large buffers on every call. Only for buffers whose every use is queued on
     the library's stream (the library's kernels and copies run there): a buffer freed while work
     that uses it is still queued may be handed out again at once, and the stream orders the new
-    work after the old. A buffer is reused for a request of at least half its size. close()
-    releases the pool (after the device has finished with it)."""
+    work after the old. A free buffer serves a request of at least half its size less 512 KiB
+    (the smallest that fits). close() releases the pool (after the device has finished with it)."""
 
     def __init__(self, lib=None):
         super().__init__(lib)
