@@ -479,9 +479,11 @@ def main():
         _, ms64 = timed_steps(lib, j6, k64, 3, *(tuple(shape6) + (0,))[:3])
         ref6 = t6.download(n6)
         f6flops = plans.plan_flops(p6)[0]
+        _, ms_default = timed_steps(lib, j6, k64, 3)  # (the library's default launch)
         res6 = {"points": n6, "plan_rows": len(p6.rows), "flops_per_point": f6flops, "tuned": best, "sweep_ms": sweep,
                 "f64_points_per_s": n6 / (ms64 * 1e-3),
-                "f64_tflops": f6flops * n6 / (ms64 * 1e-3) / 1e12}
+                "f64_tflops": f6flops * n6 / (ms64 * 1e-3) / 1e12,
+                "f64_default_launch_points_per_s": n6 / (ms_default * 1e-3)}
         for name in ("mixed", "mixed64", "compensated", "compensated64", "df64"):
             k = abi.FUSED_KERNELS[name]
             kshape, _ = autotune(lib, j6, k, shapes=TUNE_SHAPES + COOP_TUNE_SHAPES_ANY)

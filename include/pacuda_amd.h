@@ -163,8 +163,10 @@ typedef struct {
     uint32_t block;         /* threads per workgroup, a multiple of 64 up to 256, 0: 256 */
     uint32_t unit_waves;    /* 0: one wavefront per 64-point unit; 16, 8 or 4: cooperative launch (below) */
     /* all three 0: the default launch, cooperative with 8 wavefronts per unit and 16 per CU for the
-       f64, mixed, mixed64 and compensated64 kernels when every plan has a cooperative program,
-       else one wavefront per unit at the default waves_per_cu and block */
+       f64, mixed, mixed64 and compensated64 kernels when every plan has a cooperative program
+       (16 wavefronts per unit when that program would move more than 64 KiB per point through
+       the slot on average, as g g > g g g g's f64 one does), else one wavefront per unit at the
+       default waves_per_cu and block */
     void *stream;           /* hipStream_t to queue on, or NULL: the library's stream */
     float *kernel_ms;       /* not NULL: synchronise and store the fused kernel's time (HIP events) */
 } pacuda_launch;

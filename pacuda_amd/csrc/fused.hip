@@ -53,6 +53,12 @@ constexpr bool coop_built(int cls, int shape) { return cls == 0 || shape == 0 ||
 // kernels are VALU-bound and keep one wavefront per unit)
 constexpr int COOP_DEFAULT = 2;
 static_assert(COOP_WAVES[COOP_DEFAULT] == 8 && COOP_UNITS[COOP_DEFAULT] == 2, "the default cooperative shape");
+// ... and, when the default shape's programs would move more than this many bytes per point
+// through the slot (averaged over the launch's points), shape 0: one unit per CU with 16
+// wavefronts and twice the LDS per unit (measured: g g > g g g g in f64, 90 KB per point at
+// shape 2, runs 7% faster in shape 0; its f32 currents, 37 KB, and g g > g g g, 22 KB, run
+// faster in shape 2)
+constexpr uint64_t COOP_WIDE_BYTES = 64 * 1024;
 
 struct PlanDev {
     const Instr *prog_lds[LDS_VARIANTS];  // with forwarding and LDS-resident currents (LDS_CAPS[v] planes)
@@ -477,6 +483,7 @@ struct pacuda_plan {
     std::vector<void *> allocations;
     uint64_t slot_doubles = 0;
     uint64_t coop_slot_doubles[2][COOP_SHAPES] = {};  // a workgroup's slot per class and shape (0: no program)
+    uint64_t coop_slot_bytes[2][COOP_SHAPES] = {};    // the bytes a point moves through it
     bool symjit = false;  // the program holds symjit rows
 };
 
@@ -1958,6 +1965,7 @@ extern "C" int pacuda_plan_create(const pacuda_plan_desc *d, pacuda_plan **out) 
             // (the slot's current planes at 1 KB whatever the class, as fused_kernel lays them out)
             p->coop_slot_doubles[cls][s] =
                 (uint64_t)LANES * (2ull * q.slot_planes + 2ull * d->amplitude_planes + 4ull * d->form_count);
+            p->coop_slot_bytes[cls][s] = q.slot_accesses * (cls ? 8u : 16u);
         }
     p->device = upload(p, &h, 1, &ok);
     p->slot_doubles = (uint64_t)LANES * (2ull * planes + 2ull * d->amplitude_planes + 4ull * d->form_count);
@@ -2002,9 +2010,6 @@ struct Schedule {
     uint32_t slots_per_block() const { return coop >= 0 ? 1u : block / LANES; }
 };
 
-// The LDS variant of a launch, its LDS elements per wavefront and bytes per workgroup. A variant
-// qualifies when every plan has it and the workgroups a CU must hold for the requested
-// wavefronts fit in its LDS; the largest one is taken, else none (per_wave 0).
 bool coop_default(uint32_t kernel) {
     return kernel == PACMAN_RECURRENCE_KERNEL_F64 || kernel == PACMAN_RECURRENCE_KERNEL_MIXED ||
            kernel == PACMAN_RECURRENCE_KERNEL_MIXED64 || kernel == PACUDA_KERNEL_COMPENSATED64;
@@ -2016,6 +2021,9 @@ bool f32_kernel(uint32_t kernel) {
            kernel == PACUDA_KERNEL_COMPENSATED || kernel == PACUDA_KERNEL_COMPENSATED64;
 }
 
+// The LDS variant of a launch, its LDS elements per wavefront and bytes per workgroup. A variant
+// qualifies when every plan has it and the workgroups a CU must hold for the requested
+// wavefronts fit in its LDS; the largest one is taken, else none (per_wave 0).
 void lds_shape(uint32_t kernel, const Schedule &sc, uint32_t *per_wave, size_t *bytes, uint32_t *variant) {
     *per_wave = 0;
     *bytes = 0;
@@ -2053,13 +2061,21 @@ int schedule(const pacuda_item *items, uint32_t count, const pacuda_launch *l, S
         }
     } else if (!l->waves_per_cu && !l->block && coop_default(l->kernel)) {
         // the default launch: cooperative, 8 wavefronts per unit and two units per CU (the
-        // fastest shape measured for these kernels), when every plan has its program
+        // fastest shape measured for these kernels), when every plan has its program; 16
+        // wavefronts and one unit per CU when the points would move more than COOP_WIDE_BYTES
+        // each through the slot and every plan has that program too
         const int cls = f32_kernel(l->kernel) ? 1 : 0;
-        bool all = count > 0;
-        for (uint32_t k = 0; k < count && all; k++)
+        bool all = count > 0, wide_ok = true;
+        uint64_t points = 0, bytes = 0;
+        for (uint32_t k = 0; k < count && all; k++) {
             all = items[k].plan && items[k].plan->coop_slot_doubles[cls][COOP_DEFAULT];
+            if (!all) break;
+            wide_ok = wide_ok && items[k].plan->coop_slot_doubles[cls][0];
+            points += items[k].points;
+            bytes += (uint64_t)items[k].points * items[k].plan->coop_slot_bytes[cls][COOP_DEFAULT];
+        }
         if (all) {
-            sc.coop = COOP_DEFAULT;
+            sc.coop = wide_ok && bytes > COOP_WIDE_BYTES * points ? 0 : COOP_DEFAULT;
             sc.coop_class = cls;
         }
     }
