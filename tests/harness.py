@@ -38,9 +38,10 @@ class V1:
         (and the runner's lock held by this thread)"""
         self.run_plan(plan, ext, points, stride, keep=True, reduce=False)
 
-    def run_plan(self, plan, ext, points, stride=None, keep=True, reduce=True):
+    def run_plan(self, plan, ext, points, stride=None, keep=True, reduce=True, words_key=0):
         """v1 run of a generated plan with the forms on the device, then reduce_resolved from
-        zero: [points][outputs] (each output before the group factor)."""
+        zero: [points][outputs] (each output before the group factor). A nonzero words_key
+        keys the call (the HIP library runs such calls on its fused kernels)."""
         a = plan.arrays()
         stride = stride or points
         n_amp = plan.amplitude_planes
@@ -56,7 +57,7 @@ class V1:
         c.abi_version = abi.ABI
         c.struct_size = C.sizeof(abi.Call)
         c.words = ptr(a["words"], u64p)
-        c.words_key = 0
+        c.words_key = words_key
         c.word_count = len(a["words"])
         for k in range(8):
             c.width[k] = int(a["width"][k])

@@ -28,10 +28,11 @@ def _jobs(threads, per_thread):
     return plan, jobs
 
 
-def pairs(v1, threads=4, per_thread=5):
-    """(single-thread results, concurrent results) keyed by (thread, repetition)"""
+def pairs(v1, threads=4, per_thread=5, key=0):
+    """(single-thread results, concurrent results) keyed by (thread, repetition); `key` is the
+    calls' words_key"""
     plan, jobs = _jobs(threads, per_thread)
-    single = {(t, m): v1.run_plan(plan, ext, ext.shape[1]) for t, m, ext in jobs}
+    single = {(t, m): v1.run_plan(plan, ext, ext.shape[1], words_key=key) for t, m, ext in jobs}
     concurrent, errors = {}, []
     start = threading.Barrier(threads)
 
@@ -40,7 +41,7 @@ def pairs(v1, threads=4, per_thread=5):
             start.wait()
             for tt, m, ext in jobs:
                 if tt == t:
-                    concurrent[(t, m)] = v1.run_plan(plan, ext, ext.shape[1])
+                    concurrent[(t, m)] = v1.run_plan(plan, ext, ext.shape[1], words_key=key)
         except Exception as e:  # noqa: BLE001 (reported by the caller)
             errors.append(f"thread {t}: {e}")
 

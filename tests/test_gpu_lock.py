@@ -30,6 +30,20 @@ def test_concurrent_keep_reduce_pairs_equal_single_thread(v1):
         assert harness.bits_equal(single[k], concurrent[k]), k
 
 
+def test_concurrent_keyed_pairs_on_the_fused_kernels(v1):
+    """the same pairs with keyed calls, which run on the fused kernels: every run does, and the
+    results equal the unkeyed (v1 kernels') ones bit for bit"""
+    print("lock: 4 threads x 5 keyed pairs", flush=True)
+    lib = v1.lib
+    before = lib.pacuda_v1_fused_runs()
+    single, concurrent = lock_cases.pairs(v1, key=0x10C4)
+    assert lib.pacuda_v1_fused_runs() - before == 2 * len(single)
+    plain, _ = lock_cases.pairs(v1, threads=4, per_thread=5)
+    for k in single:
+        assert harness.bits_equal(single[k], concurrent[k]), k
+        assert harness.bits_equal(single[k], plain[k]), k
+
+
 def test_a_kept_run_blocks_other_threads(v1):
     print("lock: blocking", flush=True)
     reduce_at, run_at, unlocked = lock_cases.blocking(v1)
