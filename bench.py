@@ -387,7 +387,7 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
               "points with no exchange during evaluation)", file=sys.stderr)
         sys.exit(2)
 
