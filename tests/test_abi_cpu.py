@@ -201,7 +201,7 @@ def test_plan_compilation_rejects_bad_descriptions(lib):
     words = np.array(a["words"], copy=True)
     ops = words & np.uint64(15)
     assert compile_with(words=words[ops != 12])[0] == abi.INVALID
-    # symjit rows are refused (a library built without generated kernels)
+    # a malformed binding (an op-14 word made from another row's bits) is refused
     sym = np.concatenate([words[:1] & ~np.uint64(15) | np.uint64(14), words])
     assert compile_with(words=sym)[0] == abi.INVALID
     # variant out of range, missing count, too small a buffer
