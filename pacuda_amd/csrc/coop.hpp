@@ -447,6 +447,7 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
     }
     uint32_t fetch_phase = 1, fetch_wave = 0;
     uint64_t best_slack = 0;
+    std::vector<std::vector<uint64_t>> loads(P + 1);
     for (uint32_t p = 1; p <= P; p++) {
         std::vector<uint32_t> ts;
         for (uint32_t t = 0; t < T; t++)
@@ -458,6 +459,7 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
             load[w] += cost[t];
             wave_of[t] = w;
         }
+        loads[p] = load;
         const uint32_t least = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
         const uint64_t slack = *std::max_element(load.begin(), load.end()) - load[least];
         if (p == 1 || slack > best_slack) {
@@ -516,7 +518,14 @@ inline CoopProgram coop_program(const std::vector<Instr> &ops, uint32_t planes, 
             for (uint32_t t = 0; t < T; t++)
                 if (phase[t] == p && wave_of[t] == w)
                     for (uint32_t r = 0; r < task_rows[t]; r++) cp.ops.push_back(dev[task_first[t] + r]);
-            if (p < P) cp.ops.push_back(barrier);
+            if (p < P) {
+                // (the barrier row also sets the wavefront's priority for the next phase: the most
+                // loaded wavefronts, those within a tenth of the largest load, issue first)
+                Instr b = barrier;
+                const uint64_t top = *std::max_element(loads[p + 1].begin(), loads[p + 1].end());
+                b.w[1] = loads[p + 1][w] * 10 >= top * 9 ? 1u : 0u;
+                cp.ops.push_back(b);
+            }
         }
     }
     cp.start[waves] = (uint32_t)cp.ops.size();

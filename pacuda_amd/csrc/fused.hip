@@ -428,6 +428,9 @@ coop_kernel(const ItemDev *__restrict__ items, const uint32_t *__restrict__ unit
             interpret<K, SYMJIT, true>(prog + mid, end - mid, a);
         }
         __syncthreads();
+        // (wavefront 0's reduction is the unit's critical path: it issues first)
+        if (wave == 0) __builtin_amdgcn_s_setprio(2);
+        else __builtin_amdgcn_s_setprio(0);
         const uint32_t next = __builtin_amdgcn_readfirstlane(control[1]);
         ItemDev nit = it;
         uint32_t ni = 0;

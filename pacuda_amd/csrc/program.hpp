@@ -343,7 +343,11 @@ __device__ inline void interpret(const Instr *__restrict__ prog, uint32_t count,
         } else if (kind == K_SYMJIT) {
             if constexpr (SYMJIT) symjit_row<K>(a, o, flags);
         } else if (kind == K_BARRIER) {
-            if constexpr (COOP) __syncthreads();
+            if constexpr (COOP) {
+                __syncthreads();
+                if (o.w[1]) __builtin_amdgcn_s_setprio(2);
+                else __builtin_amdgcn_s_setprio(0);
+            }
         } else if (kind == K_FETCH) {
             if constexpr (COOP) a.fetch();
         } else if (kind == K_SOURCE) {  // a source row made again (programs of the f64 kernel), or a cooperative program's
