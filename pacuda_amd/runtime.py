@@ -83,6 +83,7 @@ class PooledDevice(Device):
     def __init__(self, lib=None):
         super().__init__(lib)
         self._free = []  # (nbytes, base DeviceArray) of the buffers not handed out
+        self._closed = False
 
     def empty(self, count, dtype=np.float64):
         nbytes = max(1, int(count) * np.dtype(dtype).itemsize)
@@ -93,12 +94,15 @@ class PooledDevice(Device):
         else:
             base = DeviceArray(self.lib, nbytes)
         out = DeviceArray.view(base, 0, nbytes)
-        pool = self._free
 
         def give_back(a=out, b=base):
-            if a.ptr is not None:
-                pool.append((b.nbytes, b))
-                a.ptr = None
+            if a.ptr is None:
+                return
+            a.ptr = None
+            if self._closed:  # (freed after close: straight back to the library)
+                b.free()
+            else:
+                self._free.append((b.nbytes, b))
         out.free = give_back
         return out
 
@@ -107,6 +111,7 @@ class PooledDevice(Device):
         for _, b in self._free:
             b.free()
         self._free = []
+        self._closed = True
 
 
 def plan_desc(plan, arrays, no_sort=0):

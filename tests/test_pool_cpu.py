@@ -56,6 +56,9 @@ def test_arrays_go_through_the_pool_and_close_frees_it():
     assert len(lib.live) == 2
     z = dev.empty(60, np.uint32)
     assert z.ptr in lib.live
+    late = dev.empty(64, np.uint32)  # (in use when the pool closes)
     z.free()
     dev.close()
+    assert list(lib.live) == [late.ptr]
+    late.free()
     assert lib.live == {}
