@@ -375,6 +375,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
     ap.add_argument("--only-unweighting", action="store_true", help="of the extra workloads, only the unweighting")
     ap.add_argument("--only-v1", action="store_true", help="of the extra workloads, only the v1 entry points")
+    ap.add_argument("--only-gggg", action="store_true", help="of the extra workloads, only g g > g g g g")
     ap.add_argument("--candidates", type=int, default=1 << 24)
     ap.add_argument("--train-points", type=int, default=1 << 20)
     ap.add_argument("--epochs", type=int, default=20)
@@ -439,10 +440,11 @@ def main():
             log(f"{name} {modes[name]['points_per_s']:.4g} points/s ({shape_name(mshape)})")
 
     extra = {}
-    if not args.no_extra and not args.only_unweighting:
+    only = args.only_unweighting or args.only_v1 or args.only_gggg
+    if not args.no_extra and (not only or args.only_v1):
         # the same points through the reference-compatible entry points, host memory in and out
         extra["v1_abi"] = v1_abi(lib, plan, d_ext.download((plan.legs * 4, n)), total64)
-    if not args.no_extra and not args.only_unweighting and not args.only_v1:
+    if not args.no_extra and not only:
         # the whole leading-colour sum of g g > g g g: the 12 orderings (0, s1..s4) of the legs up
         # to reflection, each the same plan on its permutation of the external legs, in one launch
         orders = [(0,) + p for p in itertools.permutations(range(1, plan.legs)) if p[0] < p[-1]]
@@ -467,6 +469,7 @@ def main():
         for a in lc_hold:
             a.free()
 
+    if not args.no_extra and (not only or args.only_gggg):
         # g g > g g g g: 2^21 points, the persistent grid swept (waves per CU, block)
         p6 = plans.gluon_plan(6)
         n6 = 1 << 21
@@ -496,6 +499,7 @@ def main():
             d.free()
         f6.destroy()
 
+    if not args.no_extra and not only:
         # p p > j j j: every subprocess, Zipf(1.2) counts summing to 2^22, one persistent launch
         subs = processes.subprocesses()
         counts = processes.zipf_counts(len(subs), 1 << 22, s=1.2, seed=1)
@@ -528,7 +532,7 @@ def main():
             t.free()
             f.destroy()
 
-    if not args.no_extra and not args.only_v1:
+    if not args.no_extra and (not only or args.only_unweighting):
         # surrogate vs plain unweighting of g g > g g g g
         t = time.perf_counter()
         uw = Unweighting(legs=6, hidden=256, layers=4, batch=args.train_batch, prior=True, lib=lib)
